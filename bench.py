@@ -1,0 +1,194 @@
+"""Benchmark: complex DOF x RHS per second of the MT forward hot path on one MI355X.
+
+Workload (adaptive-MT model, one refinement cycle at its final size): 3x3 anisotropic
+conductivity blocks with random Euler angles, a 300 m Gaussian-hill topography,
+400 surface receivers, on a synthetic graded Kuhn-tetrahedron mesh.  A step is one
+frequency of the reference's per-cycle sequence (src/em_fwd.cc:1341-1360): assemble
+C and M, build the XY / YX / dual right-hand sides, set up the preconditioner, solve
+the three systems (XY, YX to 1e-8, dual to 1e-6) and run the goal-oriented
+estimator.  The steps cycle over the model's frequencies (1e-3 .. 100 Hz).
+
+value = n_edges * 3 / (wall time of one step), with the mesh already resident in
+HBM.  The mixed path (fp32 inner FCG + auxiliary-space cycles inside fp64 FGMRES,
+3 right-hand sides batched) is the measured path; the plain path (fp64 inner solve,
+one right-hand side at a time) is timed beside it for the speed-up.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_step(ctx, m, fidx, batched=True):
+    t0 = time.perf_counter()
+    ctx.assemble_matrix(fidx)
+    ctx.assemble_rhs_mt(fidx, -1, 0)
+    ctx.assemble_rhs_mt(fidx, -2, 1)
+    ctx.assemble_rhs_dual(2)
+    t1 = time.perf_counter()
+    ctx.create_pc()
+    t2 = time.perf_counter()
+    rtols = np.array([1e-8, 1e-8, 1e-6])
+    its = []
+    if batched:
+        i, r = ctx.solve_linear_system(0, 3, 100, 1e-8, rtols=rtols)
+        its = i.tolist()
+        rr = r.tolist()
+    else:
+        rr = []
+        for s in range(3):
+            i, r = ctx.solve_linear_system(s, 1, 100, rtols[s])
+            its += i.tolist()
+            rr += r.tolist()
+    t3 = time.perf_counter()
+    ctx.estimate_error(fidx, [0, 1, 2])
+    t4 = time.perf_counter()
+    return dict(total=t4 - t0, assemble_rhs=t1 - t0, pc=t2 - t1, solve=t3 - t2, estimate=t4 - t3, its=its, relres=rr)
+
+
+def cpu_baseline(budget_s=20.0):
+    """The CPU FP64 oracle (same discretisation; sparse direct solve) on a bounded
+    sample of the same workload: one frequency of the adaptive-MT model on a 16^3-cell
+    grid, assembly + 3 solves + estimator, single thread."""
+    from emfem_amd import meshgen as mg
+    from oracle import oracle as O
+    m = mg.adaptive_mt(cells=(16, 16, 16))
+    t0 = time.perf_counter()
+    om = O.Mesh(m.points, m.tets)
+    fidx = 4
+    rp, ci, C, M, _ = O.assemble(om, m, m.freqs[fidx])
+    bg = O.background_model(om, m)
+    sols = []
+    for mode in (-1, -2):
+        sols.append(O.solve_direct(rp, ci, C, M, *O.rhs_mt(om, m, bg, m.freqs[fidx], mode)))
+        if time.perf_counter() - t0 > budget_s:
+            break
+    sols.append(O.solve_direct(rp, ci, C, M, *O.rhs_dual(om, m.rx)))
+    O.estimate_error(om, m, m.freqs[fidx], sols)
+    dt = time.perf_counter() - t0
+    return dict(value=om.n_edges * len(sols) / dt, unit="complex DOF*RHS/s", cores=1, kind="port",
+                sample="oracle on adaptive-MT 16^3 cells (%d edges), f=%.3g Hz, assembly + %d direct solves + "
+                       "estimator, %.1f s" % (om.n_edges, m.freqs[fidx], len(sols), dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=48, help="grid cells per axis")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain fp64 single-RHS comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from emfem_amd import api
+    from emfem_amd import meshgen as mg
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    t0 = time.perf_counter()
+    m = mg.adaptive_mt(cells=(args.cells, args.cells, args.cells))
+    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6")
+    ctx.load_model(m)
+    ctx.create_linear_system(3)
+    info = ctx.mesh_info()
+    log("model ready: %d tets, %d edges, nnz %d, %d colours (%.1fs)" %
+        (info["n_tets"], info["n_edges"], info["nnz"], info["n_colours"], time.perf_counter() - t0))
+    E = info["n_edges"]
+    nf = len(m.freqs)
+    for w in range(args.warmup):
+        r = run_step(ctx, m, (w + 3) % nf)
+        log("warmup %d: %.3fs its %s" % (w, r["total"], r["its"]))
+    torch.cuda.synchronize()
+    steps = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        r = run_step(ctx, m, k % nf)
+        steps.append(r)
+        log("step %d f=%.3g Hz: total %.3fs (asm+rhs %.3f pc %.3f solve %.3f est %.3f) its %s relres %s" %
+            (k, m.freqs[k % nf], r["total"], r["assemble_rhs"], r["pc"], r["solve"], r["estimate"], r["its"],
+             ["%.1e" % v for v in r["relres"]]))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed / args.steps * 1e3
+    value = E * 3 / (elapsed / args.steps)
+    solve_s = sum(s["solve"] for s in steps) / len(steps)
+
+    # roofline of the dominant kernel: the fp32 B SpMM of the inner solve
+    b_ms, b_bytes = ctx.kernel_time("b_spmm")
+    op_ms, op_bytes = ctx.kernel_time("op_spmm")
+    as_ms, as_bytes = ctx.kernel_time("assemble")
+    peak = 8000.0
+    bw = (b_bytes / (b_ms * 1e-3) / 1e9) if b_ms > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(bw, 1), "peak": peak, "unit": "GB/s", "frac": round(bw / peak, 4),
+            "traffic": None, "kernel": "k_spmm (B = C + M, fp32, 6 columns)", "launch_ms": round(b_ms, 5),
+            "algorithmic_bytes": b_bytes}
+    extra = {
+        "op_spmm_GBps": round(op_bytes / (op_ms * 1e-3) / 1e9, 1) if op_ms > 0 else None,
+        "assemble_GBps": round(as_bytes / (as_ms * 1e-3) / 1e9, 1) if as_ms > 0 else None,
+        "solve_dof_rhs_per_s": round(E * 3 / solve_s, 1),
+        "phase_s": {k: round(sum(s[k] for s in steps) / len(steps), 4)
+                    for k in ("assemble_rhs", "pc", "solve", "estimate")},
+        "outer_its": [s["its"] for s in steps],
+    }
+    if not args.no_plain:
+        ctxp = api.Context("-inner_precision fp64")
+        ctxp.load_model(m)
+        ctxp.create_linear_system(3)
+        plain = [run_step(ctxp, m, k % nf, batched=False) for k in range(min(args.steps, 2))]
+        p_total = sum(p["total"] for p in plain) / len(plain)
+        p_solve = sum(p["solve"] for p in plain) / len(plain)
+        mixed_same = sum(s["total"] for s in steps[:len(plain)]) / len(plain)
+        mixed_solve = sum(s["solve"] for s in steps[:len(plain)]) / len(plain)
+        extra["plain_fp64_single_rhs"] = {"dof_rhs_per_s": round(E * 3 / p_total, 1),
+                                          "solve_dof_rhs_per_s": round(E * 3 / p_solve, 1),
+                                          "its": [p["its"] for p in plain]}
+        extra["speedup_vs_plain_step"] = round(p_total / mixed_same, 3)
+        extra["speedup_vs_plain_solve"] = round(p_solve / mixed_solve, 3)
+        ctxp.close()
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # reported, never substituted for the GPU figure
+            cpu = {"error": str(e)}
+    line = {
+        "metric": "complex DOF*RHS/s (MT forward step: assembly + 3-RHS solve + estimator), mixed/batched path",
+        "value": round(value, 1),
+        "unit": "complex DOF*RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (outer FGMRES, assembly, estimator) + f32 (inner FCG/AMS)",
+        "data": "synthetic",
+        "config": {"workload": "adaptive-MT model, final-cycle mesh (%d^3 cells, %d edges, %d tets), "
+                               "3 RHS (XY, YX, dual), cycling over 8 frequencies 1e-3..100 Hz"
+                               % (args.cells, E, info["n_tets"]),
+                   "n_edges": E, "nnz": info["nnz"], "rhs": 3, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,351 @@
+"""ctypes binding of libemfem_amd.so (include/emfem_amd.h).
+
+`Context` mirrors the reference's EMContext entry points one to one
+(create_context / process_options / assemble_matrix / assemble_rhs_* / create_pc /
+solve_linear_system / matshell_mult_a / pc_apply_b / estimate_error /
+calculate_response_*; src/em_ctx.h:66-68, src/em_fwd.h:17-51, src/em_la.h:10-20).
+Failures raise `EmfemError` carrying the status code and the library's message,
+the way the reference propagates PetscErrorCode with its SETERRQ text.
+
+There is no fallback: if the shared library is missing, importing this module
+raises, and calls that need the GPU fail with EMFEM_ERR_HIP.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libemfem_amd.so")
+
+OK, ERR_USER, ERR_HIP, ERR_NOT_CONVERGED, ERR_STATE, ERR_ARG = 0, -1, -2, -3, -4, -5
+XY_POLAR, YX_POLAR, MT_TIDX = -1, -2, -3
+
+
+class EmfemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("emfem error %d: %s" % (code, msg))
+        self.code = code
+
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+S = ctypes.c_char_p
+
+# name -> (restype, argtypes); kept in the order of include/emfem_amd.h
+SIGNATURES = {
+    "emfem_create_context": (I, [ctypes.POINTER(P)]),
+    "emfem_destroy_context": (I, [P]),
+    "emfem_last_error": (S, [P]),
+    "emfem_process_options": (I, [P, S]),
+    "emfem_get_option": (I, [P, S, ctypes.c_char_p, I]),
+    "emfem_unused_options": (I, [P, ctypes.c_char_p, I]),
+    "emfem_set_mesh": (I, [P, I, P, I, P, P]),
+    "emfem_set_mesh_edges": (I, [P, I, P, P]),
+    "emfem_set_rho": (I, [P, I, I, P]),
+    "emfem_set_frequencies": (I, [P, I, P]),
+    "emfem_set_transmitters": (I, [P, I, P]),
+    "emfem_set_receivers": (I, [P, I, P]),
+    "emfem_get_mesh_info": (I, [P, P]),
+    "emfem_get_topology": (I, [P, P, P, P, P]),
+    "emfem_find_cell": (I, [P, P]),
+    "emfem_update_background_model": (I, [P]),
+    "emfem_create_linear_system": (I, [P, I]),
+    "emfem_destroy_linear_system": (I, [P]),
+    "emfem_assemble_matrix": (I, [P, I]),
+    "emfem_get_matrix": (I, [P, P, P, P, P]),
+    "emfem_assemble_rhs_mt": (I, [P, I, I, I]),
+    "emfem_assemble_rhs_csem": (I, [P, I, I, I]),
+    "emfem_assemble_rhs_dual": (I, [P, I]),
+    "emfem_set_rhs": (I, [P, I, P, P]),
+    "emfem_get_rhs": (I, [P, I, P, P]),
+    "emfem_get_solution": (I, [P, I, P, P]),
+    "emfem_create_pc": (I, [P]),
+    "emfem_destroy_pc": (I, [P]),
+    "emfem_solve_linear_system": (I, [P, I, I, I, D, P, P, P]),
+    "emfem_matshell_mult_a": (I, [P, I, P, P]),
+    "emfem_pc_apply_b": (I, [P, I, P, P]),
+    "emfem_estimate_error": (I, [P, I, I, P, P]),
+    "emfem_interpolate_fields": (I, [P, I, I, I, P, P, P]),
+    "emfem_calculate_response_mt": (I, [P, I, I, I, I, P, P]),
+    "emfem_calculate_response_csem": (I, [P, I, I, I, I, P, P]),
+    "emfem_refine_fixed_fraction": (I, [I, P, D, P]),
+    "emfem_refine_fixed_number": (I, [I, P, D, P]),
+    "emfem_get_stat": (I, [P, S, ctypes.POINTER(D)]),
+    "emfem_kernel_time": (I, [P, S, ctypes.POINTER(D), ctypes.POINTER(D)]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libemfem_amd.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Context:
+    """One EMFEM context: a mesh, its linear system and preconditioner on one GPU."""
+
+    def __init__(self, options: str | None = None):
+        self._L = lib()
+        h = P()
+        rc = self._L.emfem_create_context(ctypes.byref(h))
+        if rc != OK:
+            raise EmfemError(rc, "create_context failed")
+        self._h = h
+        self._keep = []
+        if options:
+            self.process_options(options)
+
+    # -- plumbing ---------------------------------------------------------------
+    def _check(self, rc):
+        if rc != OK:
+            msg = self._L.emfem_last_error(self._h)
+            raise EmfemError(rc, msg.decode() if msg else "")
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.emfem_destroy_context(self._h)
+            self._h = None
+
+    destroy_context = close
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- options ---------------------------------------------------------------
+    def process_options(self, text: str):
+        self._check(self._L.emfem_process_options(self._h, text.encode()))
+
+    def get_option(self, key: str) -> str:
+        buf = ctypes.create_string_buffer(256)
+        self._check(self._L.emfem_get_option(self._h, key.encode(), buf, 256))
+        return buf.value.decode()
+
+    def unused_options(self):
+        buf = ctypes.create_string_buffer(4096)
+        self._check(self._L.emfem_unused_options(self._h, buf, 4096))
+        return buf.value.decode().split()
+
+    # -- model -----------------------------------------------------------------
+    def set_mesh(self, points, tets, attributes):
+        pts = _c(points, np.float64)
+        tt = _c(tets, np.int32)
+        at = _c(attributes, np.int32)
+        self._check(self._L.emfem_set_mesh(self._h, pts.shape[0], _p(pts), tt.shape[0], _p(tt), _p(at)))
+
+    def set_mesh_edges(self, edgelist, tet2edge):
+        el = _c(edgelist, np.int32)
+        te = _c(tet2edge, np.int32)
+        self._check(self._L.emfem_set_mesh_edges(self._h, el.shape[0], _p(el), _p(te)))
+
+    def set_rho(self, aniso_form, rho):
+        r = _c(rho, np.float64)
+        self._check(self._L.emfem_set_rho(self._h, r.shape[0], int(aniso_form), _p(r)))
+
+    def set_frequencies(self, freqs):
+        f = _c(freqs, np.float64)
+        self._check(self._L.emfem_set_frequencies(self._h, f.size, _p(f)))
+
+    def set_transmitters(self, tx):
+        t = _c(np.asarray(tx).reshape(-1, 7), np.float64)
+        self._check(self._L.emfem_set_transmitters(self._h, t.shape[0], _p(t)))
+
+    def set_receivers(self, rx):
+        r = _c(np.asarray(rx).reshape(-1, 3), np.float64)
+        self._check(self._L.emfem_set_receivers(self._h, r.shape[0], _p(r)))
+
+    def load_model(self, m):
+        """read_mesh + read_rho + read_emd from a meshgen.Model."""
+        self.set_mesh(m.points, m.tets, m.attr)
+        self.set_rho(m.aniso_form, m.rho)
+        self.set_frequencies(m.freqs)
+        if len(m.tx):
+            self.set_transmitters(m.tx)
+        self.set_receivers(m.rx)
+
+    def mesh_info(self):
+        info = np.zeros(5, dtype=np.int32)
+        self._check(self._L.emfem_get_mesh_info(self._h, _p(info)))
+        return dict(n_vertices=int(info[0]), n_tets=int(info[1]), n_edges=int(info[2]), nnz=int(info[3]),
+                    n_colours=int(info[4]))
+
+    def topology(self):
+        inf = self.mesh_info()
+        el = np.empty((inf["n_edges"], 2), dtype=np.int32)
+        te = np.empty((inf["n_tets"], 6), dtype=np.int32)
+        nb = np.empty((inf["n_tets"], 4), dtype=np.int32)
+        eb = np.empty(inf["n_edges"], dtype=np.uint8)
+        self._check(self._L.emfem_get_topology(self._h, _p(el), _p(te), _p(nb), _p(eb)))
+        return el, te, nb, eb
+
+    def find_cell(self, p):
+        q = _c(p, np.float64)
+        return int(self._L.emfem_find_cell(self._h, _p(q)))
+
+    def update_background_model(self):
+        self._check(self._L.emfem_update_background_model(self._h))
+
+    # -- linear system -----------------------------------------------------------
+    def create_linear_system(self, n_slots):
+        self._check(self._L.emfem_create_linear_system(self._h, int(n_slots)))
+
+    def destroy_linear_system(self):
+        self._check(self._L.emfem_destroy_linear_system(self._h))
+
+    def assemble_matrix(self, fidx):
+        self._check(self._L.emfem_assemble_matrix(self._h, int(fidx)))
+
+    def get_matrix(self):
+        inf = self.mesh_info()
+        rp = np.empty(inf["n_edges"] + 1, dtype=np.int32)
+        ci = np.empty(inf["nnz"], dtype=np.int32)
+        C = np.empty(inf["nnz"])
+        M = np.empty(inf["nnz"])
+        self._check(self._L.emfem_get_matrix(self._h, _p(rp), _p(ci), _p(C), _p(M)))
+        return rp, ci, C, M
+
+    def assemble_rhs_mt(self, fidx, mode, slot):
+        self._check(self._L.emfem_assemble_rhs_mt(self._h, int(fidx), int(mode), int(slot)))
+
+    def assemble_rhs_csem(self, fidx, tidx, slot):
+        self._check(self._L.emfem_assemble_rhs_csem(self._h, int(fidx), int(tidx), int(slot)))
+
+    def assemble_rhs_dual(self, slot):
+        self._check(self._L.emfem_assemble_rhs_dual(self._h, int(slot)))
+
+    def set_rhs(self, slot, s):
+        s = np.asarray(s)
+        re = _c(s.real, np.float64)
+        im = _c(s.imag, np.float64)
+        self._check(self._L.emfem_set_rhs(self._h, int(slot), _p(re), _p(im)))
+
+    def get_rhs(self, slot):
+        n = self.mesh_info()["n_edges"]
+        re = np.empty(n)
+        im = np.empty(n)
+        self._check(self._L.emfem_get_rhs(self._h, int(slot), _p(re), _p(im)))
+        return re, im
+
+    def get_solution(self, slot):
+        n = self.mesh_info()["n_edges"]
+        re = np.empty(n)
+        im = np.empty(n)
+        self._check(self._L.emfem_get_solution(self._h, int(slot), _p(re), _p(im)))
+        return re + 1j * im
+
+    def create_pc(self):
+        self._check(self._L.emfem_create_pc(self._h))
+
+    def destroy_pc(self):
+        self._check(self._L.emfem_destroy_pc(self._h))
+
+    def solve_linear_system(self, first, n, max_it, rtol, rtols=None, check=True):
+        """Returns (iterations, true relative residuals); raises on non-convergence
+        when check is True (the reference never checks, src/em_la.cc:51-52)."""
+        its = np.zeros(n, dtype=np.int32)
+        rr = np.zeros(n)
+        rt = _c(rtols, np.float64) if rtols is not None else None
+        rc = self._L.emfem_solve_linear_system(self._h, int(first), int(n), int(max_it), float(rtol),
+                                               _p(rt) if rt is not None else None, _p(its), _p(rr))
+        if rc == ERR_NOT_CONVERGED and not check:
+            return its, rr
+        self._check(rc)
+        return its, rr
+
+    def matshell_mult_a(self, n_rhs, x_ptr, y_ptr):
+        """y = A x on device pointers ([E][2 n_rhs] interleaved, fp64)."""
+        self._check(self._L.emfem_matshell_mult_a(self._h, int(n_rhs), P(x_ptr), P(y_ptr)))
+
+    def pc_apply_b(self, n_rhs, b_ptr, x_ptr):
+        self._check(self._L.emfem_pc_apply_b(self._h, int(n_rhs), P(b_ptr), P(x_ptr)))
+
+    # -- estimator and responses ----------------------------------------------------
+    def estimate_error(self, fidx, slots):
+        nt = self.mesh_info()["n_tets"]
+        sl = _c(slots, np.int32)
+        err = np.empty((sl.size, nt))
+        self._check(self._L.emfem_estimate_error(self._h, int(fidx), sl.size, _p(sl), _p(err)))
+        return err
+
+    def interpolate_fields(self, fidx, slot, points):
+        pts = _c(np.asarray(points).reshape(-1, 3), np.float64)
+        E = np.empty((pts.shape[0], 3, 2))
+        H = np.empty((pts.shape[0], 3, 2))
+        self._check(self._L.emfem_interpolate_fields(self._h, int(fidx), int(slot), pts.shape[0], _p(pts), _p(E),
+                                                     _p(H)))
+        return E[..., 0] + 1j * E[..., 1], H[..., 0] + 1j * H[..., 1]
+
+    def calculate_response_mt(self, fidx, slot_xy, slot_yx, obs):
+        ob = _c(obs, np.int32)
+        rsp = np.full((ob.shape[0], 2), np.nan)
+        self._check(self._L.emfem_calculate_response_mt(self._h, int(fidx), int(slot_xy), int(slot_yx), ob.shape[0],
+                                                        _p(ob), _p(rsp)))
+        return rsp[:, 0] + 1j * rsp[:, 1]
+
+    def calculate_response_csem(self, fidx, tidx, slot, obs):
+        ob = _c(obs, np.int32)
+        rsp = np.full((ob.shape[0], 2), np.nan)
+        self._check(self._L.emfem_calculate_response_csem(self._h, int(fidx), int(tidx), int(slot), ob.shape[0],
+                                                          _p(ob), _p(rsp)))
+        return rsp[:, 0] + 1j * rsp[:, 1]
+
+    # -- measurement ------------------------------------------------------------------
+    def stat(self, name):
+        v = D()
+        self._check(self._L.emfem_get_stat(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def kernel_time(self, which):
+        ms = D()
+        by = D()
+        self._check(self._L.emfem_kernel_time(self._h, which.encode(), ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
+
+def refine_fixed_fraction(error, fraction, flag=None):
+    e = _c(error, np.float64)
+    f = np.zeros(e.size, dtype=np.uint8) if flag is None else flag
+    rc = lib().emfem_refine_fixed_fraction(e.size, _p(e), float(fraction), _p(f))
+    if rc != OK:
+        raise EmfemError(rc, "refine_fixed_fraction")
+    return f
+
+
+def refine_fixed_number(error, fraction, flag=None):
+    e = _c(error, np.float64)
+    f = np.zeros(e.size, dtype=np.uint8) if flag is None else flag
+    rc = lib().emfem_refine_fixed_number(e.size, _p(e), float(fraction), _p(f))
+    if rc != OK:
+        raise EmfemError(rc, "refine_fixed_number")
+    return f

@@ -1,0 +1,346 @@
+// Auxiliary-space Maxwell preconditioner: host set-up.  See amg.h.
+#include "amg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <string>
+
+#include "common.h"
+
+namespace emfem {
+
+HostCsr csr_transpose(const HostCsr &A) {
+  HostCsr T;
+  T.n = A.m;
+  T.m = A.n;
+  T.rp.assign(A.m + 1, 0);
+  for (int c : A.ci) T.rp[c + 1]++;
+  for (int i = 0; i < A.m; ++i) T.rp[i + 1] += T.rp[i];
+  T.ci.resize(A.ci.size());
+  T.v.resize(A.v.size());
+  std::vector<int> fill(T.rp.begin(), T.rp.end() - 1);
+  for (int r = 0; r < A.n; ++r)
+    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
+      int q = fill[A.ci[p]]++;
+      T.ci[q] = r;
+      T.v[q] = A.v[p];
+    }
+  return T;
+}
+
+// Row-parallel product with a per-thread dense accumulator; columns sorted per row.
+HostCsr csr_multiply(const HostCsr &A, const HostCsr &B) {
+  EMF_REQUIRE(A.m == B.n, kErrArg, "csr_multiply: shape mismatch");
+  HostCsr C;
+  C.n = A.n;
+  C.m = B.m;
+  const int nth = host_threads();
+  std::vector<std::vector<int>> rows_ci(A.n);
+  std::vector<std::vector<double>> rows_v(A.n);
+  parallel_ranges(
+      A.n,
+      [&](int64_t b, int64_t e, int) {
+        std::vector<double> acc(B.m, 0.0);
+        std::vector<int> mark(B.m, -1), cols;
+        for (int64_t r = b; r < e; ++r) {
+          cols.clear();
+          for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
+            const int k = A.ci[p];
+            const double a = A.v[p];
+            for (int q = B.rp[k]; q < B.rp[k + 1]; ++q) {
+              const int c = B.ci[q];
+              if (mark[c] != (int)r) {
+                mark[c] = (int)r;
+                acc[c] = 0.0;
+                cols.push_back(c);
+              }
+              acc[c] += a * B.v[q];
+            }
+          }
+          std::sort(cols.begin(), cols.end());
+          rows_ci[r] = cols;
+          rows_v[r].resize(cols.size());
+          for (size_t i = 0; i < cols.size(); ++i) rows_v[r][i] = acc[cols[i]];
+        }
+      },
+      nth);
+  C.rp.assign(A.n + 1, 0);
+  for (int r = 0; r < A.n; ++r) C.rp[r + 1] = C.rp[r] + (int)rows_ci[r].size();
+  C.ci.resize(C.rp[A.n]);
+  C.v.resize(C.rp[A.n]);
+  parallel_ranges(A.n, [&](int64_t b, int64_t e, int) {
+    for (int64_t r = b; r < e; ++r) {
+      std::copy(rows_ci[r].begin(), rows_ci[r].end(), C.ci.begin() + C.rp[r]);
+      std::copy(rows_v[r].begin(), rows_v[r].end(), C.v.begin() + C.rp[r]);
+    }
+  });
+  return C;
+}
+
+namespace {
+
+std::vector<double> diagonal(const HostCsr &A) {
+  std::vector<double> d(A.n, 0.0);
+  for (int r = 0; r < A.n; ++r)
+    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p)
+      if (A.ci[p] == r) d[r] = A.v[p];
+  return d;
+}
+
+std::vector<double> l1_inverse(const HostCsr &A) {
+  std::vector<double> l(A.n, 0.0);
+  for (int r = 0; r < A.n; ++r) {
+    double s = 0.0;
+    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) s += std::abs(A.v[p]);
+    l[r] = s > 0.0 ? 1.0 / s : 0.0;
+  }
+  return l;
+}
+
+// Greedy aggregation on the strength graph: roots whose strong neighbours are all
+// free seed aggregates, free nodes join a neighbouring aggregate, leftovers form
+// their own.
+int aggregate(const HostCsr &A, const std::vector<double> &d, double theta, std::vector<int> &agg) {
+  const int n = A.n;
+  std::vector<int> sp(n + 1, 0), sc;
+  sc.reserve(A.ci.size());
+  for (int r = 0; r < n; ++r) {
+    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
+      const int c = A.ci[p];
+      if (c != r && std::abs(A.v[p]) >= theta * std::sqrt(std::abs(d[r] * d[c]))) sc.push_back(c);
+    }
+    sp[r + 1] = (int)sc.size();
+  }
+  agg.assign(n, -1);
+  int na = 0;
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    bool free_nb = true;
+    for (int p = sp[i]; p < sp[i + 1]; ++p)
+      if (agg[sc[p]] >= 0) {
+        free_nb = false;
+        break;
+      }
+    if (!free_nb) continue;
+    agg[i] = na;
+    for (int p = sp[i]; p < sp[i + 1]; ++p) agg[sc[p]] = na;
+    ++na;
+  }
+  std::vector<int> pass2(agg);
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    for (int p = sp[i]; p < sp[i + 1]; ++p)
+      if (agg[sc[p]] >= 0) {
+        pass2[i] = agg[sc[p]];
+        break;
+      }
+  }
+  agg.swap(pass2);
+  for (int i = 0; i < n; ++i)
+    if (agg[i] < 0) agg[i] = na++;
+  return na;
+}
+
+double spectral_radius_dinv_a(const HostCsr &A, const std::vector<double> &d) {
+  std::vector<double> x(A.n), y(A.n);
+  uint64_t s = 88172645463325252ull;
+  for (auto &v : x) {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    v = double(s % 1000003) / 1000003.0 - 0.5;
+  }
+  double lam = 1.0;
+  for (int it = 0; it < 15; ++it) {
+    double nrm = 0.0;
+    for (int r = 0; r < A.n; ++r) {
+      double acc = 0.0;
+      for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) acc += A.v[p] * x[A.ci[p]];
+      y[r] = d[r] != 0.0 ? acc / d[r] : 0.0;
+      nrm += y[r] * y[r];
+    }
+    nrm = std::sqrt(nrm);
+    double xn = 0.0;
+    for (double v : x) xn += v * v;
+    lam = nrm / std::sqrt(xn);
+    if (nrm == 0.0) break;
+    for (int r = 0; r < A.n; ++r) x[r] = y[r] / nrm;
+  }
+  return lam;
+}
+
+// P = (I - omega D^-1 A) T with T the aggregate indicator, omega = 4 / (3 rho).
+HostCsr smoothed_prolongator(const HostCsr &A, const std::vector<double> &d, const std::vector<int> &agg, int na) {
+  const double rho = spectral_radius_dinv_a(A, d);
+  const double omega = 4.0 / (3.0 * std::max(rho, 1e-300));
+  HostCsr P;
+  P.n = A.n;
+  P.m = na;
+  std::vector<std::vector<int>> rc(A.n);
+  std::vector<std::vector<double>> rv(A.n);
+  parallel_ranges(A.n, [&](int64_t b, int64_t e, int) {
+    std::vector<std::pair<int, double>> ent;
+    for (int64_t r = b; r < e; ++r) {
+      ent.clear();
+      ent.emplace_back(agg[r], 1.0);
+      const double s = d[r] != 0.0 ? -omega / d[r] : 0.0;
+      for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) ent.emplace_back(agg[A.ci[p]], s * A.v[p]);
+      std::sort(ent.begin(), ent.end(), [](const std::pair<int, double> &x, const std::pair<int, double> &y) {
+        return x.first < y.first;
+      });
+      for (size_t i = 0; i < ent.size();) {
+        size_t j = i;
+        double v = 0.0;
+        while (j < ent.size() && ent[j].first == ent[i].first) v += ent[j++].second;
+        if (v != 0.0) {
+          rc[r].push_back(ent[i].first);
+          rv[r].push_back(v);
+        }
+        i = j;
+      }
+    }
+  });
+  P.rp.assign(A.n + 1, 0);
+  for (int r = 0; r < A.n; ++r) P.rp[r + 1] = P.rp[r] + (int)rc[r].size();
+  P.ci.resize(P.rp[A.n]);
+  P.v.resize(P.rp[A.n]);
+  for (int r = 0; r < A.n; ++r) {
+    std::copy(rc[r].begin(), rc[r].end(), P.ci.begin() + P.rp[r]);
+    std::copy(rv[r].begin(), rv[r].end(), P.v.begin() + P.rp[r]);
+  }
+  return P;
+}
+
+// Dense SPD inverse by Cholesky; a vanishing pivot (singular coarse problem) is
+// regularised, which only affects the null-space component.
+std::vector<double> dense_inverse(const HostCsr &A) {
+  const int n = A.n;
+  std::vector<double> a((size_t)n * n, 0.0);
+  double dmax = 0.0;
+  for (int r = 0; r < n; ++r)
+    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
+      a[(size_t)r * n + A.ci[p]] = A.v[p];
+      if (A.ci[p] == r) dmax = std::max(dmax, std::abs(A.v[p]));
+    }
+  // symmetrise (Galerkin products are symmetric up to rounding)
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) {
+      double v = 0.5 * (a[(size_t)i * n + j] + a[(size_t)j * n + i]);
+      a[(size_t)i * n + j] = a[(size_t)j * n + i] = v;
+    }
+  std::vector<double> L((size_t)n * n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    double s = a[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) s -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+    if (s <= 1e-14 * dmax) s = std::max(1e-14 * dmax, 1e-300);
+    const double ljj = std::sqrt(s);
+    L[(size_t)j * n + j] = ljj;
+    for (int i = j + 1; i < n; ++i) {
+      double t = a[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) t -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+      L[(size_t)i * n + j] = t / ljj;
+    }
+  }
+  std::vector<double> inv((size_t)n * n, 0.0), col(n);
+  for (int c = 0; c < n; ++c) {
+    for (int i = 0; i < n; ++i) {
+      double t = (i == c) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) t -= L[(size_t)i * n + k] * col[k];
+      col[i] = t / L[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double t = col[i];
+      for (int k = i + 1; k < n; ++k) t -= L[(size_t)k * n + i] * col[k];
+      col[i] = t / L[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = col[i];
+  }
+  return inv;
+}
+
+}  // namespace
+
+void build_amg(const HostCsr &A0, const AmgParams &prm, AmgHost &H) {
+  H.levels.clear();
+  HostCsr A = A0;
+  while (A.n > prm.max_coarse && (int)H.levels.size() < prm.max_levels) {
+    std::vector<double> d = diagonal(A);
+    std::vector<int> agg;
+    int na = aggregate(A, d, prm.theta, agg);
+    if (na >= A.n * 0.9) na = aggregate(A, d, 0.0, agg);  // weak graph: coarsen on all couplings
+    if (na >= A.n * 0.9 || na == 0) break;
+    AmgLevelHost L;
+    L.P = smoothed_prolongator(A, d, agg, na);
+    L.R = csr_transpose(L.P);
+    L.l1inv = l1_inverse(A);
+    HostCsr AP = csr_multiply(A, L.P);
+    HostCsr Ac = csr_multiply(L.R, AP);
+    L.A = std::move(A);
+    H.levels.push_back(std::move(L));
+    A = std::move(Ac);
+  }
+  EMF_REQUIRE(A.n <= 4096, kErrUser,
+              "auxiliary AMG stopped coarsening at " + std::to_string(A.n) + " unknowns");
+  H.ncoarse = A.n;
+  H.coarse_inv = dense_inverse(A);
+}
+
+void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm, AmsHost &H) {
+  // interior vertex numbering
+  std::vector<int> vmap(T.nv, -1);
+  int nv0 = 0;
+  for (int v = 0; v < T.nv; ++v)
+    if (!T.vertex_bdr[v]) vmap[v] = nv0++;
+  H.nv0 = nv0;
+  EMF_REQUIRE(nv0 > 0, kErrUser, "mesh has no interior vertex");
+  auto make = [&](int comp) {
+    HostCsr G;
+    G.n = T.ne;
+    G.m = nv0;
+    G.rp.assign(T.ne + 1, 0);
+    for (int e = 0; e < T.ne; ++e) {
+      const int a = T.edgelist[e * 2], b = T.edgelist[e * 2 + 1];
+      double va, vb;
+      if (comp < 0) {
+        va = -1.0;  // G row e: -1 at the lower vertex, +1 at the higher (src/em_la.cc:264-267)
+        vb = 1.0;
+      } else {
+        const double t = T.pts[(size_t)b * 3 + comp] - T.pts[(size_t)a * 3 + comp];
+        va = vb = 0.5 * t;
+      }
+      if (vmap[a] >= 0 && vmap[b] >= 0 && vmap[a] > vmap[b]) {
+        G.ci.push_back(vmap[b]);
+        G.v.push_back(vb);
+        G.ci.push_back(vmap[a]);
+        G.v.push_back(va);
+      } else {
+        if (vmap[a] >= 0) {
+          G.ci.push_back(vmap[a]);
+          G.v.push_back(va);
+        }
+        if (vmap[b] >= 0) {
+          G.ci.push_back(vmap[b]);
+          G.v.push_back(vb);
+        }
+      }
+      G.rp[e + 1] = (int)G.ci.size();
+    }
+    return G;
+  };
+  H.G0 = make(-1);
+  H.G0T = csr_transpose(H.G0);
+  {
+    HostCsr BG = csr_multiply(B, H.G0);
+    build_amg(csr_multiply(H.G0T, BG), prm, H.amgG);
+  }
+  for (int d = 0; d < 3; ++d) {
+    H.Pi[d] = make(d);
+    H.PiT[d] = csr_transpose(H.Pi[d]);
+    HostCsr BP = csr_multiply(B, H.Pi[d]);
+    build_amg(csr_multiply(H.PiT[d], BP), prm, H.amgPi[d]);
+  }
+  H.l1inv = l1_inverse(B);
+}
+
+}  // namespace emfem

@@ -1,0 +1,62 @@
+// Auxiliary-space Maxwell preconditioner: host set-up.
+//
+// The reference hands B = C + M, the discrete gradient G (src/em_la.cc:230-295) and
+// the vertex coordinates to HYPRE's AMS (src/em_la.cc:143-146), whose internals are
+// not part of the reference.  This is a from-scratch auxiliary-space
+// preconditioner in the same family: Chebyshev smoothing on the edge space plus
+// corrections in the nodal gradient space G and the nodal vector space Pi, each
+// solved approximately by smoothed-aggregation AMG.  Boundary vertices are left out
+// of the auxiliary spaces because the Dirichlet edges they span are decoupled in B
+// (src/em_fwd.cc:377-388).  The set-up (aggregation, prolongators, Galerkin
+// products, dense coarsest inverse) runs here on the host in fp64; the cycles run on
+// the device.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "topology.h"
+
+namespace emfem {
+
+struct HostCsr {
+  int n = 0, m = 0;  // rows, columns
+  std::vector<int> rp, ci;
+  std::vector<double> v;
+  int64_t nnz() const { return (int64_t)ci.size(); }
+};
+
+HostCsr csr_transpose(const HostCsr &A);
+HostCsr csr_multiply(const HostCsr &A, const HostCsr &B);
+
+struct AmgLevelHost {
+  HostCsr A, P, R;           // R = P^T
+  std::vector<double> l1inv; // 1 / sum_j |a_ij|
+};
+
+struct AmgHost {
+  std::vector<AmgLevelHost> levels;  // levels[0].A is the fine matrix
+  int ncoarse = 0;
+  std::vector<double> coarse_inv;    // dense ncoarse x ncoarse
+};
+
+struct AmgParams {
+  double theta = 0.08;     // strength threshold |a_ij| >= theta sqrt(a_ii a_jj)
+  int max_coarse = 256;    // dense solve below this size
+  int max_levels = 20;
+};
+
+void build_amg(const HostCsr &A, const AmgParams &p, AmgHost &H);
+
+struct AmsHost {
+  HostCsr G0, G0T;        // E x V0 gradient restricted to interior vertices, and transpose
+  HostCsr Pi[3], PiT[3];  // E x V0 nodal vector interpolation per component
+  AmgHost amgG, amgPi[3];
+  std::vector<double> l1inv;  // edge-space l1 row norms of B, inverted
+  int nv0 = 0;
+};
+
+// B given on the edge pattern (fp64 values).
+void build_ams(const Topology &T, const HostCsr &B, const AmgParams &p, AmsHost &H);
+
+}  // namespace emfem

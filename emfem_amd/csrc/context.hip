@@ -1,0 +1,873 @@
+// The EMFEM context and its C ABI (include/emfem_amd.h).
+//
+// Reference: EMContext (src/em_ctx.h:15-63), create_context / destroy_context /
+// process_options (src/em_ctx.cc:7-175) and the hot-path entry points of
+// src/em_fwd.cc and src/em_la.cc cited per function in the header.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cctype>
+#include <functional>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/emfem_amd.h"
+#include "amg.h"
+#include "common.h"
+#include "fe.h"
+#include "kernels.h"
+#include "model.h"
+#include "solver.h"
+#include "topology.h"
+
+using namespace emfem;
+
+namespace {
+
+// Option keys with their defaults: the EMFEM keys of process_options
+// (src/em_ctx.cc:100-175) and the solver keys this implementation honours.
+const std::map<std::string, std::string> &option_defaults() {
+  static const std::map<std::string, std::string> d = {
+      {"iprefix", ""},
+      {"oprefix", ""},
+      {"n_groups", "1"},
+      {"max_tx_edge_length", "-1"},
+      {"max_rx_edge_length", "-1"},
+      {"mesh_format", "mdl"},
+      {"inner_pc_type", "mixed"},
+      {"pc_threshold", "500000"},
+      {"direct_solver_type", "mumps"},
+      {"K_max_it", "100"},
+      {"e_rtol", "1e-8"},
+      {"dual_rtol", "1e-6"},
+      {"max_adaptive_refinements", "0"},
+      {"n_uniform_refinements", "0"},
+      {"max_dofs", "1000000"},
+      {"refine_fraction", "0.1"},
+      {"refine_strategy", "fixed_fraction"},
+      {"save_mesh", "true"},
+      {"n_tx_divisions", "20"},
+      // solver keys (A_ = outer FGMRES, B_ = inner FCG, src/em_la.cc:126-187)
+      {"A_ksp_gmres_restart", "30"},
+      {"B_ksp_rtol", "1e-2"},
+      {"B_ksp_max_it", "100"},
+      // this implementation
+      {"inner_precision", "fp32"},
+      {"ams_cheb_degree", "3"},
+      {"ams_cheb_ratio", "30"},
+      {"ams_theta", "0.08"},
+  };
+  return d;
+}
+
+const std::set<std::string> &enum_values(const std::string &key) {
+  static const std::map<std::string, std::set<std::string>> e = {
+      {"mesh_format", {"mdl", "mesh"}},
+      {"inner_pc_type", {"mixed", "ams", "direct"}},
+      {"direct_solver_type", {"mumps", "superlu_dist"}},
+      {"refine_strategy", {"fixed_number", "fixed_fraction"}},
+      {"inner_precision", {"fp32", "fp64"}},
+  };
+  static const std::set<std::string> none;
+  auto it = e.find(key);
+  return it == e.end() ? none : it->second;
+}
+
+bool looks_numeric(const std::string &s) {
+  if (s.empty()) return false;
+  char *end = nullptr;
+  std::strtod(s.c_str(), &end);
+  return end && *end == '\0';
+}
+
+}  // namespace
+
+struct emfem_context {
+  hipStream_t stream = nullptr;
+  bool device = false;  // false: host-only context (topology, options, point location)
+  std::string err;
+  std::map<std::string, std::string> opts, given;
+
+  Topology T;
+  bool have_mesh = false;
+  int aniso_form = -1, n_rho = 0;
+  std::vector<double> rho, s6;
+  std::vector<double> freqs, tx, rx;
+  Background bg;
+
+  // device mesh
+  DevBuf<int> d_tets, d_t2e, d_attr, d_nb, d_non, d_colour;
+  DevBuf<double> d_pts, d_s6;
+  DevBuf<uint8_t> d_ebdr;
+
+  // linear system
+  bool have_system = false;
+  int nnz = 0, nslots = 0, assembled_fidx = -1;
+  DevBuf<int> d_rp, d_ci;
+  DevBuf<double2> d_K;
+  DevBuf<double> d_S, d_X;  // [E][2 nslots]
+
+  // preconditioner
+  std::unique_ptr<AmsHost> ams_host;
+  std::unique_ptr<InnerPc> pc;
+
+  std::map<std::string, double> stats;
+  KernelClock op_clock;
+  double assemble_kernel_ms = 0.0;
+
+  std::string opt(const std::string &k) const {
+    auto it = opts.find(k);
+    return it == opts.end() ? std::string() : it->second;
+  }
+  double optd(const std::string &k) const { return std::strtod(opt(k).c_str(), nullptr); }
+  int opti(const std::string &k) const { return (int)std::strtol(opt(k).c_str(), nullptr, 10); }
+};
+
+namespace {
+
+template <typename F>
+int guarded(emfem_context *ctx, F &&f) {
+  if (!ctx) return EMFEM_ERR_ARG;
+  try {
+    f();
+    return EMFEM_OK;
+  } catch (const Error &e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    ctx->err = e.what();
+    return EMFEM_ERR_USER;
+  }
+}
+
+void require_mesh(const emfem_context *c) { EMF_REQUIRE(c->have_mesh, kErrState, "no mesh: call emfem_set_mesh first"); }
+void require_system(const emfem_context *c) {
+  EMF_REQUIRE(c->have_system, kErrState, "no linear system: call emfem_create_linear_system first");
+}
+void require_slot(const emfem_context *c, int slot) {
+  require_system(c);
+  EMF_REQUIRE(slot >= 0 && slot < c->nslots, kErrArg, "slot " + std::to_string(slot) + " out of range");
+}
+void require_freq(const emfem_context *c, int fidx) {
+  EMF_REQUIRE(fidx >= 0 && fidx < (int)c->freqs.size(), kErrArg, "frequency index " + std::to_string(fidx) + " out of range");
+}
+
+void require_device(const emfem_context *c) {
+  EMF_REQUIRE(c->device, kErrHip, "no HIP device available to this context");
+}
+
+void upload_mesh(emfem_context *c) {
+  if (!c->device) return;
+  hipStream_t s = c->stream;
+  const Topology &T = c->T;
+  c->d_tets.upload(T.tets, s);
+  c->d_t2e.upload(T.tet2edge, s);
+  c->d_attr.upload(T.attr, s);
+  c->d_nb.upload(T.neighbor, s);
+  c->d_non.upload(T.non, s);
+  c->d_colour.upload(T.color_order, s);
+  c->d_pts.upload(T.pts, s);
+  c->d_ebdr.upload(T.edge_bdr, s);
+  EMF_HIP(hipStreamSynchronize(s));
+}
+
+void check_attributes(emfem_context *c) {
+  if (!c->have_mesh || c->n_rho == 0) return;
+  for (int t = 0; t < c->T.nt; ++t)
+    EMF_REQUIRE(c->T.attr[t] >= 0 && c->T.attr[t] < c->n_rho, kErrUser,
+                "tet " + std::to_string(t) + " has attribute " + std::to_string(c->T.attr[t]) +
+                    " outside the resistivity table");
+}
+
+// Copies the (re, im) columns of `slot` between a [E][2 nslots] block and a host pair.
+void put_slot(emfem_context *c, DevBuf<double> &blk, int slot, const double *re, const double *im) {
+  std::vector<double> tmp((size_t)c->T.ne * 2);
+  for (int e = 0; e < c->T.ne; ++e) {
+    tmp[(size_t)e * 2] = re ? re[e] : 0.0;
+    tmp[(size_t)e * 2 + 1] = im ? im[e] : 0.0;
+  }
+  EMF_HIP(hipMemcpy2DAsync(blk.get() + 2 * slot, sizeof(double) * 2 * c->nslots, tmp.data(), sizeof(double) * 2,
+                           sizeof(double) * 2, c->T.ne, hipMemcpyHostToDevice, c->stream));
+  EMF_HIP(hipStreamSynchronize(c->stream));
+}
+
+void get_slot(const emfem_context *c, const DevBuf<double> &blk, int slot, double *re, double *im) {
+  std::vector<double> tmp((size_t)c->T.ne * 2);
+  EMF_HIP(hipMemcpy2DAsync(tmp.data(), sizeof(double) * 2, blk.get() + 2 * slot, sizeof(double) * 2 * c->nslots,
+                           sizeof(double) * 2, c->T.ne, hipMemcpyDeviceToHost, c->stream));
+  EMF_HIP(hipStreamSynchronize(c->stream));
+  for (int e = 0; e < c->T.ne; ++e) {
+    if (re) re[e] = tmp[(size_t)e * 2];
+    if (im) im[e] = tmp[(size_t)e * 2 + 1];
+  }
+}
+
+PcOptions pc_options(const emfem_context *c) {
+  PcOptions o;
+  o.cheb_degree = c->opti("ams_cheb_degree");
+  o.cheb_ratio = c->optd("ams_cheb_ratio");
+  o.inner_rtol = c->optd("B_ksp_rtol");
+  o.inner_max_it = c->opti("B_ksp_max_it");
+  o.amg.theta = c->optd("ams_theta");
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+int emfem_create_context(emfem_context **out) {
+  if (!out) return EMFEM_ERR_ARG;
+  *out = nullptr;
+  auto *c = new (std::nothrow) emfem_context();
+  if (!c) return EMFEM_ERR_USER;
+  c->opts = option_defaults();
+  int rc = guarded(c, [&]() {
+    // Without a device the context still serves the host-side calls; every call
+    // that needs the device then fails with EMFEM_ERR_HIP.
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) ndev = 0;
+    (void)hipGetLastError();
+    if (ndev > 0) {
+      EMF_HIP(hipStreamCreate(&c->stream));
+      c->device = true;
+    }
+  });
+  if (rc != EMFEM_OK) {
+    std::fprintf(stderr, "emfem_create_context: %s\n", c->err.c_str());
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return EMFEM_OK;
+}
+
+int emfem_destroy_context(emfem_context *c) {
+  if (!c) return EMFEM_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->pc.reset();
+  hipStream_t s = c->stream;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+  return EMFEM_OK;
+}
+
+const char *emfem_last_error(const emfem_context *c) { return c ? c->err.c_str() : "null context"; }
+
+int emfem_process_options(emfem_context *c, const char *text) {
+  return guarded(c, [&]() {
+    EMF_REQUIRE(text != nullptr, kErrArg, "options text is null");
+    // strip '#' comments per line (parse_string, src/em_utils.cc:34-58), then tokenize
+    std::stringstream in(text), clean;
+    std::string line;
+    while (std::getline(in, line)) {
+      auto p = line.find('#');
+      if (p != std::string::npos) line.erase(p);
+      clean << line << "\n";
+    }
+    std::vector<std::string> tok;
+    std::string t;
+    while (clean >> t) tok.push_back(t);
+    for (size_t i = 0; i < tok.size(); ++i) {
+      EMF_REQUIRE(tok[i].size() > 1 && tok[i][0] == '-', kErrUser, "unexpected token '" + tok[i] + "' in options");
+      std::string key = tok[i].substr(1), val = "true";
+      if (i + 1 < tok.size() && (tok[i + 1][0] != '-' || looks_numeric(tok[i + 1]))) val = tok[++i];
+      c->given[key] = val;
+      const auto &allowed = enum_values(key);
+      if (!allowed.empty())
+        EMF_REQUIRE(allowed.count(val), kErrUser, "invalid value '" + val + "' for -" + key);
+      if (c->opts.count(key)) c->opts[key] = val;
+    }
+  });
+}
+
+int emfem_get_option(const emfem_context *c, const char *key, char *buf, int buflen) {
+  if (!c || !key || !buf || buflen <= 0) return EMFEM_ERR_ARG;
+  auto it = c->opts.find(key);
+  if (it == c->opts.end()) return EMFEM_ERR_ARG;
+  std::snprintf(buf, (size_t)buflen, "%s", it->second.c_str());
+  return EMFEM_OK;
+}
+
+int emfem_unused_options(const emfem_context *c, char *buf, int buflen) {
+  if (!c || !buf || buflen <= 0) return EMFEM_ERR_ARG;
+  std::string out;
+  for (auto &kv : c->given)
+    if (!c->opts.count(kv.first)) out += (out.empty() ? "" : " ") + kv.first;
+  std::snprintf(buf, (size_t)buflen, "%s", out.c_str());
+  return EMFEM_OK;
+}
+
+int emfem_set_mesh(emfem_context *c, int n_points, const double *points, int n_tets, const int *tets,
+                   const int *attributes) {
+  return guarded(c, [&]() {
+    EMF_REQUIRE(n_points > 0 && n_tets > 0 && points && tets && attributes, kErrArg, "empty or null mesh arrays");
+    c->pc.reset();
+    c->have_system = false;
+    c->have_mesh = false;
+    c->bg = Background();
+    Topology &T = c->T;
+    T = Topology();
+    T.nv = n_points;
+    T.nt = n_tets;
+    T.pts.assign(points, points + (size_t)n_points * 3);
+    T.tets.assign(tets, tets + (size_t)n_tets * 4);
+    T.attr.assign(attributes, attributes + n_tets);
+    build_topology(T);
+    c->have_mesh = true;
+    check_attributes(c);
+    upload_mesh(c);
+  });
+}
+
+int emfem_set_mesh_edges(emfem_context *c, int n_edges, const int *edgelist, const int *tet2edge) {
+  return guarded(c, [&]() {
+    require_mesh(c);
+    EMF_REQUIRE(edgelist && tet2edge, kErrArg, "null edge arrays");
+    c->pc.reset();
+    c->have_system = false;
+    set_user_edges(c->T, n_edges, edgelist, tet2edge);
+    upload_mesh(c);
+  });
+}
+
+int emfem_set_rho(emfem_context *c, int n_rho, int aniso_form, const double *rho) {
+  return guarded(c, [&]() {
+    EMF_REQUIRE(n_rho > 0 && rho, kErrArg, "empty resistivity table");
+    const int nc = rho_columns(aniso_form);
+    EMF_REQUIRE(nc > 0, kErrUser, "Unsupported anisotropy form " + std::to_string(aniso_form) + ".");
+    c->aniso_form = aniso_form;
+    c->n_rho = n_rho;
+    c->rho.assign(rho, rho + (size_t)n_rho * nc);
+    make_sigma_table(aniso_form, n_rho, rho, c->s6);
+    if (c->device) {
+      c->d_s6.upload(c->s6, c->stream);
+      EMF_HIP(hipStreamSynchronize(c->stream));
+    }
+    check_attributes(c);
+  });
+}
+
+int emfem_set_frequencies(emfem_context *c, int n, const double *f) {
+  return guarded(c, [&]() {
+    EMF_REQUIRE(n >= 0 && (n == 0 || f), kErrArg, "bad frequency array");
+    c->freqs.assign(f, f + n);
+  });
+}
+
+int emfem_set_transmitters(emfem_context *c, int n, const double *tx) {
+  return guarded(c, [&]() {
+    EMF_REQUIRE(n >= 0 && (n == 0 || tx), kErrArg, "bad transmitter array");
+    c->tx.assign(tx, tx + (size_t)n * 7);
+  });
+}
+
+int emfem_set_receivers(emfem_context *c, int n, const double *rx) {
+  return guarded(c, [&]() {
+    EMF_REQUIRE(n >= 0 && (n == 0 || rx), kErrArg, "bad receiver array");
+    c->rx.assign(rx, rx + (size_t)n * 3);
+  });
+}
+
+int emfem_get_mesh_info(const emfem_context *c, int *info) {
+  if (!c || !info) return EMFEM_ERR_ARG;
+  info[0] = c->T.nv;
+  info[1] = c->T.nt;
+  info[2] = c->T.ne;
+  info[3] = c->have_system ? c->nnz : 0;
+  info[4] = c->T.color_ptr.empty() ? 0 : (int)c->T.color_ptr.size() - 1;
+  return EMFEM_OK;
+}
+
+int emfem_get_topology(const emfem_context *c, int *edgelist, int *tet2edge, int *neighbor, unsigned char *edge_bdr) {
+  if (!c) return EMFEM_ERR_ARG;
+  if (!c->have_mesh) return EMFEM_ERR_STATE;
+  if (edgelist) std::memcpy(edgelist, c->T.edgelist.data(), sizeof(int) * c->T.edgelist.size());
+  if (tet2edge) std::memcpy(tet2edge, c->T.tet2edge.data(), sizeof(int) * c->T.tet2edge.size());
+  if (neighbor) std::memcpy(neighbor, c->T.neighbor.data(), sizeof(int) * c->T.neighbor.size());
+  if (edge_bdr) std::memcpy(edge_bdr, c->T.edge_bdr.data(), c->T.edge_bdr.size());
+  return EMFEM_OK;
+}
+
+int emfem_find_cell(const emfem_context *c, const double *p) {
+  if (!c || !p || !c->have_mesh) return -1;
+  return find_cell(c->T, p);
+}
+
+int emfem_update_background_model(emfem_context *c) {
+  return guarded(c, [&]() {
+    require_mesh(c);
+    EMF_REQUIRE(c->n_rho > 0, kErrState, "no resistivity table");
+    update_background(c->T, c->s6, c->bg);
+  });
+}
+
+int emfem_create_linear_system(emfem_context *c, int n_slots) {
+  return guarded(c, [&]() {
+    require_mesh(c);
+    require_device(c);
+    EMF_REQUIRE(n_slots > 0 && n_slots <= 512, kErrArg, "n_slots must be in 1..512");
+    const double t0 = now_ms();
+    c->pc.reset();
+    std::vector<int> rp, ci;
+    build_pattern(c->T, rp, ci);
+    c->nnz = (int)ci.size();
+    c->d_rp.upload(rp, c->stream);
+    c->d_ci.upload(ci, c->stream);
+    c->d_K.resize(ci.size());
+    c->nslots = n_slots;
+    c->d_S.resize((size_t)c->T.ne * 2 * n_slots);
+    c->d_X.resize((size_t)c->T.ne * 2 * n_slots);
+    c->d_S.zero(c->stream);
+    c->d_X.zero(c->stream);
+    EMF_HIP(hipStreamSynchronize(c->stream));
+    c->have_system = true;
+    c->assembled_fidx = -1;
+    c->stats["create_ls_ms"] = now_ms() - t0;
+  });
+}
+
+int emfem_destroy_linear_system(emfem_context *c) {
+  return guarded(c, [&]() {
+    c->pc.reset();
+    c->d_rp.release();
+    c->d_ci.release();
+    c->d_K.release();
+    c->d_S.release();
+    c->d_X.release();
+    c->have_system = false;
+    c->nslots = 0;
+    c->nnz = 0;
+  });
+}
+
+int emfem_assemble_matrix(emfem_context *c, int fidx) {
+  return guarded(c, [&]() {
+    require_system(c);
+    require_freq(c, fidx);
+    EMF_REQUIRE(c->n_rho > 0, kErrState, "no resistivity table");
+    const double omega = 2 * kPi * c->freqs[fidx];
+    hipStream_t s = c->stream;
+    hipEvent_t e0, e1;
+    EMF_HIP(hipEventCreate(&e0));
+    EMF_HIP(hipEventCreate(&e1));
+    const double t0 = now_ms();
+    EMF_HIP(hipMemsetAsync(c->d_K.get(), 0, sizeof(double2) * c->nnz, s));
+    EMF_HIP(hipEventRecord(e0, s));
+    const auto &cp = c->T.color_ptr;
+    for (size_t k = 0; k + 1 < cp.size(); ++k)
+      assemble_colour(s, cp[k + 1] - cp[k], c->d_colour.get() + cp[k], c->d_tets.get(), c->d_t2e.get(),
+                      c->d_pts.get(), c->d_attr.get(), c->d_s6.get(), c->d_ebdr.get(), omega, c->d_rp.get(),
+                      c->d_ci.get(), c->d_K.get());
+    EMF_HIP(hipEventRecord(e1, s));
+    EMF_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    EMF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->assemble_kernel_ms = ms;
+    c->stats["assemble_ms"] = now_ms() - t0;
+    c->stats["assemble_kernel_ms"] = ms;
+    c->assembled_fidx = fidx;
+    c->pc.reset();
+  });
+}
+
+int emfem_get_matrix(const emfem_context *c, int *rowptr, int *col, double *C, double *M) {
+  if (!c) return EMFEM_ERR_ARG;
+  if (!c->have_system) return EMFEM_ERR_STATE;
+  try {
+    hipStream_t s = c->stream;
+    if (rowptr) c->d_rp.download(rowptr, c->T.ne + 1, s);
+    if (col) c->d_ci.download(col, c->nnz, s);
+    if (C || M) {
+      std::vector<double2> K(c->nnz);
+      c->d_K.download(K.data(), c->nnz, s);
+      EMF_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i < c->nnz; ++i) {
+        if (C) C[i] = K[i].x;
+        if (M) M[i] = K[i].y;
+      }
+    }
+    EMF_HIP(hipStreamSynchronize(s));
+  } catch (const Error &e) {
+    const_cast<emfem_context *>(c)->err = e.what();
+    return e.code;
+  }
+  return EMFEM_OK;
+}
+
+int emfem_assemble_rhs_mt(emfem_context *c, int fidx, int mode, int slot) {
+  return guarded(c, [&]() {
+    require_slot(c, slot);
+    require_freq(c, fidx);
+    EMF_REQUIRE(mode == EMFEM_XY_POLAR || mode == EMFEM_YX_POLAR, kErrArg, "mode must be -1 (XY) or -2 (YX)");
+    if (!c->bg.ready) update_background(c->T, c->s6, c->bg);
+    std::vector<double> re, im;
+    rhs_mt(c->T, c->s6, c->bg, c->freqs[fidx], mode, re, im);
+    put_slot(c, c->d_S, slot, re.data(), im.data());
+  });
+}
+
+int emfem_assemble_rhs_csem(emfem_context *c, int fidx, int tidx, int slot) {
+  return guarded(c, [&]() {
+    require_slot(c, slot);
+    require_freq(c, fidx);
+    EMF_REQUIRE(tidx >= 0 && (size_t)tidx * 7 < c->tx.size(), kErrArg, "transmitter index out of range");
+    std::vector<double> re, im;
+    rhs_csem(c->T, c->freqs[fidx], &c->tx[(size_t)tidx * 7], c->opti("n_tx_divisions"), re, im);
+    put_slot(c, c->d_S, slot, re.data(), im.data());
+  });
+}
+
+int emfem_assemble_rhs_dual(emfem_context *c, int slot) {
+  return guarded(c, [&]() {
+    require_slot(c, slot);
+    std::vector<double> re, im;
+    rhs_dual(c->T, c->rx, re, im);
+    put_slot(c, c->d_S, slot, re.data(), im.data());
+  });
+}
+
+int emfem_set_rhs(emfem_context *c, int slot, const double *re, const double *im) {
+  return guarded(c, [&]() {
+    require_slot(c, slot);
+    put_slot(c, c->d_S, slot, re, im);
+  });
+}
+
+int emfem_get_rhs(const emfem_context *c, int slot, double *re, double *im) {
+  auto *cc = const_cast<emfem_context *>(c);
+  return guarded(cc, [&]() {
+    require_slot(c, slot);
+    get_slot(c, c->d_S, slot, re, im);
+  });
+}
+
+int emfem_get_solution(const emfem_context *c, int slot, double *re, double *im) {
+  auto *cc = const_cast<emfem_context *>(c);
+  return guarded(cc, [&]() {
+    require_slot(c, slot);
+    get_slot(c, c->d_X, slot, re, im);
+  });
+}
+
+int emfem_create_pc(emfem_context *c) {
+  return guarded(c, [&]() {
+    require_system(c);
+    EMF_REQUIRE(c->assembled_fidx >= 0, kErrState, "matrix not assembled");
+    const double t0 = now_ms();
+    hipStream_t s = c->stream;
+    // B = C + M on the host for the auxiliary-space set-up
+    std::vector<double2> K(c->nnz);
+    c->d_K.download(K.data(), c->nnz, s);
+    HostCsr B;
+    B.n = B.m = c->T.ne;
+    B.rp.resize(c->T.ne + 1);
+    B.ci.resize(c->nnz);
+    c->d_rp.download(B.rp.data(), c->T.ne + 1, s);
+    c->d_ci.download(B.ci.data(), c->nnz, s);
+    EMF_HIP(hipStreamSynchronize(s));
+    B.v.resize(c->nnz);
+    for (int i = 0; i < c->nnz; ++i) B.v[i] = K[i].x + K[i].y;
+    PcOptions o = pc_options(c);
+    c->ams_host.reset(new AmsHost());
+    build_ams(c->T, B, o.amg, *c->ams_host);
+    const double t1 = now_ms();
+    c->pc = make_inner_pc(c->opt("inner_precision") == "fp32", *c->ams_host, c->T.ne, c->nnz, c->d_rp.get(),
+                          c->d_ci.get(), c->d_K.get(), o, s);
+    EMF_HIP(hipStreamSynchronize(s));
+    c->stats["pc_setup_host_ms"] = t1 - t0;
+    c->stats["pc_setup_ms"] = now_ms() - t0;
+  });
+}
+
+int emfem_destroy_pc(emfem_context *c) {
+  return guarded(c, [&]() {
+    c->pc.reset();
+    c->ams_host.reset();
+  });
+}
+
+int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, double rtol, const double *rtols,
+                              int *iters, double *relres) {
+  int conv_rc = EMFEM_OK;
+  int rc = guarded(c, [&]() {
+    require_system(c);
+    EMF_REQUIRE(n > 0 && first >= 0 && first + n <= c->nslots, kErrArg, "slot range out of bounds");
+    EMF_REQUIRE(c->pc, kErrState, "no preconditioner: call emfem_create_pc first");
+    EMF_REQUIRE(max_it > 0, kErrArg, "max_it must be positive");
+    hipStream_t s = c->stream;
+    const int ne = c->T.ne, nc = 2 * n;
+    DevBuf<double> b((size_t)ne * nc), x((size_t)ne * nc);
+    const size_t pitch = sizeof(double) * 2 * c->nslots;
+    EMF_HIP(hipMemcpy2DAsync(b.get(), sizeof(double) * nc, c->d_S.get() + 2 * first, pitch, sizeof(double) * nc, ne,
+                             hipMemcpyDeviceToDevice, s));
+    std::vector<double> tol(n, rtol);
+    if (rtols)
+      for (int i = 0; i < n; ++i) tol[i] = rtols[i];
+    OuterOptions oo;
+    oo.restart = c->opti("A_ksp_gmres_restart");
+    oo.max_it = max_it;
+    SolveStats st;
+    const double t0 = now_ms();
+    fgmres(s, ne, c->nnz, c->d_rp.get(), c->d_ci.get(), c->d_K.get(), *c->pc, n, b.get(), x.get(), tol, oo, st,
+           &c->op_clock);
+    EMF_HIP(hipMemcpy2DAsync(c->d_X.get() + 2 * first, pitch, x.get(), sizeof(double) * nc, sizeof(double) * nc, ne,
+                             hipMemcpyDeviceToDevice, s));
+    EMF_HIP(hipStreamSynchronize(s));
+    const double ms = now_ms() - t0;
+    c->stats["solve_ms"] = ms;
+    c->stats["outer_its"] = st.outer_its;
+    c->stats["inner_its"] = (double)st.inner_its;
+    c->stats["pc_applies"] = (double)st.pc_applies;
+    c->stats["vcycles"] = (double)st.vcycles;
+    int cnt = 0;
+    double opms = c->op_clock.average_ms(&cnt);
+    c->stats["op_spmm_avg_ms"] = opms;
+    c->stats["op_spmm_launches_timed"] = cnt;
+    c->stats["op_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * 20 + double(ne) * n * 32;
+    double bms = c->pc->b_clock().average_ms(&cnt);
+    c->stats["b_spmm_avg_ms"] = bms;
+    c->stats["b_spmm_launches_timed"] = cnt;
+    const double vb = c->pc->fp32() ? 4.0 : 8.0;
+    c->stats["b_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * (4 + vb) + double(ne) * nc * 2 * vb;
+    for (int i = 0; i < n; ++i) {
+      if (iters) iters[i] = st.its[i];
+      if (relres) relres[i] = st.relres[i];
+      if (!(st.relres[i] <= tol[i] * 1.0000001)) conv_rc = EMFEM_ERR_NOT_CONVERGED;
+    }
+    if (conv_rc != EMFEM_OK) c->err = "linear solve did not reach its tolerance within max_it";
+  });
+  return rc != EMFEM_OK ? rc : conv_rc;
+}
+
+int emfem_matshell_mult_a(emfem_context *c, int n_rhs, const double *x, double *y) {
+  return guarded(c, [&]() {
+    require_system(c);
+    EMF_REQUIRE(n_rhs > 0 && x && y && x != y, kErrArg, "bad operator arguments");
+    op_spmm<double>(c->stream, c->T.ne, c->nnz, c->d_rp.get(), c->d_ci.get(), c->d_K.get(), n_rhs, x, 2 * n_rhs, y,
+                    2 * n_rhs, 1.0, 0.0, nullptr, 0);
+    EMF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int emfem_pc_apply_b(emfem_context *c, int n_rhs, const double *b, double *x) {
+  return guarded(c, [&]() {
+    require_system(c);
+    EMF_REQUIRE(c->pc, kErrState, "no preconditioner: call emfem_create_pc first");
+    EMF_REQUIRE(n_rhs > 0 && b && x && b != x, kErrArg, "bad preconditioner arguments");
+    SolveStats st;
+    c->pc->apply64(c->stream, 2 * n_rhs, b, x, st);
+    EMF_HIP(hipStreamSynchronize(c->stream));
+    c->stats["inner_its"] = (double)st.inner_its;
+  });
+}
+
+int emfem_estimate_error(emfem_context *c, int fidx, int n, const int *slots, double *errors) {
+  return guarded(c, [&]() {
+    require_system(c);
+    require_freq(c, fidx);
+    EMF_REQUIRE(n > 0 && n <= 8 && slots && errors, kErrArg, "bad estimator arguments");
+    for (int i = 0; i < n; ++i) require_slot(c, slots[i]);
+    hipStream_t s = c->stream;
+    DevBuf<int> d_slots;
+    d_slots.upload(slots, n, s);
+    DevBuf<double> d_err((size_t)n * c->T.nt);
+    hipEvent_t e0, e1;
+    EMF_HIP(hipEventCreate(&e0));
+    EMF_HIP(hipEventCreate(&e1));
+    const double t0 = now_ms();
+    EMF_HIP(hipEventRecord(e0, s));
+    estimate(s, c->T.nt, c->d_tets.get(), c->d_t2e.get(), c->d_nb.get(), c->d_non.get(), c->d_pts.get(),
+             c->d_attr.get(), c->d_s6.get(), 2 * kPi * c->freqs[fidx], n, d_slots.get(), c->d_X.get(), 2 * c->nslots,
+             d_err.get());
+    EMF_HIP(hipEventRecord(e1, s));
+    d_err.download(errors, (size_t)n * c->T.nt, s);
+    EMF_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    EMF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->stats["estimate_kernel_ms"] = ms;
+    c->stats["estimate_ms"] = now_ms() - t0;
+  });
+}
+
+int emfem_interpolate_fields(emfem_context *c, int fidx, int slot, int n, const double *points, double *E, double *H) {
+  return guarded(c, [&]() {
+    require_slot(c, slot);
+    require_freq(c, fidx);
+    EMF_REQUIRE(n >= 0 && (n == 0 || (points && E && H)), kErrArg, "bad interpolation arguments");
+    std::vector<double> sol((size_t)c->T.ne * 2);
+    EMF_HIP(hipMemcpy2DAsync(sol.data(), sizeof(double) * 2, c->d_X.get() + 2 * slot, sizeof(double) * 2 * c->nslots,
+                             sizeof(double) * 2, c->T.ne, hipMemcpyDeviceToHost, c->stream));
+    EMF_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+      const int t = find_cell(c->T, points + (size_t)i * 3);
+      EMF_REQUIRE(t >= 0, kErrUser, "Receiver " + std::to_string(i) + " is not found in mesh.");
+      cplx Ev[3], Hv[3];
+      interpolate(c->T, t, points + (size_t)i * 3, c->freqs[fidx], sol.data(), Ev, Hv);
+      for (int k = 0; k < 3; ++k) {
+        E[(size_t)i * 6 + k * 2] = Ev[k].real();
+        E[(size_t)i * 6 + k * 2 + 1] = Ev[k].imag();
+        H[(size_t)i * 6 + k * 2] = Hv[k].real();
+        H[(size_t)i * 6 + k * 2 + 1] = Hv[k].imag();
+      }
+    }
+  });
+}
+
+int emfem_calculate_response_mt(emfem_context *c, int fidx, int slot_xy, int slot_yx, int n_obs, const int *obs,
+                                double *rsp) {
+  return guarded(c, [&]() {
+    require_slot(c, slot_xy);
+    require_slot(c, slot_yx);
+    require_freq(c, fidx);
+    EMF_REQUIRE(n_obs >= 0 && (n_obs == 0 || (obs && rsp)), kErrArg, "bad observation arrays");
+    const int nrx = (int)c->rx.size() / 3;
+    std::vector<double> exy_sol((size_t)c->T.ne * 2), eyx_sol((size_t)c->T.ne * 2);
+    for (int k = 0; k < 2; ++k) {
+      std::vector<double> &dst = k == 0 ? exy_sol : eyx_sol;
+      EMF_HIP(hipMemcpy2DAsync(dst.data(), sizeof(double) * 2, c->d_X.get() + 2 * (k == 0 ? slot_xy : slot_yx),
+                               sizeof(double) * 2 * c->nslots, sizeof(double) * 2, c->T.ne, hipMemcpyDeviceToHost,
+                               c->stream));
+    }
+    EMF_HIP(hipStreamSynchronize(c->stream));
+    std::map<int, std::array<cplx, 12>> cache;
+    const double f = c->freqs[fidx];
+    for (int o = 0; o < n_obs; ++o) {
+      const int ot = obs[o * 4], fi = obs[o * 4 + 1], ti = obs[o * 4 + 2], ri = obs[o * 4 + 3];
+      if (fi != fidx || ti != EMFEM_MT_TIDX) continue;
+      EMF_REQUIRE(ri >= 0 && ri < nrx, kErrArg, "receiver index out of range");
+      if (!cache.count(ri)) {
+        const double *p = &c->rx[(size_t)ri * 3];
+        const int t = find_cell(c->T, p);
+        EMF_REQUIRE(t >= 0, kErrUser, "Receiver " + std::to_string(ri) + " is not found in mesh.");
+        std::array<cplx, 12> v;
+        interpolate(c->T, t, p, f, exy_sol.data(), &v[0], &v[3]);
+        interpolate(c->T, t, p, f, eyx_sol.data(), &v[6], &v[9]);
+        cache[ri] = v;
+      }
+      const auto &v = cache[ri];
+      cplx out;
+      EMF_REQUIRE(mt_response(ot, f, &v[0], &v[3], &v[6], &v[9], out), kErrUser,
+                  "Data type " + std::to_string(ot) + " not implemented.");
+      rsp[o * 2] = out.real();
+      rsp[o * 2 + 1] = out.imag();
+    }
+  });
+}
+
+int emfem_calculate_response_csem(emfem_context *c, int fidx, int tidx, int slot, int n_obs, const int *obs,
+                                  double *rsp) {
+  return guarded(c, [&]() {
+    require_slot(c, slot);
+    require_freq(c, fidx);
+    EMF_REQUIRE(n_obs >= 0 && (n_obs == 0 || (obs && rsp)), kErrArg, "bad observation arrays");
+    const int nrx = (int)c->rx.size() / 3;
+    std::vector<double> sol((size_t)c->T.ne * 2);
+    EMF_HIP(hipMemcpy2DAsync(sol.data(), sizeof(double) * 2, c->d_X.get() + 2 * slot, sizeof(double) * 2 * c->nslots,
+                             sizeof(double) * 2, c->T.ne, hipMemcpyDeviceToHost, c->stream));
+    EMF_HIP(hipStreamSynchronize(c->stream));
+    std::map<int, std::array<cplx, 6>> cache;
+    const double f = c->freqs[fidx];
+    for (int o = 0; o < n_obs; ++o) {
+      const int ot = obs[o * 4], fi = obs[o * 4 + 1], ti = obs[o * 4 + 2], ri = obs[o * 4 + 3];
+      if (fi != fidx || ti != tidx) continue;
+      EMF_REQUIRE(ri >= 0 && ri < nrx, kErrArg, "receiver index out of range");
+      if (!cache.count(ri)) {
+        const double *p = &c->rx[(size_t)ri * 3];
+        const int t = find_cell(c->T, p);
+        EMF_REQUIRE(t >= 0, kErrUser, "Receiver " + std::to_string(ri) + " is not found in mesh.");
+        std::array<cplx, 6> v;
+        interpolate(c->T, t, p, f, sol.data(), &v[0], &v[3]);
+        cache[ri] = v;
+      }
+      const auto &v = cache[ri];
+      cplx out;
+      EMF_REQUIRE(csem_response(ot, &v[0], &v[3], out), kErrUser,
+                  "Data type " + std::to_string(ot) + " for tx " + std::to_string(tidx) + " is not implemented.");
+      rsp[o * 2] = out.real();
+      rsp[o * 2 + 1] = out.imag();
+    }
+  });
+}
+
+int emfem_refine_fixed_fraction(int n, const double *error, double fraction, unsigned char *flag) {
+  if (n < 0 || (n > 0 && (!error || !flag))) return EMFEM_ERR_ARG;
+  if (n == 0) return EMFEM_OK;
+  double total = 0.0;
+  std::vector<double> sorted(error, error + n);
+  for (double e : sorted) total += e;
+  std::sort(sorted.begin(), sorted.end(), std::greater<double>());
+  double acc = 0.0, thr = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (acc < fraction * total) {
+      acc += sorted[i];
+    } else {
+      thr = sorted[i];
+      break;
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    if (error[i] > thr) flag[i] = 1;
+  return EMFEM_OK;
+}
+
+int emfem_refine_fixed_number(int n, const double *error, double fraction, unsigned char *flag) {
+  if (n < 0 || (n > 0 && (!error || !flag))) return EMFEM_ERR_ARG;
+  if (n == 0) return EMFEM_OK;
+  int k = (int)(n * fraction);
+  k = std::max(0, std::min(k, n - 1));
+  std::vector<double> tmp(error, error + n);
+  std::nth_element(tmp.begin(), tmp.begin() + k, tmp.end(), std::greater<double>());
+  const double thr = tmp[k];
+  for (int i = 0; i < n; ++i)
+    if (error[i] > thr) flag[i] = 1;
+  return EMFEM_OK;
+}
+
+int emfem_get_stat(const emfem_context *c, const char *name, double *value) {
+  if (!c || !name || !value) return EMFEM_ERR_ARG;
+  auto it = c->stats.find(name);
+  if (it == c->stats.end()) return EMFEM_ERR_ARG;
+  *value = it->second;
+  return EMFEM_OK;
+}
+
+int emfem_kernel_time(const emfem_context *c, const char *which, double *ms, double *bytes) {
+  if (!c || !which || !ms || !bytes) return EMFEM_ERR_ARG;
+  std::string w(which);
+  auto get = [&](const char *k) {
+    auto it = c->stats.find(k);
+    return it == c->stats.end() ? 0.0 : it->second;
+  };
+  if (w == "assemble") {
+    *ms = get("assemble_kernel_ms");
+    const double T = c->T.nt, V = c->T.nv, E = c->T.ne, Z = c->nnz;
+    *bytes = T * (16 + 24 + 4) + V * 24 + Z * 4 + (E + 1) * 4 + Z * 16;
+  } else if (w == "op_spmm") {
+    *ms = get("op_spmm_avg_ms");
+    *bytes = get("op_spmm_bytes");
+  } else if (w == "b_spmm") {
+    *ms = get("b_spmm_avg_ms");
+    *bytes = get("b_spmm_bytes");
+  } else if (w == "estimate") {
+    *ms = get("estimate_kernel_ms");
+    *bytes = 0.0;
+  } else {
+    return EMFEM_ERR_ARG;
+  }
+  return EMFEM_OK;
+}
+
+}  // extern "C"

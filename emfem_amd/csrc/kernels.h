@@ -1,0 +1,105 @@
+// Launch wrappers of the device kernels (implemented in *.hip).
+//
+// Block vectors are row-major [rows][ld] with `nc` used columns.  Complex
+// right-hand sides are stored as interleaved (re, im) column pairs, so a block of
+// k complex vectors has nc = 2k real columns; the real-equivalent vector of the
+// reference's VecNest {re, im} (src/em_la.cc:42-47) is one such column pair.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace emfem {
+
+// CSR matrix view (device pointers).
+template <typename T>
+struct Csr {
+  int nrows = 0, ncols = 0, nnz = 0;
+  const int *rp = nullptr;
+  const int *ci = nullptr;
+  const T *v = nullptr;
+};
+
+// y = rs .* (alpha * (A x) + gamma * w) + beta * z   (rows of A; rs may be null = 1,
+// w / z may be null when their coefficient is 0).  y may alias z (same row only) but
+// must not alias x.
+template <typename T>
+void spmm(hipStream_t s, const Csr<T> &A, int nc, const T *x, int ldx, T *y, int ldy, T alpha, const T *rs,
+          T gamma, const T *w, int ldw, T beta, const T *z, int ldz);
+
+// Complex operator of the reference's MatShell A (src/em_la.cc:65-87) on nrhs
+// interleaved pairs: for each pair (xr, xi)
+//   yr = C xr - M xi,  yi = -M xr - C xi ;  y = alpha * (A x) + beta * z.
+// K holds (C, M) per nonzero.  x may be fp32 (preconditioned directions) or fp64.
+template <typename TX>
+void op_spmm(hipStream_t s, int nrows, int nnz, const int *rp, const int *ci, const double2 *K, int nrhs, const TX *x,
+             int ldx, double *y, int ldy, double alpha, double beta, const double *z, int ldz);
+
+// Element assembly of one colour of tets (atomic-free read-modify-write scatter).
+void assemble_colour(hipStream_t s, int ntets, const int *tet_ids, const int *tets, const int *tet2edge,
+                     const double *pts, const int *attr, const double *sig6, const uint8_t *edge_bdr, double omega,
+                     const int *rp, const int *ci, double2 *K);
+
+// B = C + M in precision T (the SPD operator of the inner solve, src/em_la.cc:122-128).
+template <typename T>
+void make_b(hipStream_t s, int nnz, const double2 *K, T *B);
+
+// Per-column dot products sum_r x[r][c] * y[r][c] for c < nc, written (in double) to
+// out[c] on the device.  `work` must hold dot_work_size(rows, nc) doubles.
+size_t dot_work_size(int rows, int nc);
+template <typename T>
+void coldot(hipStream_t s, int rows, int nc, const T *x, int ldx, const T *y, int ldy, double *out, double *work);
+
+// Multi-dot for Gram-Schmidt: out[i*nc + c] = sum_r V_i[r][c] w[r][c] for i < m,
+// where V_i = V + i * vstride.  work must hold dot_work_size(rows, nc) * 8 doubles.
+void mdot(hipStream_t s, int rows, int nc, const double *V, size_t vstride, int m, const double *w, double *out,
+          double *work);
+
+// w[r][c] -= sum_{i<m} h[i*hld + c] * V_i[r][c]   (V_i = V + i * vstride, type TB)
+template <typename TB>
+void mupdate(hipStream_t s, int rows, int nc, const TB *V, size_t vstride, int m, const double *h, int hld,
+             double *w, double sign);
+
+// y[r][c] = a[c] * x[r][c] + b[c] * y[r][c]  (a, b: device doubles; b may be null = 0)
+template <typename TX, typename TY>
+void axpby_cols(hipStream_t s, int rows, int nc, const double *a, const TX *x, int ldx, const double *b, TY *y,
+                int ldy);
+
+// y[r][c] = alpha * x[r][c] + beta * y[r][c] with host scalars (beta = 0 ignores y).
+template <typename TX, typename TY>
+void axpby(hipStream_t s, int rows, int nc, double alpha, const TX *x, int ldx, double beta, TY *y, int ldy);
+
+// Flexible CG updates on nc columns with device scalars:
+//   x += alpha[c] p ; r -= alpha[c] q ; rr[c] = r . r (fused reduction)
+template <typename T>
+void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const T *p, const T *q, T *x, T *r, double *rr,
+            double *work);
+
+// Chebyshev smoother helpers (Dinv = 1 / l1 row norms):
+//   init: rres = Dinv .* b ; d = rres / theta ; x = d
+//   step: d = c1 * d + c2 * rres ; x += d
+template <typename T>
+void cheb_init(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *rres, T *d, T *x, double inv_theta);
+template <typename T>
+void cheb_step(hipStream_t s, int rows, int nc, T *d, const T *rres, T *x, double c1, double c2);
+
+// x = Dinv .* b (one l1-Jacobi sweep from a zero guess)
+template <typename T>
+void diag_scale(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x);
+
+// Dense coarse solve in fp64: x[i][c] = sum_j Ainv[i][j] b[j][c].
+template <typename T>
+void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T *x);
+
+// Tiny device helpers for the Krylov scalars.
+void fcg_alpha(hipStream_t s, int nc, const double *rz, const double *pq, const uint8_t *active, double *alpha);
+void fcg_beta(hipStream_t s, int nc, const double *zq, const double *pq, const uint8_t *active, double *beta);
+
+// Estimator kernel: squared cell + face indicators per tet for ns solutions whose
+// interleaved (re, im) columns are at sol + 2*slot[s] with row stride ld.
+void estimate(hipStream_t s, int nt, const int *tets, const int *tet2edge, const int *neighbor, const int *non,
+              const double *pts, const int *attr, const double *sig6, double omega, int ns, const int *slots,
+              const double *sol, int ld, double *errors);
+
+}  // namespace emfem

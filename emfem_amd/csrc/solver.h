@@ -1,0 +1,76 @@
+// Device solvers: auxiliary-space preconditioner cycles, the inner flexible CG on
+// B = C + M and the outer FGMRES on the reference's real-equivalent operator.
+//
+// Reference: pc_apply_b (src/em_la.cc:208-228; B_ksp = FCG, rtol 1e-2, max 100,
+// unpreconditioned norm, src/em_la.cc:134-142) and solve_linear_system with the
+// FGMRES A_ksp (src/em_la.cc:35-63,175-188).  The inner solve runs in precision T
+// (fp32 in the mixed path, fp64 in the plain path); the outer FGMRES, its
+// Gram-Schmidt and the true residual are fp64.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "amg.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace emfem {
+
+struct SolveStats {
+  int outer_its = 0;         // max over right-hand sides
+  long inner_its = 0;        // FCG iterations summed over preconditioner applications
+  long pc_applies = 0;       // preconditioner applications (each on 2k columns)
+  long vcycles = 0;          // AMS cycles
+  std::vector<int> its;      // per right-hand side
+  std::vector<double> relres;
+};
+
+// Timing of a class of kernel launches with HIP events on the solver stream.
+struct KernelClock {
+  std::vector<hipEvent_t> ev;
+  double bytes_per_launch = 0.0;
+  size_t max_pairs = 256;
+  void start(hipStream_t s);
+  void stop(hipStream_t s);
+  bool armed() const { return ev.size() / 2 < max_pairs; }
+  // average ms per launch (stream must be idle), then clears
+  double average_ms(int *count);
+  ~KernelClock();
+};
+
+struct PcOptions {
+  int cheb_degree = 3;
+  double cheb_ratio = 30.0;
+  double inner_rtol = 1e-2;
+  int inner_max_it = 100;
+  AmgParams amg;
+};
+
+// Preconditioner for B on nc real columns (interface independent of precision).
+class InnerPc {
+ public:
+  virtual ~InnerPc() = default;
+  // x = approx B^-1 b by FCG + AMS; b, x are fp64 [E][nc] compact (converted inside).
+  virtual void apply64(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) = 0;
+  // FGMRES entry: b fp64, x written to the precision of the stored Krylov directions.
+  virtual void apply_dir(hipStream_t s, int nc, const double *b, void *x, SolveStats &st) = 0;
+  virtual bool fp32() const = 0;
+  virtual KernelClock &b_clock() = 0;
+};
+
+std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp,
+                                       const int *d_ci, const double2 *d_K, const PcOptions &o, hipStream_t s);
+
+struct OuterOptions {
+  int restart = 30;
+  int max_it = 100;
+};
+
+// FGMRES on nrhs interleaved right-hand sides b [E][2 nrhs] -> x [E][2 nrhs].
+void fgmres(hipStream_t s, int ne, int nnz, const int *d_rp, const int *d_ci, const double2 *d_K, InnerPc &pc,
+            int nrhs, const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o,
+            SolveStats &st, KernelClock *op_clock);
+
+}  // namespace emfem

@@ -1,0 +1,588 @@
+// Device solvers: AMG and auxiliary-space cycles, inner flexible CG, outer FGMRES.
+// See solver.h for the reference mapping.
+#include "solver.h"
+
+#include <cmath>
+#include <type_traits>
+
+namespace emfem {
+
+// ---------------------------------------------------------------------------------
+// kernel clock
+// ---------------------------------------------------------------------------------
+
+void KernelClock::start(hipStream_t s) {
+  hipEvent_t e;
+  EMF_HIP(hipEventCreate(&e));
+  EMF_HIP(hipEventRecord(e, s));
+  ev.push_back(e);
+}
+
+void KernelClock::stop(hipStream_t s) {
+  hipEvent_t e;
+  EMF_HIP(hipEventCreate(&e));
+  EMF_HIP(hipEventRecord(e, s));
+  ev.push_back(e);
+}
+
+double KernelClock::average_ms(int *count) {
+  double total = 0.0;
+  int n = 0;
+  for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) {
+      total += ms;
+      ++n;
+    }
+  }
+  for (auto e : ev) (void)hipEventDestroy(e);
+  ev.clear();
+  if (count) *count = n;
+  return n ? total / n : 0.0;
+}
+
+KernelClock::~KernelClock() {
+  for (auto e : ev) (void)hipEventDestroy(e);
+}
+
+namespace {
+
+template <typename T>
+struct DevCsr {
+  int n = 0, m = 0, nnz = 0;
+  DevBuf<int> rp, ci;
+  DevBuf<T> v;
+  void upload(const HostCsr &H, hipStream_t s) {
+    n = H.n;
+    m = H.m;
+    nnz = (int)H.ci.size();
+    rp.upload(H.rp, s);
+    ci.upload(H.ci, s);
+    std::vector<T> tv(H.v.begin(), H.v.end());
+    v.upload(tv, s);
+    EMF_HIP(hipStreamSynchronize(s));
+  }
+  Csr<T> view() const {
+    Csr<T> c;
+    c.nrows = n;
+    c.ncols = m;
+    c.nnz = nnz;
+    c.rp = rp.get();
+    c.ci = ci.get();
+    c.v = v.get();
+    return c;
+  }
+};
+
+template <typename T>
+void upload_vec(DevBuf<T> &d, const std::vector<double> &h, hipStream_t s) {
+  std::vector<T> t(h.begin(), h.end());
+  d.upload(t, s);
+  EMF_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------------
+// smoothed-aggregation V-cycle (l1-Jacobi smoothing, fp64 dense coarsest solve)
+// ---------------------------------------------------------------------------------
+
+template <typename T>
+class DevAmg {
+ public:
+  void upload(const AmgHost &H, hipStream_t s) {
+    levels_.resize(H.levels.size());
+    for (size_t l = 0; l < H.levels.size(); ++l) {
+      levels_[l].A.upload(H.levels[l].A, s);
+      levels_[l].P.upload(H.levels[l].P, s);
+      levels_[l].R.upload(H.levels[l].R, s);
+      upload_vec(levels_[l].l1inv, H.levels[l].l1inv, s);
+    }
+    ncoarse_ = H.ncoarse;
+    cinv_.upload(H.coarse_inv, s);
+    EMF_HIP(hipStreamSynchronize(s));
+  }
+  int fine_rows() const { return levels_.empty() ? ncoarse_ : levels_[0].A.n; }
+
+  void ensure(int nc) {
+    if (nc <= nc_) return;
+    nc_ = nc;
+    for (size_t l = 0; l < levels_.size(); ++l) {
+      const size_t n = (size_t)levels_[l].A.n * nc;
+      levels_[l].x.resize(n);
+      levels_[l].r.resize(n);
+      if (l > 0) {
+        levels_[l].b.resize(n);
+        levels_[l].out.resize(n);
+      }
+    }
+    coarse_b_.resize((size_t)ncoarse_ * nc);
+    coarse_x_.resize((size_t)ncoarse_ * nc);
+  }
+
+  // out = V(b) on the finest level; b, out compact [n0][nc].
+  void vcycle(hipStream_t s, int nc, const T *b, T *out) { cycle(s, 0, nc, b, out); }
+
+ private:
+  struct Level {
+    DevCsr<T> A, P, R;
+    DevBuf<T> l1inv, b, out, x, r;
+  };
+
+  void cycle(hipStream_t s, size_t l, int nc, const T *b, T *out) {
+    if (l == levels_.size()) {
+      dense_solve<T>(s, ncoarse_, nc, cinv_.get(), b, out);
+      return;
+    }
+    Level &L = levels_[l];
+    const int n = L.A.n;
+    T *x = L.x.get(), *r = L.r.get();
+    // pre-smoothing from a zero guess
+    diag_scale<T>(s, n, nc, L.l1inv.get(), b, x);
+    // r = b - A x ; restrict
+    spmm<T>(s, L.A.view(), nc, x, nc, r, nc, T(-1), nullptr, T(1), b, nc, T(0), nullptr, 0);
+    const bool last = (l + 1 == levels_.size());
+    T *cb = last ? coarse_b_.get() : levels_[l + 1].b.get();
+    T *cx = last ? coarse_x_.get() : levels_[l + 1].out.get();
+    spmm<T>(s, L.R.view(), nc, r, nc, cb, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+    cycle(s, l + 1, nc, cb, cx);
+    // x += P xc
+    spmm<T>(s, L.P.view(), nc, cx, nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), x, nc);
+    // post-smoothing: out = x + l1inv .* (b - A x)
+    spmm<T>(s, L.A.view(), nc, x, nc, out, nc, T(-1), L.l1inv.get(), T(1), b, nc, T(1), x, nc);
+  }
+
+  std::vector<Level> levels_;
+  int ncoarse_ = 0, nc_ = 0;
+  DevBuf<double> cinv_;
+  DevBuf<T> coarse_b_, coarse_x_;
+};
+
+// ---------------------------------------------------------------------------------
+// auxiliary-space cycle: Chebyshev(edge) - G - Pi - G - Chebyshev(edge)
+// ---------------------------------------------------------------------------------
+
+template <typename T>
+class DevAms {
+ public:
+  void setup(const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K, const PcOptions &o,
+             hipStream_t s) {
+    ne_ = ne;
+    B_.nrows = B_.ncols = ne;
+    B_.nnz = nnz;
+    B_.rp = rp;
+    B_.ci = ci;
+    bvals_.resize(nnz);
+    make_b<T>(s, nnz, K, bvals_.get());
+    B_.v = bvals_.get();
+    upload_vec(l1inv_, H.l1inv, s);
+    G0_.upload(H.G0, s);
+    G0T_.upload(H.G0T, s);
+    for (int d = 0; d < 3; ++d) {
+      Pi_[d].upload(H.Pi[d], s);
+      PiT_[d].upload(H.PiT[d], s);
+      amgPi_[d].upload(H.amgPi[d], s);
+    }
+    amgG_.upload(H.amgG, s);
+    nv0_ = H.nv0;
+    deg_ = std::max(1, o.cheb_degree);
+    const double lmax = 1.0;  // Gershgorin: the l1-scaled operator has spectrum in (0, 1]
+    const double lmin = lmax / o.cheb_ratio;
+    theta_ = 0.5 * (lmax + lmin);
+    delta_ = 0.5 * (lmax - lmin);
+  }
+
+  const Csr<T> &B() const { return B_; }
+
+  void ensure(int nc) {
+    if (nc <= nc_) return;
+    nc_ = nc;
+    r_.resize((size_t)ne_ * nc);
+    rres_.resize((size_t)ne_ * nc);
+    d_.resize((size_t)ne_ * nc);
+    vb_.resize((size_t)nv0_ * nc);
+    vx_.resize((size_t)nv0_ * nc);
+    amgG_.ensure(nc);
+    for (int d = 0; d < 3; ++d) amgPi_[d].ensure(nc);
+  }
+
+  // out = M_AMS b (compact [E][nc]); out must not alias b.
+  void apply(hipStream_t s, int nc, const T *b, T *out, KernelClock *clk) {
+    ensure(nc);
+    T *r = r_.get(), *rres = rres_.get(), *d = d_.get();
+    // pre-smoothing (Chebyshev from zero)
+    cheb_init<T>(s, ne_, nc, l1inv_.get(), b, rres, d, out, 1.0 / theta_);
+    cheb_iterate(s, nc, rres, d, out, clk);
+    aux(s, nc, b, out, G0_, G0T_, amgG_, clk);
+    residual(s, nc, b, out, r, clk);
+    for (int k = 0; k < 3; ++k) {
+      spmm<T>(s, PiT_[k].view(), nc, r, nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+      amgPi_[k].vcycle(s, nc, vb_.get(), vx_.get());
+      spmm<T>(s, Pi_[k].view(), nc, vx_.get(), nc, out, nc, T(1), nullptr, T(0), nullptr, 0, T(1), out, nc);
+    }
+    aux(s, nc, b, out, G0_, G0T_, amgG_, clk);
+    // post-smoothing (Chebyshev from the current iterate)
+    timed_spmm(s, nc, out, rres, T(-1), l1inv_.get(), T(1), b, T(0), nullptr, clk);
+    axpby<T, T>(s, ne_, nc, 1.0 / theta_, rres, nc, 0.0, d, nc);
+    axpby<T, T>(s, ne_, nc, 1.0, d, nc, 1.0, out, nc);
+    cheb_iterate(s, nc, rres, d, out, clk);
+  }
+
+ private:
+  void timed_spmm(hipStream_t s, int nc, const T *x, T *y, T alpha, const T *rs, T gamma, const T *w, T beta,
+                  const T *z, KernelClock *clk) {
+    const bool t = clk && clk->armed();
+    if (t) clk->start(s);
+    spmm<T>(s, B_, nc, x, nc, y, nc, alpha, rs, gamma, w, nc, beta, z, nc);
+    if (t) clk->stop(s);
+  }
+
+  void residual(hipStream_t s, int nc, const T *b, const T *x, T *r, KernelClock *clk) {
+    timed_spmm(s, nc, x, r, T(-1), nullptr, T(1), b, T(0), nullptr, clk);
+  }
+
+  void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x, KernelClock *clk) {
+    double sigma = theta_ / delta_, rho = 1.0 / sigma;
+    for (int k = 1; k < deg_; ++k) {
+      // rres -= l1inv .* (B d)
+      timed_spmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres, clk);
+      const double rho_new = 1.0 / (2.0 * sigma - rho);
+      cheb_step<T>(s, ne_, nc, d, rres, x, rho_new * rho, 2.0 * rho_new / delta_);
+      rho = rho_new;
+    }
+  }
+
+  void aux(hipStream_t s, int nc, const T *b, T *x, const DevCsr<T> &P, const DevCsr<T> &PT, DevAmg<T> &amg,
+           KernelClock *clk) {
+    residual(s, nc, b, x, r_.get(), clk);
+    spmm<T>(s, PT.view(), nc, r_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+    amg.vcycle(s, nc, vb_.get(), vx_.get());
+    spmm<T>(s, P.view(), nc, vx_.get(), nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), x, nc);
+  }
+
+  int ne_ = 0, nv0_ = 0, nc_ = 0, deg_ = 3;
+  double theta_ = 0.5, delta_ = 0.5;
+  Csr<T> B_;
+  DevBuf<T> bvals_, l1inv_, r_, rres_, d_, vb_, vx_;
+  DevCsr<T> G0_, G0T_, Pi_[3], PiT_[3];
+  DevAmg<T> amgG_, amgPi_[3];
+};
+
+// ---------------------------------------------------------------------------------
+// inner flexible CG (truncation 1) on B, per column (src/em_la.cc:134-142)
+// ---------------------------------------------------------------------------------
+
+template <typename T>
+class InnerPcImpl : public InnerPc {
+ public:
+  InnerPcImpl(const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K, const PcOptions &o,
+              hipStream_t s)
+      : ne_(ne), o_(o) {
+    ams_.setup(H, ne, nnz, rp, ci, K, o, s);
+    const double bytes_b = double(ne + 1) * 4 + double(nnz) * (4 + sizeof(T));
+    clock_.bytes_per_launch = bytes_b;  // completed with the column traffic per call
+  }
+
+  bool fp32() const override { return std::is_same<T, float>::value; }
+  KernelClock &b_clock() override { return clock_; }
+
+  void apply64(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) override {
+    ensure(nc);
+    if constexpr (std::is_same<T, double>::value) {
+      solve(s, nc, b, x, st);
+    } else {
+      axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, bt_.get(), nc);
+      solve(s, nc, bt_.get(), xt_.get(), st);
+      axpby<T, double>(s, ne_, nc, 1.0, xt_.get(), nc, 0.0, x, nc);
+    }
+  }
+
+  void apply_dir(hipStream_t s, int nc, const double *b, void *x, SolveStats &st) override {
+    ensure(nc);
+    if constexpr (std::is_same<T, double>::value) {
+      solve(s, nc, b, static_cast<T *>(x), st);
+    } else {
+      axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, bt_.get(), nc);
+      solve(s, nc, bt_.get(), static_cast<T *>(x), st);
+    }
+  }
+
+ private:
+  void ensure(int nc) {
+    if (nc <= nc_) return;
+    nc_ = nc;
+    const size_t n = (size_t)ne_ * nc;
+    r_.resize(n);
+    z_.resize(n);
+    p_.resize(n);
+    q_.resize(n);
+    if (!std::is_same<T, double>::value) {
+      bt_.resize(n);
+      xt_.resize(n);
+    }
+    for (auto *v : {&rz_, &pq_, &zq_, &alpha_, &beta_, &rr_, &ones_}) v->resize(nc);
+    active_.resize(nc);
+    work_.resize(dot_work_size(ne_, nc) * 8);
+    std::vector<double> one(nc, 1.0);
+    ones_.upload(one, 0);
+    EMF_HIP(hipDeviceSynchronize());
+    ams_.ensure(nc);
+    h_bb_.resize(nc);
+    h_rr_.resize(nc);
+    h_active_.resize(nc);
+  }
+
+  void solve(hipStream_t s, int nc, const T *b, T *x, SolveStats &st) {
+    T *r = r_.get(), *z = z_.get(), *p = p_.get(), *q = q_.get();
+    double *w = work_.get();
+    ++st.pc_applies;
+    // x = 0, r = b
+    EMF_HIP(hipMemsetAsync(x, 0, sizeof(T) * (size_t)ne_ * nc, s));
+    EMF_HIP(hipMemcpyAsync(r, b, sizeof(T) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
+    coldot<T>(s, ne_, nc, b, nc, b, nc, rr_.get(), w);
+    EMF_HIP(hipMemcpyAsync(h_bb_.data(), rr_.get(), sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    EMF_HIP(hipStreamSynchronize(s));
+    bool any = false;
+    for (int c = 0; c < nc; ++c) {
+      h_active_[c] = h_bb_[c] > 0.0;
+      any |= h_active_[c] != 0;
+    }
+    if (!any) return;
+    active_.upload(h_active_.data(), nc, s);
+    const double tol2 = o_.inner_rtol * o_.inner_rtol;
+    // z = M r ; p = z ; q = B p
+    ams_.apply(s, nc, r, z, &clock_);
+    ++st.vcycles;
+    EMF_HIP(hipMemcpyAsync(p, z, sizeof(T) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
+    bspmm(s, nc, p, q);
+    coldot<T>(s, ne_, nc, p, nc, q, nc, pq_.get(), w);
+    coldot<T>(s, ne_, nc, r, nc, z, nc, rz_.get(), w);
+    for (int it = 1; it <= o_.inner_max_it; ++it) {
+      fcg_alpha(s, nc, rz_.get(), pq_.get(), active_.get(), alpha_.get());
+      fcg_xr<T>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w);
+      ++st.inner_its;
+      EMF_HIP(hipMemcpyAsync(h_rr_.data(), rr_.get(), sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+      EMF_HIP(hipStreamSynchronize(s));
+      any = false;
+      for (int c = 0; c < nc; ++c) {
+        if (h_active_[c] && h_rr_[c] <= tol2 * h_bb_[c]) h_active_[c] = 0;
+        any |= h_active_[c] != 0;
+      }
+      if (!any || it == o_.inner_max_it) break;
+      active_.upload(h_active_.data(), nc, s);
+      ams_.apply(s, nc, r, z, &clock_);
+      ++st.vcycles;
+      coldot<T>(s, ne_, nc, z, nc, q, nc, zq_.get(), w);
+      coldot<T>(s, ne_, nc, r, nc, z, nc, rz_.get(), w);
+      fcg_beta(s, nc, zq_.get(), pq_.get(), active_.get(), beta_.get());
+      axpby_cols<T, T>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc);
+      bspmm(s, nc, p, q);
+      coldot<T>(s, ne_, nc, p, nc, q, nc, pq_.get(), w);
+    }
+  }
+
+  void bspmm(hipStream_t s, int nc, const T *x, T *y) {
+    const bool t = clock_.armed();
+    if (t) clock_.start(s);
+    spmm<T>(s, ams_.B(), nc, x, nc, y, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+    if (t) clock_.stop(s);
+  }
+
+  int ne_ = 0, nc_ = 0;
+  PcOptions o_;
+  DevAms<T> ams_;
+  DevBuf<T> r_, z_, p_, q_, bt_, xt_;
+  DevBuf<double> rz_, pq_, zq_, alpha_, beta_, rr_, ones_, work_;
+  DevBuf<uint8_t> active_;
+  std::vector<double> h_bb_, h_rr_;
+  std::vector<uint8_t> h_active_;
+  KernelClock clock_;
+};
+
+}  // namespace
+
+std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp, const int *d_ci,
+                                       const double2 *d_K, const PcOptions &o, hipStream_t s) {
+  if (fp32) return std::unique_ptr<InnerPc>(new InnerPcImpl<float>(H, ne, nnz, d_rp, d_ci, d_K, o, s));
+  return std::unique_ptr<InnerPc>(new InnerPcImpl<double>(H, ne, nnz, d_rp, d_ci, d_K, o, s));
+}
+
+// ---------------------------------------------------------------------------------
+// FGMRES (right preconditioned, classical Gram-Schmidt, PETSc's KSPFGMRES shape)
+// ---------------------------------------------------------------------------------
+
+namespace {
+
+template <typename TZ>
+void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, InnerPc &pc, int nrhs,
+                 const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o, SolveStats &st,
+                 KernelClock *clk) {
+  const int nc = 2 * nrhs;
+  const int m = std::max(1, o.restart);
+  const size_t blk = (size_t)ne * nc;
+  DevBuf<double> V((m + 1) * blk), w(blk), dots((size_t)(m + 1) * nc), coef((size_t)(m + 1) * nc), scal(nc);
+  DevBuf<TZ> Z(m * blk);
+  DevBuf<double> work(dot_work_size(ne, nc) * 8);
+  std::vector<double> h_dots((size_t)(m + 1) * nc), h_coef((size_t)(m + 1) * nc), h_scal(nc);
+  auto op = [&](const auto *in, double *out, double alpha, double beta, const double *z) {
+    const bool t = clk && clk->armed();
+    if (t) clk->start(s);
+    op_spmm(s, ne, nnz, rp, ci, K, nrhs, in, nc, out, nc, alpha, beta, z, nc);
+    if (t) clk->stop(s);
+  };
+  auto rhs_norms = [&](const double *v, std::vector<double> &out) {
+    coldot<double>(s, ne, nc, v, nc, v, nc, dots.get(), work.get());
+    EMF_HIP(hipMemcpyAsync(h_dots.data(), dots.get(), sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    EMF_HIP(hipStreamSynchronize(s));
+    out.resize(nrhs);
+    for (int r = 0; r < nrhs; ++r) out[r] = std::sqrt(h_dots[2 * r] + h_dots[2 * r + 1]);
+  };
+  auto scale_cols = [&](double *v, const std::vector<double> &fac) {
+    for (int r = 0; r < nrhs; ++r) h_scal[2 * r] = h_scal[2 * r + 1] = fac[r];
+    scal.upload(h_scal.data(), nc, s);
+    axpby_cols<double, double>(s, ne, nc, scal.get(), v, nc, nullptr, v, nc);
+  };
+
+  std::vector<double> bnorm;
+  rhs_norms(b, bnorm);
+  st.its.assign(nrhs, 0);
+  st.relres.assign(nrhs, 0.0);
+  std::vector<uint8_t> done(nrhs, 0);
+  for (int r = 0; r < nrhs; ++r) done[r] = bnorm[r] == 0.0;
+  EMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * blk, s));
+  int total = 0;
+  std::vector<double> H((size_t)nrhs * (m + 1) * m), cs((size_t)nrhs * m), sn((size_t)nrhs * m),
+      g((size_t)nrhs * (m + 1));
+  std::vector<int> kk(nrhs);
+  std::vector<uint8_t> active(nrhs);
+  while (total < o.max_it) {
+    // r = b - A x into V_0
+    double *V0 = V.get();
+    op(x, V0, -1.0, 1.0, b);
+    std::vector<double> beta;
+    rhs_norms(V0, beta);
+    bool any = false;
+    for (int r = 0; r < nrhs; ++r) {
+      if (!done[r] && beta[r] <= rtol[r] * bnorm[r]) done[r] = 1;
+      active[r] = !done[r];
+      any |= active[r] != 0;
+    }
+    if (!any) break;
+    std::vector<double> inv(nrhs);
+    for (int r = 0; r < nrhs; ++r) inv[r] = active[r] && beta[r] > 0 ? 1.0 / beta[r] : 0.0;
+    scale_cols(V0, inv);
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    for (int r = 0; r < nrhs; ++r) {
+      g[(size_t)r * (m + 1)] = active[r] ? beta[r] : 0.0;
+      kk[r] = 0;
+    }
+    int j = 0;
+    for (; j < m && total < o.max_it; ++j) {
+      ++total;
+      double *Vj = V.get() + (size_t)j * blk;
+      TZ *Zj = Z.get() + (size_t)j * blk;
+      pc.apply_dir(s, nc, Vj, Zj, st);
+      op(Zj, w.get(), 1.0, 0.0, nullptr);
+      // classical Gram-Schmidt against V_0..V_j
+      mdot(s, ne, nc, V.get(), blk, j + 1, w.get(), dots.get(), work.get());
+      EMF_HIP(hipMemcpyAsync(h_dots.data(), dots.get(), sizeof(double) * (j + 1) * nc, hipMemcpyDeviceToHost, s));
+      EMF_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i <= j; ++i)
+        for (int r = 0; r < nrhs; ++r) {
+          const double hij = active[r] ? h_dots[(size_t)i * nc + 2 * r] + h_dots[(size_t)i * nc + 2 * r + 1] : 0.0;
+          H[((size_t)r * (m + 1) + i) * m + j] = hij;
+          h_coef[(size_t)i * nc + 2 * r] = h_coef[(size_t)i * nc + 2 * r + 1] = hij;
+        }
+      coef.upload(h_coef.data(), (size_t)(j + 1) * nc, s);
+      mupdate<double>(s, ne, nc, V.get(), blk, j + 1, coef.get(), nc, w.get(), 1.0);
+      std::vector<double> hn;
+      rhs_norms(w.get(), hn);
+      std::vector<double> fac(nrhs);
+      bool still = false;
+      for (int r = 0; r < nrhs; ++r) {
+        if (!active[r]) {
+          fac[r] = 0.0;
+          continue;
+        }
+        double *Hr = &H[(size_t)r * (m + 1) * m];
+        Hr[(size_t)(j + 1) * m + j] = hn[r];
+        fac[r] = hn[r] > 0.0 ? 1.0 / hn[r] : 0.0;
+        // apply previous rotations, then a new one
+        for (int i = 0; i < j; ++i) {
+          const double a = Hr[(size_t)i * m + j], c2 = Hr[(size_t)(i + 1) * m + j];
+          Hr[(size_t)i * m + j] = cs[(size_t)r * m + i] * a + sn[(size_t)r * m + i] * c2;
+          Hr[(size_t)(i + 1) * m + j] = -sn[(size_t)r * m + i] * a + cs[(size_t)r * m + i] * c2;
+        }
+        const double a = Hr[(size_t)j * m + j], c2 = Hr[(size_t)(j + 1) * m + j];
+        const double d = std::hypot(a, c2);
+        const double cj = d > 0 ? a / d : 1.0, sj = d > 0 ? c2 / d : 0.0;
+        cs[(size_t)r * m + j] = cj;
+        sn[(size_t)r * m + j] = sj;
+        Hr[(size_t)j * m + j] = d;
+        Hr[(size_t)(j + 1) * m + j] = 0.0;
+        double *gr = &g[(size_t)r * (m + 1)];
+        gr[j + 1] = -sj * gr[j];
+        gr[j] = cj * gr[j];
+        kk[r] = j + 1;
+        st.its[r] += 1;
+        if (std::abs(gr[j + 1]) <= rtol[r] * bnorm[r] || hn[r] == 0.0) {
+          active[r] = 0;  // converged in this cycle: freeze its basis
+          done[r] = 1;
+        } else {
+          still = true;
+        }
+      }
+      scale_cols(w.get(), fac);
+      EMF_HIP(hipMemcpyAsync(V.get() + (size_t)(j + 1) * blk, w.get(), sizeof(double) * blk,
+                             hipMemcpyDeviceToDevice, s));
+      if (!still) {
+        ++j;
+        break;
+      }
+    }
+    // y = R^-1 g per right-hand side; x += sum_i y_i Z_i
+    std::fill(h_coef.begin(), h_coef.end(), 0.0);
+    int kmax = 0;
+    for (int r = 0; r < nrhs; ++r) {
+      const int k = kk[r];
+      kmax = std::max(kmax, k);
+      const double *Hr = &H[(size_t)r * (m + 1) * m];
+      const double *gr = &g[(size_t)r * (m + 1)];
+      std::vector<double> y(k, 0.0);
+      for (int i = k - 1; i >= 0; --i) {
+        double t = gr[i];
+        for (int l = i + 1; l < k; ++l) t -= Hr[(size_t)i * m + l] * y[l];
+        y[i] = Hr[(size_t)i * m + i] != 0.0 ? t / Hr[(size_t)i * m + i] : 0.0;
+      }
+      for (int i = 0; i < k; ++i) h_coef[(size_t)i * nc + 2 * r] = h_coef[(size_t)i * nc + 2 * r + 1] = y[i];
+    }
+    if (kmax > 0) {
+      coef.upload(h_coef.data(), (size_t)kmax * nc, s);
+      mupdate<TZ>(s, ne, nc, Z.get(), blk, kmax, coef.get(), nc, x, -1.0);
+    }
+    bool all = true;
+    for (int r = 0; r < nrhs; ++r) all &= done[r] != 0;
+    if (all) break;
+  }
+  // true residual
+  op(x, w.get(), -1.0, 1.0, b);
+  std::vector<double> rn;
+  rhs_norms(w.get(), rn);
+  st.outer_its = 0;
+  for (int r = 0; r < nrhs; ++r) {
+    st.relres[r] = bnorm[r] > 0 ? rn[r] / bnorm[r] : 0.0;
+    st.outer_its = std::max(st.outer_its, st.its[r]);
+  }
+}
+
+}  // namespace
+
+void fgmres(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, InnerPc &pc, int nrhs,
+            const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o, SolveStats &st,
+            KernelClock *clk) {
+  if (pc.fp32())
+    fgmres_impl<float>(s, ne, nnz, rp, ci, K, pc, nrhs, b, x, rtol, o, st, clk);
+  else
+    fgmres_impl<double>(s, ne, nnz, rp, ci, K, pc, nrhs, b, x, rtol, o, st, clk);
+}
+
+}  // namespace emfem

@@ -1,0 +1,688 @@
+// Sparse and block-vector kernels for gfx950 (64-wide wavefronts).
+//
+// SpMM strategy: rows of the edge matrices hold ~16 nonzeros.  For few columns a
+// group of LPR lanes shares a row (lanes split the nonzeros, each lane keeps all
+// columns in registers, then a butterfly reduction); for many columns a whole
+// wavefront owns a row and lanes own columns, so the column index and the matrix
+// value of a nonzero are wavefront-uniform (scalar loads) and the gathered block
+// row x[col][:] is one coalesced 64-lane access.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace emfem {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int grid_for(int64_t threads) { return (int)std::max<int64_t>(1, (threads + kBlock - 1) / kBlock); }
+
+template <typename T>
+__device__ __forceinline__ T shfl_xor(T v, int mask, int width) {
+  return __shfl_xor(v, mask, width);
+}
+
+// ------------------------------------------------------------------------------
+// real CSR x block
+// ------------------------------------------------------------------------------
+
+template <typename T, int NC, int LPR>
+__global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci, const T *__restrict__ av,
+                                                        const T *__restrict__ x, int ldx, T *y, int ldy, T alpha,
+                                                        const T *__restrict__ rs, T gamma, const T *w, int ldw,
+                                                        T beta, const T *z, int ldz) {
+  const int gid = blockIdx.x * kBlock + threadIdx.x;
+  const int row = gid / LPR;
+  const int lane = gid % LPR;
+  if (row >= nrows) return;
+  T acc[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) acc[j] = T(0);
+  const int e = rp[row + 1];
+  for (int p = rp[row] + lane; p < e; p += LPR) {
+    const T a = av[p];
+    const T *xr = x + (size_t)ci[p] * ldx;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] += a * xr[j];
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] += shfl_xor(acc[j], off, LPR);
+  }
+  const T scale = rs ? rs[row] : T(1);
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    if (lane == (j % LPR)) {
+      T v = alpha * acc[j];
+      if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
+      v *= scale;
+      if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
+      y[(size_t)row * ldy + j] = v;
+    }
+  }
+}
+
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci, const T *__restrict__ av,
+                                                      const T *__restrict__ x, int ldx, T *y, int ldy, int nc,
+                                                      T alpha, const T *__restrict__ rs, T gamma, const T *w,
+                                                      int ldw, T beta, const T *z, int ldz) {
+  int row = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+  row = __builtin_amdgcn_readfirstlane(row);
+  const int lane = threadIdx.x % 64;
+  if (row >= nrows) return;
+  T acc[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) acc[j] = T(0);
+  const int b = rp[row], e = rp[row + 1];
+  for (int p = b; p < e; ++p) {
+    const T a = av[p];
+    const T *xr = x + (size_t)ci[p] * ldx;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      const int c = lane + 64 * j;
+      if (c < nc) acc[j] += a * xr[c];
+    }
+  }
+  const T scale = rs ? rs[row] : T(1);
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    const int c = lane + 64 * j;
+    if (c < nc) {
+      T v = alpha * acc[j];
+      if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
+      v *= scale;
+      if (beta != T(0)) v += beta * z[(size_t)row * ldz + c];
+      y[(size_t)row * ldy + c] = v;
+    }
+  }
+}
+
+template <typename T, int NC>
+void launch_narrow(hipStream_t s, const Csr<T> &A, const T *x, int ldx, T *y, int ldy, T alpha, const T *rs, T gamma,
+                   const T *w, int ldw, T beta, const T *z, int ldz) {
+  const double avg = A.nrows ? double(A.nnz) / A.nrows : 0.0;
+  const int lpr = avg <= 6 ? 4 : (avg <= 20 ? 8 : (avg <= 48 ? 16 : 32));
+  const int64_t threads = (int64_t)A.nrows * lpr;
+  const int g = grid_for(threads);
+#define EMF_NARROW(L)                                                                                      \
+  hipLaunchKernelGGL((k_spmm_narrow<T, NC, L>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y, \
+                     ldy, alpha, rs, gamma, w, ldw, beta, z, ldz)
+  switch (lpr) {
+    case 4: EMF_NARROW(4); break;
+    case 8: EMF_NARROW(8); break;
+    case 16: EMF_NARROW(16); break;
+    default: EMF_NARROW(32); break;
+  }
+#undef EMF_NARROW
+}
+
+}  // namespace
+
+template <typename T>
+void spmm(hipStream_t s, const Csr<T> &A, int nc, const T *x, int ldx, T *y, int ldy, T alpha, const T *rs, T gamma,
+          const T *w, int ldw, T beta, const T *z, int ldz) {
+  if (A.nrows == 0 || nc == 0) return;
+  if (nc >= 16) {
+    const int g = (A.nrows + kBlock / 64 - 1) / (kBlock / 64);
+    if (nc <= 64)
+      hipLaunchKernelGGL((k_spmm_wide<T, 1>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y, ldy,
+                         nc, alpha, rs, gamma, w, ldw, beta, z, ldz);
+    else if (nc <= 128)
+      hipLaunchKernelGGL((k_spmm_wide<T, 2>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y, ldy,
+                         nc, alpha, rs, gamma, w, ldw, beta, z, ldz);
+    else {
+      // process 256-column slabs
+      for (int c0 = 0; c0 < nc; c0 += 256) {
+        const int n = std::min(256, nc - c0);
+        hipLaunchKernelGGL((k_spmm_wide<T, 4>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x + c0, ldx,
+                           y + c0, ldy, n, alpha, rs, gamma, w ? w + c0 : nullptr, ldw, beta, z ? z + c0 : nullptr,
+                           ldz);
+      }
+    }
+    EMF_HIP(hipGetLastError());
+    return;
+  }
+  int c0 = 0;
+  while (c0 < nc) {
+    const int left = nc - c0;
+    const int n = left >= 8 ? 8 : (left >= 6 ? 6 : (left >= 4 ? 4 : (left >= 3 ? 3 : (left >= 2 ? 2 : 1))));
+    const T *wc = w ? w + c0 : nullptr;
+    const T *zc = z ? z + c0 : nullptr;
+    switch (n) {
+      case 8: launch_narrow<T, 8>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
+      case 6: launch_narrow<T, 6>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
+      case 4: launch_narrow<T, 4>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
+      case 3: launch_narrow<T, 3>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
+      case 2: launch_narrow<T, 2>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
+      default: launch_narrow<T, 1>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
+    }
+    c0 += n;
+  }
+  EMF_HIP(hipGetLastError());
+}
+
+template void spmm<float>(hipStream_t, const Csr<float> &, int, const float *, int, float *, int, float, const float *,
+                          float, const float *, int, float, const float *, int);
+template void spmm<double>(hipStream_t, const Csr<double> &, int, const double *, int, double *, int, double,
+                           const double *, double, const double *, int, double, const double *, int);
+
+// ------------------------------------------------------------------------------
+// complex operator (C, M) on interleaved pairs
+// ------------------------------------------------------------------------------
+
+namespace {
+
+template <typename TX, int NP, int LPR>
+__global__ __launch_bounds__(kBlock) void k_op_narrow(int nrows, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                      const double2 *__restrict__ K, const TX *__restrict__ x, int ldx,
+                                                      double *y, int ldy, double alpha, double beta, const double *z,
+                                                      int ldz) {
+  const int gid = blockIdx.x * kBlock + threadIdx.x;
+  const int row = gid / LPR;
+  const int lane = gid % LPR;
+  if (row >= nrows) return;
+  double ar[NP], ai[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) ar[j] = ai[j] = 0.0;
+  const int e = rp[row + 1];
+  for (int p = rp[row] + lane; p < e; p += LPR) {
+    const double2 cm = K[p];
+    const TX *xr = x + (size_t)ci[p] * ldx;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const double re = (double)xr[2 * j], im = (double)xr[2 * j + 1];
+      ar[j] += cm.x * re - cm.y * im;
+      ai[j] += -cm.y * re - cm.x * im;
+    }
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      ar[j] += shfl_xor(ar[j], off, LPR);
+      ai[j] += shfl_xor(ai[j], off, LPR);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    if (lane == (j % LPR)) {
+      double vr = alpha * ar[j], vi = alpha * ai[j];
+      if (beta != 0.0) {
+        vr += beta * z[(size_t)row * ldz + 2 * j];
+        vi += beta * z[(size_t)row * ldz + 2 * j + 1];
+      }
+      y[(size_t)row * ldy + 2 * j] = vr;
+      y[(size_t)row * ldy + 2 * j + 1] = vi;
+    }
+  }
+}
+
+template <typename TX, int PPL>
+__global__ __launch_bounds__(kBlock) void k_op_wide(int nrows, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                    const double2 *__restrict__ K, const TX *__restrict__ x, int ldx,
+                                                    double *y, int ldy, int np, double alpha, double beta,
+                                                    const double *z, int ldz) {
+  int row = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+  row = __builtin_amdgcn_readfirstlane(row);
+  const int lane = threadIdx.x % 64;
+  if (row >= nrows) return;
+  double ar[PPL], ai[PPL];
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) ar[j] = ai[j] = 0.0;
+  const int b = rp[row], e = rp[row + 1];
+  for (int p = b; p < e; ++p) {
+    const double2 cm = K[p];
+    const TX *xr = x + (size_t)ci[p] * ldx;
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      const int q = lane + 64 * j;
+      if (q < np) {
+        const double re = (double)xr[2 * q], im = (double)xr[2 * q + 1];
+        ar[j] += cm.x * re - cm.y * im;
+        ai[j] += -cm.y * re - cm.x * im;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    const int q = lane + 64 * j;
+    if (q < np) {
+      double vr = alpha * ar[j], vi = alpha * ai[j];
+      if (beta != 0.0) {
+        vr += beta * z[(size_t)row * ldz + 2 * q];
+        vi += beta * z[(size_t)row * ldz + 2 * q + 1];
+      }
+      y[(size_t)row * ldy + 2 * q] = vr;
+      y[(size_t)row * ldy + 2 * q + 1] = vi;
+    }
+  }
+}
+
+template <typename TX, int NP>
+void launch_op_narrow(hipStream_t s, int nrows, int nnz, const int *rp, const int *ci, const double2 *K, const TX *x,
+                      int ldx, double *y, int ldy, double alpha, double beta, const double *z, int ldz) {
+  const double avg = nrows ? double(nnz) / nrows : 0.0;
+  const int lpr = avg <= 6 ? 4 : (avg <= 20 ? 8 : 16);
+  const int g = grid_for((int64_t)nrows * lpr);
+  if (lpr == 4)
+    hipLaunchKernelGGL((k_op_narrow<TX, NP, 4>), dim3(g), dim3(kBlock), 0, s, nrows, rp, ci, K, x, ldx, y, ldy, alpha,
+                       beta, z, ldz);
+  else if (lpr == 8)
+    hipLaunchKernelGGL((k_op_narrow<TX, NP, 8>), dim3(g), dim3(kBlock), 0, s, nrows, rp, ci, K, x, ldx, y, ldy, alpha,
+                       beta, z, ldz);
+  else
+    hipLaunchKernelGGL((k_op_narrow<TX, NP, 16>), dim3(g), dim3(kBlock), 0, s, nrows, rp, ci, K, x, ldx, y, ldy,
+                       alpha, beta, z, ldz);
+}
+
+}  // namespace
+
+template <typename TX>
+void op_spmm(hipStream_t s, int nrows, int nnz, const int *rp, const int *ci, const double2 *K, int nrhs, const TX *x,
+             int ldx, double *y, int ldy, double alpha, double beta, const double *z, int ldz) {
+  if (nrows == 0 || nrhs == 0) return;
+  if (nrhs >= 8) {
+    const int g = (nrows + kBlock / 64 - 1) / (kBlock / 64);
+    for (int q0 = 0; q0 < nrhs; q0 += 128) {
+      const int np = std::min(128, nrhs - q0);
+      const double *zq = z ? z + 2 * q0 : nullptr;
+      if (np <= 64)
+        hipLaunchKernelGGL((k_op_wide<TX, 1>), dim3(g), dim3(kBlock), 0, s, nrows, rp, ci, K, x + 2 * q0, ldx,
+                           y + 2 * q0, ldy, np, alpha, beta, zq, ldz);
+      else
+        hipLaunchKernelGGL((k_op_wide<TX, 2>), dim3(g), dim3(kBlock), 0, s, nrows, rp, ci, K, x + 2 * q0, ldx,
+                           y + 2 * q0, ldy, np, alpha, beta, zq, ldz);
+    }
+    EMF_HIP(hipGetLastError());
+    return;
+  }
+  int q0 = 0;
+  while (q0 < nrhs) {
+    const int left = nrhs - q0;
+    const int n = left >= 4 ? 4 : left;
+    const double *zq = z ? z + 2 * q0 : nullptr;
+    switch (n) {
+      case 4: launch_op_narrow<TX, 4>(s, nrows, nnz, rp, ci, K, x + 2 * q0, ldx, y + 2 * q0, ldy, alpha, beta, zq, ldz); break;
+      case 3: launch_op_narrow<TX, 3>(s, nrows, nnz, rp, ci, K, x + 2 * q0, ldx, y + 2 * q0, ldy, alpha, beta, zq, ldz); break;
+      case 2: launch_op_narrow<TX, 2>(s, nrows, nnz, rp, ci, K, x + 2 * q0, ldx, y + 2 * q0, ldy, alpha, beta, zq, ldz); break;
+      default: launch_op_narrow<TX, 1>(s, nrows, nnz, rp, ci, K, x + 2 * q0, ldx, y + 2 * q0, ldy, alpha, beta, zq, ldz); break;
+    }
+    q0 += n;
+  }
+  EMF_HIP(hipGetLastError());
+}
+
+template void op_spmm<float>(hipStream_t, int, int, const int *, const int *, const double2 *, int, const float *,
+                             int, double *, int, double, double, const double *, int);
+template void op_spmm<double>(hipStream_t, int, int, const int *, const int *, const double2 *, int, const double *,
+                              int, double *, int, double, double, const double *, int);
+
+// ------------------------------------------------------------------------------
+// B = C + M
+// ------------------------------------------------------------------------------
+
+namespace {
+template <typename T>
+__global__ void k_make_b(int nnz, const double2 *__restrict__ K, T *__restrict__ B) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nnz) B[i] = (T)(K[i].x + K[i].y);
+}
+}  // namespace
+
+template <typename T>
+void make_b(hipStream_t s, int nnz, const double2 *K, T *B) {
+  hipLaunchKernelGGL((k_make_b<T>), dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, K, B);
+  EMF_HIP(hipGetLastError());
+}
+template void make_b<float>(hipStream_t, int, const double2 *, float *);
+template void make_b<double>(hipStream_t, int, const double2 *, double *);
+
+// ------------------------------------------------------------------------------
+// reductions
+// ------------------------------------------------------------------------------
+
+namespace {
+
+constexpr int kDotBlocks = 512;
+
+// Threads of a block are assigned to columns c = tid % nc (block size is a multiple
+// of nc) and stride over rows; the per-column block sums go to work[block][c].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_coldot(int rows, int nc, int rows_per_pass, const T *__restrict__ x,
+                                                   int ldx, const T *__restrict__ y, int ldy, double *work) {
+  __shared__ double sh[kBlock];
+  const int per_block = (kBlock / nc) * nc;
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  if (t < per_block) {
+    const int c = t % nc;
+    const int r0 = t / nc;
+    for (int r = blockIdx.x * rows_per_pass + r0; r < rows; r += gridDim.x * rows_per_pass)
+      acc += (double)x[(size_t)r * ldx + c] * (double)y[(size_t)r * ldy + c];
+  }
+  sh[t] = acc;
+  __syncthreads();
+  if (t < nc) {
+    double sum = 0.0;
+    for (int k = t; k < per_block; k += nc) sum += sh[k];
+    work[(size_t)blockIdx.x * nc + t] = sum;
+  }
+}
+
+__global__ void k_finish_dot(int nblocks, int nc, const double *__restrict__ work, double *out) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += work[(size_t)b * nc + c];
+  out[c] = s;
+}
+
+}  // namespace
+
+size_t dot_work_size(int rows, int nc) { return (size_t)kDotBlocks * std::max(nc, 1) + 64; }
+
+template <typename T>
+void coldot(hipStream_t s, int rows, int nc, const T *x, int ldx, const T *y, int ldy, double *out, double *work) {
+  if (nc > kBlock) {
+    for (int c0 = 0; c0 < nc; c0 += kBlock)
+      coldot<T>(s, rows, std::min(kBlock, nc - c0), x + c0, ldx, y + c0, ldy, out + c0, work);
+    return;
+  }
+  const int rows_per_pass = kBlock / nc;
+  const int nb = std::max(1, std::min(kDotBlocks, (rows + rows_per_pass - 1) / rows_per_pass));
+  hipLaunchKernelGGL((k_coldot<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, rows_per_pass, x, ldx, y, ldy, work);
+  hipLaunchKernelGGL(k_finish_dot, dim3(grid_for(nc)), dim3(kBlock), 0, s, nb, nc, work, out);
+  EMF_HIP(hipGetLastError());
+}
+template void coldot<float>(hipStream_t, int, int, const float *, int, const float *, int, double *, double *);
+template void coldot<double>(hipStream_t, int, int, const double *, int, const double *, int, double *, double *);
+
+namespace {
+
+// Up to 8 basis vectors per pass; w is read once per pass.
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_mdot(int rows, int nc, int rows_per_pass, const double *__restrict__ V,
+                                                 size_t vstride, const double *__restrict__ w, double *work) {
+  __shared__ double sh[M][kBlock];
+  const int per_block = (kBlock / nc) * nc;
+  const int t = threadIdx.x;
+  double acc[M];
+#pragma unroll
+  for (int i = 0; i < M; ++i) acc[i] = 0.0;
+  if (t < per_block) {
+    const int c = t % nc;
+    const int r0 = t / nc;
+    for (int r = blockIdx.x * rows_per_pass + r0; r < rows; r += gridDim.x * rows_per_pass) {
+      const size_t o = (size_t)r * nc + c;
+      const double wv = w[o];
+#pragma unroll
+      for (int i = 0; i < M; ++i) acc[i] += V[i * vstride + o] * wv;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < M; ++i) sh[i][t] = acc[i];
+  __syncthreads();
+  if (t < nc) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      double sum = 0.0;
+      for (int k = t; k < per_block; k += nc) sum += sh[i][k];
+      work[((size_t)i * gridDim.x + blockIdx.x) * nc + t] = sum;
+    }
+  }
+}
+
+__global__ void k_finish_mdot(int nblocks, int nc, int m, const double *__restrict__ work, double *out) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= nc * m) return;
+  const int i = idx / nc, c = idx % nc;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += work[((size_t)i * nblocks + b) * nc + c];
+  out[idx] = s;
+}
+
+}  // namespace
+
+void mdot(hipStream_t s, int rows, int nc, const double *V, size_t vstride, int m, const double *w, double *out,
+          double *work) {
+  EMF_REQUIRE(nc <= kBlock, kErrArg, "mdot supports at most 256 columns per call");
+  const int rows_per_pass = kBlock / nc;
+  const int nb = std::max(1, std::min(kDotBlocks, (rows + rows_per_pass - 1) / rows_per_pass));
+  for (int i0 = 0; i0 < m; i0 += 8) {
+    const int mm = std::min(8, m - i0);
+    const double *Vi = V + (size_t)i0 * vstride;
+    switch (mm) {
+#define EMF_MDOT(K)                                                                                           \
+  case K:                                                                                                     \
+    hipLaunchKernelGGL((k_mdot<K>), dim3(nb), dim3(kBlock), 0, s, rows, nc, rows_per_pass, Vi, vstride, w, work); \
+    break;
+      EMF_MDOT(1) EMF_MDOT(2) EMF_MDOT(3) EMF_MDOT(4) EMF_MDOT(5) EMF_MDOT(6) EMF_MDOT(7) EMF_MDOT(8)
+#undef EMF_MDOT
+    }
+    hipLaunchKernelGGL(k_finish_mdot, dim3(grid_for(nc * mm)), dim3(kBlock), 0, s, nb, nc, mm, work,
+                       out + (size_t)i0 * nc);
+  }
+  EMF_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------
+// block updates
+// ------------------------------------------------------------------------------
+
+namespace {
+
+template <typename TB>
+__global__ void k_mupdate(int64_t n, int nc, const TB *__restrict__ V, size_t vstride, int m,
+                          const double *__restrict__ h, int hld, double *w, double sign) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)(i % nc);
+  double acc = 0.0;
+  for (int k = 0; k < m; ++k) acc += h[(size_t)k * hld + c] * (double)V[k * vstride + i];
+  w[i] -= sign * acc;
+}
+
+template <typename TX, typename TY>
+__global__ void k_axpby_cols(int rows, int nc, const double *__restrict__ a, const TX *__restrict__ x, int ldx,
+                             const double *__restrict__ b, TY *y, int ldy) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)rows * nc) return;
+  const int r = (int)(i / nc), c = (int)(i % nc);
+  double v = a[c] * (double)x[(size_t)r * ldx + c];
+  if (b) v += b[c] * (double)y[(size_t)r * ldy + c];
+  y[(size_t)r * ldy + c] = (TY)v;
+}
+
+template <typename TX, typename TY>
+__global__ void k_axpby(int rows, int nc, double alpha, const TX *__restrict__ x, int ldx, double beta, TY *y,
+                        int ldy) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)rows * nc) return;
+  const int r = (int)(i / nc), c = (int)(i % nc);
+  double v = alpha * (double)x[(size_t)r * ldx + c];
+  if (beta != 0.0) v += beta * (double)y[(size_t)r * ldy + c];
+  y[(size_t)r * ldy + c] = (TY)v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, int rows_per_pass, const double *__restrict__ alpha,
+                                                   const T *__restrict__ p, const T *__restrict__ q, T *x, T *r,
+                                                   double *work) {
+  __shared__ double sh[kBlock];
+  const int per_block = (kBlock / nc) * nc;
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  if (t < per_block) {
+    const int c = t % nc;
+    const int r0 = t / nc;
+    const double a = alpha[c];
+    for (int row = blockIdx.x * rows_per_pass + r0; row < rows; row += gridDim.x * rows_per_pass) {
+      const size_t o = (size_t)row * nc + c;
+      x[o] = (T)((double)x[o] + a * (double)p[o]);
+      const T rv = (T)((double)r[o] - a * (double)q[o]);
+      r[o] = rv;
+      acc += (double)rv * (double)rv;
+    }
+  }
+  sh[t] = acc;
+  __syncthreads();
+  if (t < nc) {
+    double sum = 0.0;
+    for (int k = t; k < per_block; k += nc) sum += sh[k];
+    work[(size_t)blockIdx.x * nc + t] = sum;
+  }
+}
+
+template <typename T>
+__global__ void k_cheb_init(int64_t n, int nc, const T *__restrict__ dinv, const T *__restrict__ b, T *rres, T *d,
+                            T *x, double inv_theta) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T rv = dinv[i / nc] * b[i];
+  rres[i] = rv;
+  const T dv = (T)(inv_theta * (double)rv);
+  d[i] = dv;
+  x[i] = dv;
+}
+
+template <typename T>
+__global__ void k_cheb_step(int64_t n, T *d, const T *__restrict__ rres, T *x, double c1, double c2) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const T dv = (T)(c1 * (double)d[i] + c2 * (double)rres[i]);
+  d[i] = dv;
+  x[i] += dv;
+}
+
+template <typename T>
+__global__ void k_diag_scale(int64_t n, int nc, const T *__restrict__ dinv, const T *__restrict__ b, T *x) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  x[i] = dinv[i / nc] * b[i];
+}
+
+template <typename T>
+__global__ void k_dense_solve(int n, int nc, const double *__restrict__ Ainv, const T *__restrict__ b, T *x) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n * nc) return;
+  const int i = idx / nc, c = idx % nc;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) s += Ainv[(size_t)i * n + j] * (double)b[(size_t)j * nc + c];
+  x[idx] = (T)s;
+}
+
+__global__ void k_fcg_alpha(int nc, const double *rz, const double *pq, const uint8_t *active, double *alpha) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  alpha[c] = (active[c] && pq[c] != 0.0) ? rz[c] / pq[c] : 0.0;
+}
+
+__global__ void k_fcg_beta(int nc, const double *zq, const double *pq, const uint8_t *active, double *beta) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  beta[c] = (active[c] && pq[c] != 0.0) ? -zq[c] / pq[c] : 0.0;
+}
+
+}  // namespace
+
+template <typename TB>
+void mupdate(hipStream_t s, int rows, int nc, const TB *V, size_t vstride, int m, const double *h, int hld, double *w,
+             double sign) {
+  if (m <= 0) return;
+  const int64_t n = (int64_t)rows * nc;
+  hipLaunchKernelGGL((k_mupdate<TB>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, V, vstride, m, h, hld, w, sign);
+  EMF_HIP(hipGetLastError());
+}
+template void mupdate<float>(hipStream_t, int, int, const float *, size_t, int, const double *, int, double *, double);
+template void mupdate<double>(hipStream_t, int, int, const double *, size_t, int, const double *, int, double *, double);
+
+template <typename TX, typename TY>
+void axpby_cols(hipStream_t s, int rows, int nc, const double *a, const TX *x, int ldx, const double *b, TY *y,
+                int ldy) {
+  const int64_t n = (int64_t)rows * nc;
+  hipLaunchKernelGGL((k_axpby_cols<TX, TY>), dim3(grid_for(n)), dim3(kBlock), 0, s, rows, nc, a, x, ldx, b, y, ldy);
+  EMF_HIP(hipGetLastError());
+}
+template void axpby_cols<float, float>(hipStream_t, int, int, const double *, const float *, int, const double *,
+                                       float *, int);
+template void axpby_cols<double, double>(hipStream_t, int, int, const double *, const double *, int, const double *,
+                                         double *, int);
+
+template <typename TX, typename TY>
+void axpby(hipStream_t s, int rows, int nc, double alpha, const TX *x, int ldx, double beta, TY *y, int ldy) {
+  const int64_t n = (int64_t)rows * nc;
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_axpby<TX, TY>), dim3(grid_for(n)), dim3(kBlock), 0, s, rows, nc, alpha, x, ldx, beta, y, ldy);
+  EMF_HIP(hipGetLastError());
+}
+template void axpby<float, float>(hipStream_t, int, int, double, const float *, int, double, float *, int);
+template void axpby<double, double>(hipStream_t, int, int, double, const double *, int, double, double *, int);
+template void axpby<double, float>(hipStream_t, int, int, double, const double *, int, double, float *, int);
+template void axpby<float, double>(hipStream_t, int, int, double, const float *, int, double, double *, int);
+
+template <typename T>
+void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const T *p, const T *q, T *x, T *r, double *rr,
+            double *work) {
+  EMF_REQUIRE(nc <= kBlock, kErrArg, "inner solve supports at most 256 columns");
+  const int rows_per_pass = kBlock / nc;
+  const int nb = std::max(1, std::min(kDotBlocks, (rows + rows_per_pass - 1) / rows_per_pass));
+  hipLaunchKernelGGL((k_fcg_xr<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, rows_per_pass, alpha, p, q, x, r, work);
+  hipLaunchKernelGGL(k_finish_dot, dim3(grid_for(nc)), dim3(kBlock), 0, s, nb, nc, work, rr);
+  EMF_HIP(hipGetLastError());
+}
+template void fcg_xr<float>(hipStream_t, int, int, const double *, const float *, const float *, float *, float *,
+                            double *, double *);
+template void fcg_xr<double>(hipStream_t, int, int, const double *, const double *, const double *, double *, double *,
+                             double *, double *);
+
+template <typename T>
+void cheb_init(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *rres, T *d, T *x, double inv_theta) {
+  const int64_t n = (int64_t)rows * nc;
+  hipLaunchKernelGGL((k_cheb_init<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, dinv, b, rres, d, x, inv_theta);
+  EMF_HIP(hipGetLastError());
+}
+template <typename T>
+void cheb_step(hipStream_t s, int rows, int nc, T *d, const T *rres, T *x, double c1, double c2) {
+  const int64_t n = (int64_t)rows * nc;
+  hipLaunchKernelGGL((k_cheb_step<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, rres, x, c1, c2);
+  EMF_HIP(hipGetLastError());
+}
+template <typename T>
+void diag_scale(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x) {
+  const int64_t n = (int64_t)rows * nc;
+  hipLaunchKernelGGL((k_diag_scale<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, dinv, b, x);
+  EMF_HIP(hipGetLastError());
+}
+template <typename T>
+void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T *x) {
+  hipLaunchKernelGGL((k_dense_solve<T>), dim3(grid_for((int64_t)n * nc)), dim3(kBlock), 0, s, n, nc, Ainv, b, x);
+  EMF_HIP(hipGetLastError());
+}
+
+#define EMF_INST(T)                                                                                        \
+  template void cheb_init<T>(hipStream_t, int, int, const T *, const T *, T *, T *, T *, double);           \
+  template void cheb_step<T>(hipStream_t, int, int, T *, const T *, T *, double, double);                  \
+  template void diag_scale<T>(hipStream_t, int, int, const T *, const T *, T *);                           \
+  template void dense_solve<T>(hipStream_t, int, int, const double *, const T *, T *);
+EMF_INST(float)
+EMF_INST(double)
+#undef EMF_INST
+
+void fcg_alpha(hipStream_t s, int nc, const double *rz, const double *pq, const uint8_t *active, double *alpha) {
+  hipLaunchKernelGGL(k_fcg_alpha, dim3(grid_for(nc)), dim3(kBlock), 0, s, nc, rz, pq, active, alpha);
+  EMF_HIP(hipGetLastError());
+}
+void fcg_beta(hipStream_t s, int nc, const double *zq, const double *pq, const uint8_t *active, double *beta) {
+  hipLaunchKernelGGL(k_fcg_beta, dim3(grid_for(nc)), dim3(kBlock), 0, s, nc, zq, pq, active, beta);
+  EMF_HIP(hipGetLastError());
+}
+
+}  // namespace emfem

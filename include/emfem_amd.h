@@ -1,0 +1,210 @@
+/*
+ * emfem_amd: MI355X-native implementation of EMFEM's forward-modelling hot path.
+ *
+ * C ABI.  Plain pointers and sizes only; every call returns an int status
+ * (0 on success, negative on failure) like the reference's PetscErrorCode
+ * functions, and the message of the last failure is available from
+ * emfem_last_error().  One context owns one HIP stream and all device buffers;
+ * calls on a context are synchronous at return and a context must be used by
+ * one host thread at a time (the reference's PETSc objects carry the same rule).
+ *
+ * Indexing is zero-based int32 (PetscInt default, TetGen `z` switch,
+ * src/em_mesh.cc:79-81).  Complex vectors cross the boundary as split real and
+ * imaginary arrays (PETScBlockVector {re, im}, src/em_defs.h:33-35) on the host,
+ * or as interleaved (re, im) pairs per edge and right-hand side on the device
+ * (row-major [n_edges][2 * n_rhs]).
+ *
+ * Each declaration names the reference interface it replaces (file:line,
+ * relative to the reference checkout).
+ */
+#ifndef EMFEM_AMD_H
+#define EMFEM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  EMFEM_ERR_USER matches EM_ERR_USER (src/em_defs.h:76). */
+#define EMFEM_OK 0
+#define EMFEM_ERR_USER (-1)
+#define EMFEM_ERR_HIP (-2)
+#define EMFEM_ERR_NOT_CONVERGED (-3)
+#define EMFEM_ERR_STATE (-4)
+#define EMFEM_ERR_ARG (-5)
+
+/* Polarisation codes (PolarType, src/em_defs.h:39) and the MT transmitter index
+ * (src/em_fwd.cc:766). */
+#define EMFEM_XY_POLAR (-1)
+#define EMFEM_YX_POLAR (-2)
+#define EMFEM_MT_TIDX (-3)
+
+typedef struct emfem_context emfem_context;
+
+/* ---- context and options --------------------------------------------------- */
+
+/* create_context (src/em_ctx.cc:7-84): allocates the context and its HIP stream on
+ * the current device. */
+int emfem_create_context(emfem_context **ctx);
+
+/* destroy_context (src/em_ctx.cc:86-98). */
+int emfem_destroy_context(emfem_context *ctx);
+
+/* Message of the last failed call on ctx (SETERRQ text in the reference). */
+const char *emfem_last_error(const emfem_context *ctx);
+
+/* process_options (src/em_ctx.cc:100-175): parses PETSc options-file text
+ * ("-key value" tokens, '#' comments) for the EMFEM keys and the A_/B_ solver
+ * keys this implementation honours; unknown keys are kept and reported by
+ * emfem_unused_options.  May be called more than once; later values win. */
+int emfem_process_options(emfem_context *ctx, const char *options_text);
+
+/* Current value of an option as text (after defaults); returns EMFEM_ERR_ARG for
+ * an unknown key. */
+int emfem_get_option(const emfem_context *ctx, const char *key, char *buf, int buflen);
+
+/* Space-separated list of option keys that were given but are not used. */
+int emfem_unused_options(const emfem_context *ctx, char *buf, int buflen);
+
+/* ---- mesh, model and survey ------------------------------------------------- */
+
+/* read_mesh + Mesh::create (src/em_io.cc:155-200, src/em_mesh.cc:56-85): points
+ * [n_points*3], tets [n_tets*4] (det([1; x_v]) > 0), attributes [n_tets] (row of
+ * the resistivity table).  Derives edges (ascending orientation,
+ * src/em_mesh.cc:448-455), face neighbours (src/em_mesh.cc:463-481) and boundary
+ * markers (src/em_mesh.cc:483-502), and uploads the mesh to the device.  Edges are
+ * numbered in lexicographic order of (lower vertex, higher vertex) unless
+ * emfem_set_mesh_edges supplies them. */
+int emfem_set_mesh(emfem_context *ctx, int n_points, const double *points, int n_tets, const int *tets,
+                   const int *attributes);
+
+/* Supplies TetGen's edge list and tet->edge map (tetgenio edgelist /
+ * tet2edgelist, src/em_mesh.cc:448-461) so that edge numbering matches a TetGen
+ * run exactly.  Call after emfem_set_mesh. */
+int emfem_set_mesh_edges(emfem_context *ctx, int n_edges, const int *edgelist, const int *tet2edge);
+
+/* read_rho (src/em_io.cc:9-86): aniso_form 0..3 with {1,2,3,6} columns per row. */
+int emfem_set_rho(emfem_context *ctx, int n_rho, int aniso_form, const double *rho);
+
+/* read_emd parts (src/em_io.cc:202-235): frequencies (Hz), transmitters (7 numbers
+ * each: x y z azimuth dip current length, src/em_defs.h:25) and receivers. */
+int emfem_set_frequencies(emfem_context *ctx, int n, const double *freqs);
+int emfem_set_transmitters(emfem_context *ctx, int n, const double *tx);
+int emfem_set_receivers(emfem_context *ctx, int n, const double *rx);
+
+/* info[0..4] = n_vertices, n_tets, n_edges, nnz (0 before create_linear_system),
+ * n_colours of the atomic-free assembly schedule. */
+int emfem_get_mesh_info(const emfem_context *ctx, int *info);
+
+/* Topology as the context holds it: edgelist [n_edges*2], tet2edge [n_tets*6],
+ * neighbor [n_tets*4], edge_bdr [n_edges].  Any pointer may be NULL. */
+int emfem_get_topology(const emfem_context *ctx, int *edgelist, int *tet2edge, int *neighbor,
+                       unsigned char *edge_bdr);
+
+/* Mesh::find_cell_around_point (src/em_mesh.cc:527-566): tet containing p, or -1. */
+int emfem_find_cell(const emfem_context *ctx, const double *p);
+
+/* update_background_model (src/em_fwd.cc:135-159): 1-D layered columns under the
+ * four top corners, used by the MT boundary data. */
+int emfem_update_background_model(emfem_context *ctx);
+
+/* ---- linear system ---------------------------------------------------------- */
+
+/* create_linear_system (src/em_fwd.cc:248-295): builds the CSR pattern
+ * (src/em_fwd.cc:191-246) on the device and allocates n_slots right-hand-side and
+ * solution columns. */
+int emfem_create_linear_system(emfem_context *ctx, int n_slots);
+
+/* destroy_linear_system (src/em_fwd.cc:297-316). */
+int emfem_destroy_linear_system(emfem_context *ctx);
+
+/* assemble_matrix (src/em_fwd.cc:318-400): C and M on the device for frequency fidx. */
+int emfem_assemble_matrix(emfem_context *ctx, int fidx);
+
+/* Copies the assembled CSR to the host: rowptr [n_edges+1], col [nnz], C [nnz],
+ * M [nnz].  Any pointer may be NULL. */
+int emfem_get_matrix(const emfem_context *ctx, int *rowptr, int *col, double *C, double *M);
+
+/* assemble_rhs_mt (src/em_fwd.cc:488-580) for polarisation mode (-1 XY, -2 YX). */
+int emfem_assemble_rhs_mt(emfem_context *ctx, int fidx, int mode, int slot);
+
+/* assemble_rhs_csem (src/em_fwd.cc:616-671). */
+int emfem_assemble_rhs_csem(emfem_context *ctx, int fidx, int tidx, int slot);
+
+/* assemble_rhs_dual (src/em_fwd.cc:673-717). */
+int emfem_assemble_rhs_dual(emfem_context *ctx, int slot);
+
+/* Host copies of a right-hand side / solution slot (s.re, s.im / e.re, e.im). */
+int emfem_set_rhs(emfem_context *ctx, int slot, const double *re, const double *im);
+int emfem_get_rhs(const emfem_context *ctx, int slot, double *re, double *im);
+int emfem_get_solution(const emfem_context *ctx, int slot, double *re, double *im);
+
+/* create_pc (src/em_la.cc:113-191): B = C + M, the auxiliary-space (AMS)
+ * preconditioner for B and the outer FGMRES set-up.  Precision of the inner solve
+ * follows the option -inner_precision (fp32 | fp64). */
+int emfem_create_pc(emfem_context *ctx);
+
+/* destroy_pc (src/em_la.cc:193-206). */
+int emfem_destroy_pc(emfem_context *ctx);
+
+/* solve_linear_system (src/em_la.cc:35-63), batched: solves slots
+ * [first, first+n) together, each to its own relative tolerance rtols[i] (or
+ * rtol for all when rtols is NULL), zero initial guess, at most max_it outer
+ * iterations.  iters[i] and relres[i] (true FP64 residual ||s - A e|| / ||s||)
+ * are written when the pointers are non-NULL.  Returns EMFEM_ERR_NOT_CONVERGED
+ * (after storing the partial solution) if any right-hand side missed its
+ * tolerance, which the reference never reports. */
+int emfem_solve_linear_system(emfem_context *ctx, int first, int n, int max_it, double rtol,
+                              const double *rtols, int *iters, double *relres);
+
+/* matshell_mult_a (src/em_la.cc:65-87) on device arrays: y = A x for n_rhs
+ * interleaved right-hand sides ([n_edges][2*n_rhs]).  Also MATOP_MULT_TRANSPOSE
+ * (A is symmetric, src/em_la.cc:169). */
+int emfem_matshell_mult_a(emfem_context *ctx, int n_rhs, const double *x_dev, double *y_dev);
+
+/* pc_apply_b (src/em_la.cc:208-228) on device arrays: x = diag(B,B)^-1 b by the
+ * inner Krylov solve (FCG + AMS) for n_rhs interleaved right-hand sides. */
+int emfem_pc_apply_b(emfem_context *ctx, int n_rhs, const double *b_dev, double *x_dev);
+
+/* ---- estimator, responses, marking ----------------------------------------- */
+
+/* estimate_error (src/em_fwd.cc:1102-1170): squared indicators per tet for each of
+ * the n solution slots; errors is [n][n_tets] on the host. */
+int emfem_estimate_error(emfem_context *ctx, int fidx, int n, const int *slots, double *errors);
+
+/* interpolate_fields (src/em_fwd.cc:719-744) at n points: E and H as
+ * [n][3][2] (re, im) arrays.  The containing tet is found by
+ * emfem_find_cell's rule. */
+int emfem_interpolate_fields(emfem_context *ctx, int fidx, int slot, int n, const double *points,
+                             double *E, double *H);
+
+/* calculate_response_mt (src/em_fwd.cc:746-802): obs is [n_obs][4] =
+ * (otype, fidx, tidx, ridx); rsp [n_obs][2] is written for rows with this fidx and
+ * tidx -3.  Apparent resistivity and phase (codes 212, 222) are answered as
+ * log(rho_a) + i*phi, which the reference leaves unimplemented. */
+int emfem_calculate_response_mt(emfem_context *ctx, int fidx, int slot_xy, int slot_yx, int n_obs,
+                                const int *obs, double *rsp);
+
+/* calculate_response_csem (src/em_fwd.cc:804-871). */
+int emfem_calculate_response_csem(emfem_context *ctx, int fidx, int tidx, int slot, int n_obs,
+                                  const int *obs, double *rsp);
+
+/* refine_fixed_fraction / refine_fixed_number (src/em_fwd.cc:1182-1240); flags are
+ * OR-ed into flag[n]. */
+int emfem_refine_fixed_fraction(int n, const double *error, double fraction, unsigned char *flag);
+int emfem_refine_fixed_number(int n, const double *error, double fraction, unsigned char *flag);
+
+/* ---- measurement ---------------------------------------------------------- */
+
+/* Named counters and timers of the last calls (e.g. "assemble_ms", "solve_ms",
+ * "outer_its", "inner_its", "vcycles"); returns EMFEM_ERR_ARG for an unknown name. */
+int emfem_get_stat(const emfem_context *ctx, const char *name, double *value);
+
+/* Average device time (ms) of the hot kernels of the last solve, measured with HIP
+ * events on the context's stream: which = "assemble", "op_spmm", "b_spmm". */
+int emfem_kernel_time(const emfem_context *ctx, const char *which, double *ms, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,203 @@
+"""GPU parity: the HIP path through the C ABI against the CPU FP64 oracle.
+
+Bars: topology/pattern bit-exact; assembled C and M within 1e-12 (relative to the
+largest entry); right-hand sides within 1e-12; FGMRES solutions within the
+discretisation-independent bound set by the requested residual; receiver responses
+within 1e-6 relative of the responses of the exact discrete solution.
+"""
+import numpy as np
+import pytest
+
+from emfem_amd import api
+from emfem_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
+    return torch
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = np.max(np.abs(b)) if b.size else 1.0
+    return float(np.max(np.abs(a - b)) / scale) if b.size else 0.0
+
+
+@pytest.fixture(scope="module")
+def tiny(oracle_mod):
+    m = mg.tiny_mt(n=3, aniso_form=3, seed=5, freq=0.7)
+    om = oracle_mod.Mesh(m.points, m.tets)
+    return m, om
+
+
+def _ctx(m, opts=""):
+    c = api.Context(opts)
+    c.load_model(m)
+    return c
+
+
+def test_assembly_matches_oracle(tiny, oracle_mod):
+    m, om = tiny
+    ctx = _ctx(m)
+    ctx.create_linear_system(3)
+    ctx.assemble_matrix(0)
+    rp, ci, C, M = ctx.get_matrix()
+    orp, oci = om.pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci), "CSR pattern differs"
+    _, _, oC, oM, _ = oracle_mod.assemble(om, m, m.freqs[0], (orp, oci))
+    assert _rel(C, oC) < 1e-12
+    assert _rel(M, oM) < 1e-12
+    # bitwise reproducible (colour-ordered, atomic-free scatter)
+    ctx.assemble_matrix(0)
+    _, _, C2, M2 = ctx.get_matrix()
+    assert np.array_equal(C, C2) and np.array_equal(M, M2)
+
+
+@pytest.mark.parametrize("form", [0, 1, 2, 3])
+def test_assembly_all_aniso_forms(form, oracle_mod):
+    m = mg.tiny_mt(n=2, aniso_form=form, seed=11, freq=3.0)
+    om = oracle_mod.Mesh(m.points, m.tets)
+    ctx = _ctx(m)
+    ctx.create_linear_system(1)
+    ctx.assemble_matrix(0)
+    rp, ci, C, M = ctx.get_matrix()
+    _, _, oC, oM, _ = oracle_mod.assemble(om, m, m.freqs[0], (rp, ci))
+    assert _rel(C, oC) < 1e-12 and _rel(M, oM) < 1e-12
+
+
+def test_operator_matches_oracle(tiny, oracle_mod):
+    torch = _torch()
+    m, om = tiny
+    ctx = _ctx(m)
+    ctx.create_linear_system(1)
+    ctx.assemble_matrix(0)
+    rp, ci, C, M = ctx.get_matrix()
+    ne = rp.size - 1
+    rng = np.random.default_rng(3)
+    for k in (1, 3, 8, 20):
+        x = rng.standard_normal((ne, 2 * k))
+        xd = torch.tensor(x, device="cuda", dtype=torch.float64).contiguous()
+        yd = torch.empty_like(xd)
+        torch.cuda.synchronize()
+        ctx.matshell_mult_a(k, xd.data_ptr(), yd.data_ptr())
+        y = yd.cpu().numpy()
+        for j in range(k):
+            yr, yi = oracle_mod.apply_operator(rp, ci, C, M, x[:, 2 * j], x[:, 2 * j + 1])
+            assert _rel(y[:, 2 * j], yr) < 1e-13
+            assert _rel(y[:, 2 * j + 1], yi) < 1e-13
+
+
+def test_rhs_matches_oracle(tiny, oracle_mod):
+    m, om = tiny
+    m = mg.tiny_mt(n=3, aniso_form=3, seed=5, freq=0.7)
+    m.tx = np.array([[100.0, -200.0, 500.0, 30.0, 10.0, 2.0, 0.0],
+                     [0.0, 0.0, 800.0, 0.0, 0.0, 1.0, 900.0]])
+    ctx = _ctx(m, "-n_tx_divisions 7")
+    ctx.create_linear_system(5)
+    bg = oracle_mod.background_model(om, m)
+    for slot, mode in ((0, api.XY_POLAR), (1, api.YX_POLAR)):
+        ctx.assemble_rhs_mt(0, mode, slot)
+        re, im = ctx.get_rhs(slot)
+        ore, oim = oracle_mod.rhs_mt(om, m, bg, m.freqs[0], mode)
+        assert _rel(re, ore) < 1e-12 and _rel(im, oim) < 1e-12
+    ctx.assemble_rhs_dual(2)
+    re, im = ctx.get_rhs(2)
+    ore, oim = oracle_mod.rhs_dual(om, m.rx)
+    assert _rel(re, ore) < 1e-12 and _rel(im, oim) < 1e-12
+    for t in range(2):
+        ctx.assemble_rhs_csem(0, t, 3 + t)
+        re, im = ctx.get_rhs(3 + t)
+        ore, oim = oracle_mod.rhs_csem(om, m.freqs[0], m.tx[t], n_div=7)
+        assert np.all(re == 0.0)
+        assert _rel(im, oim) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_mt_solve_and_responses(precision, oracle_mod):
+    m = mg.halfspace_mt(n=4, n_rx=3)
+    om = oracle_mod.Mesh(m.points, m.tets)
+    ctx = _ctx(m, "-inner_precision %s" % precision)
+    ctx.create_linear_system(3)
+    ctx.assemble_matrix(0)
+    ctx.assemble_rhs_mt(0, api.XY_POLAR, 0)
+    ctx.assemble_rhs_mt(0, api.YX_POLAR, 1)
+    ctx.assemble_rhs_dual(2)
+    ctx.create_pc()
+    its, rr = ctx.solve_linear_system(0, 3, 200, 1e-10, rtols=np.array([1e-10, 1e-10, 1e-10]))
+    assert np.all(rr <= 1e-10 * 1.0001), rr
+    rp, ci, C, M = ctx.get_matrix()
+    sols = []
+    for slot in range(3):
+        sr, si = ctx.get_rhs(slot)
+        e_direct = oracle_mod.solve_direct(rp, ci, C, M, sr, si)
+        e_gpu = ctx.get_solution(slot)
+        assert np.linalg.norm(e_gpu - e_direct) <= 1e-6 * np.linalg.norm(e_direct)
+        sols.append(e_direct)
+    rsp = ctx.calculate_response_mt(0, 0, 1, m.obs)
+    ref = oracle_mod.responses_mt(om, m, 0, sols[0], sols[1])
+    for o, v in ref.items():
+        assert abs(rsp[o] - v) <= 1e-6 * max(abs(v), 1e-12), (m.obs[o], rsp[o], v)
+    # apparent resistivity of a half-space approaches its resistivity (coarse mesh: within 40%)
+    zxy = rsp[[o for o in range(len(m.obs)) if m.obs[o][0] == mg.Z_XY_RI]]
+    rho_a = np.abs(zxy) ** 2 / (2 * np.pi * m.freqs[0] * 4e-7 * np.pi)
+    assert np.all(np.abs(rho_a / 100.0 - 1) < 0.4)
+
+
+def test_estimator_matches_oracle(oracle_mod):
+    m = mg.tiny_mt(n=3, aniso_form=2, seed=7, freq=2.0)
+    om = oracle_mod.Mesh(m.points, m.tets)
+    ctx = _ctx(m, "-inner_precision fp64")
+    ctx.create_linear_system(3)
+    ctx.assemble_matrix(0)
+    ctx.assemble_rhs_mt(0, api.XY_POLAR, 0)
+    ctx.assemble_rhs_mt(0, api.YX_POLAR, 1)
+    ctx.assemble_rhs_dual(2)
+    ctx.create_pc()
+    ctx.solve_linear_system(0, 3, 200, 1e-10)
+    sols = [ctx.get_solution(s) for s in range(3)]
+    err = ctx.estimate_error(0, [0, 1, 2])
+    oerr = oracle_mod.estimate_error(om, m, m.freqs[0], sols)
+    for s in range(3):
+        assert _rel(err[s], oerr[s]) < 1e-12
+    # the per-tet indicator the reference refines on (src/em_fwd.cc:1172-1177)
+    ixy = np.sqrt(err[0] * err[2])
+    oxy = np.sqrt(oerr[0] * oerr[2])
+    assert _rel(ixy, oxy) < 1e-12
+
+
+def test_pc_apply_matches_fcg_tolerance(tiny, oracle_mod):
+    """pc_apply_b gives x with ||b - B x|| <= rtol ||b|| per column (src/em_la.cc:208-228)."""
+    torch = _torch()
+    m, om = tiny
+    for prec, tol in (("fp64", 1e-2), ("fp32", 1e-2)):
+        ctx = _ctx(m, "-inner_precision %s -B_ksp_rtol %g" % (prec, tol))
+        ctx.create_linear_system(1)
+        ctx.assemble_matrix(0)
+        ctx.create_pc()
+        rp, ci, C, M = ctx.get_matrix()
+        B = oracle_mod.csr(rp, ci, C + M, rp.size - 1)
+        rng = np.random.default_rng(9)
+        b = rng.standard_normal((rp.size - 1, 4))
+        bd = torch.tensor(b, device="cuda", dtype=torch.float64).contiguous()
+        xd = torch.empty_like(bd)
+        torch.cuda.synchronize()
+        ctx.pc_apply_b(2, bd.data_ptr(), xd.data_ptr())
+        x = xd.cpu().numpy()
+        for c in range(4):
+            r = b[:, c] - B @ x[:, c]
+            assert np.linalg.norm(r) <= tol * np.linalg.norm(b[:, c]) * 1.01
+
+
+def test_not_converged_is_reported(oracle_mod):
+    m = mg.tiny_mt(n=3)
+    ctx = _ctx(m, "-B_ksp_max_it 1")
+    ctx.create_linear_system(1)
+    ctx.assemble_matrix(0)
+    ctx.assemble_rhs_mt(0, api.XY_POLAR, 0)
+    ctx.create_pc()
+    with pytest.raises(api.EmfemError) as e:
+        ctx.solve_linear_system(0, 1, 1, 1e-12)
+    assert e.value.code == api.ERR_NOT_CONVERGED
