@@ -74,6 +74,15 @@ SIGNATURES = {
     "emfem_calculate_response_csem": (I, [P, I, I, I, I, P, P]),
     "emfem_refine_fixed_fraction": (I, [I, P, D, P]),
     "emfem_refine_fixed_number": (I, [I, P, D, P]),
+    "emfem_read_mesh": (I, [P, S]),
+    "emfem_read_emd": (I, [P, S]),
+    "emfem_set_observations": (I, [P, I, P, P]),
+    "emfem_save_rsp": (I, [P, S]),
+    "emfem_get_responses": (I, [P, P]),
+    "emfem_forward": (I, [P]),
+    "emfem_refine_mesh": (I, [P, P]),
+    "emfem_refine_uniform": (I, [P]),
+    "emfem_get_mesh": (I, [P, P, P, P]),
     "emfem_get_stat": (I, [P, S, ctypes.POINTER(D)]),
     "emfem_kernel_time": (I, [P, S, ctypes.POINTER(D), ctypes.POINTER(D)]),
 }
@@ -319,6 +328,46 @@ class Context:
         self._check(self._L.emfem_calculate_response_csem(self._h, int(fidx), int(tidx), int(slot), ob.shape[0],
                                                           _p(ob), _p(rsp)))
         return rsp[:, 0] + 1j * rsp[:, 1]
+
+    # -- inputs, outputs, refinement, forward driver -----------------------------------
+    def read_mesh(self, path):
+        self._check(self._L.emfem_read_mesh(self._h, str(path).encode()))
+
+    def read_emd(self, path):
+        self._check(self._L.emfem_read_emd(self._h, str(path).encode()))
+
+    def set_observations(self, obs, values=None):
+        ob = _c(np.asarray(obs).reshape(-1, 4), np.int32)
+        va = _c(np.asarray(values).reshape(-1, 4), np.float64) if values is not None else None
+        self._check(self._L.emfem_set_observations(self._h, ob.shape[0], _p(ob), _p(va) if va is not None else None))
+        self._nobs = ob.shape[0]
+
+    def save_rsp(self, path):
+        self._check(self._L.emfem_save_rsp(self._h, str(path).encode()))
+
+    def responses(self, n_obs=None):
+        n = self._nobs if n_obs is None else n_obs
+        r = np.empty((n, 2))
+        self._check(self._L.emfem_get_responses(self._h, _p(r)))
+        return r[:, 0] + 1j * r[:, 1]
+
+    def forward(self):
+        self._check(self._L.emfem_forward(self._h))
+
+    def refine_mesh(self, flags):
+        f = _c(flags, np.uint8)
+        self._check(self._L.emfem_refine_mesh(self._h, _p(f)))
+
+    def refine_uniform(self):
+        self._check(self._L.emfem_refine_uniform(self._h))
+
+    def get_mesh(self):
+        inf = self.mesh_info()
+        pts = np.empty((inf["n_vertices"], 3))
+        tets = np.empty((inf["n_tets"], 4), dtype=np.int32)
+        attr = np.empty(inf["n_tets"], dtype=np.int32)
+        self._check(self._L.emfem_get_mesh(self._h, _p(pts), _p(tets), _p(attr)))
+        return pts, tets, attr
 
     # -- measurement ------------------------------------------------------------------
     def stat(self, name):

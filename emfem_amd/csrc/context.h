@@ -1,0 +1,162 @@
+// The EMFEM context shared by the C-ABI translation units.
+//
+// Reference: EMContext (src/em_ctx.h:15-63).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "amg.h"
+#include "common.h"
+#include "model.h"
+#include "solver.h"
+#include "topology.h"
+#include "fe.h"
+#include "kernels.h"
+#include "../../include/emfem_amd.h"
+
+using namespace emfem;
+
+struct emfem_context {
+  hipStream_t stream = nullptr;
+  bool device = false;  // false: host-only context (topology, options, point location)
+  std::string err;
+  std::map<std::string, std::string> opts, given;
+
+  Topology T;
+  bool have_mesh = false;
+  int aniso_form = -1, n_rho = 0;
+  std::vector<double> rho, s6;
+  std::vector<double> freqs, tx, rx;
+  Background bg;
+  // observations (read_emd, src/em_io.cc:237-284): otype fidx tidx ridx, obs, obserr, rsp
+  std::vector<int> obs4;
+  std::vector<std::complex<double>> obs, obserr, rsp;
+  // the mesh as read (coarse_mesh, src/em_ctx.h:29) for per-task refinement
+  std::vector<double> coarse_pts;
+  std::vector<int> coarse_tets, coarse_attr;
+
+  // device mesh
+  DevBuf<int> d_tets, d_t2e, d_attr, d_nb, d_non, d_colour;
+  DevBuf<double> d_pts, d_s6;
+  DevBuf<uint8_t> d_ebdr;
+
+  // linear system
+  bool have_system = false;
+  int nnz = 0, nslots = 0, assembled_fidx = -1;
+  DevBuf<int> d_rp, d_ci;
+  DevBuf<double2> d_K;
+  DevBuf<double> d_S, d_X;  // [E][2 nslots]
+
+  // preconditioner
+  std::unique_ptr<AmsHost> ams_host;
+  std::unique_ptr<InnerPc> pc;
+
+  std::map<std::string, double> stats;
+  KernelClock op_clock;
+  double assemble_kernel_ms = 0.0;
+
+  std::string opt(const std::string &k) const {
+    auto it = opts.find(k);
+    return it == opts.end() ? std::string() : it->second;
+  }
+  double optd(const std::string &k) const { return std::strtod(opt(k).c_str(), nullptr); }
+  int opti(const std::string &k) const { return (int)std::strtol(opt(k).c_str(), nullptr, 10); }
+};
+using namespace emfem;
+
+namespace emfem_ctx {
+
+template <typename F>
+inline int guarded(emfem_context *ctx, F &&f) {
+  if (!ctx) return EMFEM_ERR_ARG;
+  try {
+    f();
+    return EMFEM_OK;
+  } catch (const Error &e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    ctx->err = e.what();
+    return EMFEM_ERR_USER;
+  }
+}
+
+inline void require_mesh(const emfem_context *c) { EMF_REQUIRE(c->have_mesh, kErrState, "no mesh: call emfem_set_mesh first"); }
+inline void require_system(const emfem_context *c) {
+  EMF_REQUIRE(c->have_system, kErrState, "no linear system: call emfem_create_linear_system first");
+}
+inline void require_slot(const emfem_context *c, int slot) {
+  require_system(c);
+  EMF_REQUIRE(slot >= 0 && slot < c->nslots, kErrArg, "slot " + std::to_string(slot) + " out of range");
+}
+inline void require_freq(const emfem_context *c, int fidx) {
+  EMF_REQUIRE(fidx >= 0 && fidx < (int)c->freqs.size(), kErrArg, "frequency index " + std::to_string(fidx) + " out of range");
+}
+
+inline void require_device(const emfem_context *c) {
+  EMF_REQUIRE(c->device, kErrHip, "no HIP device available to this context");
+}
+
+inline void upload_mesh(emfem_context *c) {
+  if (!c->device) return;
+  hipStream_t s = c->stream;
+  const Topology &T = c->T;
+  c->d_tets.upload(T.tets, s);
+  c->d_t2e.upload(T.tet2edge, s);
+  c->d_attr.upload(T.attr, s);
+  c->d_nb.upload(T.neighbor, s);
+  c->d_non.upload(T.non, s);
+  c->d_colour.upload(T.color_order, s);
+  c->d_pts.upload(T.pts, s);
+  c->d_ebdr.upload(T.edge_bdr, s);
+  EMF_HIP(hipStreamSynchronize(s));
+}
+
+inline void check_attributes(emfem_context *c) {
+  if (!c->have_mesh || c->n_rho == 0) return;
+  for (int t = 0; t < c->T.nt; ++t)
+    EMF_REQUIRE(c->T.attr[t] >= 0 && c->T.attr[t] < c->n_rho, kErrUser,
+                "tet " + std::to_string(t) + " has attribute " + std::to_string(c->T.attr[t]) +
+                    " outside the resistivity table");
+}
+
+// Copies the (re, im) columns of `slot` between a [E][2 nslots] block and a host pair.
+inline void put_slot(emfem_context *c, DevBuf<double> &blk, int slot, const double *re, const double *im) {
+  std::vector<double> tmp((size_t)c->T.ne * 2);
+  for (int e = 0; e < c->T.ne; ++e) {
+    tmp[(size_t)e * 2] = re ? re[e] : 0.0;
+    tmp[(size_t)e * 2 + 1] = im ? im[e] : 0.0;
+  }
+  EMF_HIP(hipMemcpy2DAsync(blk.get() + 2 * slot, sizeof(double) * 2 * c->nslots, tmp.data(), sizeof(double) * 2,
+                           sizeof(double) * 2, c->T.ne, hipMemcpyHostToDevice, c->stream));
+  EMF_HIP(hipStreamSynchronize(c->stream));
+}
+
+inline void get_slot(const emfem_context *c, const DevBuf<double> &blk, int slot, double *re, double *im) {
+  std::vector<double> tmp((size_t)c->T.ne * 2);
+  EMF_HIP(hipMemcpy2DAsync(tmp.data(), sizeof(double) * 2, blk.get() + 2 * slot, sizeof(double) * 2 * c->nslots,
+                           sizeof(double) * 2, c->T.ne, hipMemcpyDeviceToHost, c->stream));
+  EMF_HIP(hipStreamSynchronize(c->stream));
+  for (int e = 0; e < c->T.ne; ++e) {
+    if (re) re[e] = tmp[(size_t)e * 2];
+    if (im) im[e] = tmp[(size_t)e * 2 + 1];
+  }
+}
+
+inline PcOptions pc_options(const emfem_context *c) {
+  PcOptions o;
+  o.cheb_degree = c->opti("ams_cheb_degree");
+  o.cheb_ratio = c->optd("ams_cheb_ratio");
+  o.inner_rtol = c->optd("B_ksp_rtol");
+  o.inner_max_it = c->opti("B_ksp_max_it");
+  o.amg.theta = c->optd("ams_theta");
+  return o;
+}
+
+}  // namespace emfem_ctx

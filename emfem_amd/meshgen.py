@@ -168,6 +168,42 @@ def surface_grid(lo, hi, n, z=0.0):
 
 
 # ----------------------------------------------------------------------------------------
+# Input files in the reference's text formats (src/em_io.cc:155-287)
+# ----------------------------------------------------------------------------------------
+
+def write_mesh(m: Model, path, bounds=(1.0e-2, 1.0e5)):
+    """`.mesh`: points, tets with attribute, then the resistivity block
+    (n_rho aniso_form; each row its values and two bounds)."""
+    with open(path, "w") as f:
+        f.write("# points\n%d\n" % m.n_vertices)
+        np.savetxt(f, m.points, fmt="%.17g")
+        f.write("# tets: 4 vertex ids and the attribute\n%d\n" % m.n_tets)
+        np.savetxt(f, np.concatenate([m.tets, m.attr[:, None]], axis=1), fmt="%d")
+        f.write("# resistivity\n%d %d\n" % (m.rho.shape[0], m.aniso_form))
+        for row in m.rho:
+            f.write(" ".join("%.17g" % v for v in row) + " %g %g\n" % bounds)
+
+
+def write_emd(m: Model, path, values=None):
+    """`.emd`: frequencies, transmitters, receivers, observations (otype fidx tidx ridx
+    re im err_re err_im)."""
+    with open(path, "w") as f:
+        f.write("# frequencies\n%d\n" % len(m.freqs))
+        for v in m.freqs:
+            f.write("%.17g\n" % v)
+        f.write("# transmitters\n%d\n" % len(m.tx))
+        for row in m.tx:
+            f.write(" ".join("%.17g" % v for v in row) + "\n")
+        f.write("# receivers\n%d\n" % len(m.rx))
+        for row in m.rx:
+            f.write(" ".join("%.17g" % v for v in row) + "\n")
+        f.write("# observations\n%d\n" % len(m.obs))
+        vals = np.ones((len(m.obs), 4)) if values is None else values
+        for o, v in zip(m.obs, vals):
+            f.write("%d %d %d %d %.17g %.17g %.17g %.17g\n" % (tuple(o) + tuple(v)))
+
+
+# ----------------------------------------------------------------------------------------
 # Workloads
 # ----------------------------------------------------------------------------------------
 

@@ -193,6 +193,48 @@ int emfem_calculate_response_csem(emfem_context *ctx, int fidx, int tidx, int sl
 int emfem_refine_fixed_fraction(int n, const double *error, double fraction, unsigned char *flag);
 int emfem_refine_fixed_number(int n, const double *error, double fraction, unsigned char *flag);
 
+/* ---- inputs, outputs, refinement and the forward driver -------------------- */
+
+/* read_mesh (src/em_io.cc:155-200) with read_rho (src/em_io.cc:9-86): the .mesh
+ * text format (points, tets with attribute, resistivity block); '#' starts a
+ * comment (parse_string, src/em_utils.cc:34-58).  Also keeps the mesh as the
+ * coarse mesh that emfem_forward restarts every task from (src/em_fwd.cc:1330). */
+int emfem_read_mesh(emfem_context *ctx, const char *path);
+
+/* read_emd (src/em_io.cc:202-287): frequencies, transmitters, receivers and the
+ * observation list of an .emd file. */
+int emfem_read_emd(emfem_context *ctx, const char *path);
+
+/* The observation part of read_emd: obs [n][4] = (otype, fidx, tidx, ridx) and
+ * values [n][4] = (re, im, err_re, err_im) as in the file (amplitude/phase codes
+ * are stored as log amplitude + i phase, src/em_io.cc:249-261); values may be
+ * NULL. */
+int emfem_set_observations(emfem_context *ctx, int n, const int *obs, const double *values);
+
+/* save_rsp (src/em_io.cc:316-399): writes the .rsp response file. */
+int emfem_save_rsp(emfem_context *ctx, const char *path);
+
+/* Responses of the last emfem_forward as [n_obs][2] (re, im). */
+int emfem_get_responses(const emfem_context *ctx, double *rsp);
+
+/* em_forward (src/em_fwd.cc:1441-1471) after the inputs are loaded: background
+ * model, task distribution (src/em_fwd.cc:1273-1318), forward_mt / forward_csem
+ * (src/em_fwd.cc:1320-1439) with receiver/transmitter-area refinement, uniform
+ * refinement and goal-oriented adaptive cycles.  CSEM transmitters of one
+ * frequency that share a mesh are solved as one block of right-hand sides. */
+int emfem_forward(emfem_context *ctx);
+
+/* refine_mesh's mesh update (src/em_fwd.cc:1268 -> Mesh::refine_tetgen,
+ * src/em_mesh.cc:101-139): every flagged tet is halved by conforming longest-edge
+ * bisection (TetGen is not used); topology is rebuilt. */
+int emfem_refine_mesh(emfem_context *ctx, const unsigned char *flags);
+
+/* Mesh::refine_uniform (src/em_mesh.cc:141-182): 1 -> 8 split of every tet. */
+int emfem_refine_uniform(emfem_context *ctx);
+
+/* The current mesh: points [n_vertices*3], tets [n_tets*4], attributes [n_tets]. */
+int emfem_get_mesh(const emfem_context *ctx, double *points, int *tets, int *attributes);
+
 /* ---- measurement ---------------------------------------------------------- */
 
 /* Named counters and timers of the last calls (e.g. "assemble_ms", "solve_ms",

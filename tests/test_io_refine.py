@@ -1,0 +1,115 @@
+"""CPU tests of the input/output formats and of mesh refinement (host logic of the
+C-ABI library; no kernels run)."""
+import numpy as np
+import pytest
+
+from emfem_amd import api
+from emfem_amd import meshgen as mg
+
+
+def _volumes(pts, tets):
+    p = pts[tets]
+    return np.einsum("ij,ij->i", p[:, 1] - p[:, 0], np.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 0])) / 6.0
+
+
+def _check_conforming(pts, tets, oracle_mod):
+    """Every face is shared by two tets except faces on the bounding box."""
+    om = oracle_mod.Mesh(pts, tets)
+    lo, hi = pts.min(0), pts.max(0)
+    t, f = np.nonzero(om.neighbor < 0)
+    for tt, ff in zip(t, f):
+        tri = pts[tets[tt, [(ff + 1) % 4, (ff + 2) % 4, (ff + 3) % 4]]]
+        assert any(np.all(tri[:, k] == lo[k]) or np.all(tri[:, k] == hi[k]) for k in range(3)), (tt, ff)
+
+
+def test_read_mesh_and_emd(tmp_path, oracle_mod):
+    m = mg.tiny_mt(n=3, aniso_form=3, seed=4)
+    m.tx = np.array([[0.0, 0.0, 100.0, 0.0, 0.0, 1.0, 0.0]])
+    mg.write_mesh(m, tmp_path / "model.mesh")
+    with open(tmp_path / "model.mesh", "a") as f:
+        f.write("# trailing comment\n")
+    vals = np.tile([[10.0, 45.0, 5.0, 2.0]], (len(m.obs), 1))
+    mg.write_emd(m, tmp_path / "model.emd", vals)
+    ctx = api.Context()
+    ctx.read_mesh(tmp_path / "model.mesh")
+    ctx.read_emd(tmp_path / "model.emd")
+    pts, tets, attr = ctx.get_mesh()
+    assert np.array_equal(pts, m.points) and np.array_equal(tets, m.tets) and np.array_equal(attr, m.attr)
+    el, te, _, _ = ctx.topology()
+    om = oracle_mod.Mesh(m.points, m.tets)
+    assert np.array_equal(el, om.edgelist) and np.array_equal(te, om.tet2edge)
+    ctx.set_observations(m.obs, vals)
+    ctx.save_rsp(tmp_path / "out.rsp")
+    text = open(tmp_path / "out.rsp").read().splitlines()
+    assert text[0] == "# frequencies" and text[1] == "1"
+    assert text[2] == "%.4E" % m.freqs[0]
+    i = text.index("# observations")
+    assert int(text[i + 1]) == len(m.obs)
+    assert text[i + 2].split() == ["#", "otype", "fidx", "tidx", "ridx", "amp", "phase", "amp`", "phase`"]
+    first = text[i + 3].split()
+    assert [int(v) for v in first[:4]] == list(m.obs[0])
+    # Z codes are real/imaginary pairs: the observed value round-trips
+    assert float(first[4]) == 10.0 and float(first[5]) == 45.0
+
+
+def test_read_errors(tmp_path):
+    ctx = api.Context()
+    with pytest.raises(api.EmfemError) as e:
+        ctx.read_mesh(tmp_path / "missing.mesh")
+    assert e.value.code == api.ERR_USER and "Unable to open" in str(e.value)
+    (tmp_path / "bad.mesh").write_text("4\n0 0 0\n1 0 0\n")
+    with pytest.raises(api.EmfemError):
+        ctx.read_mesh(tmp_path / "bad.mesh")
+
+
+def test_amplitude_phase_observations(tmp_path):
+    m = mg.tiny_mt(n=2)
+    m.obs = np.array([[mg.R_XY_AP, 0, -3, 0], [mg.Z_XY_RI, 0, -3, 1]], dtype=np.int32)
+    vals = np.array([[100.0, 45.0, 5.0, 2.0], [0.1, 0.2, 0.01, 0.01]])
+    ctx = api.Context()
+    ctx.load_model(m)
+    ctx.set_observations(m.obs, vals)
+    ctx.save_rsp(tmp_path / "o.rsp")
+    lines = open(tmp_path / "o.rsp").read().splitlines()
+    row = lines[lines.index("# observations") + 3].split()
+    # amplitude/phase rows print exp(log amp) and the phase in degrees (src/em_io.cc:362-366)
+    assert abs(float(row[4]) - 100.0) < 1e-3 and abs(float(row[5]) - 45.0) < 1e-3
+
+
+def test_bisection_refinement(oracle_mod):
+    m = mg.tiny_mt(n=3)
+    ctx = api.Context()
+    ctx.load_model(m)
+    v0 = _volumes(m.points, m.tets)
+    rng = np.random.default_rng(1)
+    flags = (rng.random(m.n_tets) < 0.2).astype(np.uint8)
+    ctx.refine_mesh(flags)
+    pts, tets, attr = ctx.get_mesh()
+    v1 = _volumes(pts, tets)
+    assert np.all(v1 > 0)
+    assert tets.shape[0] > m.n_tets
+    assert abs(v1.sum() - v0.sum()) < 1e-9 * v0.sum()
+    for a in np.unique(m.attr):  # attributes are inherited: volume per attribute is kept
+        assert abs(v1[attr == a].sum() - v0[m.attr == a].sum()) < 1e-9 * v0.sum()
+    _check_conforming(pts, tets, oracle_mod)
+    # every flagged tet lost at least half of its volume: its centroid now lies in a
+    # tet of at most half its volume
+    c = m.points[m.tets[flags == 1]].mean(axis=1)
+    for p, vol in zip(c, v0[flags == 1]):
+        t = ctx.find_cell(p)
+        assert v1[t] <= 0.5 * vol * (1 + 1e-12)
+
+
+def test_uniform_refinement(oracle_mod):
+    m = mg.tiny_mt(n=2)
+    ctx = api.Context()
+    ctx.load_model(m)
+    ctx.refine_uniform()
+    pts, tets, attr = ctx.get_mesh()
+    assert tets.shape[0] == 8 * m.n_tets
+    v0 = _volumes(m.points, m.tets)
+    v1 = _volumes(pts, tets)
+    assert np.all(v1 > 0)
+    assert np.allclose(v1.reshape(-1, 8).sum(axis=1), v0, rtol=1e-12)
+    assert np.array_equal(attr.reshape(-1, 8)[:, 0], m.attr)
+    _check_conforming(pts, tets, oracle_mod)

@@ -1,0 +1,365 @@
+"""CPU checks that pin the oracle (oracle/oracle.c) without the reference.
+
+The reference ships no tests or golden vectors, so each oracle function is pinned
+by mathematics that a plausible slip would break: closed-form Whitney integrals,
+curl(grad) = 0, the Dirichlet row structure, rotation invariants of the
+conductivity, the 1-D plane-wave ODE and interface conditions, closed-form
+estimator values for constant fields, brute-force point location and marking.
+"""
+import numpy as np
+import pytest
+
+from emfem_amd import meshgen as mg
+
+MU = 4e-7 * np.pi
+
+
+def rand_tet(rng):
+    while True:
+        x = rng.uniform(-1000, 1000, (4, 3))
+        if np.linalg.det(np.stack([x[1] - x[0], x[2] - x[0], x[3] - x[0]], axis=1)) > 1e6:
+            return x
+
+
+def rand_spd(rng):
+    A = rng.standard_normal((3, 3))
+    return A @ A.T + 0.1 * np.eye(3)
+
+
+def closed_form_element(x, vid, sigma, omega):
+    """Whitney element matrices from first principles (independent of the oracle)."""
+    J = np.stack([x[1] - x[0], x[2] - x[0], x[3] - x[0]], axis=1)
+    V = np.linalg.det(J) / 6.0
+    Ji = np.linalg.inv(J)
+    g = np.vstack([-Ji.sum(axis=0), Ji])          # rows: grad lambda_0..3
+    m = V / 20.0 * (np.ones((4, 4)) + np.eye(4))   # int lambda_p lambda_q
+    S = g @ sigma @ g.T
+    pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    ends = [(p, q) if vid[p] < vid[q] else (q, p) for p, q in pairs]
+    C = np.zeros((6, 6))
+    M = np.zeros((6, 6))
+    for i, (a, b) in enumerate(ends):
+        ci = 2 * np.cross(g[a], g[b])
+        for j, (c, d) in enumerate(ends):
+            cj = 2 * np.cross(g[c], g[d])
+            C[i, j] = V * ci @ cj / MU
+            M[i, j] = omega * (m[a, c] * S[b, d] - m[a, d] * S[b, c] - m[b, c] * S[a, d] + m[b, d] * S[a, c])
+    return C, M, ends
+
+
+@pytest.fixture(scope="module")
+def single_tet(oracle_mod):
+    rng = np.random.default_rng(0)
+    x = rand_tet(rng)
+    vid = np.array([7, 2, 9, 4])
+    pts = np.zeros((10, 3))
+    pts[vid] = x
+    mesh = oracle_mod.Mesh(pts, vid.reshape(1, 4).astype(np.int32))
+    return mesh, x, vid, rng
+
+
+def test_element_matrices_closed_form(single_tet, oracle_mod):
+    mesh, x, vid, rng = single_tet
+    sigma = rand_spd(rng)
+    omega = 2 * np.pi * 3.7
+    C, M = oracle_mod.element_matrices(mesh, 0, sigma, omega)
+    Cr, Mr, _ = closed_form_element(x, vid, sigma, omega)
+    # the oracle's local edge order is the mesh's tet2edge order; map it to the pairs above
+    pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    order = []
+    for e in mesh.tet2edge[0]:
+        a, b = mesh.edgelist[e]
+        la, lb = list(vid).index(a), list(vid).index(b)
+        order.append(pairs.index(tuple(sorted((la, lb)))))
+    Cr, Mr = Cr[np.ix_(order, order)], Mr[np.ix_(order, order)]
+    assert np.max(np.abs(C - Cr)) <= 1e-13 * np.max(np.abs(Cr))
+    assert np.max(np.abs(M - Mr)) <= 1e-13 * np.max(np.abs(Mr))
+    assert np.allclose(C, C.T, rtol=0, atol=0) and np.allclose(M, M.T, rtol=0, atol=0)
+    assert np.all(np.linalg.eigvalsh(M) > 0)
+
+
+def test_curl_of_gradient_vanishes(single_tet, oracle_mod):
+    mesh, x, vid, rng = single_tet
+    C, _ = oracle_mod.element_matrices(mesh, 0, np.eye(3), 1.0)
+    phi = rng.standard_normal(10)
+    g = np.array([phi[b] - phi[a] for a, b in mesh.edgelist[mesh.tet2edge[0]]])
+    assert np.max(np.abs(C @ g)) <= 1e-12 * np.max(np.abs(C)) * np.max(np.abs(g))
+
+
+def test_whitney_tangential_moment(oracle_mod):
+    """A constant field E0 interpolated by DOF_e = E0 . t_e is reproduced exactly:
+    the interpolated field at any point equals E0 (Whitney completeness)."""
+    m = mg.tiny_mt(n=2)
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    E0 = np.array([1.5, -0.3, 2.0]) * (1 + 0.5j)
+    t = mesh.points[mesh.edgelist[:, 1]] - mesh.points[mesh.edgelist[:, 0]]
+    e = t @ E0
+    rng = np.random.default_rng(4)
+    for tt in rng.choice(mesh.n_tets, 10, replace=False):
+        c = mesh.points[mesh.tets[tt]].mean(axis=0)
+        Ev, Hv = oracle_mod.interpolate_fields(mesh, tt, c, 1.0, e)
+        assert np.allclose(Ev, E0, rtol=1e-12, atol=1e-12 * np.abs(E0).max())
+        assert np.allclose(Hv, 0, atol=1e-12)
+
+
+def test_sigma_forms(oracle_mod):
+    s = oracle_mod.sigma(0, [4.0])
+    assert np.allclose(s, np.eye(3) / 4.0)
+    s = oracle_mod.sigma(1, [4.0, 8.0])
+    assert np.allclose(s, np.diag([0.25, 0.25, 0.125]))
+    s = oracle_mod.sigma(2, [2.0, 4.0, 8.0])
+    assert np.allclose(s, np.diag([0.5, 0.25, 0.125]))
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        rho = rng.uniform(1, 100, 3)
+        ang = rng.uniform(0, 360, 3)
+        s = oracle_mod.sigma(3, list(rho) + list(ang))
+        assert np.allclose(s, s.T, atol=1e-15)
+        assert np.allclose(np.sort(np.linalg.eigvalsh(s)), np.sort(1 / rho), rtol=1e-12)
+    # a 90 degree rotation about z exchanges the x and y principal values
+    s = oracle_mod.sigma(3, [1.0, 2.0, 4.0, 0.0, 0.0, 90.0])
+    assert np.allclose(np.diag(s), [0.5, 1.0, 0.25], atol=1e-15)
+    # about x exchanges y and z
+    s = oracle_mod.sigma(3, [1.0, 2.0, 4.0, 90.0, 0.0, 0.0])
+    assert np.allclose(np.diag(s), [1.0, 0.25, 0.5], atol=1e-15)
+    with pytest.raises(ValueError):
+        oracle_mod.sigma(4, [1.0])
+
+
+def test_kuhn_topology_counts(oracle_mod):
+    m = mg.plain_mt()
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    assert (mesh.n_vertices, mesh.n_tets, mesh.n_edges) == (9702, 50400, 62581)
+    rp, ci = mesh.pattern()
+    assert ci.size == 984661
+    assert np.all(mesh.edgelist[:, 0] < mesh.edgelist[:, 1])
+    # neighbour relation is symmetric and reversible
+    t, f = np.nonzero(mesh.neighbor >= 0)
+    n = mesh.neighbor[t, f]
+    assert np.all(mesh.neighbor[n, mesh.non[t, f]] == t)
+    # boundary edges are exactly the edges lying on the bounding box
+    p = mesh.points
+    lo, hi = p.min(axis=0), p.max(axis=0)
+    a, b = p[mesh.edgelist[:, 0]], p[mesh.edgelist[:, 1]]
+    on = np.zeros(mesh.n_edges, bool)
+    for k in range(3):
+        on |= (a[:, k] == lo[k]) & (b[:, k] == lo[k])
+        on |= (a[:, k] == hi[k]) & (b[:, k] == hi[k])
+    assert np.array_equal(on, mesh.edge_bdr.astype(bool))
+
+
+def test_assembly_invariants(oracle_mod):
+    m = mg.tiny_mt(n=3, aniso_form=3, seed=2, freq=0.3)
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    rp, ci, C, M, bdr = oracle_mod.assemble(mesh, m, m.freqs[0])
+    n = mesh.n_edges
+    Cm = oracle_mod.csr(rp, ci, C, n)
+    Mm = oracle_mod.csr(rp, ci, M, n)
+    assert abs(Cm - Cm.T).max() == 0 and abs(Mm - Mm.T).max() == 0
+    eb = mesh.edge_bdr.astype(bool)
+    # Dirichlet rows: C keeps only its diagonal, M is empty
+    Cb = Cm[eb].tocoo()
+    assert np.all((Cb.data == 0) | (np.nonzero(eb)[0][Cb.row] == Cb.col))
+    assert abs(Mm[eb]).max() == 0
+    # C G phi = 0 for nodal phi vanishing on the boundary (curl grad = 0)
+    rng = np.random.default_rng(3)
+    phi = rng.standard_normal(mesh.n_vertices) * (mesh.vertex_bdr == 0)
+    g = mesh.gradient_matrix() @ phi
+    assert np.max(np.abs(Cm @ g)) <= 1e-12 * abs(Cm).max() * np.max(np.abs(g))
+    # B = C + M is symmetric positive definite
+    B = (Cm + Mm).toarray()
+    assert np.linalg.eigvalsh(B).min() > 0
+    # bdr cells are exactly the tets touching a boundary edge
+    assert np.array_equal(bdr.astype(bool), eb[mesh.tet2edge].any(axis=1))
+
+
+def test_plane_wave_halfspace(oracle_mod):
+    sig, f = 0.01, 2.0
+    omega = 2 * np.pi * f
+    k = np.sqrt(1j * omega * MU * sig)     # e^{+i w t}: fields decay as exp(-k z)
+    for mode in (-1, -2):
+        e0, _ = oracle_mod.pw_eval([0.0], [sig], f, 0.0, mode)
+        comp = 0 if mode == -1 else 1
+        for z in (10.0, 250.0, 1300.0):
+            e, h = oracle_mod.pw_eval([0.0], [sig], f, z, mode)
+            assert abs(e[comp] / e0[comp] - np.exp(-k * z)) < 1e-12
+            # impedance E/H of a half-space is sqrt(i w mu / sigma) up to the field orientation
+            hc = 1 if mode == -1 else 0
+            assert abs(abs(e[comp] / h[hc]) - abs(np.sqrt(1j * omega * MU / sig))) < 1e-9 * abs(np.sqrt(1j * omega * MU / sig))
+
+
+def test_plane_wave_layers_satisfy_ode(oracle_mod):
+    """E is continuous with continuous dE/dz across interfaces, solves E'' = i w mu sigma E
+    inside layers and has no up-going wave in the bottom half-space."""
+    ztop = np.array([-5000.0, 0.0, 800.0, 2000.0])
+    lsig = np.array([1e-6, 0.05, 0.002, 0.2])
+    f = 0.5
+    omega = 2 * np.pi * f
+    ev = lambda z: oracle_mod.pw_eval(ztop, lsig, f, z, -1)[0][0]
+    for zi in ztop[1:]:
+        d = 1e-3
+        assert abs(ev(zi - d) - ev(zi)) < 1e-5 * abs(ev(zi))
+        dl = (ev(zi - d) - ev(zi - 2 * d)) / d
+        dr = (ev(zi + 2 * d) - ev(zi + d)) / d
+        assert abs(dl - dr) < 1e-3 * max(abs(dl), 1e-30)
+    for z, s in ((400.0, 0.05), (1500.0, 0.002), (3000.0, 0.2)):
+        h = 1.0
+        d2 = (ev(z + h) - 2 * ev(z) + ev(z - h)) / h ** 2
+        # e^{+i w t} convention: E'' = i w mu sigma E
+        assert abs(d2 - 1j * omega * MU * s * ev(z)) < 1e-5 * abs(omega * MU * s * ev(z))
+    kb = np.sqrt(1j * omega * MU * lsig[-1])
+    assert abs(ev(3500.0) / ev(3000.0) - np.exp(-kb * 500.0)) < 1e-10
+
+
+def _constant_field_indicator(mesh, m, freq, E0):
+    """Closed form of the estimator for E = E0 everywhere: face jumps of sigma E . n only
+    (curl E = 0), face weights summing to 6 x area (src/em_fe.cc:68)."""
+    from oracle import oracle as O
+    omega = 2 * np.pi * freq
+    nt = mesh.n_tets
+    out = np.zeros(nt)
+    sig = [O.sigma(m.aniso_form, m.rho[a]) for a in m.attr]
+    for t in range(nt):
+        x = mesh.points[mesh.tets[t]]
+        V = np.linalg.det(np.stack([x[1] - x[0], x[2] - x[0], x[3] - x[0]], axis=1)) / 6
+        h = O.lib().or_tet_diameter(O._p(mesh.points), O._p(mesh.tets), int(t))
+        out[t] = h * h * omega ** 2 * np.linalg.norm(sig[t] @ E0) ** 2 * V
+        for f in range(4):
+            n = mesh.neighbor[t, f]
+            if n < 0:
+                continue
+            tri = x[[(f + 1) % 4, (f + 2) % 4, (f + 3) % 4]]
+            nrm = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+            area = 0.5 * np.linalg.norm(nrm)
+            nrm /= np.linalg.norm(nrm)
+            a, b, c = (np.linalg.norm(tri[1] - tri[2]), np.linalg.norm(tri[0] - tri[2]),
+                       np.linalg.norm(tri[0] - tri[1]))
+            R = a * b * c / (4 * area)
+            jump = abs(((sig[t] - sig[n]) @ E0) @ nrm) ** 2
+            out[t] += 6 * area * jump * R / 2
+    return out
+
+
+def test_estimator_constant_field_closed_form(oracle_mod):
+    m = mg.tiny_mt(n=2, aniso_form=2, seed=3, freq=1.3)
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    E0 = np.array([0.7 - 0.2j, 1.1 + 0.4j, -0.5 + 0.9j])
+    t = mesh.points[mesh.edgelist[:, 1]] - mesh.points[mesh.edgelist[:, 0]]
+    e = t @ E0
+    err = oracle_mod.estimate_error(mesh, m, m.freqs[0], [e])[0]
+    ref = _constant_field_indicator(mesh, m, m.freqs[0], E0)
+    assert np.max(np.abs(err - ref)) <= 1e-10 * np.max(ref)
+
+
+def test_estimator_gradient_field_has_no_curl_jump(oracle_mod):
+    """For E = grad(phi) with phi linear the curl vanishes, so in a homogeneous model
+    only the volume term remains."""
+    m = mg.tiny_mt(n=2)
+    m.rho = np.array([[50.0], [50.0], [50.0]])
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    c = np.array([0.3, -1.2, 0.8])
+    phi = mesh.points @ c
+    e = (phi[mesh.edgelist[:, 1]] - phi[mesh.edgelist[:, 0]]).astype(complex)
+    err = oracle_mod.estimate_error(mesh, m, 0.9, [e])[0]
+    ref = _constant_field_indicator(mesh, m, 0.9, c.astype(complex))
+    assert np.max(np.abs(err - ref)) <= 1e-10 * np.max(ref)
+
+
+def test_point_location_rule(oracle_mod):
+    m = mg.tiny_mt(n=3)
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        p = rng.uniform(mesh.points.min(0), mesh.points.max(0))
+        t = mesh.find_cell(p)
+        assert t >= 0 and mesh.contains(t, p)
+    # a point on a vertex: the last tet (highest index) that lists the vertex
+    for v in rng.choice(mesh.n_vertices, 10, replace=False):
+        t = mesh.find_cell(mesh.points[v])
+        assert t == max(np.nonzero((mesh.tets == v).any(axis=1))[0])
+    assert mesh.find_cell(mesh.points.max(0) + 1.0) == -1
+
+
+def test_marking_brute_force(oracle_mod):
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        err = rng.exponential(size=rng.integers(5, 200))
+        frac = rng.uniform(0.05, 0.9)
+        flag = oracle_mod.refine_fixed_fraction(err, frac)
+        # brute force: the smallest prefix of the descending order whose sum reaches
+        # frac * total, then one more value as the (exclusive) threshold
+        s = np.sort(err)[::-1]
+        c = np.concatenate([[0.0], np.cumsum(s)])
+        k = np.argmax(c >= frac * err.sum())
+        thr = s[k] if k < s.size else 0.0
+        assert np.array_equal(flag.astype(bool), err > thr)
+        flag = oracle_mod.refine_fixed_number(err, frac)
+        kk = int(err.size * frac)
+        assert np.array_equal(flag.astype(bool), err > s[kk])
+
+
+def test_mt_response_formulas(oracle_mod):
+    Z = 0.02 + 0.015j
+    f = 1.0
+    exy = np.array([1.0, 0, 0], complex)
+    hxy = np.array([0, 1.0 / Z, 0], complex)
+    eyx = np.array([0, 1.0, 0], complex)
+    hyx = np.array([-1.0 / Z, 0, 0], complex)
+    assert abs(oracle_mod.mt_response(mg.Z_XY_RI, f, exy, hxy, eyx, hyx) - Z) < 1e-15
+    assert abs(oracle_mod.mt_response(mg.Z_YX_RI, f, exy, hxy, eyx, hyx) + Z) < 1e-15
+    assert abs(oracle_mod.mt_response(mg.Z_XX_RI, f, exy, hxy, eyx, hyx)) < 1e-15
+    assert abs(oracle_mod.mt_response(mg.Z_YY_RI, f, exy, hxy, eyx, hyx)) < 1e-15
+    assert abs(oracle_mod.mt_response(mg.T_ZX_RI, f, exy, hxy, eyx, hyx)) < 1e-15
+    r = oracle_mod.mt_response(mg.R_XY_AP, f, exy, hxy, eyx, hyx)
+    assert abs(np.exp(r.real) - abs(Z) ** 2 / (2 * np.pi * f * MU)) < 1e-9
+    assert abs(r.imag - np.angle(Z)) < 1e-15
+    with pytest.raises(ValueError):
+        oracle_mod.mt_response(999, f, exy, hxy, eyx, hyx)
+
+
+def test_csem_dipole_rhs(oracle_mod):
+    """A point dipole d at c gives b_im[i] = w I N_i(c).d: summing b_im * DOF of a
+    constant field E0 (DOF = E0 . t_e) yields w I E0.d (Whitney completeness)."""
+    m = mg.tiny_mt(n=2)
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    tx = np.array([120.0, -340.0, 560.0, 30.0, 20.0, 2.5, 0.0])
+    f = 0.8
+    sr, si = oracle_mod.rhs_csem(mesh, f, tx)
+    assert np.all(sr == 0)
+    d = np.array([np.cos(np.radians(30)) * np.cos(np.radians(20)), np.sin(np.radians(30)) * np.cos(np.radians(20)),
+                  np.sin(np.radians(20))])
+    E0 = np.array([0.4, -1.3, 0.7])
+    t = mesh.points[mesh.edgelist[:, 1]] - mesh.points[mesh.edgelist[:, 0]]
+    assert abs(si @ (t @ E0) - 2 * np.pi * f * 2.5 * E0 @ d) < 1e-10
+    # a finite line source split into 10 dipoles carries the same total moment
+    tx2 = tx.copy()
+    tx2[6] = 400.0
+    sr2, si2 = oracle_mod.rhs_csem(mesh, f, tx2, n_div=10)
+    assert abs(si2 @ (t @ E0) - 2 * np.pi * f * 2.5 * E0 @ d) < 1e-10
+
+
+def test_dual_rhs_sums_components(oracle_mod):
+    m = mg.tiny_mt(n=2)
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    sr, si = oracle_mod.rhs_dual(mesh, m.rx)
+    assert np.array_equal(sr, si)
+    E0 = np.array([0.4, -1.3, 0.7])
+    t = mesh.points[mesh.edgelist[:, 1]] - mesh.points[mesh.edgelist[:, 0]]
+    # sum_j b_j DOF_j(E0) = sum_rx sum_components E0 = n_rx * sum(E0)
+    assert abs(sr @ (t @ E0) - len(m.rx) * E0.sum()) < 1e-10
+
+
+def test_direct_solution_solves_real_equivalent_system(oracle_mod):
+    m = mg.tiny_mt(n=2, aniso_form=1, seed=1)
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    rp, ci, C, M, _ = oracle_mod.assemble(mesh, m, m.freqs[0])
+    bg = oracle_mod.background_model(mesh, m)
+    sr, si = oracle_mod.rhs_mt(mesh, m, bg, m.freqs[0], -1)
+    e = oracle_mod.solve_direct(rp, ci, C, M, sr, si)
+    yr, yi = oracle_mod.apply_operator(rp, ci, C, M, e.real, e.imag)
+    assert np.linalg.norm(np.concatenate([yr - sr, yi - si])) <= 1e-10 * np.linalg.norm(np.concatenate([sr, si]))
+    # boundary DOFs take the Dirichlet values exactly
+    bv, have = oracle_mod.boundary_values(mesh, bg, m.freqs[0], -1)
+    eb = mesh.edge_bdr.astype(bool)
+    assert np.array_equal(have.astype(bool), eb)
+    assert np.allclose(e[eb], bv[eb], rtol=1e-12, atol=1e-14 * np.abs(bv).max())
