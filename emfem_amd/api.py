@@ -83,6 +83,10 @@ SIGNATURES = {
     "emfem_refine_mesh": (I, [P, P]),
     "emfem_refine_uniform": (I, [P]),
     "emfem_get_mesh": (I, [P, P, P, P]),
+    "emfem_device_alloc": (I, [P, ctypes.c_ulonglong, ctypes.POINTER(P)]),
+    "emfem_device_free": (I, [P, P]),
+    "emfem_copy_to_device": (I, [P, P, P, ctypes.c_ulonglong]),
+    "emfem_copy_to_host": (I, [P, P, P, ctypes.c_ulonglong]),
     "emfem_get_stat": (I, [P, S, ctypes.POINTER(D)]),
     "emfem_kernel_time": (I, [P, S, ctypes.POINTER(D), ctypes.POINTER(D)]),
 }
@@ -369,6 +373,11 @@ class Context:
         self._check(self._L.emfem_get_mesh(self._h, _p(pts), _p(tets), _p(attr)))
         return pts, tets, attr
 
+    # -- device buffers -----------------------------------------------------------------
+    def device_array(self, host):
+        """A DeviceArray holding a copy of the float64 host array."""
+        return DeviceArray(self, np.ascontiguousarray(host, dtype=np.float64))
+
     # -- measurement ------------------------------------------------------------------
     def stat(self, name):
         v = D()
@@ -380,6 +389,36 @@ class Context:
         by = D()
         self._check(self._L.emfem_kernel_time(self._h, which.encode(), ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+
+class DeviceArray:
+    """A float64 device buffer owned through the context (emfem_device_alloc)."""
+
+    def __init__(self, ctx, host):
+        self.ctx = ctx
+        self.shape = host.shape
+        self.nbytes = host.nbytes
+        p = P()
+        ctx._check(ctx._L.emfem_device_alloc(ctx._h, ctypes.c_ulonglong(self.nbytes), ctypes.byref(p)))
+        self.ptr = p.value
+        ctx._check(ctx._L.emfem_copy_to_device(ctx._h, P(self.ptr), _p(host), ctypes.c_ulonglong(self.nbytes)))
+
+    def numpy(self):
+        out = np.empty(self.shape)
+        self.ctx._check(self.ctx._L.emfem_copy_to_host(self.ctx._h, _p(out), P(self.ptr),
+                                                       ctypes.c_ulonglong(self.nbytes)))
+        return out
+
+    def free(self):
+        if self.ptr and getattr(self.ctx, "_h", None):
+            self.ctx._L.emfem_device_free(self.ctx._h, P(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def refine_fixed_fraction(error, fraction, flag=None):

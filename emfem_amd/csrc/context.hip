@@ -509,11 +509,16 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
     c->stats["op_spmm_avg_ms"] = opms;
     c->stats["op_spmm_launches_timed"] = cnt;
     c->stats["op_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * 20 + double(ne) * n * 32;
-    double bms = c->pc->b_clock().average_ms(&cnt);
-    c->stats["b_spmm_avg_ms"] = bms;
+    // smoother SpMM in the cycle precision, FCG SpMM in fp64; both on 2n columns
+    double sms = c->pc->smoother_clock().average_ms(&cnt);
+    c->stats["b_spmm_avg_ms"] = sms;
     c->stats["b_spmm_launches_timed"] = cnt;
     const double vb = c->pc->fp32() ? 4.0 : 8.0;
     c->stats["b_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * (4 + vb) + double(ne) * nc * 2 * vb;
+    double fms = c->pc->fcg_clock().average_ms(&cnt);
+    c->stats["fcg_spmm_avg_ms"] = fms;
+    c->stats["fcg_spmm_launches_timed"] = cnt;
+    c->stats["fcg_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * 12 + double(ne) * nc * 16;
     for (int i = 0; i < n; ++i) {
       if (iters) iters[i] = st.its[i];
       if (relres) relres[i] = st.relres[i];
@@ -710,6 +715,39 @@ int emfem_refine_fixed_number(int n, const double *error, double fraction, unsig
   return EMFEM_OK;
 }
 
+int emfem_device_alloc(emfem_context *c, unsigned long long bytes, void **ptr) {
+  return guarded(c, [&]() {
+    require_device(c);
+    EMF_REQUIRE(ptr != nullptr, kErrArg, "null output pointer");
+    *ptr = nullptr;
+    EMF_HIP(hipMalloc(ptr, (size_t)std::max<unsigned long long>(bytes, 1)));
+  });
+}
+
+int emfem_device_free(emfem_context *c, void *ptr) {
+  return guarded(c, [&]() {
+    require_device(c);
+    EMF_HIP(hipStreamSynchronize(c->stream));
+    if (ptr) EMF_HIP(hipFree(ptr));
+  });
+}
+
+int emfem_copy_to_device(emfem_context *c, void *dst, const void *src, unsigned long long bytes) {
+  return guarded(c, [&]() {
+    require_device(c);
+    EMF_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    EMF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int emfem_copy_to_host(emfem_context *c, void *dst, const void *src, unsigned long long bytes) {
+  return guarded(c, [&]() {
+    require_device(c);
+    EMF_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+    EMF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
 int emfem_get_stat(const emfem_context *c, const char *name, double *value) {
   if (!c || !name || !value) return EMFEM_ERR_ARG;
   auto it = c->stats.find(name);
@@ -735,6 +773,9 @@ int emfem_kernel_time(const emfem_context *c, const char *which, double *ms, dou
   } else if (w == "b_spmm") {
     *ms = get("b_spmm_avg_ms");
     *bytes = get("b_spmm_bytes");
+  } else if (w == "fcg_spmm") {
+    *ms = get("fcg_spmm_avg_ms");
+    *bytes = get("fcg_spmm_bytes");
   } else if (w == "estimate") {
     *ms = get("estimate_kernel_ms");
     *bytes = 0.0;

@@ -24,9 +24,17 @@ struct Csr {
 // y = rs .* (alpha * (A x) + gamma * w) + beta * z   (rows of A; rs may be null = 1,
 // w / z may be null when their coefficient is 0).  y may alias z (same row only) but
 // must not alias x.
+// Matrix values TA, input TX, output and accumulation T; instantiated for
+// (f32,f32,f32), (f64,f64,f64), (f64,f32,f64) and (f32,f64,f32).
+template <typename TA, typename TX, typename T>
+void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
+                T gamma, const T *w, int ldw, T beta, const T *z, int ldz);
+
 template <typename T>
-void spmm(hipStream_t s, const Csr<T> &A, int nc, const T *x, int ldx, T *y, int ldy, T alpha, const T *rs,
-          T gamma, const T *w, int ldw, T beta, const T *z, int ldz);
+inline void spmm(hipStream_t s, const Csr<T> &A, int nc, const T *x, int ldx, T *y, int ldy, T alpha, const T *rs,
+                 T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
+  spmm_mixed<T, T, T>(s, A, nc, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz);
+}
 
 // Complex operator of the reference's MatShell A (src/em_la.cc:65-87) on nrhs
 // interleaved pairs: for each pair (xr, xi)

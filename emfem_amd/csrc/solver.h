@@ -57,7 +57,9 @@ class InnerPc {
   // FGMRES entry: b fp64, x written to the precision of the stored Krylov directions.
   virtual void apply_dir(hipStream_t s, int nc, const double *b, void *x, SolveStats &st) = 0;
   virtual bool fp32() const = 0;
-  virtual KernelClock &b_clock() = 0;
+  // B SpMMs of the Chebyshev smoother (precision of the cycle) and of the fp64 FCG
+  virtual KernelClock &smoother_clock() = 0;
+  virtual KernelClock &fcg_clock() = 0;
 };
 
 std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp,

@@ -1,5 +1,12 @@
 // Device solvers: AMG and auxiliary-space cycles, inner flexible CG, outer FGMRES.
 // See solver.h for the reference mapping.
+//
+// Precision.  In air (sigma ~ 1e-8 S/m) the mass term of B = C + M is 1e-7 to 1e-10
+// of the curl-curl term, so B rounded to fp32 cannot represent B on the gradient
+// space: the rounding residue of C grad(phi) exceeds M grad(phi).  The mixed path
+// therefore keeps every quantity that sees that cancellation in fp64 -- the flexible
+// CG recurrence on B and the residuals that feed the auxiliary spaces -- and runs
+// the Chebyshev edge smoothing, the auxiliary transfers and the AMG V-cycles in fp32.
 #include "solver.h"
 
 #include <cmath>
@@ -100,7 +107,6 @@ class DevAmg {
     cinv_.upload(H.coarse_inv, s);
     EMF_HIP(hipStreamSynchronize(s));
   }
-  int fine_rows() const { return levels_.empty() ? ncoarse_ : levels_[0].A.n; }
 
   void ensure(int nc) {
     if (nc <= nc_) return;
@@ -166,13 +172,29 @@ class DevAms {
   void setup(const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K, const PcOptions &o,
              hipStream_t s) {
     ne_ = ne;
-    B_.nrows = B_.ncols = ne;
-    B_.nnz = nnz;
-    B_.rp = rp;
-    B_.ci = ci;
+    auto view = [&](const void *v) {
+      Csr<T> c;
+      c.nrows = c.ncols = ne;
+      c.nnz = nnz;
+      c.rp = rp;
+      c.ci = ci;
+      c.v = static_cast<const T *>(v);
+      return c;
+    };
     bvals_.resize(nnz);
     make_b<T>(s, nnz, K, bvals_.get());
-    B_.v = bvals_.get();
+    B_ = view(bvals_.get());
+    B64_.nrows = B64_.ncols = ne;
+    B64_.nnz = nnz;
+    B64_.rp = rp;
+    B64_.ci = ci;
+    if constexpr (std::is_same<T, double>::value) {
+      B64_.v = bvals_.get();
+    } else {
+      b64vals_.resize(nnz);
+      make_b<double>(s, nnz, K, b64vals_.get());
+      B64_.v = b64vals_.get();
+    }
     upload_vec(l1inv_, H.l1inv, s);
     G0_.upload(H.G0, s);
     G0T_.upload(H.G0T, s);
@@ -188,39 +210,49 @@ class DevAms {
     const double lmin = lmax / o.cheb_ratio;
     theta_ = 0.5 * (lmax + lmin);
     delta_ = 0.5 * (lmax - lmin);
+    EMF_HIP(hipStreamSynchronize(s));
   }
 
-  const Csr<T> &B() const { return B_; }
+  const Csr<double> &B64() const { return B64_; }
 
   void ensure(int nc) {
     if (nc <= nc_) return;
     nc_ = nc;
-    r_.resize((size_t)ne_ * nc);
+    r64_.resize((size_t)ne_ * nc);
     rres_.resize((size_t)ne_ * nc);
     d_.resize((size_t)ne_ * nc);
+    if (!std::is_same<T, double>::value) bt_.resize((size_t)ne_ * nc);
     vb_.resize((size_t)nv0_ * nc);
     vx_.resize((size_t)nv0_ * nc);
     amgG_.ensure(nc);
     for (int d = 0; d < 3; ++d) amgPi_[d].ensure(nc);
   }
 
-  // out = M_AMS b (compact [E][nc]); out must not alias b.
-  void apply(hipStream_t s, int nc, const T *b, T *out, KernelClock *clk) {
+  // out = M_AMS b: b fp64 compact [E][nc], out in T; out must not alias b.
+  void apply(hipStream_t s, int nc, const double *b, T *out, KernelClock *clk) {
     ensure(nc);
-    T *r = r_.get(), *rres = rres_.get(), *d = d_.get();
+    T *rres = rres_.get(), *d = d_.get();
+    const T *bt;
+    if constexpr (std::is_same<T, double>::value) {
+      bt = b;
+    } else {
+      axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, bt_.get(), nc);
+      bt = bt_.get();
+    }
     // pre-smoothing (Chebyshev from zero)
-    cheb_init<T>(s, ne_, nc, l1inv_.get(), b, rres, d, out, 1.0 / theta_);
+    cheb_init<T>(s, ne_, nc, l1inv_.get(), bt, rres, d, out, 1.0 / theta_);
     cheb_iterate(s, nc, rres, d, out, clk);
-    aux(s, nc, b, out, G0_, G0T_, amgG_, clk);
-    residual(s, nc, b, out, r, clk);
+    aux_gradient(s, nc, b, out);
+    residual64(s, nc, b, out);
     for (int k = 0; k < 3; ++k) {
-      spmm<T>(s, PiT_[k].view(), nc, r, nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+      spmm_mixed<T, double, T>(s, PiT_[k].view(), nc, r64_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0,
+                               T(0), nullptr, 0);
       amgPi_[k].vcycle(s, nc, vb_.get(), vx_.get());
       spmm<T>(s, Pi_[k].view(), nc, vx_.get(), nc, out, nc, T(1), nullptr, T(0), nullptr, 0, T(1), out, nc);
     }
-    aux(s, nc, b, out, G0_, G0T_, amgG_, clk);
+    aux_gradient(s, nc, b, out);
     // post-smoothing (Chebyshev from the current iterate)
-    timed_spmm(s, nc, out, rres, T(-1), l1inv_.get(), T(1), b, T(0), nullptr, clk);
+    timed_spmm(s, nc, out, rres, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr, clk);
     axpby<T, T>(s, ne_, nc, 1.0 / theta_, rres, nc, 0.0, d, nc);
     axpby<T, T>(s, ne_, nc, 1.0, d, nc, 1.0, out, nc);
     cheb_iterate(s, nc, rres, d, out, clk);
@@ -235,8 +267,9 @@ class DevAms {
     if (t) clk->stop(s);
   }
 
-  void residual(hipStream_t s, int nc, const T *b, const T *x, T *r, KernelClock *clk) {
-    timed_spmm(s, nc, x, r, T(-1), nullptr, T(1), b, T(0), nullptr, clk);
+  // r64 = b - B x in fp64 (the residual the auxiliary spaces see)
+  void residual64(hipStream_t s, int nc, const double *b, const T *x) {
+    spmm_mixed<double, T, double>(s, B64_, nc, x, nc, r64_.get(), nc, -1.0, nullptr, 1.0, b, nc, 0.0, nullptr, 0);
   }
 
   void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x, KernelClock *clk) {
@@ -250,24 +283,27 @@ class DevAms {
     }
   }
 
-  void aux(hipStream_t s, int nc, const T *b, T *x, const DevCsr<T> &P, const DevCsr<T> &PT, DevAmg<T> &amg,
-           KernelClock *clk) {
-    residual(s, nc, b, x, r_.get(), clk);
-    spmm<T>(s, PT.view(), nc, r_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
-    amg.vcycle(s, nc, vb_.get(), vx_.get());
-    spmm<T>(s, P.view(), nc, vx_.get(), nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), x, nc);
+  void aux_gradient(hipStream_t s, int nc, const double *b, T *x) {
+    residual64(s, nc, b, x);
+    spmm_mixed<T, double, T>(s, G0T_.view(), nc, r64_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0, T(0),
+                             nullptr, 0);
+    amgG_.vcycle(s, nc, vb_.get(), vx_.get());
+    spmm<T>(s, G0_.view(), nc, vx_.get(), nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), x, nc);
   }
 
   int ne_ = 0, nv0_ = 0, nc_ = 0, deg_ = 3;
   double theta_ = 0.5, delta_ = 0.5;
   Csr<T> B_;
-  DevBuf<T> bvals_, l1inv_, r_, rres_, d_, vb_, vx_;
+  Csr<double> B64_;
+  DevBuf<T> bvals_, l1inv_, rres_, d_, bt_, vb_, vx_;
+  DevBuf<double> b64vals_, r64_;
   DevCsr<T> G0_, G0T_, Pi_[3], PiT_[3];
   DevAmg<T> amgG_, amgPi_[3];
 };
 
 // ---------------------------------------------------------------------------------
-// inner flexible CG (truncation 1) on B, per column (src/em_la.cc:134-142)
+// inner flexible CG (truncation 1) on B in fp64, per column (src/em_la.cc:134-142),
+// preconditioned by the auxiliary-space cycle in precision T
 // ---------------------------------------------------------------------------------
 
 template <typename T>
@@ -275,33 +311,26 @@ class InnerPcImpl : public InnerPc {
  public:
   InnerPcImpl(const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K, const PcOptions &o,
               hipStream_t s)
-      : ne_(ne), o_(o) {
+      : ne_(ne), nnz_(nnz), o_(o) {
     ams_.setup(H, ne, nnz, rp, ci, K, o, s);
-    const double bytes_b = double(ne + 1) * 4 + double(nnz) * (4 + sizeof(T));
-    clock_.bytes_per_launch = bytes_b;  // completed with the column traffic per call
   }
 
   bool fp32() const override { return std::is_same<T, float>::value; }
-  KernelClock &b_clock() override { return clock_; }
+  KernelClock &smoother_clock() override { return smooth_clock_; }
+  KernelClock &fcg_clock() override { return fcg_clock_; }
 
   void apply64(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) override {
     ensure(nc);
-    if constexpr (std::is_same<T, double>::value) {
-      solve(s, nc, b, x, st);
-    } else {
-      axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, bt_.get(), nc);
-      solve(s, nc, bt_.get(), xt_.get(), st);
-      axpby<T, double>(s, ne_, nc, 1.0, xt_.get(), nc, 0.0, x, nc);
-    }
+    solve(s, nc, b, x, st);
   }
 
   void apply_dir(hipStream_t s, int nc, const double *b, void *x, SolveStats &st) override {
     ensure(nc);
     if constexpr (std::is_same<T, double>::value) {
-      solve(s, nc, b, static_cast<T *>(x), st);
+      solve(s, nc, b, static_cast<double *>(x), st);
     } else {
-      axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, bt_.get(), nc);
-      solve(s, nc, bt_.get(), static_cast<T *>(x), st);
+      solve(s, nc, b, xd_.get(), st);
+      axpby<double, T>(s, ne_, nc, 1.0, xd_.get(), nc, 0.0, static_cast<T *>(x), nc);
     }
   }
 
@@ -315,8 +344,8 @@ class InnerPcImpl : public InnerPc {
     p_.resize(n);
     q_.resize(n);
     if (!std::is_same<T, double>::value) {
-      bt_.resize(n);
-      xt_.resize(n);
+      xd_.resize(n);
+      zt_.resize(n);
     }
     for (auto *v : {&rz_, &pq_, &zq_, &alpha_, &beta_, &rr_, &ones_}) v->resize(nc);
     active_.resize(nc);
@@ -330,14 +359,24 @@ class InnerPcImpl : public InnerPc {
     h_active_.resize(nc);
   }
 
-  void solve(hipStream_t s, int nc, const T *b, T *x, SolveStats &st) {
-    T *r = r_.get(), *z = z_.get(), *p = p_.get(), *q = q_.get();
+  void precondition(hipStream_t s, int nc, const double *r, double *z, SolveStats &st) {
+    ++st.vcycles;
+    if constexpr (std::is_same<T, double>::value) {
+      ams_.apply(s, nc, r, z, &smooth_clock_);
+    } else {
+      ams_.apply(s, nc, r, zt_.get(), &smooth_clock_);
+      axpby<T, double>(s, ne_, nc, 1.0, zt_.get(), nc, 0.0, z, nc);
+    }
+  }
+
+  void solve(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) {
+    double *r = r_.get(), *z = z_.get(), *p = p_.get(), *q = q_.get();
     double *w = work_.get();
     ++st.pc_applies;
     // x = 0, r = b
-    EMF_HIP(hipMemsetAsync(x, 0, sizeof(T) * (size_t)ne_ * nc, s));
-    EMF_HIP(hipMemcpyAsync(r, b, sizeof(T) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
-    coldot<T>(s, ne_, nc, b, nc, b, nc, rr_.get(), w);
+    EMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * (size_t)ne_ * nc, s));
+    EMF_HIP(hipMemcpyAsync(r, b, sizeof(double) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
+    coldot<double>(s, ne_, nc, b, nc, b, nc, rr_.get(), w);
     EMF_HIP(hipMemcpyAsync(h_bb_.data(), rr_.get(), sizeof(double) * nc, hipMemcpyDeviceToHost, s));
     EMF_HIP(hipStreamSynchronize(s));
     bool any = false;
@@ -349,15 +388,14 @@ class InnerPcImpl : public InnerPc {
     active_.upload(h_active_.data(), nc, s);
     const double tol2 = o_.inner_rtol * o_.inner_rtol;
     // z = M r ; p = z ; q = B p
-    ams_.apply(s, nc, r, z, &clock_);
-    ++st.vcycles;
-    EMF_HIP(hipMemcpyAsync(p, z, sizeof(T) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
+    precondition(s, nc, r, z, st);
+    EMF_HIP(hipMemcpyAsync(p, z, sizeof(double) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
     bspmm(s, nc, p, q);
-    coldot<T>(s, ne_, nc, p, nc, q, nc, pq_.get(), w);
-    coldot<T>(s, ne_, nc, r, nc, z, nc, rz_.get(), w);
+    coldot<double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w);
+    coldot<double>(s, ne_, nc, r, nc, z, nc, rz_.get(), w);
     for (int it = 1; it <= o_.inner_max_it; ++it) {
       fcg_alpha(s, nc, rz_.get(), pq_.get(), active_.get(), alpha_.get());
-      fcg_xr<T>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w);
+      fcg_xr<double>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w);
       ++st.inner_its;
       EMF_HIP(hipMemcpyAsync(h_rr_.data(), rr_.get(), sizeof(double) * nc, hipMemcpyDeviceToHost, s));
       EMF_HIP(hipStreamSynchronize(s));
@@ -368,33 +406,33 @@ class InnerPcImpl : public InnerPc {
       }
       if (!any || it == o_.inner_max_it) break;
       active_.upload(h_active_.data(), nc, s);
-      ams_.apply(s, nc, r, z, &clock_);
-      ++st.vcycles;
-      coldot<T>(s, ne_, nc, z, nc, q, nc, zq_.get(), w);
-      coldot<T>(s, ne_, nc, r, nc, z, nc, rz_.get(), w);
+      precondition(s, nc, r, z, st);
+      coldot<double>(s, ne_, nc, z, nc, q, nc, zq_.get(), w);
+      coldot<double>(s, ne_, nc, r, nc, z, nc, rz_.get(), w);
       fcg_beta(s, nc, zq_.get(), pq_.get(), active_.get(), beta_.get());
-      axpby_cols<T, T>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc);
+      axpby_cols<double, double>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc);
       bspmm(s, nc, p, q);
-      coldot<T>(s, ne_, nc, p, nc, q, nc, pq_.get(), w);
+      coldot<double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w);
     }
   }
 
-  void bspmm(hipStream_t s, int nc, const T *x, T *y) {
-    const bool t = clock_.armed();
-    if (t) clock_.start(s);
-    spmm<T>(s, ams_.B(), nc, x, nc, y, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
-    if (t) clock_.stop(s);
+  void bspmm(hipStream_t s, int nc, const double *x, double *y) {
+    const bool t = fcg_clock_.armed();
+    if (t) fcg_clock_.start(s);
+    spmm<double>(s, ams_.B64(), nc, x, nc, y, nc, 1.0, nullptr, 0.0, nullptr, 0, 0.0, nullptr, 0);
+    if (t) fcg_clock_.stop(s);
   }
 
-  int ne_ = 0, nc_ = 0;
+  int ne_ = 0, nnz_ = 0, nc_ = 0;
   PcOptions o_;
   DevAms<T> ams_;
-  DevBuf<T> r_, z_, p_, q_, bt_, xt_;
+  DevBuf<double> r_, z_, p_, q_, xd_;
+  DevBuf<T> zt_;
   DevBuf<double> rz_, pq_, zq_, alpha_, beta_, rr_, ones_, work_;
   DevBuf<uint8_t> active_;
   std::vector<double> h_bb_, h_rr_;
   std::vector<uint8_t> h_active_;
-  KernelClock clock_;
+  KernelClock smooth_clock_, fcg_clock_;
 };
 
 }  // namespace
@@ -435,10 +473,11 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
     out.resize(nrhs);
     for (int r = 0; r < nrhs; ++r) out[r] = std::sqrt(h_dots[2 * r] + h_dots[2 * r + 1]);
   };
-  auto scale_cols = [&](double *v, const std::vector<double> &fac) {
+  // dst = fac .* src per right-hand side (in place allowed)
+  auto scale_cols = [&](const double *src, double *dst, const std::vector<double> &fac) {
     for (int r = 0; r < nrhs; ++r) h_scal[2 * r] = h_scal[2 * r + 1] = fac[r];
     scal.upload(h_scal.data(), nc, s);
-    axpby_cols<double, double>(s, ne, nc, scal.get(), v, nc, nullptr, v, nc);
+    axpby_cols<double, double>(s, ne, nc, scal.get(), src, nc, nullptr, dst, nc);
   };
 
   std::vector<double> bnorm;
@@ -468,15 +507,14 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
     if (!any) break;
     std::vector<double> inv(nrhs);
     for (int r = 0; r < nrhs; ++r) inv[r] = active[r] && beta[r] > 0 ? 1.0 / beta[r] : 0.0;
-    scale_cols(V0, inv);
+    scale_cols(V0, V0, inv);
     std::fill(H.begin(), H.end(), 0.0);
     std::fill(g.begin(), g.end(), 0.0);
     for (int r = 0; r < nrhs; ++r) {
       g[(size_t)r * (m + 1)] = active[r] ? beta[r] : 0.0;
       kk[r] = 0;
     }
-    int j = 0;
-    for (; j < m && total < o.max_it; ++j) {
+    for (int j = 0; j < m && total < o.max_it; ++j) {
       ++total;
       double *Vj = V.get() + (size_t)j * blk;
       TZ *Zj = Z.get() + (size_t)j * blk;
@@ -531,13 +569,8 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
           still = true;
         }
       }
-      scale_cols(w.get(), fac);
-      EMF_HIP(hipMemcpyAsync(V.get() + (size_t)(j + 1) * blk, w.get(), sizeof(double) * blk,
-                             hipMemcpyDeviceToDevice, s));
-      if (!still) {
-        ++j;
-        break;
-      }
+      scale_cols(w.get(), V.get() + (size_t)(j + 1) * blk, fac);
+      if (!still) break;
     }
     // y = R^-1 g per right-hand side; x += sum_i y_i Z_i
     std::fill(h_coef.begin(), h_coef.end(), 0.0);

@@ -30,10 +30,11 @@ __device__ __forceinline__ T shfl_xor(T v, int mask, int width) {
 // real CSR x block
 // ------------------------------------------------------------------------------
 
-template <typename T, int NC, int LPR>
+// Matrix values TA, input block TX, output/accumulation TY.
+template <typename TA, typename TX, typename T, int NC, int LPR>
 __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__restrict__ rp,
-                                                        const int *__restrict__ ci, const T *__restrict__ av,
-                                                        const T *__restrict__ x, int ldx, T *y, int ldy, T alpha,
+                                                        const int *__restrict__ ci, const TA *__restrict__ av,
+                                                        const TX *__restrict__ x, int ldx, T *y, int ldy, T alpha,
                                                         const T *__restrict__ rs, T gamma, const T *w, int ldw,
                                                         T beta, const T *z, int ldz) {
   const int gid = blockIdx.x * kBlock + threadIdx.x;
@@ -45,10 +46,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
   for (int j = 0; j < NC; ++j) acc[j] = T(0);
   const int e = rp[row + 1];
   for (int p = rp[row] + lane; p < e; p += LPR) {
-    const T a = av[p];
-    const T *xr = x + (size_t)ci[p] * ldx;
+    const T a = (T)av[p];
+    const TX *xr = x + (size_t)ci[p] * ldx;
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] += a * xr[j];
+    for (int j = 0; j < NC; ++j) acc[j] += a * (T)xr[j];
   }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) {
@@ -68,10 +69,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
   }
 }
 
-template <typename T, int CPL>
+template <typename TA, typename TX, typename T, int CPL>
 __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__restrict__ rp,
-                                                      const int *__restrict__ ci, const T *__restrict__ av,
-                                                      const T *__restrict__ x, int ldx, T *y, int ldy, int nc,
+                                                      const int *__restrict__ ci, const TA *__restrict__ av,
+                                                      const TX *__restrict__ x, int ldx, T *y, int ldy, int nc,
                                                       T alpha, const T *__restrict__ rs, T gamma, const T *w,
                                                       int ldw, T beta, const T *z, int ldz) {
   int row = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
@@ -83,12 +84,12 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
   for (int j = 0; j < CPL; ++j) acc[j] = T(0);
   const int b = rp[row], e = rp[row + 1];
   for (int p = b; p < e; ++p) {
-    const T a = av[p];
-    const T *xr = x + (size_t)ci[p] * ldx;
+    const T a = (T)av[p];
+    const TX *xr = x + (size_t)ci[p] * ldx;
 #pragma unroll
     for (int j = 0; j < CPL; ++j) {
       const int c = lane + 64 * j;
-      if (c < nc) acc[j] += a * xr[c];
+      if (c < nc) acc[j] += a * (T)xr[c];
     }
   }
   const T scale = rs ? rs[row] : T(1);
@@ -105,16 +106,16 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
   }
 }
 
-template <typename T, int NC>
-void launch_narrow(hipStream_t s, const Csr<T> &A, const T *x, int ldx, T *y, int ldy, T alpha, const T *rs, T gamma,
-                   const T *w, int ldw, T beta, const T *z, int ldz) {
+template <typename TA, typename TX, typename T, int NC>
+void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
+                   T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
   const double avg = A.nrows ? double(A.nnz) / A.nrows : 0.0;
   const int lpr = avg <= 6 ? 4 : (avg <= 20 ? 8 : (avg <= 48 ? 16 : 32));
   const int64_t threads = (int64_t)A.nrows * lpr;
   const int g = grid_for(threads);
-#define EMF_NARROW(L)                                                                                      \
-  hipLaunchKernelGGL((k_spmm_narrow<T, NC, L>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y, \
-                     ldy, alpha, rs, gamma, w, ldw, beta, z, ldz)
+#define EMF_NARROW(L)                                                                                              \
+  hipLaunchKernelGGL((k_spmm_narrow<TA, TX, T, NC, L>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, \
+                     y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz)
   switch (lpr) {
     case 4: EMF_NARROW(4); break;
     case 8: EMF_NARROW(8); break;
@@ -126,25 +127,25 @@ void launch_narrow(hipStream_t s, const Csr<T> &A, const T *x, int ldx, T *y, in
 
 }  // namespace
 
-template <typename T>
-void spmm(hipStream_t s, const Csr<T> &A, int nc, const T *x, int ldx, T *y, int ldy, T alpha, const T *rs, T gamma,
-          const T *w, int ldw, T beta, const T *z, int ldz) {
+template <typename TA, typename TX, typename T>
+void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
+                T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
   if (A.nrows == 0 || nc == 0) return;
   if (nc >= 16) {
     const int g = (A.nrows + kBlock / 64 - 1) / (kBlock / 64);
     if (nc <= 64)
-      hipLaunchKernelGGL((k_spmm_wide<T, 1>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y, ldy,
-                         nc, alpha, rs, gamma, w, ldw, beta, z, ldz);
+      hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, 1>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y,
+                         ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz);
     else if (nc <= 128)
-      hipLaunchKernelGGL((k_spmm_wide<T, 2>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y, ldy,
-                         nc, alpha, rs, gamma, w, ldw, beta, z, ldz);
+      hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, 2>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y,
+                         ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz);
     else {
-      // process 256-column slabs
+      // 256-column slabs
       for (int c0 = 0; c0 < nc; c0 += 256) {
         const int n = std::min(256, nc - c0);
-        hipLaunchKernelGGL((k_spmm_wide<T, 4>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x + c0, ldx,
-                           y + c0, ldy, n, alpha, rs, gamma, w ? w + c0 : nullptr, ldw, beta, z ? z + c0 : nullptr,
-                           ldz);
+        hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, 4>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x + c0,
+                           ldx, y + c0, ldy, n, alpha, rs, gamma, w ? w + c0 : nullptr, ldw, beta,
+                           z ? z + c0 : nullptr, ldz);
       }
     }
     EMF_HIP(hipGetLastError());
@@ -156,23 +157,34 @@ void spmm(hipStream_t s, const Csr<T> &A, int nc, const T *x, int ldx, T *y, int
     const int n = left >= 8 ? 8 : (left >= 6 ? 6 : (left >= 4 ? 4 : (left >= 3 ? 3 : (left >= 2 ? 2 : 1))));
     const T *wc = w ? w + c0 : nullptr;
     const T *zc = z ? z + c0 : nullptr;
+#define EMF_CASE(K)                                                                                                 \
+  case K:                                                                                                           \
+    launch_narrow<TA, TX, T, K>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz);        \
+    break;
     switch (n) {
-      case 8: launch_narrow<T, 8>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
-      case 6: launch_narrow<T, 6>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
-      case 4: launch_narrow<T, 4>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
-      case 3: launch_narrow<T, 3>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
-      case 2: launch_narrow<T, 2>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
-      default: launch_narrow<T, 1>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz); break;
+      EMF_CASE(8)
+      EMF_CASE(6)
+      EMF_CASE(4)
+      EMF_CASE(3)
+      EMF_CASE(2)
+      default:
+        launch_narrow<TA, TX, T, 1>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz);
+        break;
     }
+#undef EMF_CASE
     c0 += n;
   }
   EMF_HIP(hipGetLastError());
 }
 
-template void spmm<float>(hipStream_t, const Csr<float> &, int, const float *, int, float *, int, float, const float *,
-                          float, const float *, int, float, const float *, int);
-template void spmm<double>(hipStream_t, const Csr<double> &, int, const double *, int, double *, int, double,
-                           const double *, double, const double *, int, double, const double *, int);
+#define EMF_SPMM_INST(TA, TX, T)                                                                                    \
+  template void spmm_mixed<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, int, T *, int, T, const T *, T, \
+                                      const T *, int, T, const T *, int);
+EMF_SPMM_INST(float, float, float)
+EMF_SPMM_INST(double, double, double)
+EMF_SPMM_INST(double, float, double)
+EMF_SPMM_INST(float, double, float)
+#undef EMF_SPMM_INST
 
 // ------------------------------------------------------------------------------
 // complex operator (C, M) on interleaved pairs

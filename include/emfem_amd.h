@@ -235,6 +235,16 @@ int emfem_refine_uniform(emfem_context *ctx);
 /* The current mesh: points [n_vertices*3], tets [n_tets*4], attributes [n_tets]. */
 int emfem_get_mesh(const emfem_context *ctx, double *points, int *tets, int *attributes);
 
+/* ---- device buffers -------------------------------------------------------- */
+
+/* Device memory on the context's GPU for callers that have no GPU framework of
+ * their own (the device-pointer entry points above take such buffers).  No
+ * reference counterpart: PETSc would hand its own device vectors. */
+int emfem_device_alloc(emfem_context *ctx, unsigned long long bytes, void **ptr);
+int emfem_device_free(emfem_context *ctx, void *ptr);
+int emfem_copy_to_device(emfem_context *ctx, void *dst, const void *src, unsigned long long bytes);
+int emfem_copy_to_host(emfem_context *ctx, void *dst, const void *src, unsigned long long bytes);
+
 /* ---- measurement ---------------------------------------------------------- */
 
 /* Named counters and timers of the last calls (e.g. "assemble_ms", "solve_ms",

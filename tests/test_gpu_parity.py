@@ -14,12 +14,6 @@ from emfem_amd import meshgen as mg
 pytestmark = pytest.mark.gpu
 
 
-def _torch():
-    import torch
-    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
-    return torch
-
-
 def _rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     scale = np.max(np.abs(b)) if b.size else 1.0
@@ -69,7 +63,6 @@ def test_assembly_all_aniso_forms(form, oracle_mod):
 
 
 def test_operator_matches_oracle(tiny, oracle_mod):
-    torch = _torch()
     m, om = tiny
     ctx = _ctx(m)
     ctx.create_linear_system(1)
@@ -77,13 +70,12 @@ def test_operator_matches_oracle(tiny, oracle_mod):
     rp, ci, C, M = ctx.get_matrix()
     ne = rp.size - 1
     rng = np.random.default_rng(3)
-    for k in (1, 3, 8, 20):
+    for k in (1, 3, 8, 20, 70):
         x = rng.standard_normal((ne, 2 * k))
-        xd = torch.tensor(x, device="cuda", dtype=torch.float64).contiguous()
-        yd = torch.empty_like(xd)
-        torch.cuda.synchronize()
-        ctx.matshell_mult_a(k, xd.data_ptr(), yd.data_ptr())
-        y = yd.cpu().numpy()
+        xd = ctx.device_array(x)
+        yd = ctx.device_array(np.zeros_like(x))
+        ctx.matshell_mult_a(k, xd.ptr, yd.ptr)
+        y = yd.numpy()
         for j in range(k):
             yr, yi = oracle_mod.apply_operator(rp, ci, C, M, x[:, 2 * j], x[:, 2 * j + 1])
             assert _rel(y[:, 2 * j], yr) < 1e-13
@@ -115,8 +107,21 @@ def test_rhs_matches_oracle(tiny, oracle_mod):
         assert _rel(im, oim) < 1e-12
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_mt_solve_and_responses(precision, oracle_mod):
+def check_mt_responses(obs, rsp, ref, tol=1e-6):
+    """Per receiver: the impedance tensor within tol of its largest entry (Z_xx and
+    Z_yy are differences of near-equal products), the tipper within tol absolute
+    (dimensionless)."""
+    for r in np.unique(obs[:, 3]):
+        rows = [o for o in ref if obs[o][3] == r]
+        zrows = [o for o in rows if obs[o][0] in (311, 321, 331, 341)]
+        scale = max(abs(ref[o]) for o in zrows) if zrows else 1.0
+        for o in rows:
+            s = scale if o in zrows else 1.0
+            assert abs(rsp[o] - ref[o]) <= tol * s, (obs[o], rsp[o], ref[o])
+
+
+@pytest.mark.parametrize("precision,rtol", [("fp64", 1e-10), ("fp32", 1e-8)])
+def test_mt_solve_and_responses(precision, rtol, oracle_mod):
     m = mg.halfspace_mt(n=4, n_rx=3)
     om = oracle_mod.Mesh(m.points, m.tets)
     ctx = _ctx(m, "-inner_precision %s" % precision)
@@ -126,20 +131,20 @@ def test_mt_solve_and_responses(precision, oracle_mod):
     ctx.assemble_rhs_mt(0, api.YX_POLAR, 1)
     ctx.assemble_rhs_dual(2)
     ctx.create_pc()
-    its, rr = ctx.solve_linear_system(0, 3, 200, 1e-10, rtols=np.array([1e-10, 1e-10, 1e-10]))
-    assert np.all(rr <= 1e-10 * 1.0001), rr
+    its, rr = ctx.solve_linear_system(0, 3, 300, rtol, check=False)
+    print(precision, "outer its", its, "true relres", rr, "inner its", ctx.stat("inner_its"))
+    assert np.all(rr <= rtol * 1.0001), rr
     rp, ci, C, M = ctx.get_matrix()
     sols = []
     for slot in range(3):
         sr, si = ctx.get_rhs(slot)
         e_direct = oracle_mod.solve_direct(rp, ci, C, M, sr, si)
         e_gpu = ctx.get_solution(slot)
-        assert np.linalg.norm(e_gpu - e_direct) <= 1e-6 * np.linalg.norm(e_direct)
+        assert np.linalg.norm(e_gpu - e_direct) <= 1e-5 * np.linalg.norm(e_direct)
         sols.append(e_direct)
     rsp = ctx.calculate_response_mt(0, 0, 1, m.obs)
     ref = oracle_mod.responses_mt(om, m, 0, sols[0], sols[1])
-    for o, v in ref.items():
-        assert abs(rsp[o] - v) <= 1e-6 * max(abs(v), 1e-12), (m.obs[o], rsp[o], v)
+    check_mt_responses(m.obs, rsp, ref)
     # apparent resistivity of a half-space approaches its resistivity (coarse mesh: within 40%)
     zxy = rsp[[o for o in range(len(m.obs)) if m.obs[o][0] == mg.Z_XY_RI]]
     rho_a = np.abs(zxy) ** 2 / (2 * np.pi * m.freqs[0] * 4e-7 * np.pi)
@@ -170,9 +175,8 @@ def test_estimator_matches_oracle(oracle_mod):
 
 def test_pc_apply_matches_fcg_tolerance(tiny, oracle_mod):
     """pc_apply_b gives x with ||b - B x|| <= rtol ||b|| per column (src/em_la.cc:208-228)."""
-    torch = _torch()
     m, om = tiny
-    for prec, tol in (("fp64", 1e-2), ("fp32", 1e-2)):
+    for prec, tol in (("fp64", 1e-2), ("fp32", 1e-2), ("fp64", 1e-6)):
         ctx = _ctx(m, "-inner_precision %s -B_ksp_rtol %g" % (prec, tol))
         ctx.create_linear_system(1)
         ctx.assemble_matrix(0)
@@ -181,11 +185,10 @@ def test_pc_apply_matches_fcg_tolerance(tiny, oracle_mod):
         B = oracle_mod.csr(rp, ci, C + M, rp.size - 1)
         rng = np.random.default_rng(9)
         b = rng.standard_normal((rp.size - 1, 4))
-        bd = torch.tensor(b, device="cuda", dtype=torch.float64).contiguous()
-        xd = torch.empty_like(bd)
-        torch.cuda.synchronize()
-        ctx.pc_apply_b(2, bd.data_ptr(), xd.data_ptr())
-        x = xd.cpu().numpy()
+        bd = ctx.device_array(b)
+        xd = ctx.device_array(np.zeros_like(b))
+        ctx.pc_apply_b(2, bd.ptr, xd.ptr)
+        x = xd.numpy()
         for c in range(4):
             r = b[:, c] - B @ x[:, c]
             assert np.linalg.norm(r) <= tol * np.linalg.norm(b[:, c]) * 1.01
