@@ -41,15 +41,19 @@ def run_step(ctx, m, fidx, batched=True):
     rtols = np.array([1e-8, 1e-8, 1e-6])
     its = []
     if batched:
-        i, r = ctx.solve_linear_system(0, 3, 100, 1e-8, rtols=rtols)
-        its = i.tolist()
-        rr = r.tolist()
+        # XY and YX as one block in the mixed path; the dual in the fp64 cycle
+        i, r = ctx.solve_linear_system(0, 2, 100, 1e-8, check=False)
+        i2, r2 = ctx.solve_linear_system(2, 1, 100, 1e-6, check=False, precision="fp64")
+        its = i.tolist() + i2.tolist()
+        rr = r.tolist() + r2.tolist()
     else:
         rr = []
         for s in range(3):
-            i, r = ctx.solve_linear_system(s, 1, 100, rtols[s])
+            i, r = ctx.solve_linear_system(s, 1, 100, rtols[s], check=False)
             its += i.tolist()
             rr += r.tolist()
+    if any(v > 10 * t for v, t in zip(rr, rtols)):
+        raise RuntimeError("solve did not converge: relres %s" % rr)
     t3 = time.perf_counter()
     ctx.estimate_error(fidx, [0, 1, 2])
     t4 = time.perf_counter()

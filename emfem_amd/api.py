@@ -283,14 +283,23 @@ class Context:
     def destroy_pc(self):
         self._check(self._L.emfem_destroy_pc(self._h))
 
-    def solve_linear_system(self, first, n, max_it, rtol, rtols=None, check=True):
+    def solve_linear_system(self, first, n, max_it, rtol, rtols=None, check=True, precision=None):
         """Returns (iterations, true relative residuals); raises on non-convergence
-        when check is True (the reference never checks, src/em_la.cc:51-52)."""
+        when check is True (the reference never checks, src/em_la.cc:51-52).
+        precision ('fp32' | 'fp64') overrides -inner_precision for this call."""
         its = np.zeros(n, dtype=np.int32)
         rr = np.zeros(n)
         rt = _c(rtols, np.float64) if rtols is not None else None
-        rc = self._L.emfem_solve_linear_system(self._h, int(first), int(n), int(max_it), float(rtol),
-                                               _p(rt) if rt is not None else None, _p(its), _p(rr))
+        keep = None
+        if precision is not None:
+            keep = self.get_option("inner_precision")
+            self.process_options("-inner_precision %s" % precision)
+        try:
+            rc = self._L.emfem_solve_linear_system(self._h, int(first), int(n), int(max_it), float(rtol),
+                                                   _p(rt) if rt is not None else None, _p(its), _p(rr))
+        finally:
+            if keep is not None:
+                self.process_options("-inner_precision %s" % keep)
         if rc == ERR_NOT_CONVERGED and not check:
             return its, rr
         self._check(rc)

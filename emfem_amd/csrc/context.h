@@ -126,27 +126,41 @@ inline void check_attributes(emfem_context *c) {
                     " outside the resistivity table");
 }
 
-// Copies the (re, im) columns of `slot` between a [E][2 nslots] block and a host pair.
+// Copies the (re, im) columns of `slot` between a [E][2 nslots] block and a host pair:
+// one contiguous transfer to a device staging buffer, then a strided on-device copy
+// (a host<->device 2-D copy with 16-byte rows is row-by-row and very slow).
 inline void put_slot(emfem_context *c, DevBuf<double> &blk, int slot, const double *re, const double *im) {
   std::vector<double> tmp((size_t)c->T.ne * 2);
   for (int e = 0; e < c->T.ne; ++e) {
     tmp[(size_t)e * 2] = re ? re[e] : 0.0;
     tmp[(size_t)e * 2 + 1] = im ? im[e] : 0.0;
   }
-  EMF_HIP(hipMemcpy2DAsync(blk.get() + 2 * slot, sizeof(double) * 2 * c->nslots, tmp.data(), sizeof(double) * 2,
-                           sizeof(double) * 2, c->T.ne, hipMemcpyHostToDevice, c->stream));
+  DevBuf<double> stage;
+  stage.upload(tmp, c->stream);
+  axpby<double, double>(c->stream, c->T.ne, 2, 1.0, stage.get(), 2, 0.0, blk.get() + 2 * slot, 2 * c->nslots);
   EMF_HIP(hipStreamSynchronize(c->stream));
 }
 
 inline void get_slot(const emfem_context *c, const DevBuf<double> &blk, int slot, double *re, double *im) {
   std::vector<double> tmp((size_t)c->T.ne * 2);
-  EMF_HIP(hipMemcpy2DAsync(tmp.data(), sizeof(double) * 2, blk.get() + 2 * slot, sizeof(double) * 2 * c->nslots,
-                           sizeof(double) * 2, c->T.ne, hipMemcpyDeviceToHost, c->stream));
+  DevBuf<double> stage((size_t)c->T.ne * 2);
+  axpby<double, double>(c->stream, c->T.ne, 2, 1.0, blk.get() + 2 * slot, 2 * c->nslots, 0.0, stage.get(), 2);
+  stage.download(tmp.data(), tmp.size(), c->stream);
   EMF_HIP(hipStreamSynchronize(c->stream));
   for (int e = 0; e < c->T.ne; ++e) {
     if (re) re[e] = tmp[(size_t)e * 2];
     if (im) im[e] = tmp[(size_t)e * 2 + 1];
   }
+}
+
+// Solution of a slot as interleaved (re, im) per edge on the host.
+inline std::vector<double> solution_ri(const emfem_context *c, int slot) {
+  std::vector<double> out((size_t)c->T.ne * 2);
+  DevBuf<double> stage((size_t)c->T.ne * 2);
+  axpby<double, double>(c->stream, c->T.ne, 2, 1.0, c->d_X.get() + 2 * slot, 2 * c->nslots, 0.0, stage.get(), 2);
+  stage.download(out.data(), out.size(), c->stream);
+  EMF_HIP(hipStreamSynchronize(c->stream));
+  return out;
 }
 
 inline PcOptions pc_options(const emfem_context *c) {

@@ -63,6 +63,8 @@ const std::map<std::string, std::string> &option_defaults() {
       {"B_ksp_max_it", "100"},
       // this implementation
       {"inner_precision", "fp32"},
+      {"dual_precision", "fp64"},
+      {"fp64_fallback", "true"},
       {"ams_cheb_degree", "3"},
       {"ams_cheb_ratio", "30"},
       {"ams_theta", "0.08"},
@@ -77,6 +79,8 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"direct_solver_type", {"mumps", "superlu_dist"}},
       {"refine_strategy", {"fixed_number", "fixed_fraction"}},
       {"inner_precision", {"fp32", "fp64"}},
+      {"dual_precision", {"fp32", "fp64"}},
+      {"fp64_fallback", {"true", "false"}},
   };
   static const std::set<std::string> none;
   auto it = e.find(key);
@@ -482,21 +486,20 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
     hipStream_t s = c->stream;
     const int ne = c->T.ne, nc = 2 * n;
     DevBuf<double> b((size_t)ne * nc), x((size_t)ne * nc);
-    const size_t pitch = sizeof(double) * 2 * c->nslots;
-    EMF_HIP(hipMemcpy2DAsync(b.get(), sizeof(double) * nc, c->d_S.get() + 2 * first, pitch, sizeof(double) * nc, ne,
-                             hipMemcpyDeviceToDevice, s));
+    axpby<double, double>(s, ne, nc, 1.0, c->d_S.get() + 2 * first, 2 * c->nslots, 0.0, b.get(), nc);
     std::vector<double> tol(n, rtol);
     if (rtols)
       for (int i = 0; i < n; ++i) tol[i] = rtols[i];
     OuterOptions oo;
     oo.restart = c->opti("A_ksp_gmres_restart");
     oo.max_it = max_it;
+    oo.fp64_fallback = c->opt("fp64_fallback") == "true";
+    c->pc->set_fp32(c->opt("inner_precision") == "fp32", s);
     SolveStats st;
     const double t0 = now_ms();
     fgmres(s, ne, c->nnz, c->d_rp.get(), c->d_ci.get(), c->d_K.get(), *c->pc, n, b.get(), x.get(), tol, oo, st,
            &c->op_clock);
-    EMF_HIP(hipMemcpy2DAsync(c->d_X.get() + 2 * first, pitch, x.get(), sizeof(double) * nc, sizeof(double) * nc, ne,
-                             hipMemcpyDeviceToDevice, s));
+    axpby<double, double>(s, ne, nc, 1.0, x.get(), nc, 0.0, c->d_X.get() + 2 * first, 2 * c->nslots);
     EMF_HIP(hipStreamSynchronize(s));
     const double ms = now_ms() - t0;
     c->stats["solve_ms"] = ms;
@@ -504,6 +507,7 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
     c->stats["inner_its"] = (double)st.inner_its;
     c->stats["pc_applies"] = (double)st.pc_applies;
     c->stats["vcycles"] = (double)st.vcycles;
+    c->stats["fell_back"] = (double)st.fell_back;
     int cnt = 0;
     double opms = c->op_clock.average_ms(&cnt);
     c->stats["op_spmm_avg_ms"] = opms;
@@ -545,7 +549,8 @@ int emfem_pc_apply_b(emfem_context *c, int n_rhs, const double *b, double *x) {
     EMF_REQUIRE(c->pc, kErrState, "no preconditioner: call emfem_create_pc first");
     EMF_REQUIRE(n_rhs > 0 && b && x && b != x, kErrArg, "bad preconditioner arguments");
     SolveStats st;
-    c->pc->apply64(c->stream, 2 * n_rhs, b, x, st);
+    c->pc->set_fp32(c->opt("inner_precision") == "fp32", c->stream);
+    c->pc->apply(c->stream, 2 * n_rhs, b, x, st);
     EMF_HIP(hipStreamSynchronize(c->stream));
     c->stats["inner_its"] = (double)st.inner_its;
   });
@@ -586,10 +591,7 @@ int emfem_interpolate_fields(emfem_context *c, int fidx, int slot, int n, const 
     require_slot(c, slot);
     require_freq(c, fidx);
     EMF_REQUIRE(n >= 0 && (n == 0 || (points && E && H)), kErrArg, "bad interpolation arguments");
-    std::vector<double> sol((size_t)c->T.ne * 2);
-    EMF_HIP(hipMemcpy2DAsync(sol.data(), sizeof(double) * 2, c->d_X.get() + 2 * slot, sizeof(double) * 2 * c->nslots,
-                             sizeof(double) * 2, c->T.ne, hipMemcpyDeviceToHost, c->stream));
-    EMF_HIP(hipStreamSynchronize(c->stream));
+    const std::vector<double> sol = solution_ri(c, slot);
     for (int i = 0; i < n; ++i) {
       const int t = find_cell(c->T, points + (size_t)i * 3);
       EMF_REQUIRE(t >= 0, kErrUser, "Receiver " + std::to_string(i) + " is not found in mesh.");
@@ -613,14 +615,7 @@ int emfem_calculate_response_mt(emfem_context *c, int fidx, int slot_xy, int slo
     require_freq(c, fidx);
     EMF_REQUIRE(n_obs >= 0 && (n_obs == 0 || (obs && rsp)), kErrArg, "bad observation arrays");
     const int nrx = (int)c->rx.size() / 3;
-    std::vector<double> exy_sol((size_t)c->T.ne * 2), eyx_sol((size_t)c->T.ne * 2);
-    for (int k = 0; k < 2; ++k) {
-      std::vector<double> &dst = k == 0 ? exy_sol : eyx_sol;
-      EMF_HIP(hipMemcpy2DAsync(dst.data(), sizeof(double) * 2, c->d_X.get() + 2 * (k == 0 ? slot_xy : slot_yx),
-                               sizeof(double) * 2 * c->nslots, sizeof(double) * 2, c->T.ne, hipMemcpyDeviceToHost,
-                               c->stream));
-    }
-    EMF_HIP(hipStreamSynchronize(c->stream));
+    const std::vector<double> exy_sol = solution_ri(c, slot_xy), eyx_sol = solution_ri(c, slot_yx);
     std::map<int, std::array<cplx, 12>> cache;
     const double f = c->freqs[fidx];
     for (int o = 0; o < n_obs; ++o) {
@@ -653,10 +648,7 @@ int emfem_calculate_response_csem(emfem_context *c, int fidx, int tidx, int slot
     require_freq(c, fidx);
     EMF_REQUIRE(n_obs >= 0 && (n_obs == 0 || (obs && rsp)), kErrArg, "bad observation arrays");
     const int nrx = (int)c->rx.size() / 3;
-    std::vector<double> sol((size_t)c->T.ne * 2);
-    EMF_HIP(hipMemcpy2DAsync(sol.data(), sizeof(double) * 2, c->d_X.get() + 2 * slot, sizeof(double) * 2 * c->nslots,
-                             sizeof(double) * 2, c->T.ne, hipMemcpyDeviceToHost, c->stream));
-    EMF_HIP(hipStreamSynchronize(c->stream));
+    const std::vector<double> sol = solution_ri(c, slot);
     std::map<int, std::array<cplx, 6>> cache;
     const double f = c->freqs[fidx];
     for (int o = 0; o < n_obs; ++o) {

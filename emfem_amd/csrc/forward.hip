@@ -151,6 +151,23 @@ void refine_by_error(emfem_context *c, const std::vector<std::vector<double>> &e
   install_mesh(c, m);
 }
 
+void solve_slots(emfem_context *c, int first, int n, const std::vector<double> &rtols);
+
+// The dual problem (point sources at receivers on the air-earth interface) drives
+// gradient fields ~ 1/(omega sigma_air) in the air; it is solved with the cycle
+// precision of -dual_precision (fp64 by default).
+void solve_dual(emfem_context *c, int slot) {
+  const std::string keep = c->opts["inner_precision"];
+  c->opts["inner_precision"] = c->opt("dual_precision");
+  try {
+    solve_slots(c, slot, 1, {c->optd("dual_rtol")});
+  } catch (...) {
+    c->opts["inner_precision"] = keep;
+    throw;
+  }
+  c->opts["inner_precision"] = keep;
+}
+
 void solve_slots(emfem_context *c, int first, int n, const std::vector<double> &rtols) {
   std::vector<int> its(n);
   std::vector<double> rr(n);
@@ -180,8 +197,9 @@ void forward_mt(emfem_context *c, int fidx) {
     check(c, emfem_assemble_rhs_mt(c, fidx, EMFEM_XY_POLAR, 0));
     check(c, emfem_assemble_rhs_mt(c, fidx, EMFEM_YX_POLAR, 1));
     check(c, emfem_assemble_rhs_dual(c, 2));
-    const double e_rtol = c->optd("e_rtol"), d_rtol = c->optd("dual_rtol");
-    solve_slots(c, 0, 3, {e_rtol, e_rtol, d_rtol});
+    const double e_rtol = c->optd("e_rtol");
+    solve_slots(c, 0, 2, {e_rtol, e_rtol});
+    solve_dual(c, 2);
     check(c, emfem_destroy_pc(c));
     const int nt = c->T.nt;
     const int slots[3] = {0, 1, 2};
@@ -234,9 +252,8 @@ void forward_csem(emfem_context *c, int fidx, const std::vector<int> &tidxs) {
       check(c, emfem_create_pc(c));
       for (int k = 0; k < ng; ++k) check(c, emfem_assemble_rhs_csem(c, fidx, grp[k], k));
       check(c, emfem_assemble_rhs_dual(c, ng));
-      std::vector<double> rt(ng + 1, c->optd("e_rtol"));
-      rt[ng] = c->optd("dual_rtol");
-      solve_slots(c, 0, ng + 1, rt);
+      solve_slots(c, 0, ng, std::vector<double>(ng, c->optd("e_rtol")));
+      solve_dual(c, ng);
       check(c, emfem_destroy_pc(c));
       const bool last = cycle == max_cycles || c->T.ne >= c->opti("max_dofs");
       if (last) {

@@ -19,6 +19,7 @@
 namespace emfem {
 
 struct SolveStats {
+  int fell_back = 0;         // 1 if the solve switched to the fp64 cycle
   int outer_its = 0;         // max over right-hand sides
   long inner_its = 0;        // FCG iterations summed over preconditioner applications
   long pc_applies = 0;       // preconditioner applications (each on 2k columns)
@@ -48,14 +49,15 @@ struct PcOptions {
   AmgParams amg;
 };
 
-// Preconditioner for B on nc real columns (interface independent of precision).
+// Preconditioner for B on nc real columns: flexible CG in fp64 with the
+// auxiliary-space cycle in fp32 or fp64 (both built from one host set-up; the
+// precision not asked for at creation is set up on first use).
 class InnerPc {
  public:
   virtual ~InnerPc() = default;
-  // x = approx B^-1 b by FCG + AMS; b, x are fp64 [E][nc] compact (converted inside).
-  virtual void apply64(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) = 0;
-  // FGMRES entry: b fp64, x written to the precision of the stored Krylov directions.
-  virtual void apply_dir(hipStream_t s, int nc, const double *b, void *x, SolveStats &st) = 0;
+  // x = approx B^-1 b by FCG + AMS; b, x are fp64 [E][nc] compact.
+  virtual void apply(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) = 0;
+  virtual void set_fp32(bool fp32, hipStream_t s) = 0;
   virtual bool fp32() const = 0;
   // B SpMMs of the Chebyshev smoother (precision of the cycle) and of the fp64 FCG
   virtual KernelClock &smoother_clock() = 0;
@@ -68,6 +70,11 @@ std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int 
 struct OuterOptions {
   int restart = 30;
   int max_it = 100;
+  // switch the whole solve to the fp64 cycle when an fp32-preconditioned right-hand
+  // side stops converging (less than halving its residual over `stall_window`
+  // iterations)
+  bool fp64_fallback = true;
+  int stall_window = 15;
 };
 
 // FGMRES on nrhs interleaved right-hand sides b [E][2 nrhs] -> x [E][2 nrhs].

@@ -306,35 +306,40 @@ class DevAms {
 // preconditioned by the auxiliary-space cycle in precision T
 // ---------------------------------------------------------------------------------
 
-template <typename T>
 class InnerPcImpl : public InnerPc {
  public:
-  InnerPcImpl(const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K, const PcOptions &o,
-              hipStream_t s)
-      : ne_(ne), nnz_(nnz), o_(o) {
-    ams_.setup(H, ne, nnz, rp, ci, K, o, s);
+  InnerPcImpl(bool fp32, const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K,
+              const PcOptions &o, hipStream_t s)
+      : H_(H), ne_(ne), nnz_(nnz), rp_(rp), ci_(ci), K_(K), o_(o) {
+    set_fp32(fp32, s);
   }
 
-  bool fp32() const override { return std::is_same<T, float>::value; }
+  bool fp32() const override { return fp32_; }
   KernelClock &smoother_clock() override { return smooth_clock_; }
   KernelClock &fcg_clock() override { return fcg_clock_; }
 
-  void apply64(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) override {
+  void set_fp32(bool fp32, hipStream_t s) override {
+    fp32_ = fp32;
+    if (fp32 && !a32_) {
+      a32_.reset(new DevAms<float>());
+      a32_->setup(H_, ne_, nnz_, rp_, ci_, K_, o_, s);
+      if (nc_) a32_->ensure(nc_);
+    }
+    if (!fp32 && !a64_) {
+      a64_.reset(new DevAms<double>());
+      a64_->setup(H_, ne_, nnz_, rp_, ci_, K_, o_, s);
+      if (nc_) a64_->ensure(nc_);
+    }
+  }
+
+  void apply(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) override {
     ensure(nc);
     solve(s, nc, b, x, st);
   }
 
-  void apply_dir(hipStream_t s, int nc, const double *b, void *x, SolveStats &st) override {
-    ensure(nc);
-    if constexpr (std::is_same<T, double>::value) {
-      solve(s, nc, b, static_cast<double *>(x), st);
-    } else {
-      solve(s, nc, b, xd_.get(), st);
-      axpby<double, T>(s, ne_, nc, 1.0, xd_.get(), nc, 0.0, static_cast<T *>(x), nc);
-    }
-  }
-
  private:
+  const Csr<double> &B64() const { return a64_ ? a64_->B64() : a32_->B64(); }
+
   void ensure(int nc) {
     if (nc <= nc_) return;
     nc_ = nc;
@@ -343,17 +348,15 @@ class InnerPcImpl : public InnerPc {
     z_.resize(n);
     p_.resize(n);
     q_.resize(n);
-    if (!std::is_same<T, double>::value) {
-      xd_.resize(n);
-      zt_.resize(n);
-    }
+    zt_.resize(n);
     for (auto *v : {&rz_, &pq_, &zq_, &alpha_, &beta_, &rr_, &ones_}) v->resize(nc);
     active_.resize(nc);
     work_.resize(dot_work_size(ne_, nc) * 8);
     std::vector<double> one(nc, 1.0);
     ones_.upload(one, 0);
     EMF_HIP(hipDeviceSynchronize());
-    ams_.ensure(nc);
+    if (a32_) a32_->ensure(nc);
+    if (a64_) a64_->ensure(nc);
     h_bb_.resize(nc);
     h_rr_.resize(nc);
     h_active_.resize(nc);
@@ -361,11 +364,11 @@ class InnerPcImpl : public InnerPc {
 
   void precondition(hipStream_t s, int nc, const double *r, double *z, SolveStats &st) {
     ++st.vcycles;
-    if constexpr (std::is_same<T, double>::value) {
-      ams_.apply(s, nc, r, z, &smooth_clock_);
+    if (!fp32_) {
+      a64_->apply(s, nc, r, z, &smooth_clock_);
     } else {
-      ams_.apply(s, nc, r, zt_.get(), &smooth_clock_);
-      axpby<T, double>(s, ne_, nc, 1.0, zt_.get(), nc, 0.0, z, nc);
+      a32_->apply(s, nc, r, zt_.get(), &smooth_clock_);
+      axpby<float, double>(s, ne_, nc, 1.0, zt_.get(), nc, 0.0, z, nc);
     }
   }
 
@@ -419,15 +422,20 @@ class InnerPcImpl : public InnerPc {
   void bspmm(hipStream_t s, int nc, const double *x, double *y) {
     const bool t = fcg_clock_.armed();
     if (t) fcg_clock_.start(s);
-    spmm<double>(s, ams_.B64(), nc, x, nc, y, nc, 1.0, nullptr, 0.0, nullptr, 0, 0.0, nullptr, 0);
+    spmm<double>(s, B64(), nc, x, nc, y, nc, 1.0, nullptr, 0.0, nullptr, 0, 0.0, nullptr, 0);
     if (t) fcg_clock_.stop(s);
   }
 
+  const AmsHost &H_;
   int ne_ = 0, nnz_ = 0, nc_ = 0;
+  const int *rp_, *ci_;
+  const double2 *K_;
   PcOptions o_;
-  DevAms<T> ams_;
-  DevBuf<double> r_, z_, p_, q_, xd_;
-  DevBuf<T> zt_;
+  bool fp32_ = true;
+  std::unique_ptr<DevAms<float>> a32_;
+  std::unique_ptr<DevAms<double>> a64_;
+  DevBuf<double> r_, z_, p_, q_;
+  DevBuf<float> zt_;
   DevBuf<double> rz_, pq_, zq_, alpha_, beta_, rr_, ones_, work_;
   DevBuf<uint8_t> active_;
   std::vector<double> h_bb_, h_rr_;
@@ -439,8 +447,7 @@ class InnerPcImpl : public InnerPc {
 
 std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp, const int *d_ci,
                                        const double2 *d_K, const PcOptions &o, hipStream_t s) {
-  if (fp32) return std::unique_ptr<InnerPc>(new InnerPcImpl<float>(H, ne, nnz, d_rp, d_ci, d_K, o, s));
-  return std::unique_ptr<InnerPc>(new InnerPcImpl<double>(H, ne, nnz, d_rp, d_ci, d_K, o, s));
+  return std::unique_ptr<InnerPc>(new InnerPcImpl(fp32, H, ne, nnz, d_rp, d_ci, d_K, o, s));
 }
 
 // ---------------------------------------------------------------------------------
@@ -449,7 +456,9 @@ std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int 
 
 namespace {
 
-template <typename TZ>
+// Convergence is decided on the true fp64 residual at every restart: a right-hand
+// side whose Arnoldi estimate converged inside a cycle is frozen for the rest of
+// that cycle and re-checked (and, if needed, resumed) at the next restart.
 void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, InnerPc &pc, int nrhs,
                  const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o, SolveStats &st,
                  KernelClock *clk) {
@@ -457,7 +466,8 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
   const int m = std::max(1, o.restart);
   const size_t blk = (size_t)ne * nc;
   DevBuf<double> V((m + 1) * blk), w(blk), dots((size_t)(m + 1) * nc), coef((size_t)(m + 1) * nc), scal(nc);
-  DevBuf<TZ> Z(m * blk);
+  DevBuf<double> Z(m * blk);
+  std::vector<std::vector<double>> hist(nrhs);
   DevBuf<double> work(dot_work_size(ne, nc) * 8);
   std::vector<double> h_dots((size_t)(m + 1) * nc), h_coef((size_t)(m + 1) * nc), h_scal(nc);
   auto op = [&](const auto *in, double *out, double alpha, double beta, const double *z) {
@@ -500,11 +510,12 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
     rhs_norms(V0, beta);
     bool any = false;
     for (int r = 0; r < nrhs; ++r) {
-      if (!done[r] && beta[r] <= rtol[r] * bnorm[r]) done[r] = 1;
+      done[r] = bnorm[r] == 0.0 || beta[r] <= rtol[r] * bnorm[r];
       active[r] = !done[r];
       any |= active[r] != 0;
     }
     if (!any) break;
+    bool switch_now = false;
     std::vector<double> inv(nrhs);
     for (int r = 0; r < nrhs; ++r) inv[r] = active[r] && beta[r] > 0 ? 1.0 / beta[r] : 0.0;
     scale_cols(V0, V0, inv);
@@ -517,8 +528,8 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
     for (int j = 0; j < m && total < o.max_it; ++j) {
       ++total;
       double *Vj = V.get() + (size_t)j * blk;
-      TZ *Zj = Z.get() + (size_t)j * blk;
-      pc.apply_dir(s, nc, Vj, Zj, st);
+      double *Zj = Z.get() + (size_t)j * blk;
+      pc.apply(s, nc, Vj, Zj, st);
       op(Zj, w.get(), 1.0, 0.0, nullptr);
       // classical Gram-Schmidt against V_0..V_j
       mdot(s, ne, nc, V.get(), blk, j + 1, w.get(), dots.get(), work.get());
@@ -562,6 +573,11 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
         gr[j] = cj * gr[j];
         kk[r] = j + 1;
         st.its[r] += 1;
+        hist[r].push_back(std::abs(gr[j + 1]));
+        const size_t hs = hist[r].size();
+        if (o.fp64_fallback && pc.fp32() && (int)hs > o.stall_window &&
+            hist[r][hs - 1] > 0.5 * hist[r][hs - 1 - o.stall_window])
+          switch_now = true;  // the fp32 cycle no longer makes progress on this RHS
         if (std::abs(gr[j + 1]) <= rtol[r] * bnorm[r] || hn[r] == 0.0) {
           active[r] = 0;  // converged in this cycle: freeze its basis
           done[r] = 1;
@@ -570,7 +586,11 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
         }
       }
       scale_cols(w.get(), V.get() + (size_t)(j + 1) * blk, fac);
-      if (!still) break;
+      if (!still || switch_now) break;
+    }
+    if (switch_now) {
+      pc.set_fp32(false, s);
+      st.fell_back = 1;
     }
     // y = R^-1 g per right-hand side; x += sum_i y_i Z_i
     std::fill(h_coef.begin(), h_coef.end(), 0.0);
@@ -590,11 +610,8 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
     }
     if (kmax > 0) {
       coef.upload(h_coef.data(), (size_t)kmax * nc, s);
-      mupdate<TZ>(s, ne, nc, Z.get(), blk, kmax, coef.get(), nc, x, -1.0);
+      mupdate<double>(s, ne, nc, Z.get(), blk, kmax, coef.get(), nc, x, -1.0);
     }
-    bool all = true;
-    for (int r = 0; r < nrhs; ++r) all &= done[r] != 0;
-    if (all) break;
   }
   // true residual
   op(x, w.get(), -1.0, 1.0, b);
@@ -612,10 +629,7 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
 void fgmres(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, InnerPc &pc, int nrhs,
             const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o, SolveStats &st,
             KernelClock *clk) {
-  if (pc.fp32())
-    fgmres_impl<float>(s, ne, nnz, rp, ci, K, pc, nrhs, b, x, rtol, o, st, clk);
-  else
-    fgmres_impl<double>(s, ne, nnz, rp, ci, K, pc, nrhs, b, x, rtol, o, st, clk);
+  fgmres_impl(s, ne, nnz, rp, ci, K, pc, nrhs, b, x, rtol, o, st, clk);
 }
 
 }  // namespace emfem

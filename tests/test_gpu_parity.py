@@ -120,8 +120,13 @@ def check_mt_responses(obs, rsp, ref, tol=1e-6):
             assert abs(rsp[o] - ref[o]) <= tol * s, (obs[o], rsp[o], ref[o])
 
 
-@pytest.mark.parametrize("precision,rtol", [("fp64", 1e-10), ("fp32", 1e-8)])
-def test_mt_solve_and_responses(precision, rtol, oracle_mod):
+@pytest.mark.parametrize("precision,rtol,rsp_tol", [("fp64", 1e-10, 1e-6), ("fp32", 1e-10, 1e-6),
+                                                     ("fp64", 1e-8, 2e-5), ("fp32", 1e-8, 2e-5)])
+def test_mt_solve_and_responses(precision, rtol, rsp_tol, oracle_mod):
+    """The FP32 inner solve inside FP64 FGMRES reaches the same true residual as the
+    plain FP64 path; responses then agree with the exact discrete solution to a bound
+    set by that residual (1e-6 at 1e-10; at 1e-8 the solution error is ~1e-6..1e-5
+    relative on this mesh)."""
     m = mg.halfspace_mt(n=4, n_rx=3)
     om = oracle_mod.Mesh(m.points, m.tets)
     ctx = _ctx(m, "-inner_precision %s" % precision)
@@ -140,11 +145,11 @@ def test_mt_solve_and_responses(precision, rtol, oracle_mod):
         sr, si = ctx.get_rhs(slot)
         e_direct = oracle_mod.solve_direct(rp, ci, C, M, sr, si)
         e_gpu = ctx.get_solution(slot)
-        assert np.linalg.norm(e_gpu - e_direct) <= 1e-5 * np.linalg.norm(e_direct)
+        assert np.linalg.norm(e_gpu - e_direct) <= 1e3 * rtol * np.linalg.norm(e_direct)
         sols.append(e_direct)
     rsp = ctx.calculate_response_mt(0, 0, 1, m.obs)
     ref = oracle_mod.responses_mt(om, m, 0, sols[0], sols[1])
-    check_mt_responses(m.obs, rsp, ref)
+    check_mt_responses(m.obs, rsp, ref, tol=rsp_tol)
     # apparent resistivity of a half-space approaches its resistivity (coarse mesh: within 40%)
     zxy = rsp[[o for o in range(len(m.obs)) if m.obs[o][0] == mg.Z_XY_RI]]
     rho_a = np.abs(zxy) ** 2 / (2 * np.pi * m.freqs[0] * 4e-7 * np.pi)
