@@ -42,18 +42,16 @@ def run_step(ctx, m, fidx, batched=True):
     its = []
     if batched:
         # XY and YX as one block in the mixed path; the dual in the fp64 cycle
-        i, r = ctx.solve_linear_system(0, 2, 100, 1e-8, check=False)
-        i2, r2 = ctx.solve_linear_system(2, 1, 100, 1e-6, check=False, precision="fp64")
+        i, r = ctx.solve_linear_system(0, 2, 100, 1e-8)
+        i2, r2 = ctx.solve_linear_system(2, 1, 100, 1e-6, precision="fp64")
         its = i.tolist() + i2.tolist()
         rr = r.tolist() + r2.tolist()
     else:
         rr = []
         for s in range(3):
-            i, r = ctx.solve_linear_system(s, 1, 100, rtols[s], check=False)
+            i, r = ctx.solve_linear_system(s, 1, 100, rtols[s])
             its += i.tolist()
             rr += r.tolist()
-    if any(v > 10 * t for v, t in zip(rr, rtols)):
-        raise RuntimeError("solve did not converge: relres %s" % rr)
     t3 = time.perf_counter()
     ctx.estimate_error(fidx, [0, 1, 2])
     t4 = time.perf_counter()
@@ -128,8 +126,14 @@ def main():
     value = E * 3 / (elapsed / args.steps)
     solve_s = sum(s["solve"] for s in steps) / len(steps)
 
-    # roofline of the dominant kernel: the fp32 B SpMM of the inner solve
+    # roofline of the dominant kernel: the fp32 B SpMM of the inner solve.  The timed
+    # steps replay captured cycles, so the per-launch times come from one extra,
+    # untimed solve with plain launches bracketed by HIP events on the solver stream.
+    ctx.process_options("-use_graph false")
+    ctx.solve_linear_system(0, 2, 100, 1e-8)
+    ctx.process_options("-use_graph true")
     b_ms, b_bytes = ctx.kernel_time("b_spmm")
+    fcg_ms, fcg_bytes = ctx.kernel_time("fcg_spmm")
     op_ms, op_bytes = ctx.kernel_time("op_spmm")
     as_ms, as_bytes = ctx.kernel_time("assemble")
     peak = 8000.0
@@ -139,6 +143,7 @@ def main():
             "algorithmic_bytes": b_bytes}
     extra = {
         "op_spmm_GBps": round(op_bytes / (op_ms * 1e-3) / 1e9, 1) if op_ms > 0 else None,
+        "fcg_spmm_fp64_GBps": round(fcg_bytes / (fcg_ms * 1e-3) / 1e9, 1) if fcg_ms > 0 else None,
         "assemble_GBps": round(as_bytes / (as_ms * 1e-3) / 1e9, 1) if as_ms > 0 else None,
         "solve_dof_rhs_per_s": round(E * 3 / solve_s, 1),
         "phase_s": {k: round(sum(s[k] for s in steps) / len(steps), 4)

@@ -328,17 +328,29 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm, AmsHos
     }
     return G;
   };
+  double t0 = now_ms();
   H.G0 = make(-1);
   H.G0T = csr_transpose(H.G0);
+  H.timing["aux_build_ms"] += now_ms() - t0;
   {
-    HostCsr BG = csr_multiply(B, H.G0);
-    build_amg(csr_multiply(H.G0T, BG), prm, H.amgG);
+    t0 = now_ms();
+    HostCsr AG = csr_multiply(H.G0T, csr_multiply(B, H.G0));
+    H.timing["galerkin_ms"] += now_ms() - t0;
+    t0 = now_ms();
+    build_amg(AG, prm, H.amgG);
+    H.timing["amg_ms"] += now_ms() - t0;
   }
   for (int d = 0; d < 3; ++d) {
+    t0 = now_ms();
     H.Pi[d] = make(d);
     H.PiT[d] = csr_transpose(H.Pi[d]);
-    HostCsr BP = csr_multiply(B, H.Pi[d]);
-    build_amg(csr_multiply(H.PiT[d], BP), prm, H.amgPi[d]);
+    H.timing["aux_build_ms"] += now_ms() - t0;
+    t0 = now_ms();
+    HostCsr AP = csr_multiply(H.PiT[d], csr_multiply(B, H.Pi[d]));
+    H.timing["galerkin_ms"] += now_ms() - t0;
+    t0 = now_ms();
+    build_amg(AP, prm, H.amgPi[d]);
+    H.timing["amg_ms"] += now_ms() - t0;
   }
   H.l1inv = l1_inverse(B);
 }

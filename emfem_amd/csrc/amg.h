@@ -13,6 +13,8 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "topology.h"
@@ -54,6 +56,7 @@ struct AmsHost {
   AmgHost amgG, amgPi[3];
   std::vector<double> l1inv;  // edge-space l1 row norms of B, inverted
   int nv0 = 0;
+  std::map<std::string, double> timing;  // set-up phase times (ms)
 };
 
 // B given on the edge pattern (fp64 values).

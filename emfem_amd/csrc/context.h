@@ -170,6 +170,7 @@ inline PcOptions pc_options(const emfem_context *c) {
   o.inner_rtol = c->optd("B_ksp_rtol");
   o.inner_max_it = c->opti("B_ksp_max_it");
   o.amg.theta = c->optd("ams_theta");
+  o.use_graph = c->opt("use_graph") == "true";
   return o;
 }
 

@@ -65,6 +65,7 @@ const std::map<std::string, std::string> &option_defaults() {
       {"inner_precision", "fp32"},
       {"dual_precision", "fp64"},
       {"fp64_fallback", "true"},
+      {"use_graph", "true"},
       {"ams_cheb_degree", "3"},
       {"ams_cheb_ratio", "30"},
       {"ams_theta", "0.08"},
@@ -81,6 +82,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"inner_precision", {"fp32", "fp64"}},
       {"dual_precision", {"fp32", "fp64"}},
       {"fp64_fallback", {"true", "false"}},
+      {"use_graph", {"true", "false"}},
   };
   static const std::set<std::string> none;
   auto it = e.find(key);
@@ -458,8 +460,11 @@ int emfem_create_pc(emfem_context *c) {
     for (int i = 0; i < c->nnz; ++i) B.v[i] = K[i].x + K[i].y;
     PcOptions o = pc_options(c);
     c->ams_host.reset(new AmsHost());
+    const double tb = now_ms();
     build_ams(c->T, B, o.amg, *c->ams_host);
     const double t1 = now_ms();
+    c->stats["pc_download_ms"] = tb - t0;
+    for (auto &kv : c->ams_host->timing) c->stats["pc_" + kv.first] = kv.second;
     c->pc = make_inner_pc(c->opt("inner_precision") == "fp32", *c->ams_host, c->T.ne, c->nnz, c->d_rp.get(),
                           c->d_ci.get(), c->d_K.get(), o, s);
     EMF_HIP(hipStreamSynchronize(s));
@@ -495,6 +500,7 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
     oo.max_it = max_it;
     oo.fp64_fallback = c->opt("fp64_fallback") == "true";
     c->pc->set_fp32(c->opt("inner_precision") == "fp32", s);
+    c->pc->set_use_graph(c->opt("use_graph") == "true");
     SolveStats st;
     const double t0 = now_ms();
     fgmres(s, ne, c->nnz, c->d_rp.get(), c->d_ci.get(), c->d_K.get(), *c->pc, n, b.get(), x.get(), tol, oo, st,
@@ -526,7 +532,7 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
     for (int i = 0; i < n; ++i) {
       if (iters) iters[i] = st.its[i];
       if (relres) relres[i] = st.relres[i];
-      if (!(st.relres[i] <= tol[i] * 1.0000001)) conv_rc = EMFEM_ERR_NOT_CONVERGED;
+      if (!st.converged[i]) conv_rc = EMFEM_ERR_NOT_CONVERGED;
     }
     if (conv_rc != EMFEM_OK) c->err = "linear solve did not reach its tolerance within max_it";
   });
@@ -550,6 +556,7 @@ int emfem_pc_apply_b(emfem_context *c, int n_rhs, const double *b, double *x) {
     EMF_REQUIRE(n_rhs > 0 && b && x && b != x, kErrArg, "bad preconditioner arguments");
     SolveStats st;
     c->pc->set_fp32(c->opt("inner_precision") == "fp32", c->stream);
+    c->pc->set_use_graph(c->opt("use_graph") == "true");
     c->pc->apply(c->stream, 2 * n_rhs, b, x, st);
     EMF_HIP(hipStreamSynchronize(c->stream));
     c->stats["inner_its"] = (double)st.inner_its;

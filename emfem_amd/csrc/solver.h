@@ -26,6 +26,7 @@ struct SolveStats {
   long vcycles = 0;          // AMS cycles
   std::vector<int> its;      // per right-hand side
   std::vector<double> relres;
+  std::vector<uint8_t> converged;  // the reference's stopping test was met
 };
 
 // Timing of a class of kernel launches with HIP events on the solver stream.
@@ -47,6 +48,7 @@ struct PcOptions {
   double inner_rtol = 1e-2;
   int inner_max_it = 100;
   AmgParams amg;
+  bool use_graph = true;  // replay each auxiliary-space cycle as one captured hipGraph
 };
 
 // Preconditioner for B on nc real columns: flexible CG in fp64 with the
@@ -59,6 +61,8 @@ class InnerPc {
   virtual void apply(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) = 0;
   virtual void set_fp32(bool fp32, hipStream_t s) = 0;
   virtual bool fp32() const = 0;
+  // replay the cycles as captured hipGraphs (off: plain launches, timed per SpMM)
+  virtual void set_use_graph(bool g) = 0;
   // B SpMMs of the Chebyshev smoother (precision of the cycle) and of the fp64 FCG
   virtual KernelClock &smoother_clock() = 0;
   virtual KernelClock &fcg_clock() = 0;

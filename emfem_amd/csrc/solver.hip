@@ -10,6 +10,8 @@
 #include "solver.h"
 
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <type_traits>
 
 namespace emfem {
@@ -228,9 +230,36 @@ class DevAms {
     for (int d = 0; d < 3; ++d) amgPi_[d].ensure(nc);
   }
 
-  // out = M_AMS b: b fp64 compact [E][nc], out in T; out must not alias b.
-  void apply(hipStream_t s, int nc, const double *b, T *out, KernelClock *clk) {
+  ~DevAms() {
+    for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  }
+
+  // out = M_AMS b: b fp64 compact [E][nc], out in T; out must not alias b.  With
+  // use_graph the ~100 launches of one cycle are captured once per (nc, b, out)
+  // and replayed as a single hipGraph launch (no memset nodes are captured).
+  void apply(hipStream_t s, int nc, const double *b, T *out, KernelClock *clk, bool use_graph) {
     ensure(nc);
+    if (!use_graph) {
+      launch_cycle(s, nc, b, out, clk);
+      return;
+    }
+    const auto key = std::make_tuple(nc, (const void *)b, (const void *)out);
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+      hipGraph_t g;
+      EMF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      launch_cycle(s, nc, b, out, nullptr);
+      EMF_HIP(hipStreamEndCapture(s, &g));
+      hipGraphExec_t ex;
+      EMF_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      it = graphs_.emplace(key, ex).first;
+    }
+    EMF_HIP(hipGraphLaunch(it->second, s));
+  }
+
+ private:
+  void launch_cycle(hipStream_t s, int nc, const double *b, T *out, KernelClock *clk) {
     T *rres = rres_.get(), *d = d_.get();
     const T *bt;
     if constexpr (std::is_same<T, double>::value) {
@@ -258,7 +287,6 @@ class DevAms {
     cheb_iterate(s, nc, rres, d, out, clk);
   }
 
- private:
   void timed_spmm(hipStream_t s, int nc, const T *x, T *y, T alpha, const T *rs, T gamma, const T *w, T beta,
                   const T *z, KernelClock *clk) {
     const bool t = clk && clk->armed();
@@ -299,6 +327,7 @@ class DevAms {
   DevBuf<double> b64vals_, r64_;
   DevCsr<T> G0_, G0T_, Pi_[3], PiT_[3];
   DevAmg<T> amgG_, amgPi_[3];
+  std::map<std::tuple<int, const void *, const void *>, hipGraphExec_t> graphs_;
 };
 
 // ---------------------------------------------------------------------------------
@@ -315,6 +344,7 @@ class InnerPcImpl : public InnerPc {
   }
 
   bool fp32() const override { return fp32_; }
+  void set_use_graph(bool g) override { o_.use_graph = g; }
   KernelClock &smoother_clock() override { return smooth_clock_; }
   KernelClock &fcg_clock() override { return fcg_clock_; }
 
@@ -365,9 +395,9 @@ class InnerPcImpl : public InnerPc {
   void precondition(hipStream_t s, int nc, const double *r, double *z, SolveStats &st) {
     ++st.vcycles;
     if (!fp32_) {
-      a64_->apply(s, nc, r, z, &smooth_clock_);
+      a64_->apply(s, nc, r, z, &smooth_clock_, o_.use_graph);
     } else {
-      a32_->apply(s, nc, r, zt_.get(), &smooth_clock_);
+      a32_->apply(s, nc, r, zt_.get(), &smooth_clock_, o_.use_graph);
       axpby<float, double>(s, ne_, nc, 1.0, zt_.get(), nc, 0.0, z, nc);
     }
   }
@@ -494,7 +524,9 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
   rhs_norms(b, bnorm);
   st.its.assign(nrhs, 0);
   st.relres.assign(nrhs, 0.0);
-  std::vector<uint8_t> done(nrhs, 0);
+  st.converged.assign(nrhs, 0);
+  std::vector<uint8_t> done(nrhs, 0), est_ok(nrhs, 0);
+  std::vector<double> last_true(nrhs, 0.0);
   for (int r = 0; r < nrhs; ++r) done[r] = bnorm[r] == 0.0;
   EMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * blk, s));
   int total = 0;
@@ -510,7 +542,18 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
     rhs_norms(V0, beta);
     bool any = false;
     for (int r = 0; r < nrhs; ++r) {
-      done[r] = bnorm[r] == 0.0 || beta[r] <= rtol[r] * bnorm[r];
+      const bool true_ok = bnorm[r] == 0.0 || beta[r] <= rtol[r] * bnorm[r];
+      // The reference stops on FGMRES's implicit residual (PETSc's test).  When that
+      // test passed but the true residual did not, restart as iterative refinement
+      // until the true residual stops improving (its evaluation floor, e.g. fields
+      // of ~1/(omega sigma_air) in the air), then accept.
+      bool floor = false;
+      if (!true_ok && est_ok[r]) {
+        floor = last_true[r] > 0.0 && beta[r] > 0.5 * last_true[r];
+        last_true[r] = beta[r];
+      }
+      done[r] = true_ok || floor;
+      if (done[r]) st.converged[r] = 1;
       active[r] = !done[r];
       any |= active[r] != 0;
     }
@@ -581,6 +624,7 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
         if (std::abs(gr[j + 1]) <= rtol[r] * bnorm[r] || hn[r] == 0.0) {
           active[r] = 0;  // converged in this cycle: freeze its basis
           done[r] = 1;
+          est_ok[r] = 1;
         } else {
           still = true;
         }

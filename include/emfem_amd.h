@@ -149,10 +149,13 @@ int emfem_destroy_pc(emfem_context *ctx);
 /* solve_linear_system (src/em_la.cc:35-63), batched: solves slots
  * [first, first+n) together, each to its own relative tolerance rtols[i] (or
  * rtol for all when rtols is NULL), zero initial guess, at most max_it outer
- * iterations.  iters[i] and relres[i] (true FP64 residual ||s - A e|| / ||s||)
- * are written when the pointers are non-NULL.  Returns EMFEM_ERR_NOT_CONVERGED
- * (after storing the partial solution) if any right-hand side missed its
- * tolerance, which the reference never reports. */
+ * iterations.  The stopping test is the reference's (FGMRES's implicit residual
+ * <= rtol ||s||); when it passes but the true residual does not, the solve restarts
+ * from the true residual until that stops improving.  iters[i] and relres[i] (the
+ * true FP64 residual ||s - A e|| / ||s||) are written when the pointers are
+ * non-NULL.  Returns EMFEM_ERR_NOT_CONVERGED (after storing the partial solution)
+ * if any right-hand side missed the stopping test within max_it, which the
+ * reference never reports. */
 int emfem_solve_linear_system(emfem_context *ctx, int first, int n, int max_it, double rtol,
                               const double *rtols, int *iters, double *relres);
 

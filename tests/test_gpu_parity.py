@@ -199,6 +199,33 @@ def test_pc_apply_matches_fcg_tolerance(tiny, oracle_mod):
             assert np.linalg.norm(r) <= tol * np.linalg.norm(b[:, c]) * 1.01
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_graph_replay_matches_plain_launches(tiny, precision):
+    """The captured auxiliary-space cycle gives bitwise the launches' result on every
+    replay, with eager work and synchronisation between replays."""
+    m, om = tiny
+    ctx = _ctx(m, "-inner_precision %s -B_ksp_rtol 1e-6" % precision)
+    ctx.create_linear_system(2)
+    ctx.assemble_matrix(0)
+    ctx.create_pc()
+    rng = np.random.default_rng(12)
+    ne = ctx.mesh_info()["n_edges"]
+    b = rng.standard_normal((ne, 4))
+    bd = ctx.device_array(b)
+    outs = {}
+    for graph in ("false", "true"):
+        ctx.process_options("-use_graph %s" % graph)
+        res = []
+        for rep in range(3):
+            xd = ctx.device_array(np.zeros_like(b))
+            ctx.pc_apply_b(2, bd.ptr, xd.ptr)
+            res.append(xd.numpy())
+            ctx.matshell_mult_a(2, bd.ptr, xd.ptr)  # eager work between replays
+        outs[graph] = res
+    for r in outs["true"] + outs["false"][1:]:
+        assert np.array_equal(r, outs["false"][0])
+
+
 def test_not_converged_is_reported(oracle_mod):
     m = mg.tiny_mt(n=3)
     ctx = _ctx(m, "-B_ksp_max_it 1")
