@@ -78,6 +78,61 @@ HostCsr csr_multiply(const HostCsr &A, const HostCsr &B) {
   return C;
 }
 
+// R A P row by row without forming A P: row i of the product accumulates
+// sum_k R_ik sum_j A_kj P_j. into a per-thread dense accumulator; two passes (count,
+// fill) write the CSR directly.
+HostCsr csr_triple(const HostCsr &R, const HostCsr &A, const HostCsr &P) {
+  EMF_REQUIRE(R.m == A.n && A.m == P.n, kErrArg, "csr_triple: shape mismatch");
+  HostCsr C;
+  C.n = R.n;
+  C.m = P.m;
+  C.rp.assign(R.n + 1, 0);
+  auto row = [&](int64_t i, std::vector<double> &acc, std::vector<int> &mark, std::vector<int> &cols) {
+    cols.clear();
+    for (int p = R.rp[i]; p < R.rp[i + 1]; ++p) {
+      const int k = R.ci[p];
+      const double r = R.v[p];
+      for (int q = A.rp[k]; q < A.rp[k + 1]; ++q) {
+        const int j = A.ci[q];
+        const double ra = r * A.v[q];
+        for (int s = P.rp[j]; s < P.rp[j + 1]; ++s) {
+          const int c = P.ci[s];
+          if (mark[c] != (int)i) {
+            mark[c] = (int)i;
+            acc[c] = 0.0;
+            cols.push_back(c);
+          }
+          acc[c] += ra * P.v[s];
+        }
+      }
+    }
+  };
+  parallel_ranges(R.n, [&](int64_t b, int64_t e, int) {
+    std::vector<double> acc(P.m);
+    std::vector<int> mark(P.m, -1), cols;
+    for (int64_t i = b; i < e; ++i) {
+      row(i, acc, mark, cols);
+      C.rp[i + 1] = (int)cols.size();
+    }
+  });
+  for (int i = 0; i < R.n; ++i) C.rp[i + 1] += C.rp[i];
+  C.ci.resize(C.rp[R.n]);
+  C.v.resize(C.rp[R.n]);
+  parallel_ranges(R.n, [&](int64_t b, int64_t e, int) {
+    std::vector<double> acc(P.m);
+    std::vector<int> mark(P.m, -1), cols;
+    for (int64_t i = b; i < e; ++i) {
+      row(i, acc, mark, cols);
+      std::sort(cols.begin(), cols.end());
+      for (size_t t = 0; t < cols.size(); ++t) {
+        C.ci[C.rp[i] + t] = cols[t];
+        C.v[C.rp[i] + t] = acc[cols[t]];
+      }
+    }
+  });
+  return C;
+}
+
 namespace {
 
 std::vector<double> diagonal(const HostCsr &A) {
@@ -264,18 +319,41 @@ std::vector<double> dense_inverse(const HostCsr &A) {
 void build_amg(const HostCsr &A0, const AmgParams &prm, AmgHost &H) {
   H.levels.clear();
   HostCsr A = A0;
+  auto tick = [&](const char *k, double t0) {
+    if (prm.timing) (*prm.timing)[k] += now_ms() - t0;
+  };
   while (A.n > prm.max_coarse && (int)H.levels.size() < prm.max_levels) {
+    double t0 = now_ms();
     std::vector<double> d = diagonal(A);
     std::vector<int> agg;
     int na = aggregate(A, d, prm.theta, agg);
     if (na >= A.n * 0.9) na = aggregate(A, d, 0.0, agg);  // weak graph: coarsen on all couplings
+    tick("amg_aggregate_ms", t0);
     if (na >= A.n * 0.9 || na == 0) break;
     AmgLevelHost L;
-    L.P = smoothed_prolongator(A, d, agg, na);
+    t0 = now_ms();
+    if ((int)H.levels.size() < prm.smooth_levels) {
+      L.P = smoothed_prolongator(A, d, agg, na);
+    } else {
+      // plain aggregation: one unit entry per row keeps every coarse stencil as sparse
+      // as the fine one (smoothed prolongators grew 14 -> 350+ nonzeros per row here
+      // without changing the auxiliary-space iteration count)
+      L.P.n = A.n;
+      L.P.m = na;
+      L.P.rp.resize(A.n + 1);
+      L.P.ci.resize(A.n);
+      L.P.v.assign(A.n, 1.0);
+      for (int i = 0; i <= A.n; ++i) L.P.rp[i] = i;
+      for (int i = 0; i < A.n; ++i) L.P.ci[i] = agg[i];
+    }
+    tick("amg_prolongator_ms", t0);
+    t0 = now_ms();
     L.R = csr_transpose(L.P);
     L.l1inv = l1_inverse(A);
-    HostCsr AP = csr_multiply(A, L.P);
-    HostCsr Ac = csr_multiply(L.R, AP);
+    tick("amg_transpose_ms", t0);
+    t0 = now_ms();
+    HostCsr Ac = csr_triple(L.R, A, L.P);
+    tick("amg_rap_ms", t0);
     L.A = std::move(A);
     H.levels.push_back(std::move(L));
     A = std::move(Ac);
@@ -283,10 +361,14 @@ void build_amg(const HostCsr &A0, const AmgParams &prm, AmgHost &H) {
   EMF_REQUIRE(A.n <= 4096, kErrUser,
               "auxiliary AMG stopped coarsening at " + std::to_string(A.n) + " unknowns");
   H.ncoarse = A.n;
+  const double t0 = now_ms();
   H.coarse_inv = dense_inverse(A);
+  tick("amg_coarse_ms", t0);
 }
 
-void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm, AmsHost &H) {
+void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, AmsHost &H) {
+  AmgParams prm = prm_in;
+  prm.timing = &H.timing;
   // interior vertex numbering
   std::vector<int> vmap(T.nv, -1);
   int nv0 = 0;
@@ -334,7 +416,7 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm, AmsHos
   H.timing["aux_build_ms"] += now_ms() - t0;
   {
     t0 = now_ms();
-    HostCsr AG = csr_multiply(H.G0T, csr_multiply(B, H.G0));
+    HostCsr AG = csr_triple(H.G0T, B, H.G0);
     H.timing["galerkin_ms"] += now_ms() - t0;
     t0 = now_ms();
     build_amg(AG, prm, H.amgG);
@@ -346,7 +428,7 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm, AmsHos
     H.PiT[d] = csr_transpose(H.Pi[d]);
     H.timing["aux_build_ms"] += now_ms() - t0;
     t0 = now_ms();
-    HostCsr AP = csr_multiply(H.PiT[d], csr_multiply(B, H.Pi[d]));
+    HostCsr AP = csr_triple(H.PiT[d], B, H.Pi[d]);
     H.timing["galerkin_ms"] += now_ms() - t0;
     t0 = now_ms();
     build_amg(AP, prm, H.amgPi[d]);

@@ -30,6 +30,7 @@ struct HostCsr {
 
 HostCsr csr_transpose(const HostCsr &A);
 HostCsr csr_multiply(const HostCsr &A, const HostCsr &B);
+HostCsr csr_triple(const HostCsr &R, const HostCsr &A, const HostCsr &P);
 
 struct AmgLevelHost {
   HostCsr A, P, R;           // R = P^T
@@ -46,6 +47,8 @@ struct AmgParams {
   double theta = 0.08;     // strength threshold |a_ij| >= theta sqrt(a_ii a_jj)
   int max_coarse = 256;    // dense solve below this size
   int max_levels = 20;
+  int smooth_levels = 0;   // levels with a Jacobi-smoothed prolongator (then plain aggregation)
+  std::map<std::string, double> *timing = nullptr;  // optional phase times (ms)
 };
 
 void build_amg(const HostCsr &A, const AmgParams &p, AmgHost &H);
