@@ -60,11 +60,11 @@ def run_step(ctx, m, fidx, batched=True):
 
 def cpu_baseline(budget_s=20.0):
     """The CPU FP64 oracle (same discretisation; sparse direct solve) on a bounded
-    sample of the same workload: one frequency of the adaptive-MT model on a 16^3-cell
+    sample of the same workload: one frequency of the adaptive-MT model on a 13^3-cell
     grid, assembly + 3 solves + estimator, single thread."""
     from emfem_amd import meshgen as mg
     from oracle import oracle as O
-    m = mg.adaptive_mt(cells=(16, 16, 16))
+    m = mg.adaptive_mt(cells=(13, 13, 13))
     t0 = time.perf_counter()
     om = O.Mesh(m.points, m.tets)
     fidx = 4
@@ -79,7 +79,7 @@ def cpu_baseline(budget_s=20.0):
     O.estimate_error(om, m, m.freqs[fidx], sols)
     dt = time.perf_counter() - t0
     return dict(value=om.n_edges * len(sols) / dt, unit="complex DOF*RHS/s", cores=1, kind="port",
-                sample="oracle on adaptive-MT 16^3 cells (%d edges), f=%.3g Hz, assembly + %d direct solves + "
+                sample="oracle on adaptive-MT 13^3 cells (%d edges), f=%.3g Hz, assembly + %d direct solves + "
                        "estimator, %.1f s" % (om.n_edges, m.freqs[fidx], len(sols), dt))
 
 
@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--cells", type=int, default=48, help="grid cells per axis")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain fp64 single-RHS comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="plain launches instead of hipGraph-captured cycles (for kernel tracers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -102,7 +104,8 @@ def main():
 
     t0 = time.perf_counter()
     m = mg.adaptive_mt(cells=(args.cells, args.cells, args.cells))
-    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6")
+    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6 -use_graph %s"
+                      % ("false" if args.no_graph else "true"))
     ctx.load_model(m)
     ctx.create_linear_system(3)
     info = ctx.mesh_info()
@@ -131,7 +134,7 @@ def main():
     # untimed solve with plain launches bracketed by HIP events on the solver stream.
     ctx.process_options("-use_graph false")
     ctx.solve_linear_system(0, 2, 100, 1e-8)
-    ctx.process_options("-use_graph true")
+    ctx.process_options("-use_graph %s" % ("false" if args.no_graph else "true"))
     b_ms, b_bytes = ctx.kernel_time("b_spmm")
     fcg_ms, fcg_bytes = ctx.kernel_time("fcg_spmm")
     op_ms, op_bytes = ctx.kernel_time("op_spmm")
@@ -139,7 +142,7 @@ def main():
     peak = 8000.0
     bw = (b_bytes / (b_ms * 1e-3) / 1e9) if b_ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(bw, 1), "peak": peak, "unit": "GB/s", "frac": round(bw / peak, 4),
-            "traffic": None, "kernel": "k_spmm (B = C + M, fp32, 6 columns)", "launch_ms": round(b_ms, 5),
+            "traffic": None, "kernel": "k_spmm_narrow role 1 (B = C + M, fp32, XY+YX = 4 real columns)", "launch_ms": round(b_ms, 5),
             "algorithmic_bytes": b_bytes}
     extra = {
         "op_spmm_GBps": round(op_bytes / (op_ms * 1e-3) / 1e9, 1) if op_ms > 0 else None,

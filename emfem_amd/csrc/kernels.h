@@ -19,7 +19,15 @@ struct Csr {
   const int *rp = nullptr;
   const int *ci = nullptr;
   const T *v = nullptr;
+  int lpr = 0;  // lanes per row of the narrow SpMM kernel; 0 = chosen from the row length
+  int role = 0; // 1 = edge-space smoother operator (a separate kernel instance in traces)
 };
+
+// Picks the fastest lanes-per-row (4, 8, 16, 32) of the narrow SpMM for A on nc
+// columns by timing each with HIP events on stream s; x and y are scratch blocks
+// of [A.ncols][nc] and [A.nrows][nc].  Returns 0 when nc uses the wide kernel.
+template <typename TA, typename TX, typename T>
+int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y);
 
 // y = rs .* (alpha * (A x) + gamma * w) + beta * z   (rows of A; rs may be null = 1,
 // w / z may be null when their coefficient is 0).  y may alias z (same row only) but

@@ -186,6 +186,7 @@ class DevAms {
     bvals_.resize(nnz);
     make_b<T>(s, nnz, K, bvals_.get());
     B_ = view(bvals_.get());
+    B_.role = 1;
     B64_.nrows = B64_.ncols = ne;
     B64_.nnz = nnz;
     B64_.rp = rp;
@@ -230,6 +231,15 @@ class DevAms {
     for (int d = 0; d < 3; ++d) amgPi_[d].ensure(nc);
   }
 
+  // Lanes per row of the smoother SpMM for nc columns, timed once per nc (outside
+  // any graph capture) on scratch blocks.
+  void tune(hipStream_t s, int nc) {
+    ensure(nc);
+    if (lpr_.count(nc)) return;
+    d_.zero(s);
+    lpr_[nc] = tune_lpr<T, T, T>(s, B_, nc, d_.get(), rres_.get());
+  }
+
   ~DevAms() {
     for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   }
@@ -238,7 +248,7 @@ class DevAms {
   // use_graph the ~100 launches of one cycle are captured once per (nc, b, out)
   // and replayed as a single hipGraph launch (no memset nodes are captured).
   void apply(hipStream_t s, int nc, const double *b, T *out, KernelClock *clk, bool use_graph) {
-    ensure(nc);
+    tune(s, nc);
     if (!use_graph) {
       launch_cycle(s, nc, b, out, clk);
       return;
@@ -291,7 +301,9 @@ class DevAms {
                   const T *z, KernelClock *clk) {
     const bool t = clk && clk->armed();
     if (t) clk->start(s);
-    spmm<T>(s, B_, nc, x, nc, y, nc, alpha, rs, gamma, w, nc, beta, z, nc);
+    Csr<T> B = B_;
+    B.lpr = lpr_[nc];
+    spmm<T>(s, B, nc, x, nc, y, nc, alpha, rs, gamma, w, nc, beta, z, nc);
     if (t) clk->stop(s);
   }
 
@@ -328,6 +340,7 @@ class DevAms {
   DevCsr<T> G0_, G0T_, Pi_[3], PiT_[3];
   DevAmg<T> amgG_, amgPi_[3];
   std::map<std::tuple<int, const void *, const void *>, hipGraphExec_t> graphs_;
+  std::map<int, int> lpr_;
 };
 
 // ---------------------------------------------------------------------------------

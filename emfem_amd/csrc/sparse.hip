@@ -30,8 +30,48 @@ __device__ __forceinline__ T shfl_xor(T v, int mask, int width) {
 // real CSR x block
 // ------------------------------------------------------------------------------
 
-// Matrix values TA, input block TX, output/accumulation TY.
-template <typename TA, typename TX, typename T, int NC, int LPR>
+// Sums the NC per-lane partials over the LPR lanes of a row.  For power-of-two NC
+// the sum is reduce-scattered (each exchange halves the values a lane keeps, so
+// NC/2 + NC/4 + ... shuffles instead of NC per step); the lane then owns columns
+// [col0, col0 + KEND).  Other NC use a full butterfly and every lane owns all.
+template <typename T, int NC, int LPR>
+__device__ __forceinline__ int row_reduce(T (&acc)[NC], int lane) {
+  constexpr bool kScatter = (NC & (NC - 1)) == 0 && NC > 1;
+  int col0 = 0;
+  if constexpr (kScatter) {
+    int K = NC;
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+      if (K > 1) {
+        const bool hi = (lane & off) != 0;
+        const int h = K / 2;
+#pragma unroll
+        for (int i = 0; i < NC / 2; ++i) {
+          if (i < h) {
+            const T send = hi ? acc[i] : acc[h + i];
+            const T keep = hi ? acc[h + i] : acc[i];
+            acc[i] = keep + shfl_xor(send, off, LPR);
+          }
+        }
+        col0 += hi ? h : 0;
+        K = h;
+      } else {
+        acc[0] += shfl_xor(acc[0], off, LPR);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] += shfl_xor(acc[j], off, LPR);
+    }
+  }
+  return col0;
+}
+
+// Matrix values TA, input block TX, output/accumulation T.  ROLE only separates
+// the edge-space smoother launches (ROLE 1) from the rest in kernel traces.
+template <typename TA, typename TX, typename T, int NC, int LPR, int ROLE>
 __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__restrict__ rp,
                                                         const int *__restrict__ ci, const TA *__restrict__ av,
                                                         const TX *__restrict__ x, int ldx, T *y, int ldy, T alpha,
@@ -45,26 +85,46 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
 #pragma unroll
   for (int j = 0; j < NC; ++j) acc[j] = T(0);
   const int e = rp[row + 1];
-  for (int p = rp[row] + lane; p < e; p += LPR) {
-    const T a = (T)av[p];
-    const TX *xr = x + (size_t)ci[p] * ldx;
+  // two nonzeros per lane per trip: both column loads, then both row gathers, are
+  // in flight together
+  for (int p = rp[row] + lane; p < e; p += 2 * LPR) {
+    const bool two = p + LPR < e;
+    const int c0 = ci[p];
+    const int c1 = two ? ci[p + LPR] : c0;
+    const T a0 = (T)av[p];
+    const T a1 = two ? (T)av[p + LPR] : T(0);
+    const TX *x0 = x + (size_t)c0 * ldx;
+    const TX *x1 = x + (size_t)c1 * ldx;
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] += a * (T)xr[j];
+    for (int j = 0; j < NC; ++j) acc[j] += a0 * (T)x0[j] + a1 * (T)x1[j];
   }
-#pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) {
-#pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] += shfl_xor(acc[j], off, LPR);
-  }
+  const int col0 = row_reduce<T, NC, LPR>(acc, lane);
   const T scale = rs ? rs[row] : T(1);
+  constexpr bool kScatter = (NC & (NC - 1)) == 0 && NC > 1;
+  if constexpr (kScatter) {
+    // lanes that own a column after the reduce-scatter
+    constexpr int kOwn = NC >= LPR ? NC / LPR : 1;
+    constexpr int kRep = NC >= LPR ? 1 : LPR / NC;
+    if ((lane & (kRep - 1)) != 0) return;
 #pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    if (lane == (j % LPR)) {
-      T v = alpha * acc[j];
+    for (int i = 0; i < kOwn; ++i) {
+      const int j = col0 + i;
+      T v = alpha * acc[i];
       if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
       v *= scale;
       if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
       y[(size_t)row * ldy + j] = v;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      if (lane == (j % LPR)) {
+        T v = alpha * acc[j];
+        if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
+        v *= scale;
+        if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
+        y[(size_t)row * ldy + j] = v;
+      }
     }
   }
 }
@@ -110,12 +170,16 @@ template <typename TA, typename TX, typename T, int NC>
 void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
                    T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
   const double avg = A.nrows ? double(A.nnz) / A.nrows : 0.0;
-  const int lpr = avg <= 6 ? 4 : (avg <= 20 ? 8 : (avg <= 48 ? 16 : 32));
+  const int lpr = A.lpr > 0 ? A.lpr : (avg <= 6 ? 4 : (avg <= 20 ? 8 : (avg <= 48 ? 16 : 32)));
   const int64_t threads = (int64_t)A.nrows * lpr;
   const int g = grid_for(threads);
-#define EMF_NARROW(L)                                                                                              \
-  hipLaunchKernelGGL((k_spmm_narrow<TA, TX, T, NC, L>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, \
-                     y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz)
+#define EMF_NARROW(L)                                                                                        \
+  if (A.role == 1)                                                                                           \
+    hipLaunchKernelGGL((k_spmm_narrow<TA, TX, T, NC, L, 1>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, \
+                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz);                         \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_spmm_narrow<TA, TX, T, NC, L, 0>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, \
+                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz)
   switch (lpr) {
     case 4: EMF_NARROW(4); break;
     case 8: EMF_NARROW(8); break;
@@ -177,9 +241,39 @@ void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T
   EMF_HIP(hipGetLastError());
 }
 
+template <typename TA, typename TX, typename T>
+int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
+  if (nc >= 16 || A.nrows == 0) return 0;
+  hipEvent_t e0, e1;
+  EMF_HIP(hipEventCreate(&e0));
+  EMF_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int lpr : {4, 8, 16, 32}) {
+    Csr<TA> B = A;
+    B.lpr = lpr;
+    spmm_mixed<TA, TX, T>(s, B, nc, x, nc, y, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+    EMF_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < 5; ++r)
+      spmm_mixed<TA, TX, T>(s, B, nc, x, nc, y, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+    EMF_HIP(hipEventRecord(e1, s));
+    EMF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    EMF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = lpr;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
 #define EMF_SPMM_INST(TA, TX, T)                                                                                    \
   template void spmm_mixed<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, int, T *, int, T, const T *, T, \
-                                      const T *, int, T, const T *, int);
+                                      const T *, int, T, const T *, int);                                           \
+  template int tune_lpr<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, T *);
 EMF_SPMM_INST(float, float, float)
 EMF_SPMM_INST(double, double, double)
 EMF_SPMM_INST(double, float, double)

@@ -110,8 +110,8 @@ int emfem_update_background_model(emfem_context *ctx);
 /* ---- linear system ---------------------------------------------------------- */
 
 /* create_linear_system (src/em_fwd.cc:248-295): builds the CSR pattern
- * (src/em_fwd.cc:191-246) on the device and allocates n_slots right-hand-side and
- * solution columns. */
+ * (src/em_fwd.cc:191-246) on the host, uploads it, and allocates n_slots
+ * right-hand-side and solution columns on the device. */
 int emfem_create_linear_system(emfem_context *ctx, int n_slots);
 
 /* destroy_linear_system (src/em_fwd.cc:297-316). */
