@@ -83,6 +83,56 @@ def cpu_baseline(budget_s=20.0):
                        "estimator, %.1f s" % (om.n_edges, m.freqs[fidx], len(sols), dt))
 
 
+def csem_leg(api, mg, cells, n_tx=64, n_plain=4, fidx=1):
+    """Marine-CSEM batched multi-RHS solve (one frequency, n_tx x-directed dipoles as
+    one block of right-hand sides, fp32 inner cycle in fp64 FGMRES) against the plain
+    fp64 single-RHS path on the first n_plain transmitters of the same operator.
+    Returns DOF*RHS/s of both, the speed-up, and the largest receiver-field difference
+    between the two paths relative to the field's norm over the receivers."""
+    m = mg.marine_csem(cells=cells, n_tx=n_tx)
+    out = {}
+    fields = {}
+    for name, opts, batched in (("mixed_batched", "-inner_precision fp32", True),
+                                ("plain_fp64_single_rhs", "-inner_precision fp64", False)):
+        ctx = api.Context(opts + " -e_rtol 1e-8")
+        ctx.load_model(m)
+        nt = n_tx if batched else n_plain
+        ctx.create_linear_system(nt)
+        E = ctx.mesh_info()["n_edges"]
+        ctx.assemble_matrix(fidx)
+        t0 = time.perf_counter()
+        ctx.create_pc()
+        t1 = time.perf_counter()
+        for t in range(nt):
+            ctx.assemble_rhs_csem(fidx, t, t)
+        t2 = time.perf_counter()
+        if batched:
+            its, rr = ctx.solve_linear_system(0, nt, 200, 1e-8)
+            its, rr = its.tolist(), rr.tolist()
+        else:
+            its, rr = [], []
+            for t in range(nt):
+                i, r = ctx.solve_linear_system(t, 1, 200, 1e-8)
+                its += i.tolist()
+                rr += r.tolist()
+        t3 = time.perf_counter()
+        fields[name] = [np.concatenate(ctx.interpolate_fields(fidx, t, m.rx), axis=1) for t in range(n_plain)]
+        out[name] = {"dof_rhs_per_s": round(E * nt / (t3 - t2), 1), "rhs": nt, "solve_s": round(t3 - t2, 3),
+                     "pc_s": round(t1 - t0, 3), "max_outer_its": int(max(its)), "max_true_relres": float(max(rr))}
+        log("csem %s: E=%d rhs=%d pc %.2fs solve %.2fs its max %d relres max %.1e" %
+            (name, E, nt, t1 - t0, t3 - t2, max(its), max(rr)))
+        ctx.close()
+    err = 0.0
+    for a, b in zip(fields["mixed_batched"], fields["plain_fp64_single_rhs"]):
+        for c in range(a.shape[1]):
+            err = max(err, float(np.linalg.norm(a[:, c] - b[:, c]) / max(np.linalg.norm(b[:, c]), 1e-300)))
+    out["workload"] = ("marine CSEM %dx%dx%d cells, %d edges, f=%.2f Hz, %d x-dipoles batched"
+                       % (cells[0], cells[1], cells[2], E, m.freqs[fidx], n_tx))
+    out["speedup"] = round(out["mixed_batched"]["dof_rhs_per_s"] / out["plain_fp64_single_rhs"]["dof_rhs_per_s"], 3)
+    out["max_receiver_rel_diff"] = err
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +141,8 @@ def main():
     ap.add_argument("--cells", type=int, default=48, help="grid cells per axis")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain fp64 single-RHS comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-csem", action="store_true", help="skip the batched marine-CSEM comparison")
+    ap.add_argument("--csem-cells", type=str, default="48,48,40", help="marine-CSEM grid cells x,y,z")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of hipGraph-captured cycles (for kernel tracers)")
     args = ap.parse_args()
@@ -168,6 +220,8 @@ def main():
         extra["speedup_vs_plain_step"] = round(p_total / mixed_same, 3)
         extra["speedup_vs_plain_solve"] = round(p_solve / mixed_solve, 3)
         ctxp.close()
+    if not args.no_csem:
+        extra["csem_batched_64tx"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")))
     cpu = None
     if not args.no_cpu:
         try:

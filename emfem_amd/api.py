@@ -79,6 +79,8 @@ SIGNATURES = {
     "emfem_set_observations": (I, [P, I, P, P]),
     "emfem_save_rsp": (I, [P, S]),
     "emfem_get_responses": (I, [P, P]),
+    "emfem_locate_points": (I, [P, I, P, P]),
+    "emfem_save_mesh": (I, [P, S, I, P, P]),
     "emfem_forward": (I, [P]),
     "emfem_refine_mesh": (I, [P, P]),
     "emfem_refine_uniform": (I, [P]),
@@ -226,6 +228,13 @@ class Context:
         q = _c(p, np.float64)
         return int(self._L.emfem_find_cell(self._h, _p(q)))
 
+    def locate_points(self, points):
+        """Batch point location on the device (find_cell_around_point per point)."""
+        q = _c(np.asarray(points, dtype=np.float64).reshape(-1, 3), np.float64)
+        out = np.empty(q.shape[0], dtype=np.int32)
+        self._check(self._L.emfem_locate_points(self._h, q.shape[0], _p(q), _p(out)))
+        return out
+
     def update_background_model(self):
         self._check(self._L.emfem_update_background_model(self._h))
 
@@ -357,6 +366,14 @@ class Context:
 
     def save_rsp(self, path):
         self._check(self._L.emfem_save_rsp(self._h, str(path).encode()))
+
+    def save_mesh(self, path, tidx, error0, error1=None):
+        """save_mesh (src/em_io.cc:289-314): legacy VTK with error indicators
+        (MT: tidx < 0, error0 = XY, error1 = YX; CSEM: error0)."""
+        e0 = _c(error0, np.float64)
+        e1 = _c(error1, np.float64) if error1 is not None else None
+        self._check(self._L.emfem_save_mesh(self._h, str(path).encode(), int(tidx), _p(e0),
+                                            _p(e1) if e1 is not None else None))
 
     def responses(self, n_obs=None):
         n = self._nobs if n_obs is None else n_obs

@@ -10,22 +10,56 @@
 
 namespace emfem {
 
+// Two-pass transpose over row chunks: per-chunk column counts, then each chunk
+// fills its rows at its own offsets, so every column lists rows in ascending order.
 HostCsr csr_transpose(const HostCsr &A) {
   HostCsr T;
   T.n = A.m;
   T.m = A.n;
   T.rp.assign(A.m + 1, 0);
-  for (int c : A.ci) T.rp[c + 1]++;
-  for (int i = 0; i < A.m; ++i) T.rp[i + 1] += T.rp[i];
   T.ci.resize(A.ci.size());
   T.v.resize(A.v.size());
-  std::vector<int> fill(T.rp.begin(), T.rp.end() - 1);
-  for (int r = 0; r < A.n; ++r)
-    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
-      int q = fill[A.ci[p]]++;
-      T.ci[q] = r;
-      T.v[q] = A.v[p];
+  const int nch = A.ci.size() < 200000 ? 1 : host_threads();
+  std::vector<std::vector<int>> off(nch, std::vector<int>(A.m, 0));
+  auto run = [&](auto &&f) {
+    if (nch == 1) {
+      f(0);
+      return;
     }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nch; ++t) th.emplace_back([&, t]() { f(t); });
+    for (auto &x : th) x.join();
+  };
+  auto rows = [&](int t, int &b, int &e) {
+    b = (int)((int64_t)A.n * t / nch);
+    e = (int)((int64_t)A.n * (t + 1) / nch);
+  };
+  run([&](int t) {
+    int b, e;
+    rows(t, b, e);
+    int *cnt = off[t].data();
+    for (int p = A.rp[b]; p < A.rp[e]; ++p) cnt[A.ci[p]]++;
+  });
+  int run_sum = 0;
+  for (int c = 0; c < A.m; ++c) {
+    for (int t = 0; t < nch; ++t) {
+      const int k = off[t][c];
+      off[t][c] = run_sum;
+      run_sum += k;
+    }
+    T.rp[c + 1] = run_sum;
+  }
+  run([&](int t) {
+    int b, e;
+    rows(t, b, e);
+    int *pos = off[t].data();
+    for (int r = b; r < e; ++r)
+      for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
+        const int q = pos[A.ci[p]]++;
+        T.ci[q] = r;
+        T.v[q] = A.v[p];
+      }
+  });
   return T;
 }
 
@@ -381,59 +415,106 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, Ams
     G.n = T.ne;
     G.m = nv0;
     G.rp.assign(T.ne + 1, 0);
-    for (int e = 0; e < T.ne; ++e) {
-      const int a = T.edgelist[e * 2], b = T.edgelist[e * 2 + 1];
-      double va, vb;
-      if (comp < 0) {
-        va = -1.0;  // G row e: -1 at the lower vertex, +1 at the higher (src/em_la.cc:264-267)
-        vb = 1.0;
-      } else {
-        const double t = T.pts[(size_t)b * 3 + comp] - T.pts[(size_t)a * 3 + comp];
-        va = vb = 0.5 * t;
-      }
-      if (vmap[a] >= 0 && vmap[b] >= 0 && vmap[a] > vmap[b]) {
-        G.ci.push_back(vmap[b]);
-        G.v.push_back(vb);
-        G.ci.push_back(vmap[a]);
-        G.v.push_back(va);
-      } else {
-        if (vmap[a] >= 0) {
-          G.ci.push_back(vmap[a]);
-          G.v.push_back(va);
+    for (int e = 0; e < T.ne; ++e)
+      G.rp[e + 1] = G.rp[e] + (vmap[T.edgelist[e * 2]] >= 0) + (vmap[T.edgelist[e * 2 + 1]] >= 0);
+    G.ci.resize(G.rp[T.ne]);
+    G.v.resize(G.rp[T.ne]);
+    parallel_ranges(T.ne, [&](int64_t eb, int64_t ee, int) {
+      for (int64_t e = eb; e < ee; ++e) {
+        const int a = T.edgelist[e * 2], b = T.edgelist[e * 2 + 1];
+        double va, vb;
+        if (comp < 0) {
+          va = -1.0;  // G row e: -1 at the lower vertex, +1 at the higher (src/em_la.cc:264-267)
+          vb = 1.0;
+        } else {
+          const double t = T.pts[(size_t)b * 3 + comp] - T.pts[(size_t)a * 3 + comp];
+          va = vb = 0.5 * t;
         }
-        if (vmap[b] >= 0) {
-          G.ci.push_back(vmap[b]);
-          G.v.push_back(vb);
+        int q = G.rp[e];
+        // columns ascending within the row
+        const bool swap = vmap[a] >= 0 && vmap[b] >= 0 && vmap[a] > vmap[b];
+        const int c0 = swap ? b : a, c1 = swap ? a : b;
+        const double v0 = swap ? vb : va, v1 = swap ? va : vb;
+        if (vmap[c0] >= 0) {
+          G.ci[q] = vmap[c0];
+          G.v[q++] = v0;
+        }
+        if (vmap[c1] >= 0) {
+          G.ci[q] = vmap[c1];
+          G.v[q++] = v1;
         }
       }
-      G.rp[e + 1] = (int)G.ci.size();
-    }
+    });
     return G;
   };
   double t0 = now_ms();
   H.G0 = make(-1);
   H.G0T = csr_transpose(H.G0);
   H.timing["aux_build_ms"] += now_ms() - t0;
-  {
+  t0 = now_ms();
+  HostCsr AG = csr_triple(H.G0T, B, H.G0);
+  H.timing["galerkin_ms"] += now_ms() - t0;
+  std::vector<HostCsr> P(3), A(3);
+  for (int d = 0; d < 3; ++d) {
     t0 = now_ms();
-    HostCsr AG = csr_triple(H.G0T, B, H.G0);
+    P[d] = make(d);
+    const HostCsr PT = csr_transpose(P[d]);
+    H.timing["aux_build_ms"] += now_ms() - t0;
+    t0 = now_ms();
+    A[d] = csr_triple(PT, B, P[d]);
     H.timing["galerkin_ms"] += now_ms() - t0;
+  }
+  H.additive = prm.additive;
+  std::vector<const HostCsr *> pb, ab;
+  if (prm.additive) {
+    pb.push_back(&H.G0);
+    ab.push_back(&AG);
+  } else {
     t0 = now_ms();
     build_amg(AG, prm, H.amgG);
     H.timing["amg_ms"] += now_ms() - t0;
   }
   for (int d = 0; d < 3; ++d) {
-    t0 = now_ms();
-    H.Pi[d] = make(d);
-    H.PiT[d] = csr_transpose(H.Pi[d]);
-    H.timing["aux_build_ms"] += now_ms() - t0;
-    t0 = now_ms();
-    HostCsr AP = csr_triple(H.PiT[d], B, H.Pi[d]);
-    H.timing["galerkin_ms"] += now_ms() - t0;
-    t0 = now_ms();
-    build_amg(AP, prm, H.amgPi[d]);
-    H.timing["amg_ms"] += now_ms() - t0;
+    pb.push_back(&P[d]);
+    ab.push_back(&A[d]);
   }
+  H.nblk = (int)pb.size();
+  t0 = now_ms();
+  const int nb = H.nblk;
+  H.Pi.n = T.ne;
+  H.Pi.m = nb * nv0;
+  H.Pi.rp.assign(T.ne + 1, 0);
+  for (int e = 0; e < T.ne; ++e) {
+    int k = 0;
+    for (const HostCsr *q : pb) k += q->rp[e + 1] - q->rp[e];
+    H.Pi.rp[e + 1] = H.Pi.rp[e] + k;
+  }
+  H.Pi.ci.resize(H.Pi.rp[T.ne]);
+  H.Pi.v.resize(H.Pi.rp[T.ne]);
+  parallel_ranges(T.ne, [&](int64_t eb, int64_t ee, int) {
+    for (int64_t e = eb; e < ee; ++e) {
+      int q = H.Pi.rp[e];
+      for (int d = 0; d < nb; ++d)
+        for (int k = pb[d]->rp[e]; k < pb[d]->rp[e + 1]; ++k) {
+          H.Pi.ci[q] = d * nv0 + pb[d]->ci[k];
+          H.Pi.v[q++] = pb[d]->v[k];
+        }
+    }
+  });
+  H.PiT = csr_transpose(H.Pi);
+  HostCsr AP;
+  AP.n = AP.m = nb * nv0;
+  AP.rp.assign(AP.n + 1, 0);
+  for (int d = 0, base = 0; d < nb; base += ab[d]->rp[nv0], ++d) {
+    for (int r = 0; r < nv0; ++r) AP.rp[d * nv0 + r + 1] = base + ab[d]->rp[r + 1];
+    AP.ci.insert(AP.ci.end(), ab[d]->ci.begin(), ab[d]->ci.end());
+    AP.v.insert(AP.v.end(), ab[d]->v.begin(), ab[d]->v.end());
+    for (size_t k = base; k < AP.ci.size(); ++k) AP.ci[k] += d * nv0;
+  }
+  H.timing["aux_build_ms"] += now_ms() - t0;
+  t0 = now_ms();
+  build_amg(AP, prm, H.amgPi);
+  H.timing["amg_ms"] += now_ms() - t0;
   H.l1inv = l1_inverse(B);
 }
 

@@ -5,7 +5,7 @@
 // not part of the reference.  This is a from-scratch auxiliary-space
 // preconditioner in the same family: Chebyshev smoothing on the edge space plus
 // corrections in the nodal gradient space G and the nodal vector space Pi, each
-// solved approximately by smoothed-aggregation AMG.  Boundary vertices are left out
+// solved approximately by one V-cycle of aggregation AMG.  Boundary vertices are left out
 // of the auxiliary spaces because the Dirichlet edges they span are decoupled in B
 // (src/em_fwd.cc:377-388).  The set-up (aggregation, prolongators, Galerkin
 // products, dense coarsest inverse) runs here on the host in fp64; the cycles run on
@@ -48,15 +48,27 @@ struct AmgParams {
   int max_coarse = 256;    // dense solve below this size
   int max_levels = 20;
   int smooth_levels = 0;   // levels with a Jacobi-smoothed prolongator (then plain aggregation)
+  bool additive = true;    // auxiliary cycle 0-(G+Pi)-0 instead of 0-G-Pi-G-0
   std::map<std::string, double> *timing = nullptr;  // optional phase times (ms)
 };
 
 void build_amg(const HostCsr &A, const AmgParams &p, AmgHost &H);
 
 struct AmsHost {
-  HostCsr G0, G0T;        // E x V0 gradient restricted to interior vertices, and transpose
-  HostCsr Pi[3], PiT[3];  // E x V0 nodal vector interpolation per component
-  AmgHost amgG, amgPi[3];
+  // E x V0 gradient restricted to interior vertices, its transpose and the AMG of
+  // G0^T B G0 (multiplicative cycle only).
+  HostCsr G0, G0T;
+  AmgHost amgG;
+  // Auxiliary blocks side by side: E x (nblk V0), column k * V0 + v for block k of
+  // interior vertex v.  Blocks are the three nodal-vector components Pi_d
+  // (multiplicative cycle) or G0 followed by Pi_x, Pi_y, Pi_z (additive cycle).
+  // Corrections of these blocks see the same residual, so their auxiliary problems
+  // form one block-diagonal matrix diag(P_k^T B P_k) with one AMG hierarchy whose
+  // aggregates never cross blocks.
+  HostCsr Pi, PiT;
+  AmgHost amgPi;
+  int nblk = 3;
+  bool additive = false;
   std::vector<double> l1inv;  // edge-space l1 row norms of B, inverted
   int nv0 = 0;
   std::map<std::string, double> timing;  // set-up phase times (ms)

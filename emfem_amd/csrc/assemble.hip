@@ -15,6 +15,7 @@
 
 #include "common.h"
 #include "fe.h"
+#include "device.h"
 #include "kernels.h"
 
 namespace emfem {
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ntets, const int *__res
                                                      const double *__restrict__ sig6, const uint8_t *__restrict__ edge_bdr,
                                                      double omega, const int *__restrict__ rp,
                                                      const int *__restrict__ ci, double2 *K) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_block() * kBlock + threadIdx.x;
   if (i >= ntets) return;
   const int t = tet_ids[i];
   int vid[4];
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(128) void k_estimate(int nt, const int *__restrict_
                                                   const double *__restrict__ sig6, double omega, int ns,
                                                   const int *__restrict__ slots, const double *__restrict__ sol, int ld,
                                                   double *errors) {
-  const int t = blockIdx.x * 128 + threadIdx.x;
+  const int t = xcd_block() * 128 + threadIdx.x;
   if (t >= nt) return;
   TetData D;
   load_full(t, tets, tet2edge, pts, attr, sig6, D);

@@ -13,6 +13,7 @@
 
 #include "amg.h"
 #include "common.h"
+#include "locate.h"
 #include "model.h"
 #include "solver.h"
 #include "topology.h"
@@ -42,7 +43,7 @@ struct emfem_context {
   std::vector<int> coarse_tets, coarse_attr;
 
   // device mesh
-  DevBuf<int> d_tets, d_t2e, d_attr, d_nb, d_non, d_colour;
+  DevBuf<int> d_tets, d_t2e, d_attr, d_nb, d_non, d_colour, d_v2t;
   DevBuf<double> d_pts, d_s6;
   DevBuf<uint8_t> d_ebdr;
 
@@ -115,7 +116,31 @@ inline void upload_mesh(emfem_context *c) {
   c->d_colour.upload(T.color_order, s);
   c->d_pts.upload(T.pts, s);
   c->d_ebdr.upload(T.edge_bdr, s);
+  c->d_v2t.upload(T.v2t, s);
   EMF_HIP(hipStreamSynchronize(s));
+}
+
+// Batch point location: on the device (locate.hip) when the context has one, else
+// the host search of a host-only context.
+inline Locate locator(emfem_context *c) {
+  if (!c->device) return host_locate(c->T);
+  return [c](const std::vector<double> &pts) {
+    const int n = (int)(pts.size() / 3);
+    std::vector<int> out(n);
+    if (n == 0) return out;
+    hipStream_t s = c->stream;
+    DevBuf<double> q;
+    q.upload(pts, s);
+    DevBuf<int> nearest((size_t)n), queue((size_t)n * kLocateQueue), res((size_t)n);
+    locate_points(s, n, q.get(), c->T.nv, c->d_pts.get(), c->d_tets.get(), c->d_nb.get(), c->d_v2t.get(),
+                  nearest.get(), queue.get(), res.get());
+    res.download(out.data(), (size_t)n, s);
+    for (int i = 0; i < n; ++i)
+      EMF_REQUIRE(out[i] != -2, kErrUser,
+                  "point location queued " + std::to_string(kLocateQueue) + " tets without an answer for point " +
+                      std::to_string(i));
+    return out;
+  };
 }
 
 inline void check_attributes(emfem_context *c) {
@@ -170,6 +195,7 @@ inline PcOptions pc_options(const emfem_context *c) {
   o.inner_rtol = c->optd("B_ksp_rtol");
   o.inner_max_it = c->opti("B_ksp_max_it");
   o.amg.theta = c->optd("ams_theta");
+  o.amg.additive = c->opt("ams_cycle") == "additive";
   o.use_graph = c->opt("use_graph") == "true";
   return o;
 }

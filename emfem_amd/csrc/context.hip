@@ -69,6 +69,7 @@ const std::map<std::string, std::string> &option_defaults() {
       {"ams_cheb_degree", "3"},
       {"ams_cheb_ratio", "30"},
       {"ams_theta", "0.08"},
+      {"ams_cycle", "additive"},
   };
   return d;
 }
@@ -83,6 +84,8 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"dual_precision", {"fp32", "fp64"}},
       {"fp64_fallback", {"true", "false"}},
       {"use_graph", {"true", "false"}},
+      {"ams_cycle", {"multiplicative", "additive"}},
+      {"save_mesh", {"true", "false"}},
   };
   static const std::set<std::string> none;
   auto it = e.find(key);
@@ -282,6 +285,15 @@ int emfem_find_cell(const emfem_context *c, const double *p) {
   return find_cell(c->T, p);
 }
 
+int emfem_locate_points(emfem_context *c, int n, const double *points, int *tets) {
+  return guarded(c, [&]() {
+    require_mesh(c);
+    EMF_REQUIRE(n >= 0 && (n == 0 || (points && tets)), kErrArg, "bad point arrays");
+    const std::vector<int> cells = locator(c)(std::vector<double>(points, points + (size_t)n * 3));
+    std::copy(cells.begin(), cells.end(), tets);
+  });
+}
+
 int emfem_update_background_model(emfem_context *c) {
   return guarded(c, [&]() {
     require_mesh(c);
@@ -403,7 +415,7 @@ int emfem_assemble_rhs_csem(emfem_context *c, int fidx, int tidx, int slot) {
     require_freq(c, fidx);
     EMF_REQUIRE(tidx >= 0 && (size_t)tidx * 7 < c->tx.size(), kErrArg, "transmitter index out of range");
     std::vector<double> re, im;
-    rhs_csem(c->T, c->freqs[fidx], &c->tx[(size_t)tidx * 7], c->opti("n_tx_divisions"), re, im);
+    rhs_csem(c->T, c->freqs[fidx], &c->tx[(size_t)tidx * 7], c->opti("n_tx_divisions"), locator(c), re, im);
     put_slot(c, c->d_S, slot, re.data(), im.data());
   });
 }
@@ -412,7 +424,7 @@ int emfem_assemble_rhs_dual(emfem_context *c, int slot) {
   return guarded(c, [&]() {
     require_slot(c, slot);
     std::vector<double> re, im;
-    rhs_dual(c->T, c->rx, re, im);
+    rhs_dual(c->T, c->rx, locator(c), re, im);
     put_slot(c, c->d_S, slot, re.data(), im.data());
   });
 }
@@ -599,8 +611,9 @@ int emfem_interpolate_fields(emfem_context *c, int fidx, int slot, int n, const 
     require_freq(c, fidx);
     EMF_REQUIRE(n >= 0 && (n == 0 || (points && E && H)), kErrArg, "bad interpolation arguments");
     const std::vector<double> sol = solution_ri(c, slot);
+    const std::vector<int> cells = locator(c)(std::vector<double>(points, points + (size_t)n * 3));
     for (int i = 0; i < n; ++i) {
-      const int t = find_cell(c->T, points + (size_t)i * 3);
+      const int t = cells[i];
       EMF_REQUIRE(t >= 0, kErrUser, "Receiver " + std::to_string(i) + " is not found in mesh.");
       cplx Ev[3], Hv[3];
       interpolate(c->T, t, points + (size_t)i * 3, c->freqs[fidx], sol.data(), Ev, Hv);
@@ -623,6 +636,7 @@ int emfem_calculate_response_mt(emfem_context *c, int fidx, int slot_xy, int slo
     EMF_REQUIRE(n_obs >= 0 && (n_obs == 0 || (obs && rsp)), kErrArg, "bad observation arrays");
     const int nrx = (int)c->rx.size() / 3;
     const std::vector<double> exy_sol = solution_ri(c, slot_xy), eyx_sol = solution_ri(c, slot_yx);
+    const std::vector<int> cells = n_obs > 0 ? locator(c)(c->rx) : std::vector<int>();
     std::map<int, std::array<cplx, 12>> cache;
     const double f = c->freqs[fidx];
     for (int o = 0; o < n_obs; ++o) {
@@ -631,7 +645,7 @@ int emfem_calculate_response_mt(emfem_context *c, int fidx, int slot_xy, int slo
       EMF_REQUIRE(ri >= 0 && ri < nrx, kErrArg, "receiver index out of range");
       if (!cache.count(ri)) {
         const double *p = &c->rx[(size_t)ri * 3];
-        const int t = find_cell(c->T, p);
+        const int t = cells[ri];
         EMF_REQUIRE(t >= 0, kErrUser, "Receiver " + std::to_string(ri) + " is not found in mesh.");
         std::array<cplx, 12> v;
         interpolate(c->T, t, p, f, exy_sol.data(), &v[0], &v[3]);
@@ -656,6 +670,7 @@ int emfem_calculate_response_csem(emfem_context *c, int fidx, int tidx, int slot
     EMF_REQUIRE(n_obs >= 0 && (n_obs == 0 || (obs && rsp)), kErrArg, "bad observation arrays");
     const int nrx = (int)c->rx.size() / 3;
     const std::vector<double> sol = solution_ri(c, slot);
+    const std::vector<int> cells = n_obs > 0 ? locator(c)(c->rx) : std::vector<int>();
     std::map<int, std::array<cplx, 6>> cache;
     const double f = c->freqs[fidx];
     for (int o = 0; o < n_obs; ++o) {
@@ -664,7 +679,7 @@ int emfem_calculate_response_csem(emfem_context *c, int fidx, int tidx, int slot
       EMF_REQUIRE(ri >= 0 && ri < nrx, kErrArg, "receiver index out of range");
       if (!cache.count(ri)) {
         const double *p = &c->rx[(size_t)ri * 3];
-        const int t = find_cell(c->T, p);
+        const int t = cells[ri];
         EMF_REQUIRE(t >= 0, kErrUser, "Receiver " + std::to_string(ri) + " is not found in mesh.");
         std::array<cplx, 6> v;
         interpolate(c->T, t, p, f, sol.data(), &v[0], &v[3]);

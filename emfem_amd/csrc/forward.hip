@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <map>
 #include <set>
 #include <sstream>
 #include <tuple>
@@ -93,8 +94,9 @@ void refine_points(emfem_context *c, const std::vector<double> &pts, double L, c
   for (int pass = 0; pass < 64; ++pass) {
     std::vector<uint8_t> flag(c->T.nt, 0);
     bool any = false;
+    const std::vector<int> cells = locator(c)(pts);
     for (size_t i = 0; i < pts.size() / 3; ++i) {
-      const int t = find_cell(c->T, &pts[i * 3]);
+      const int t = cells[i];
       EMF_REQUIRE(t >= 0, kErrUser, std::string(what) + " " + std::to_string(i) + " is not found in mesh.");
       if (tet_volume(c->T, t) > L * L * L) {
         flag[t] = 1;
@@ -135,6 +137,69 @@ std::vector<double> dipole_points(const double *tx, int n_div) {
 
 void check(emfem_context *c, int rc) {
   if (rc != EMFEM_OK) throw Error(rc, c->err);
+}
+
+// save_mesh (src/em_io.cc:289-314) + Mesh::save_vtk (src/em_mesh.cc:184-248): legacy
+// ASCII VTK with the subdomain (0: one process), the attribute, every resistivity
+// column of the cell's row and the error indicators.  The data blocks follow the
+// reference's std::map, i.e. they are ordered by name.
+void write_vtk(const emfem_context *c, const std::string &path, int tidx, const double *err0, const double *err1) {
+  static const char *kNames[4][6] = {{"rho"},
+                                     {"rho_h", "rho_v"},
+                                     {"rho_x", "rho_y", "rho_z"},
+                                     {"rho_x", "rho_y", "rho_z", "eta_x", "eta_y", "eta_z"}};
+  static const int kCols[4] = {1, 2, 3, 6};
+  const Topology &T = c->T;
+  EMF_REQUIRE(c->aniso_form >= 0 && c->aniso_form < 4 && c->n_rho > 0, kErrState, "no resistivity table");
+  EMF_REQUIRE(err0 != nullptr && (tidx >= 0 || err1 != nullptr), kErrArg, "missing error indicators");
+  const int nc = kCols[c->aniso_form];
+  std::map<std::string, std::vector<double>> data;
+  for (int i = 0; i < nc; ++i) {
+    auto &d = data[kNames[c->aniso_form][i]];
+    d.resize(T.nt);
+    for (int t = 0; t < T.nt; ++t) d[t] = c->rho[(size_t)T.attr[t] * nc + i];
+  }
+  if (tidx < 0) {
+    data["error_xy"].assign(err0, err0 + T.nt);
+    data["error_yx"].assign(err1, err1 + T.nt);
+  } else {
+    data["error"].assign(err0, err0 + T.nt);
+  }
+  FILE *fp = std::fopen(path.c_str(), "w");
+  EMF_REQUIRE(fp != nullptr, kErrUser, "Unable to open file " + path + ".");
+  std::fprintf(fp, "# vtk DataFile Version 2.0\nUnstructured Grid\nASCII\nDATASET UNSTRUCTURED_GRID\n");
+  std::fprintf(fp, "POINTS %d double\n", T.nv);
+  for (int v = 0; v < T.nv; ++v)
+    std::fprintf(fp, "%.17g %.17g %.17g\n", T.pts[(size_t)v * 3], T.pts[(size_t)v * 3 + 1], T.pts[(size_t)v * 3 + 2]);
+  std::fprintf(fp, "\nCELLS %d %d\n", T.nt, T.nt * 5);
+  for (int t = 0; t < T.nt; ++t) {
+    std::fprintf(fp, "4");
+    for (int k = 0; k < 4; ++k) std::fprintf(fp, " %4d", T.tets[(size_t)t * 4 + k]);
+    std::fprintf(fp, "\n");
+  }
+  std::fprintf(fp, "\nCELL_TYPES %d\n", T.nt);
+  for (int t = 0; t < T.nt; ++t) std::fprintf(fp, "10\n");
+  std::fprintf(fp, "\nCELL_DATA %d\n", T.nt);
+  auto block = [&](const char *name, auto value) {
+    std::fprintf(fp, "SCALARS %s double 1\nLOOKUP_TABLE default\n", name);
+    for (int t = 0; t < T.nt; ++t) std::fprintf(fp, "%.15g\n", value(t));
+    std::fprintf(fp, "\n");
+  };
+  block("subdomain", [](int) { return 0.0; });
+  block("tet_attr", [&](int t) { return (double)T.attr[t]; });
+  for (const auto &kv : data) block(kv.first.c_str(), [&](int t) { return kv.second[t]; });
+  std::fclose(fp);
+}
+
+// Per-cycle VTK output of the drivers (src/em_fwd.cc:1361-1363,1420-1422).  The
+// reference requires -oprefix; here an empty prefix turns the output off.
+bool saving_mesh(const emfem_context *c) { return c->opt("save_mesh") == "true" && !c->opt("oprefix").empty(); }
+
+std::string vtk_name(const emfem_context *c, int fidx, int tidx, int cycle) {
+  char buf[64];
+  if (tidx < 0) std::snprintf(buf, sizeof(buf), "-%02d-%02d.vtk", fidx, cycle);
+  else std::snprintf(buf, sizeof(buf), "-%02d-%03d-%02d.vtk", fidx, tidx, cycle);
+  return c->opt("oprefix") + buf;
 }
 
 // Marks and refines by the squared indicators (refine_mesh, src/em_fwd.cc:1242-1271).
@@ -205,6 +270,12 @@ void forward_mt(emfem_context *c, int fidx) {
     const int slots[3] = {0, 1, 2};
     std::vector<double> err((size_t)3 * nt);
     check(c, emfem_estimate_error(c, fidx, 3, slots, err.data()));
+    std::vector<double> exy(nt), eyx(nt);
+    for (int t = 0; t < nt; ++t) {
+      exy[t] = std::sqrt(err[t] * err[(size_t)2 * nt + t]);
+      eyx[t] = std::sqrt(err[(size_t)nt + t] * err[(size_t)2 * nt + t]);
+    }
+    if (saving_mesh(c)) write_vtk(c, vtk_name(c, fidx, -3, cycle), -3, exy.data(), eyx.data());
     if (cycle == max_cycles || c->T.ne >= c->opti("max_dofs")) {
       std::vector<double> rsp(c->obs4.size() / 2, NAN);
       check(c, emfem_calculate_response_mt(c, fidx, 0, 1, (int)c->obs4.size() / 4, c->obs4.data(), rsp.data()));
@@ -213,11 +284,6 @@ void forward_mt(emfem_context *c, int fidx) {
           c->rsp[o] = cplx(rsp[o * 2], rsp[o * 2 + 1]);
       check(c, emfem_destroy_linear_system(c));
       break;
-    }
-    std::vector<double> exy(nt), eyx(nt);
-    for (int t = 0; t < nt; ++t) {
-      exy[t] = std::sqrt(err[t] * err[(size_t)2 * nt + t]);
-      eyx[t] = std::sqrt(err[(size_t)nt + t] * err[(size_t)2 * nt + t]);
     }
     check(c, emfem_destroy_linear_system(c));
     refine_by_error(c, {exy, eyx});
@@ -256,6 +322,16 @@ void forward_csem(emfem_context *c, int fidx, const std::vector<int> &tidxs) {
       solve_dual(c, ng);
       check(c, emfem_destroy_pc(c));
       const bool last = cycle == max_cycles || c->T.ne >= c->opti("max_dofs");
+      if (saving_mesh(c)) {
+        const int nt = c->T.nt;
+        std::vector<double> err((size_t)2 * nt), ind(nt);
+        for (int k = 0; k < ng; ++k) {
+          const int slots[2] = {k, ng};
+          check(c, emfem_estimate_error(c, fidx, 2, slots, err.data()));
+          for (int t = 0; t < nt; ++t) ind[t] = std::sqrt(err[t] * err[(size_t)nt + t]);
+          write_vtk(c, vtk_name(c, fidx, grp[k], cycle), grp[k], ind.data(), nullptr);
+        }
+      }
       if (last) {
         for (int k = 0; k < ng; ++k) {
           std::vector<double> rsp(c->obs4.size() / 2, NAN);
@@ -395,6 +471,14 @@ int emfem_save_rsp(emfem_context *c, const char *path) {
       }
     }
     std::fclose(fp);
+  });
+}
+
+int emfem_save_mesh(emfem_context *c, const char *path, int tidx, const double *error0, const double *error1) {
+  return guarded(c, [&]() {
+    require_mesh(c);
+    EMF_REQUIRE(path != nullptr, kErrArg, "null path");
+    write_vtk(c, path, tidx, error0, error1);
   });
 }
 

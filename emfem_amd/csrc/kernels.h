@@ -23,7 +23,7 @@ struct Csr {
   int role = 0; // 1 = edge-space smoother operator (a separate kernel instance in traces)
 };
 
-// Picks the fastest lanes-per-row (4, 8, 16, 32) of the narrow SpMM for A on nc
+// Picks the fastest lanes-per-row (2, 4, 8, 16) of the narrow SpMM for A on nc
 // columns by timing each with HIP events on stream s; x and y are scratch blocks
 // of [A.ncols][nc] and [A.nrows][nc].  Returns 0 when nc uses the wide kernel.
 template <typename TA, typename TX, typename T>
@@ -37,6 +37,13 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y);
 template <typename TA, typename TX, typename T>
 void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
                 T gamma, const T *w, int ldw, T beta, const T *z, int ldz);
+
+// spmm_mixed with two optional per-row scalings: the input is read as
+// xs[col] * x[col][:] (the operator A diag(xs)) and z as zs[row] * z[row][:]; either
+// may be null (= 1).
+template <typename TA, typename TX, typename T>
+void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
+             T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs);
 
 template <typename T>
 inline void spmm(hipStream_t s, const Csr<T> &A, int nc, const T *x, int ldx, T *y, int ldy, T alpha, const T *rs,
@@ -104,7 +111,9 @@ void cheb_step(hipStream_t s, int rows, int nc, T *d, const T *rres, T *x, doubl
 template <typename T>
 void diag_scale(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x);
 
-// Dense coarse solve in fp64: x[i][c] = sum_j Ainv[i][j] b[j][c].
+// Dense coarse solve in fp64: x[i][c] = sum_j AinvT[j][i] b[j][c], AinvT being the
+// transposed inverse (row j holds column j of the inverse); n <= kDenseMax.
+constexpr int kDenseMax = 4096;
 template <typename T>
 void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T *x);
 

@@ -299,8 +299,16 @@ void rhs_mt(const Topology &T, const std::vector<double> &s6, const Background &
   if (bval_out) bval_out->swap(bval);
 }
 
-void rhs_csem(const Topology &T, double freq, const double *tx, int n_div, std::vector<double> &s_re,
-              std::vector<double> &s_im) {
+Locate host_locate(const Topology &T) {
+  return [&T](const std::vector<double> &pts) {
+    std::vector<int> t(pts.size() / 3);
+    for (size_t i = 0; i < t.size(); ++i) t[i] = find_cell(T, &pts[i * 3]);
+    return t;
+  };
+}
+
+void rhs_csem(const Topology &T, double freq, const double *tx, int n_div, const Locate &loc,
+              std::vector<double> &s_re, std::vector<double> &s_im) {
   s_re.assign(T.ne, 0.0);
   s_im.assign(T.ne, 0.0);
   const double omega = 2 * kPi * freq;
@@ -324,9 +332,10 @@ void rhs_csem(const Topology &T, double freq, const double *tx, int n_div, std::
       for (int k = 0; k < 3; ++k) A[k] = B[k];
     }
   }
+  const std::vector<int> cells = loc(pos);
   for (size_t j = 0; j < wt.size(); ++j) {
     const double *c = &pos[j * 3];
-    int t = find_cell(T, c);
+    const int t = cells[j];
     EMF_REQUIRE(t >= 0, kErrUser, "Transmitter dipole " + std::to_string(j) + " is not found in mesh.");
     TetLocal L;
     load_tet(T, t, L);
@@ -342,13 +351,14 @@ void rhs_csem(const Topology &T, double freq, const double *tx, int n_div, std::
   }
 }
 
-void rhs_dual(const Topology &T, const std::vector<double> &rx, std::vector<double> &s_re,
+void rhs_dual(const Topology &T, const std::vector<double> &rx, const Locate &loc, std::vector<double> &s_re,
               std::vector<double> &s_im) {
   s_re.assign(T.ne, 0.0);
   s_im.assign(T.ne, 0.0);
+  const std::vector<int> cells = loc(rx);
   for (size_t r = 0; r < rx.size() / 3; ++r) {
     const double *p = &rx[r * 3];
-    int t = find_cell(T, p);
+    const int t = cells[r];
     EMF_REQUIRE(t >= 0, kErrUser, "Receiver " + std::to_string(r) + " is not found in mesh.");
     TetLocal L;
     load_tet(T, t, L);

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <complex>
+#include <functional>
 #include <vector>
 
 #include "topology.h"
@@ -58,12 +59,19 @@ class PlaneWave {
 void rhs_mt(const Topology &T, const std::vector<double> &s6, const Background &bg, double freq, int mode,
             std::vector<double> &s_re, std::vector<double> &s_im, std::vector<cplx> *bval = nullptr);
 
+// Point location for a batch of points [n][3]: containing tet per point, -1 when
+// outside (find_cell_around_point semantics).
+using Locate = std::function<std::vector<int>(const std::vector<double> &pts)>;
+
+// Locate by the host breadth-first search, one point at a time.
+Locate host_locate(const Topology &T);
+
 // CSEM right-hand side for one transmitter record (7 numbers).
-void rhs_csem(const Topology &T, double freq, const double *tx, int n_div, std::vector<double> &s_re,
-              std::vector<double> &s_im);
+void rhs_csem(const Topology &T, double freq, const double *tx, int n_div, const Locate &loc,
+              std::vector<double> &s_re, std::vector<double> &s_im);
 
 // Dual right-hand side summed over the receivers.
-void rhs_dual(const Topology &T, const std::vector<double> &rx, std::vector<double> &s_re,
+void rhs_dual(const Topology &T, const std::vector<double> &rx, const Locate &loc, std::vector<double> &s_re,
               std::vector<double> &s_im);
 
 // E and H at p inside tet t for the complex edge solution e (interleaved re, im).

@@ -91,7 +91,7 @@ void upload_vec(DevBuf<T> &d, const std::vector<double> &h, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------------
-// smoothed-aggregation V-cycle (l1-Jacobi smoothing, fp64 dense coarsest solve)
+// aggregation-AMG V-cycle (l1-Jacobi smoothing, fp64 dense coarsest solve)
 // ---------------------------------------------------------------------------------
 
 template <typename T>
@@ -106,7 +106,10 @@ class DevAmg {
       upload_vec(levels_[l].l1inv, H.levels[l].l1inv, s);
     }
     ncoarse_ = H.ncoarse;
-    cinv_.upload(H.coarse_inv, s);
+    std::vector<double> t((size_t)ncoarse_ * ncoarse_);
+    for (int i = 0; i < ncoarse_; ++i)
+      for (int j = 0; j < ncoarse_; ++j) t[(size_t)j * ncoarse_ + i] = H.coarse_inv[(size_t)i * ncoarse_ + j];
+    cinv_.upload(t, s);  // transposed for coalesced reads in the dense solve
     EMF_HIP(hipStreamSynchronize(s));
   }
 
@@ -143,17 +146,17 @@ class DevAmg {
     Level &L = levels_[l];
     const int n = L.A.n;
     T *x = L.x.get(), *r = L.r.get();
-    // pre-smoothing from a zero guess
-    diag_scale<T>(s, n, nc, L.l1inv.get(), b, x);
-    // r = b - A x ; restrict
-    spmm<T>(s, L.A.view(), nc, x, nc, r, nc, T(-1), nullptr, T(1), b, nc, T(0), nullptr, 0);
+    const T *dinv = L.l1inv.get();
+    // pre-smoothing from a zero guess, x0 = Dinv b, folded into the next two
+    // launches: r = b - A Dinv b ; restrict
+    spmm_ex<T, T, T>(s, L.A.view(), nc, b, nc, r, nc, T(-1), nullptr, T(1), b, nc, T(0), nullptr, 0, dinv, nullptr);
     const bool last = (l + 1 == levels_.size());
     T *cb = last ? coarse_b_.get() : levels_[l + 1].b.get();
     T *cx = last ? coarse_x_.get() : levels_[l + 1].out.get();
     spmm<T>(s, L.R.view(), nc, r, nc, cb, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
     cycle(s, l + 1, nc, cb, cx);
-    // x += P xc
-    spmm<T>(s, L.P.view(), nc, cx, nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), x, nc);
+    // x = Dinv b + P xc
+    spmm_ex<T, T, T>(s, L.P.view(), nc, cx, nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), b, nc, nullptr, dinv);
     // post-smoothing: out = x + l1inv .* (b - A x)
     spmm<T>(s, L.A.view(), nc, x, nc, out, nc, T(-1), L.l1inv.get(), T(1), b, nc, T(1), x, nc);
   }
@@ -199,15 +202,17 @@ class DevAms {
       B64_.v = b64vals_.get();
     }
     upload_vec(l1inv_, H.l1inv, s);
-    G0_.upload(H.G0, s);
-    G0T_.upload(H.G0T, s);
-    for (int d = 0; d < 3; ++d) {
-      Pi_[d].upload(H.Pi[d], s);
-      PiT_[d].upload(H.PiT[d], s);
-      amgPi_[d].upload(H.amgPi[d], s);
+    additive_ = H.additive;
+    if (!additive_) {
+      G0_.upload(H.G0, s);
+      G0T_.upload(H.G0T, s);
+      amgG_.upload(H.amgG, s);
     }
-    amgG_.upload(H.amgG, s);
+    Pi_.upload(H.Pi, s);
+    PiT_.upload(H.PiT, s);
+    amgPi_.upload(H.amgPi, s);
     nv0_ = H.nv0;
+    nblk_ = H.nblk;
     deg_ = std::max(1, o.cheb_degree);
     const double lmax = 1.0;  // Gershgorin: the l1-scaled operator has spectrum in (0, 1]
     const double lmin = lmax / o.cheb_ratio;
@@ -225,10 +230,10 @@ class DevAms {
     rres_.resize((size_t)ne_ * nc);
     d_.resize((size_t)ne_ * nc);
     if (!std::is_same<T, double>::value) bt_.resize((size_t)ne_ * nc);
-    vb_.resize((size_t)nv0_ * nc);
-    vx_.resize((size_t)nv0_ * nc);
-    amgG_.ensure(nc);
-    for (int d = 0; d < 3; ++d) amgPi_[d].ensure(nc);
+    vb_.resize((size_t)nblk_ * nv0_ * nc);
+    vx_.resize((size_t)nblk_ * nv0_ * nc);
+    if (!additive_) amgG_.ensure(nc);
+    amgPi_.ensure(nc);
   }
 
   // Lanes per row of the smoother SpMM for nc columns, timed once per nc (outside
@@ -281,15 +286,15 @@ class DevAms {
     // pre-smoothing (Chebyshev from zero)
     cheb_init<T>(s, ne_, nc, l1inv_.get(), bt, rres, d, out, 1.0 / theta_);
     cheb_iterate(s, nc, rres, d, out, clk);
-    aux_gradient(s, nc, b, out);
+    if (!additive_) aux_gradient(s, nc, b, out);
     residual64(s, nc, b, out);
-    for (int k = 0; k < 3; ++k) {
-      spmm_mixed<T, double, T>(s, PiT_[k].view(), nc, r64_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0,
-                               T(0), nullptr, 0);
-      amgPi_[k].vcycle(s, nc, vb_.get(), vx_.get());
-      spmm<T>(s, Pi_[k].view(), nc, vx_.get(), nc, out, nc, T(1), nullptr, T(0), nullptr, 0, T(1), out, nc);
-    }
-    aux_gradient(s, nc, b, out);
+    // the nodal-vector components (and, additive, the gradient) in one
+    // block-diagonal auxiliary solve
+    spmm_mixed<T, double, T>(s, PiT_.view(), nc, r64_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0, T(0),
+                             nullptr, 0);
+    amgPi_.vcycle(s, nc, vb_.get(), vx_.get());
+    spmm<T>(s, Pi_.view(), nc, vx_.get(), nc, out, nc, T(1), nullptr, T(0), nullptr, 0, T(1), out, nc);
+    if (!additive_) aux_gradient(s, nc, b, out);
     // post-smoothing (Chebyshev from the current iterate)
     timed_spmm(s, nc, out, rres, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr, clk);
     axpby<T, T>(s, ne_, nc, 1.0 / theta_, rres, nc, 0.0, d, nc);
@@ -331,14 +336,15 @@ class DevAms {
     spmm<T>(s, G0_.view(), nc, vx_.get(), nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), x, nc);
   }
 
-  int ne_ = 0, nv0_ = 0, nc_ = 0, deg_ = 3;
+  int ne_ = 0, nv0_ = 0, nc_ = 0, deg_ = 3, nblk_ = 3;
+  bool additive_ = false;
   double theta_ = 0.5, delta_ = 0.5;
   Csr<T> B_;
   Csr<double> B64_;
   DevBuf<T> bvals_, l1inv_, rres_, d_, bt_, vb_, vx_;
   DevBuf<double> b64vals_, r64_;
-  DevCsr<T> G0_, G0T_, Pi_[3], PiT_[3];
-  DevAmg<T> amgG_, amgPi_[3];
+  DevCsr<T> G0_, G0T_, Pi_, PiT_;
+  DevAmg<T> amgG_, amgPi_;
   std::map<std::tuple<int, const void *, const void *>, hipGraphExec_t> graphs_;
   std::map<int, int> lpr_;
 };

@@ -11,6 +11,7 @@
 #include <algorithm>
 
 #include "common.h"
+#include "device.h"
 #include "kernels.h"
 
 namespace emfem {
@@ -76,8 +77,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
                                                         const int *__restrict__ ci, const TA *__restrict__ av,
                                                         const TX *__restrict__ x, int ldx, T *y, int ldy, T alpha,
                                                         const T *__restrict__ rs, T gamma, const T *w, int ldw,
-                                                        T beta, const T *z, int ldz) {
-  const int gid = blockIdx.x * kBlock + threadIdx.x;
+                                                        T beta, const T *z, int ldz, const T *__restrict__ xs,
+                                                        const T *__restrict__ zs) {
+  const int gid = xcd_block() * kBlock + threadIdx.x;
   const int row = gid / LPR;
   const int lane = gid % LPR;
   if (row >= nrows) return;
@@ -85,21 +87,33 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
 #pragma unroll
   for (int j = 0; j < NC; ++j) acc[j] = T(0);
   const int e = rp[row + 1];
-  // two nonzeros per lane per trip: both column loads, then both row gathers, are
-  // in flight together
-  for (int p = rp[row] + lane; p < e; p += 2 * LPR) {
-    const bool two = p + LPR < e;
-    const int c0 = ci[p];
-    const int c1 = two ? ci[p + LPR] : c0;
-    const T a0 = (T)av[p];
-    const T a1 = two ? (T)av[p + LPR] : T(0);
-    const TX *x0 = x + (size_t)c0 * ldx;
-    const TX *x1 = x + (size_t)c1 * ldx;
+  // kU nonzeros per lane per trip: all column/value loads, then all row gathers,
+  // are in flight together (a typical 16-nonzero edge row is one trip at LPR 4)
+  constexpr int kU = 4;
+  for (int p0 = rp[row] + lane; p0 < e; p0 += kU * LPR) {
+    int c[kU];
+    T a[kU];
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] += a0 * (T)x0[j] + a1 * (T)x1[j];
+    for (int u = 0; u < kU; ++u) {
+      const int p = p0 + u * LPR;
+      const bool ok = p < e;
+      c[u] = ok ? ci[p] : 0;
+      a[u] = ok ? (T)av[p] : T(0);
+    }
+    if (xs) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) a[u] *= xs[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const TX *xr = x + (size_t)c[u] * ldx;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] += a[u] * (T)xr[j];
+    }
   }
   const int col0 = row_reduce<T, NC, LPR>(acc, lane);
   const T scale = rs ? rs[row] : T(1);
+  const T zscale = zs ? beta * zs[row] : beta;
   constexpr bool kScatter = (NC & (NC - 1)) == 0 && NC > 1;
   if constexpr (kScatter) {
     // lanes that own a column after the reduce-scatter
@@ -112,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
       T v = alpha * acc[i];
       if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
       v *= scale;
-      if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
+      if (beta != T(0)) v += zscale * z[(size_t)row * ldz + j];
       y[(size_t)row * ldy + j] = v;
     }
   } else {
@@ -122,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
         T v = alpha * acc[j];
         if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
         v *= scale;
-        if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
+        if (beta != T(0)) v += zscale * z[(size_t)row * ldz + j];
         y[(size_t)row * ldy + j] = v;
       }
     }
@@ -134,8 +148,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
                                                       const int *__restrict__ ci, const TA *__restrict__ av,
                                                       const TX *__restrict__ x, int ldx, T *y, int ldy, int nc,
                                                       T alpha, const T *__restrict__ rs, T gamma, const T *w,
-                                                      int ldw, T beta, const T *z, int ldz) {
-  int row = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+                                                      int ldw, T beta, const T *z, int ldz, const T *__restrict__ xs,
+                                                      const T *__restrict__ zs) {
+  int row = xcd_block() * (kBlock / 64) + threadIdx.x / 64;
   row = __builtin_amdgcn_readfirstlane(row);
   const int lane = threadIdx.x % 64;
   if (row >= nrows) return;
@@ -144,8 +159,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
   for (int j = 0; j < CPL; ++j) acc[j] = T(0);
   const int b = rp[row], e = rp[row + 1];
   for (int p = b; p < e; ++p) {
-    const T a = (T)av[p];
-    const TX *xr = x + (size_t)ci[p] * ldx;
+    const int c0 = ci[p];
+    const T a = xs ? (T)av[p] * xs[c0] : (T)av[p];
+    const TX *xr = x + (size_t)c0 * ldx;
 #pragma unroll
     for (int j = 0; j < CPL; ++j) {
       const int c = lane + 64 * j;
@@ -153,6 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
     }
   }
   const T scale = rs ? rs[row] : T(1);
+  const T zscale = zs ? beta * zs[row] : beta;
 #pragma unroll
   for (int j = 0; j < CPL; ++j) {
     const int c = lane + 64 * j;
@@ -160,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
       T v = alpha * acc[j];
       if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
       v *= scale;
-      if (beta != T(0)) v += beta * z[(size_t)row * ldz + c];
+      if (beta != T(0)) v += zscale * z[(size_t)row * ldz + c];
       y[(size_t)row * ldy + c] = v;
     }
   }
@@ -168,19 +185,21 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
 
 template <typename TA, typename TX, typename T, int NC>
 void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
-                   T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
+                   T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs) {
   const double avg = A.nrows ? double(A.nnz) / A.nrows : 0.0;
-  const int lpr = A.lpr > 0 ? A.lpr : (avg <= 6 ? 4 : (avg <= 20 ? 8 : (avg <= 48 ? 16 : 32)));
+  // one trip of 4 nonzeros per lane covers a row of average length
+  const int lpr = A.lpr > 0 ? A.lpr : (avg <= 8 ? 2 : (avg <= 16 ? 4 : (avg <= 32 ? 8 : (avg <= 64 ? 16 : 32))));
   const int64_t threads = (int64_t)A.nrows * lpr;
   const int g = grid_for(threads);
 #define EMF_NARROW(L)                                                                                        \
   if (A.role == 1)                                                                                           \
     hipLaunchKernelGGL((k_spmm_narrow<TA, TX, T, NC, L, 1>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, \
-                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz);                         \
+                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs);                 \
   else                                                                                                       \
     hipLaunchKernelGGL((k_spmm_narrow<TA, TX, T, NC, L, 0>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, \
-                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz)
+                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs)
   switch (lpr) {
+    case 2: EMF_NARROW(2); break;
     case 4: EMF_NARROW(4); break;
     case 8: EMF_NARROW(8); break;
     case 16: EMF_NARROW(16); break;
@@ -192,24 +211,24 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
 }  // namespace
 
 template <typename TA, typename TX, typename T>
-void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
-                T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
+void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
+             T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs) {
   if (A.nrows == 0 || nc == 0) return;
   if (nc >= 16) {
     const int g = (A.nrows + kBlock / 64 - 1) / (kBlock / 64);
     if (nc <= 64)
       hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, 1>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y,
-                         ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz);
+                         ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs);
     else if (nc <= 128)
       hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, 2>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y,
-                         ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz);
+                         ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs);
     else {
       // 256-column slabs
       for (int c0 = 0; c0 < nc; c0 += 256) {
         const int n = std::min(256, nc - c0);
         hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, 4>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x + c0,
                            ldx, y + c0, ldy, n, alpha, rs, gamma, w ? w + c0 : nullptr, ldw, beta,
-                           z ? z + c0 : nullptr, ldz);
+                           z ? z + c0 : nullptr, ldz, xs, zs);
       }
     }
     EMF_HIP(hipGetLastError());
@@ -223,7 +242,7 @@ void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T
     const T *zc = z ? z + c0 : nullptr;
 #define EMF_CASE(K)                                                                                                 \
   case K:                                                                                                           \
-    launch_narrow<TA, TX, T, K>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz);        \
+    launch_narrow<TA, TX, T, K>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz, xs, zs); \
     break;
     switch (n) {
       EMF_CASE(8)
@@ -232,13 +251,19 @@ void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T
       EMF_CASE(3)
       EMF_CASE(2)
       default:
-        launch_narrow<TA, TX, T, 1>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz);
+        launch_narrow<TA, TX, T, 1>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz, xs, zs);
         break;
     }
 #undef EMF_CASE
     c0 += n;
   }
   EMF_HIP(hipGetLastError());
+}
+
+template <typename TA, typename TX, typename T>
+void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
+                T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
+  spmm_ex<TA, TX, T>(s, A, nc, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, nullptr, nullptr);
 }
 
 template <typename TA, typename TX, typename T>
@@ -249,7 +274,7 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
   EMF_HIP(hipEventCreate(&e1));
   int best = 0;
   float best_ms = 1e30f;
-  for (int lpr : {4, 8, 16, 32}) {
+  for (int lpr : {2, 4, 8, 16}) {
     Csr<TA> B = A;
     B.lpr = lpr;
     spmm_mixed<TA, TX, T>(s, B, nc, x, nc, y, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
@@ -273,6 +298,8 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
 #define EMF_SPMM_INST(TA, TX, T)                                                                                    \
   template void spmm_mixed<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, int, T *, int, T, const T *, T, \
                                       const T *, int, T, const T *, int);                                           \
+  template void spmm_ex<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, int, T *, int, T, const T *, T,    \
+                                   const T *, int, T, const T *, int, const T *, const T *);                        \
   template int tune_lpr<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, T *);
 EMF_SPMM_INST(float, float, float)
 EMF_SPMM_INST(double, double, double)
@@ -291,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_op_narrow(int nrows, const int *__re
                                                       const double2 *__restrict__ K, const TX *__restrict__ x, int ldx,
                                                       double *y, int ldy, double alpha, double beta, const double *z,
                                                       int ldz) {
-  const int gid = blockIdx.x * kBlock + threadIdx.x;
+  const int gid = xcd_block() * kBlock + threadIdx.x;
   const int row = gid / LPR;
   const int lane = gid % LPR;
   if (row >= nrows) return;
@@ -336,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_op_wide(int nrows, const int *__rest
                                                     const double2 *__restrict__ K, const TX *__restrict__ x, int ldx,
                                                     double *y, int ldy, int np, double alpha, double beta,
                                                     const double *z, int ldz) {
-  int row = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+  int row = xcd_block() * (kBlock / 64) + threadIdx.x / 64;
   row = __builtin_amdgcn_readfirstlane(row);
   const int lane = threadIdx.x % 64;
   if (row >= nrows) return;
@@ -483,12 +510,28 @@ __global__ __launch_bounds__(kBlock) void k_coldot(int rows, int nc, int rows_pe
   }
 }
 
-__global__ void k_finish_dot(int nblocks, int nc, const double *__restrict__ work, double *out) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= nc) return;
+// Fixed-order block sum of sh[0..kBlock) (tree over LDS); result in sh[0].
+__device__ __forceinline__ void block_tree_sum(double *sh) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    __syncthreads();
+    if (t < w) sh[t] += sh[t + w];
+  }
+  __syncthreads();
+}
+
+// One block per column: the per-block partials are summed in a fixed order (strided
+// per thread, then a tree), so results do not depend on scheduling.
+__global__ __launch_bounds__(kBlock) void k_finish_dot(int nblocks, int nc, const double *__restrict__ work,
+                                                       double *out) {
+  __shared__ double sh[kBlock];
+  const int c = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += work[(size_t)b * nc + c];
-  out[c] = s;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) s += work[(size_t)b * nc + c];
+  sh[threadIdx.x] = s;
+  block_tree_sum(sh);
+  if (threadIdx.x == 0) out[c] = sh[0];
 }
 
 }  // namespace
@@ -505,7 +548,7 @@ void coldot(hipStream_t s, int rows, int nc, const T *x, int ldx, const T *y, in
   const int rows_per_pass = kBlock / nc;
   const int nb = std::max(1, std::min(kDotBlocks, (rows + rows_per_pass - 1) / rows_per_pass));
   hipLaunchKernelGGL((k_coldot<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, rows_per_pass, x, ldx, y, ldy, work);
-  hipLaunchKernelGGL(k_finish_dot, dim3(grid_for(nc)), dim3(kBlock), 0, s, nb, nc, work, out);
+  hipLaunchKernelGGL(k_finish_dot, dim3(nc), dim3(kBlock), 0, s, nb, nc, work, out);
   EMF_HIP(hipGetLastError());
 }
 template void coldot<float>(hipStream_t, int, int, const float *, int, const float *, int, double *, double *);
@@ -546,13 +589,17 @@ __global__ __launch_bounds__(kBlock) void k_mdot(int rows, int nc, int rows_per_
   }
 }
 
-__global__ void k_finish_mdot(int nblocks, int nc, int m, const double *__restrict__ work, double *out) {
-  const int idx = blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= nc * m) return;
+// One block per (basis vector i, column c), fixed-order sum as in k_finish_dot.
+__global__ __launch_bounds__(kBlock) void k_finish_mdot(int nblocks, int nc, int m, const double *__restrict__ work,
+                                                        double *out) {
+  __shared__ double sh[kBlock];
+  const int idx = blockIdx.x;
   const int i = idx / nc, c = idx % nc;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += work[((size_t)i * nblocks + b) * nc + c];
-  out[idx] = s;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) s += work[((size_t)i * nblocks + b) * nc + c];
+  sh[threadIdx.x] = s;
+  block_tree_sum(sh);
+  if (threadIdx.x == 0) out[idx] = sh[0];
 }
 
 }  // namespace
@@ -573,7 +620,7 @@ void mdot(hipStream_t s, int rows, int nc, const double *V, size_t vstride, int 
       EMF_MDOT(1) EMF_MDOT(2) EMF_MDOT(3) EMF_MDOT(4) EMF_MDOT(5) EMF_MDOT(6) EMF_MDOT(7) EMF_MDOT(8)
 #undef EMF_MDOT
     }
-    hipLaunchKernelGGL(k_finish_mdot, dim3(grid_for(nc * mm)), dim3(kBlock), 0, s, nb, nc, mm, work,
+    hipLaunchKernelGGL(k_finish_mdot, dim3(nc * mm), dim3(kBlock), 0, s, nb, nc, mm, work,
                        out + (size_t)i0 * nc);
   }
   EMF_HIP(hipGetLastError());
@@ -676,13 +723,21 @@ __global__ void k_diag_scale(int64_t n, int nc, const T *__restrict__ dinv, cons
 }
 
 template <typename T>
-__global__ void k_dense_solve(int n, int nc, const double *__restrict__ Ainv, const T *__restrict__ b, T *x) {
-  const int idx = blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n * nc) return;
-  const int i = idx / nc, c = idx % nc;
-  double s = 0.0;
-  for (int j = 0; j < n; ++j) s += Ainv[(size_t)i * n + j] * (double)b[(size_t)j * nc + c];
-  x[idx] = (T)s;
+// One block per column c: b[:, c] is staged in LDS, thread i accumulates row i
+// reading the transposed inverse AinvT[j][i] (consecutive threads, consecutive i:
+// coalesced).  n <= kDenseMax.
+__global__ __launch_bounds__(kBlock) void k_dense_solve(int n, int nc, const double *__restrict__ AinvT,
+                                                        const T *__restrict__ b, T *x) {
+  __shared__ double bs[kDenseMax];
+  const int c = blockIdx.x;
+  for (int j = threadIdx.x; j < n; j += kBlock) bs[j] = (double)b[(size_t)j * nc + c];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    double s = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < n; ++j) s += AinvT[(size_t)j * n + i] * bs[j];
+    x[(size_t)i * nc + c] = (T)s;
+  }
 }
 
 __global__ void k_fcg_alpha(int nc, const double *rz, const double *pq, const uint8_t *active, double *alpha) {
@@ -741,7 +796,7 @@ void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const T *p, co
   const int rows_per_pass = kBlock / nc;
   const int nb = std::max(1, std::min(kDotBlocks, (rows + rows_per_pass - 1) / rows_per_pass));
   hipLaunchKernelGGL((k_fcg_xr<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, rows_per_pass, alpha, p, q, x, r, work);
-  hipLaunchKernelGGL(k_finish_dot, dim3(grid_for(nc)), dim3(kBlock), 0, s, nb, nc, work, rr);
+  hipLaunchKernelGGL(k_finish_dot, dim3(nc), dim3(kBlock), 0, s, nb, nc, work, rr);
   EMF_HIP(hipGetLastError());
 }
 template void fcg_xr<float>(hipStream_t, int, int, const double *, const float *, const float *, float *, float *,
@@ -769,7 +824,8 @@ void diag_scale(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x
 }
 template <typename T>
 void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T *x) {
-  hipLaunchKernelGGL((k_dense_solve<T>), dim3(grid_for((int64_t)n * nc)), dim3(kBlock), 0, s, n, nc, Ainv, b, x);
+  EMF_REQUIRE(n <= kDenseMax, kErrArg, "dense coarse solve larger than its LDS stage");
+  hipLaunchKernelGGL((k_dense_solve<T>), dim3(nc), dim3(kBlock), 0, s, n, nc, Ainv, b, x);
   EMF_HIP(hipGetLastError());
 }
 

@@ -275,8 +275,9 @@ int nearest_vertex(const Topology &T, const double *p) {
   int best = 0;
   double bd = 1e300;
   for (int v = 0; v < T.nv; ++v) {
+    // squared distance: the kd-tree's L2 metric (src/em_mesh.h:60-61)
     double dx = T.pts[v * 3] - p[0], dy = T.pts[v * 3 + 1] - p[1], dz = T.pts[v * 3 + 2] - p[2];
-    double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+    double d = dx * dx + dy * dy + dz * dz;
     if (d < bd) {
       bd = d;
       best = v;

@@ -100,8 +100,16 @@ int emfem_get_mesh_info(const emfem_context *ctx, int *info);
 int emfem_get_topology(const emfem_context *ctx, int *edgelist, int *tet2edge, int *neighbor,
                        unsigned char *edge_bdr);
 
-/* Mesh::find_cell_around_point (src/em_mesh.cc:527-566): tet containing p, or -1. */
+/* Mesh::find_cell_around_point (src/em_mesh.cc:527-566): tet containing p, or -1
+ * (host search). */
 int emfem_find_cell(const emfem_context *ctx, const double *p);
+
+/* Mesh::find_closest_vertex + find_cell_around_point (src/em_mesh.cc:518-566) for n
+ * points [n*3] at once, on the device: tets[i] = containing tet or -1.  The
+ * breadth-first search per point is bounded at 4096 queued tets; a point that
+ * reaches the bound fails the call with EMFEM_ERR_USER.  The receivers and dipoles of
+ * the right-hand sides and responses are located this way. */
+int emfem_locate_points(emfem_context *ctx, int n, const double *points, int *tets);
 
 /* update_background_model (src/em_fwd.cc:135-159): 1-D layered columns under the
  * four top corners, used by the MT boundary data. */
@@ -219,6 +227,15 @@ int emfem_save_rsp(emfem_context *ctx, const char *path);
 
 /* Responses of the last emfem_forward as [n_obs][2] (re, im). */
 int emfem_get_responses(const emfem_context *ctx, double *rsp);
+
+/* save_mesh (src/em_io.cc:289-314) with Mesh::save_vtk (src/em_mesh.cc:184-248):
+ * legacy ASCII VTK of the current mesh with the subdomain, attribute, resistivity
+ * columns and error indicators.  tidx < 0 (MT): error0 = error_xy and error1 =
+ * error_yx; tidx >= 0 (CSEM): error0 = error, error1 unused.  Indicator arrays hold
+ * one value per tet.  emfem_forward writes these per cycle when -save_mesh is true
+ * and -oprefix is set. */
+int emfem_save_mesh(emfem_context *ctx, const char *path, int tidx, const double *error0,
+                    const double *error1);
 
 /* em_forward (src/em_fwd.cc:1441-1471) after the inputs are loaded: background
  * model, task distribution (src/em_fwd.cc:1273-1318), forward_mt / forward_csem

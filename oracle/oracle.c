@@ -575,7 +575,7 @@ int or_nearest_vertex(int nv, const double *pts, const double *p) {
   double bd = 1e300;
   for (v = 0; v < nv; ++v) {
     double dx = pts[v * 3] - p[0], dy = pts[v * 3 + 1] - p[1], dz = pts[v * 3 + 2] - p[2];
-    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    double d = dx * dx + dy * dy + dz * dz; /* squared: the kd-tree's L2 metric, src/em_mesh.h:60-61 */
     if (d < bd) { bd = d; best = v; }
   }
   return best;

@@ -113,3 +113,48 @@ def test_uniform_refinement(oracle_mod):
     assert np.allclose(v1.reshape(-1, 8).sum(axis=1), v0, rtol=1e-12)
     assert np.array_equal(attr.reshape(-1, 8)[:, 0], m.attr)
     _check_conforming(pts, tets, oracle_mod)
+
+
+def _vtk_blocks(path):
+    lines = open(path).read().splitlines()
+    blocks = {}
+    order = []
+    for i, l in enumerate(lines):
+        if l.startswith("SCALARS "):
+            name = l.split()[1]
+            assert lines[i + 1] == "LOOKUP_TABLE default"
+            order.append(name)
+            blocks[name] = lines[i + 2:]
+    return lines, order, blocks
+
+
+@pytest.mark.parametrize("form", [0, 3])
+def test_save_mesh_vtk(tmp_path, form):
+    """save_mesh / Mesh::save_vtk (src/em_io.cc:289-314, src/em_mesh.cc:184-248): header,
+    cells, the data blocks in name order and their values."""
+    m = mg.tiny_mt(n=3, aniso_form=form, seed=5)
+    ctx = api.Context()
+    ctx.load_model(m)
+    nt, nv = len(m.tets), len(m.points)
+    rng = np.random.default_rng(1)
+    exy, eyx = rng.random(nt) * 1e-3, rng.random(nt)
+    ctx.save_mesh(tmp_path / "mt.vtk", -3, exy, eyx)
+    lines, order, blocks = _vtk_blocks(tmp_path / "mt.vtk")
+    assert lines[:5] == ["# vtk DataFile Version 2.0", "Unstructured Grid", "ASCII", "DATASET UNSTRUCTURED_GRID",
+                         "POINTS %d double" % nv]
+    assert [float(v) for v in lines[5].split()] == list(m.points[0])
+    i = lines.index("CELLS %d %d" % (nt, 5 * nt))
+    assert lines[i + 1] == "4" + "".join(" %4d" % v for v in m.tets[0])
+    assert lines[lines.index("CELL_TYPES %d" % nt) + 1] == "10"
+    names = {0: ["rho"], 3: ["rho_x", "rho_y", "rho_z", "eta_x", "eta_y", "eta_z"]}[form]
+    assert order == ["subdomain", "tet_attr"] + sorted(["error_xy", "error_yx"] + names)
+    col = lambda n: np.array([float(v) for v in blocks[n][:nt]])
+    assert np.all(col("subdomain") == 0) and np.array_equal(col("tet_attr"), m.attr)
+    assert np.allclose(col("error_xy"), exy, rtol=1e-14) and np.allclose(col("error_yx"), eyx, rtol=1e-14)
+    for k, n in enumerate(names):
+        assert np.allclose(col(n), m.rho[m.attr, k], rtol=1e-14, atol=0)
+    ctx.save_mesh(tmp_path / "csem.vtk", 0, exy)
+    _, order, blocks = _vtk_blocks(tmp_path / "csem.vtk")
+    assert order == ["subdomain", "tet_attr"] + sorted(["error"] + names)
+    with pytest.raises(api.EmfemError):
+        ctx.save_mesh(tmp_path / "bad.vtk", -3, exy)
