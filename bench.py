@@ -133,6 +133,102 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=4, fidx=1):
     return out
 
 
+def batch_leg(api, mg, n_models, n_single=8):
+    """Uneven multi-model batch: n_models independent MT models (edges log-uniform in
+    2e4..2e5, one random frequency each), XY and YX per model, solved as one
+    block-diagonal batch (one launch per operation for all models, per-model Krylov
+    processes) against the same models solved one at a time (the first n_single)."""
+    models = mg.uneven_mt_batch(n_models=n_models)
+    t0 = time.perf_counter()
+    ctxs = []
+    for m in models:
+        c = api.Context("-inner_precision fp32")
+        c.load_model(m)
+        c.create_linear_system(2)
+        c.assemble_matrix(0)
+        c.assemble_rhs_mt(0, api.XY_POLAR, 0)
+        c.assemble_rhs_mt(0, api.YX_POLAR, 1)
+        ctxs.append(c)
+    t1 = time.perf_counter()
+    edges = np.array([c.mesh_info()["n_edges"] for c in ctxs])
+    its, rr, ms = api.solve_batch(ctxs, 0, 2, 300, 1e-8)
+    t2 = time.perf_counter()
+    setup = ctxs[0].stat("batch_setup_ms") / 1e3
+    solve = ctxs[0].stat("batch_solve_ms") / 1e3
+    per_model = ms.max(axis=1)
+    log("batch: %d models, %d edges in all, contexts %.1fs, set-up %.2fs, solve %.2fs, its max %d, relres max %.1e"
+        % (len(ctxs), edges.sum(), t1 - t0, setup, solve, its.max(), rr.max()))
+    seq = []
+    for c in ctxs[:n_single]:
+        ta = time.perf_counter()
+        c.create_pc()
+        c.solve_linear_system(0, 2, 300, 1e-8)
+        seq.append(time.perf_counter() - ta)
+        c.destroy_pc()
+    seq_rate = 2.0 * edges[:n_single].sum() / sum(seq)
+    out = {
+        "workload": "%d uneven MT models, %d..%d edges (%d in all), f log-uniform 1e-2..10 Hz, XY+YX"
+                    % (len(ctxs), edges.min(), edges.max(), edges.sum()),
+        "dof_rhs_per_s": round(2.0 * edges.sum() / (setup + solve), 1),
+        "solve_dof_rhs_per_s": round(2.0 * edges.sum() / solve, 1),
+        "setup_s": round(setup, 3), "solve_s": round(solve, 3),
+        "per_model_ms_p50": round(float(np.percentile(per_model, 50)), 1),
+        "per_model_ms_p99": round(float(np.percentile(per_model, 99)), 1),
+        "max_outer_its": int(its.max()), "max_true_relres": float(rr.max()),
+        "one_at_a_time_dof_rhs_per_s": round(seq_rate, 1),
+        "one_at_a_time_models": n_single,
+    }
+    out["speedup_vs_one_at_a_time"] = round(out["dof_rhs_per_s"] / seq_rate, 3)
+    for c in ctxs:
+        c.close()
+    return out
+
+
+def time_to_1pct_leg(api, mg, cells=10, max_cycles=5, fraction=0.2):
+    """Time-to-1% receiver error of the goal-oriented adaptive MT loop: the adaptive-MT
+    model on a cells^3 grid at two frequencies (1e-3 and 1 Hz class), XY + YX + dual per
+    cycle.  The target is a fine-mesh FP64 forward on the same model refined uniformly
+    twice; the adaptive loop (em_forward with k refinement cycles, each run timed from
+    scratch) is run for k = 0.. until every Z_xy and Z_yx is within 1% of the target,
+    with the mixed and with the plain FP64 inner solve."""
+    m = mg.adaptive_mt(cells=(cells, cells, cells), n_freqs=8, n_rx_side=6)
+    m.freqs = m.freqs[[2, 6]]
+    m.obs = mg.mt_obs(len(m.freqs), len(m.rx), types=(mg.Z_XY_RI, mg.Z_YX_RI))
+    ref = api.Context("-n_uniform_refinements 2 -max_adaptive_refinements 0 -inner_precision fp64 "
+                      "-e_rtol 1e-8 -save_mesh false")
+    ref.load_model(m)
+    ref.set_observations(m.obs)
+    t0 = time.perf_counter()
+    ref.forward()
+    t_ref = time.perf_counter() - t0
+    target = ref.responses()
+    fine_edges = ref.mesh_info()["n_edges"]
+    ref.close()
+    out = {"workload": "adaptive-MT %d^3 cells, 2 frequencies, %d receivers; target: 2x uniform refinement "
+                       "(%d edges), fp64, %.1f s" % (cells, len(m.rx), fine_edges, t_ref)}
+    for prec in ("fp32", "fp64"):
+        trace = []
+        for k in range(max_cycles + 1):
+            c = api.Context("-max_adaptive_refinements %d -refine_fraction %g -inner_precision %s -e_rtol 1e-8 "
+                            "-max_dofs 100000000 -save_mesh false" % (k, fraction, prec))
+            c.load_model(m)
+            c.set_observations(m.obs)
+            t0 = time.perf_counter()
+            c.forward()
+            dt = time.perf_counter() - t0
+            err = float(np.max(np.abs(c.responses() - target) / np.abs(target)))
+            trace.append({"cycles": k, "edges": c.mesh_info()["n_edges"], "s": round(dt, 3), "max_rel_err": err})
+            c.close()
+            log("time-to-1%% %s: %d cycles, %d edges, %.2fs, max rel err %.2e" % (prec, k, trace[-1]["edges"], dt, err))
+            if err <= 0.01:
+                break
+        hit = [t for t in trace if t["max_rel_err"] <= 0.01]
+        out[prec] = {"time_to_1pct_s": hit[0]["s"] if hit else None, "trace": trace}
+    a, b = out["fp32"]["time_to_1pct_s"], out["fp64"]["time_to_1pct_s"]
+    out["mixed_vs_plain"] = round(a / b, 3) if a and b else None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +239,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-csem", action="store_true", help="skip the batched marine-CSEM comparison")
     ap.add_argument("--csem-cells", type=str, default="48,48,40", help="marine-CSEM grid cells x,y,z")
+    ap.add_argument("--no-adaptive", action="store_true", help="skip the time-to-1%% adaptive comparison")
+    ap.add_argument("--batch-models", type=int, default=64,
+                    help="models of the uneven multi-model batch (0 skips it)")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of hipGraph-captured cycles (for kernel tracers)")
     args = ap.parse_args()
@@ -194,7 +293,7 @@ def main():
     peak = 8000.0
     bw = (b_bytes / (b_ms * 1e-3) / 1e9) if b_ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(bw, 1), "peak": peak, "unit": "GB/s", "frac": round(bw / peak, 4),
-            "traffic": None, "kernel": "k_spmm_narrow role 1 (B = C + M, fp32, XY+YX = 4 real columns)", "launch_ms": round(b_ms, 5),
+            "traffic": None, "kernel": "k_spmm_narrow role 1: Chebyshev sweep rres -= Dinv .* (B d), B = C + M fp32, XY+YX = 4 real columns", "launch_ms": round(b_ms, 5),
             "algorithmic_bytes": b_bytes}
     extra = {
         "op_spmm_GBps": round(op_bytes / (op_ms * 1e-3) / 1e9, 1) if op_ms > 0 else None,
@@ -222,6 +321,10 @@ def main():
         ctxp.close()
     if not args.no_csem:
         extra["csem_batched_64tx"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")))
+    if not args.no_adaptive:
+        extra["time_to_1pct"] = time_to_1pct_leg(api, mg)
+    if args.batch_models > 0:
+        extra["uneven_batch"] = batch_leg(api, mg, args.batch_models)
     cpu = None
     if not args.no_cpu:
         try:

@@ -80,6 +80,7 @@ SIGNATURES = {
     "emfem_save_rsp": (I, [P, S]),
     "emfem_get_responses": (I, [P, P]),
     "emfem_locate_points": (I, [P, I, P, P]),
+    "emfem_solve_batch": (I, [P, I, I, I, I, D, P, P, P]),
     "emfem_save_mesh": (I, [P, S, I, P, P]),
     "emfem_forward": (I, [P]),
     "emfem_refine_mesh": (I, [P, P]),
@@ -463,3 +464,19 @@ def refine_fixed_number(error, fraction, flag=None):
     if rc != OK:
         raise EmfemError(rc, "refine_fixed_number")
     return f
+
+
+def solve_batch(contexts, first, n, max_it, rtol, check=True):
+    """Multi-model batch solve (emfem_solve_batch): slots [first, first+n) of every
+    context as one block-diagonal system with per-model Krylov processes.  Returns
+    (iterations [k, n], true relative residuals [k, n], acceptance times in ms [k, n]).
+    Options are taken from contexts[0]."""
+    k = len(contexts)
+    hs = (P * k)(*[c._h for c in contexts])
+    its = np.zeros((k, n), dtype=np.int32)
+    rr = np.zeros((k, n))
+    ms = np.zeros((k, n))
+    rc = lib().emfem_solve_batch(hs, k, int(first), int(n), int(max_it), float(rtol), _p(its), _p(rr), _p(ms))
+    if rc != OK and (check or rc != ERR_NOT_CONVERGED):
+        contexts[0]._check(rc)
+    return its, rr, ms

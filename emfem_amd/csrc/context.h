@@ -59,7 +59,6 @@ struct emfem_context {
   std::unique_ptr<InnerPc> pc;
 
   std::map<std::string, double> stats;
-  KernelClock op_clock;
   double assemble_kernel_ms = 0.0;
 
   std::string opt(const std::string &k) const {

@@ -68,26 +68,42 @@ void assemble_colour(hipStream_t s, int ntets, const int *tet_ids, const int *te
 template <typename T>
 void make_b(hipStream_t s, int nnz, const double2 *K, T *B);
 
-// Per-column dot products sum_r x[r][c] * y[r][c] for c < nc, written (in double) to
-// out[c] on the device.  `work` must hold dot_work_size(rows, nc) doubles.
-size_t dot_work_size(int rows, int nc);
+// Rows split into independent segments (the models of a multi-model batch, stacked
+// block-diagonally).  Per-column scalars are then per (segment, column) at
+// seg * nc + c ("logical columns", nl = nseg * nc), and reductions run over chunks
+// of rows that never straddle a segment.  The default is one segment with no tables.
+struct Seg {
+  int nseg = 1;
+  int nchunks = 0;                 // reduction chunks (0: an even split of all rows)
+  const int *rseg = nullptr;       // [rows] segment of each row (null: all 0)
+  const int *chunk_row = nullptr;  // [nchunks + 1] first row of each chunk
+  const int *seg_chunk = nullptr;  // [nseg + 1] first chunk of each segment
+};
+
+// Per-column dot products sum_r x[r][c] * y[r][c], written (in double) to
+// out[seg * nc + c] on the device.  `work` must hold dot_work_size(rows, nc, sg)
+// doubles.  Results are bitwise reproducible (fixed summation order).
+size_t dot_work_size(int rows, int nc, const Seg &sg = Seg());
 template <typename T>
-void coldot(hipStream_t s, int rows, int nc, const T *x, int ldx, const T *y, int ldy, double *out, double *work);
+void coldot(hipStream_t s, int rows, int nc, const T *x, int ldx, const T *y, int ldy, double *out, double *work,
+            const Seg &sg = Seg());
 
-// Multi-dot for Gram-Schmidt: out[i*nc + c] = sum_r V_i[r][c] w[r][c] for i < m,
-// where V_i = V + i * vstride.  work must hold dot_work_size(rows, nc) * 8 doubles.
+// Multi-dot for Gram-Schmidt: out[i*nl + seg*nc + c] = sum_r V_i[r][c] w[r][c] for
+// i < m, where V_i = V + i * vstride.  work must hold dot_work_size(rows, nc, sg) * 8
+// doubles.
 void mdot(hipStream_t s, int rows, int nc, const double *V, size_t vstride, int m, const double *w, double *out,
-          double *work);
+          double *work, const Seg &sg = Seg());
 
-// w[r][c] -= sum_{i<m} h[i*hld + c] * V_i[r][c]   (V_i = V + i * vstride, type TB)
+// w[r][c] -= sign * sum_{i<m} h[i*hld + seg(r)*nc + c] * V_i[r][c]
 template <typename TB>
 void mupdate(hipStream_t s, int rows, int nc, const TB *V, size_t vstride, int m, const double *h, int hld,
-             double *w, double sign);
+             double *w, double sign, const Seg &sg = Seg());
 
-// y[r][c] = a[c] * x[r][c] + b[c] * y[r][c]  (a, b: device doubles; b may be null = 0)
+// y[r][c] = a[k] * x[r][c] + b[k] * y[r][c], k = seg(r)*nc + c (a, b: device
+// doubles; b may be null = 0)
 template <typename TX, typename TY>
 void axpby_cols(hipStream_t s, int rows, int nc, const double *a, const TX *x, int ldx, const double *b, TY *y,
-                int ldy);
+                int ldy, const Seg &sg = Seg());
 
 // y[r][c] = alpha * x[r][c] + beta * y[r][c] with host scalars (beta = 0 ignores y).
 template <typename TX, typename TY>
@@ -97,7 +113,7 @@ void axpby(hipStream_t s, int rows, int nc, double alpha, const TX *x, int ldx, 
 //   x += alpha[c] p ; r -= alpha[c] q ; rr[c] = r . r (fused reduction)
 template <typename T>
 void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const T *p, const T *q, T *x, T *r, double *rr,
-            double *work);
+            double *work, const Seg &sg = Seg());
 
 // Chebyshev smoother helpers (Dinv = 1 / l1 row norms):
 //   init: rres = Dinv .* b ; d = rres / theta ; x = d

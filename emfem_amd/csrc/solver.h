@@ -27,20 +27,13 @@ struct SolveStats {
   std::vector<int> its;      // per right-hand side
   std::vector<double> relres;
   std::vector<uint8_t> converged;  // the reference's stopping test was met
+  std::vector<double> done_ms;     // per right-hand side: time from the start until it was accepted
 };
 
-// Timing of a class of kernel launches with HIP events on the solver stream.
-struct KernelClock {
-  std::vector<hipEvent_t> ev;
-  double bytes_per_launch = 0.0;
-  size_t max_pairs = 256;
-  void start(hipStream_t s);
-  void stop(hipStream_t s);
-  bool armed() const { return ev.size() / 2 < max_pairs; }
-  // average ms per launch (stream must be idle), then clears
-  double average_ms(int *count);
-  ~KernelClock();
-};
+// ms per launch of the complex operator on nrhs interleaved pairs (x -> y), from HIP
+// events around `reps` back-to-back launches on stream s.
+double time_operator(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, int nrhs,
+                     const double *x, double *y, int reps);
 
 struct PcOptions {
   int cheb_degree = 3;
@@ -59,13 +52,17 @@ class InnerPc {
   virtual ~InnerPc() = default;
   // x = approx B^-1 b by FCG + AMS; b, x are fp64 [E][nc] compact.
   virtual void apply(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) = 0;
+  // rows split into independent segments (a multi-model batch): the FCG runs per
+  // (segment, column); the cycle itself acts on the block-diagonal operator as is
+  virtual void set_segments(const Seg &sg) = 0;
   virtual void set_fp32(bool fp32, hipStream_t s) = 0;
   virtual bool fp32() const = 0;
   // replay the cycles as captured hipGraphs (off: plain launches, timed per SpMM)
   virtual void set_use_graph(bool g) = 0;
-  // B SpMMs of the Chebyshev smoother (precision of the cycle) and of the fp64 FCG
-  virtual KernelClock &smoother_clock() = 0;
-  virtual KernelClock &fcg_clock() = 0;
+  // ms per launch of the B SpMM of the Chebyshev smoother (cycle precision) and of
+  // the fp64 FCG on nc columns, from HIP events around `reps` back-to-back launches
+  virtual double time_smoother(hipStream_t s, int nc, int reps) = 0;
+  virtual double time_fcg(hipStream_t s, int nc, int reps) = 0;
 };
 
 std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp,
@@ -81,9 +78,12 @@ struct OuterOptions {
   int stall_window = 15;
 };
 
-// FGMRES on nrhs interleaved right-hand sides b [E][2 nrhs] -> x [E][2 nrhs].
+// FGMRES on nrhs interleaved right-hand sides b [E][2 nrhs] -> x [E][2 nrhs].  With
+// segments (a block-diagonal multi-model batch) every (segment, right-hand side) is
+// an independent system with its own Arnoldi process and stopping test; rtol and the
+// per-RHS statistics are then indexed seg * nrhs + k.
 void fgmres(hipStream_t s, int ne, int nnz, const int *d_rp, const int *d_ci, const double2 *d_K, InnerPc &pc,
             int nrhs, const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o,
-            SolveStats &st, KernelClock *op_clock);
+            SolveStats &st, const Seg &sg = Seg());
 
 }  // namespace emfem

@@ -17,41 +17,31 @@
 namespace emfem {
 
 // ---------------------------------------------------------------------------------
-// kernel clock
+// kernel timing: HIP events around back-to-back launches on the launch stream
 // ---------------------------------------------------------------------------------
 
-void KernelClock::start(hipStream_t s) {
-  hipEvent_t e;
-  EMF_HIP(hipEventCreate(&e));
-  EMF_HIP(hipEventRecord(e, s));
-  ev.push_back(e);
+template <typename F>
+double time_launches(hipStream_t s, int reps, F &&launch) {
+  launch();  // warm: caches, first-touch, lanes-per-row tuning
+  hipEvent_t e0, e1;
+  EMF_HIP(hipEventCreate(&e0));
+  EMF_HIP(hipEventCreate(&e1));
+  EMF_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) launch();
+  EMF_HIP(hipEventRecord(e1, s));
+  EMF_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  EMF_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return reps > 0 ? ms / reps : 0.0;
 }
 
-void KernelClock::stop(hipStream_t s) {
-  hipEvent_t e;
-  EMF_HIP(hipEventCreate(&e));
-  EMF_HIP(hipEventRecord(e, s));
-  ev.push_back(e);
-}
-
-double KernelClock::average_ms(int *count) {
-  double total = 0.0;
-  int n = 0;
-  for (size_t i = 0; i + 1 < ev.size(); i += 2) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) {
-      total += ms;
-      ++n;
-    }
-  }
-  for (auto e : ev) (void)hipEventDestroy(e);
-  ev.clear();
-  if (count) *count = n;
-  return n ? total / n : 0.0;
-}
-
-KernelClock::~KernelClock() {
-  for (auto e : ev) (void)hipEventDestroy(e);
+double time_operator(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, int nrhs,
+                     const double *x, double *y, int reps) {
+  return time_launches(s, reps, [&]() {
+    op_spmm<double>(s, ne, nnz, rp, ci, K, nrhs, x, 2 * nrhs, y, 2 * nrhs, 1.0, 0.0, nullptr, 0);
+  });
 }
 
 namespace {
@@ -252,10 +242,10 @@ class DevAms {
   // out = M_AMS b: b fp64 compact [E][nc], out in T; out must not alias b.  With
   // use_graph the ~100 launches of one cycle are captured once per (nc, b, out)
   // and replayed as a single hipGraph launch (no memset nodes are captured).
-  void apply(hipStream_t s, int nc, const double *b, T *out, KernelClock *clk, bool use_graph) {
+  void apply(hipStream_t s, int nc, const double *b, T *out, bool use_graph) {
     tune(s, nc);
     if (!use_graph) {
-      launch_cycle(s, nc, b, out, clk);
+      launch_cycle(s, nc, b, out);
       return;
     }
     const auto key = std::make_tuple(nc, (const void *)b, (const void *)out);
@@ -263,7 +253,7 @@ class DevAms {
     if (it == graphs_.end()) {
       hipGraph_t g;
       EMF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      launch_cycle(s, nc, b, out, nullptr);
+      launch_cycle(s, nc, b, out);
       EMF_HIP(hipStreamEndCapture(s, &g));
       hipGraphExec_t ex;
       EMF_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
@@ -273,8 +263,16 @@ class DevAms {
     EMF_HIP(hipGraphLaunch(it->second, s));
   }
 
+  // ms per launch of the smoother SpMM in its Chebyshev form, rres -= Dinv .* (B d),
+  // on nc columns (lanes per row as tuned), on the cycle's own buffers
+  double time_b(hipStream_t s, int nc, int reps) {
+    tune(s, nc);
+    return time_launches(s, reps,
+                         [&]() { bspmm(s, nc, d_.get(), rres_.get(), T(-1), l1inv_.get(), T(0), nullptr, T(1), rres_.get()); });
+  }
+
  private:
-  void launch_cycle(hipStream_t s, int nc, const double *b, T *out, KernelClock *clk) {
+  void launch_cycle(hipStream_t s, int nc, const double *b, T *out) {
     T *rres = rres_.get(), *d = d_.get();
     const T *bt;
     if constexpr (std::is_same<T, double>::value) {
@@ -285,7 +283,7 @@ class DevAms {
     }
     // pre-smoothing (Chebyshev from zero)
     cheb_init<T>(s, ne_, nc, l1inv_.get(), bt, rres, d, out, 1.0 / theta_);
-    cheb_iterate(s, nc, rres, d, out, clk);
+    cheb_iterate(s, nc, rres, d, out);
     if (!additive_) aux_gradient(s, nc, b, out);
     residual64(s, nc, b, out);
     // the nodal-vector components (and, additive, the gradient) in one
@@ -296,20 +294,17 @@ class DevAms {
     spmm<T>(s, Pi_.view(), nc, vx_.get(), nc, out, nc, T(1), nullptr, T(0), nullptr, 0, T(1), out, nc);
     if (!additive_) aux_gradient(s, nc, b, out);
     // post-smoothing (Chebyshev from the current iterate)
-    timed_spmm(s, nc, out, rres, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr, clk);
+    bspmm(s, nc, out, rres, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr);
     axpby<T, T>(s, ne_, nc, 1.0 / theta_, rres, nc, 0.0, d, nc);
     axpby<T, T>(s, ne_, nc, 1.0, d, nc, 1.0, out, nc);
-    cheb_iterate(s, nc, rres, d, out, clk);
+    cheb_iterate(s, nc, rres, d, out);
   }
 
-  void timed_spmm(hipStream_t s, int nc, const T *x, T *y, T alpha, const T *rs, T gamma, const T *w, T beta,
-                  const T *z, KernelClock *clk) {
-    const bool t = clk && clk->armed();
-    if (t) clk->start(s);
+  void bspmm(hipStream_t s, int nc, const T *x, T *y, T alpha, const T *rs, T gamma, const T *w, T beta,
+             const T *z) {
     Csr<T> B = B_;
     B.lpr = lpr_[nc];
     spmm<T>(s, B, nc, x, nc, y, nc, alpha, rs, gamma, w, nc, beta, z, nc);
-    if (t) clk->stop(s);
   }
 
   // r64 = b - B x in fp64 (the residual the auxiliary spaces see)
@@ -317,11 +312,11 @@ class DevAms {
     spmm_mixed<double, T, double>(s, B64_, nc, x, nc, r64_.get(), nc, -1.0, nullptr, 1.0, b, nc, 0.0, nullptr, 0);
   }
 
-  void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x, KernelClock *clk) {
+  void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x) {
     double sigma = theta_ / delta_, rho = 1.0 / sigma;
     for (int k = 1; k < deg_; ++k) {
       // rres -= l1inv .* (B d)
-      timed_spmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres, clk);
+      bspmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres);
       const double rho_new = 1.0 / (2.0 * sigma - rho);
       cheb_step<T>(s, ne_, nc, d, rres, x, rho_new * rho, 2.0 * rho_new / delta_);
       rho = rho_new;
@@ -364,8 +359,14 @@ class InnerPcImpl : public InnerPc {
 
   bool fp32() const override { return fp32_; }
   void set_use_graph(bool g) override { o_.use_graph = g; }
-  KernelClock &smoother_clock() override { return smooth_clock_; }
-  KernelClock &fcg_clock() override { return fcg_clock_; }
+  double time_smoother(hipStream_t s, int nc, int reps) override {
+    ensure(nc);
+    return fp32_ ? a32_->time_b(s, nc, reps) : a64_->time_b(s, nc, reps);
+  }
+  double time_fcg(hipStream_t s, int nc, int reps) override {
+    ensure(nc);
+    return time_launches(s, reps, [&]() { bspmm(s, nc, p_.get(), q_.get()); });
+  }
 
   void set_fp32(bool fp32, hipStream_t s) override {
     fp32_ = fp32;
@@ -386,96 +387,103 @@ class InnerPcImpl : public InnerPc {
     solve(s, nc, b, x, st);
   }
 
+  void set_segments(const Seg &sg) override {
+    sg_ = sg;
+    nc_ = 0;  // scalar buffers are sized per logical column
+  }
+
  private:
   const Csr<double> &B64() const { return a64_ ? a64_->B64() : a32_->B64(); }
 
   void ensure(int nc) {
     if (nc <= nc_) return;
     nc_ = nc;
+    const int nl = sg_.nseg * nc;
     const size_t n = (size_t)ne_ * nc;
     r_.resize(n);
     z_.resize(n);
     p_.resize(n);
     q_.resize(n);
     zt_.resize(n);
-    for (auto *v : {&rz_, &pq_, &zq_, &alpha_, &beta_, &rr_, &ones_}) v->resize(nc);
-    active_.resize(nc);
-    work_.resize(dot_work_size(ne_, nc) * 8);
-    std::vector<double> one(nc, 1.0);
+    for (auto *v : {&rz_, &pq_, &zq_, &alpha_, &beta_, &rr_, &ones_}) v->resize(nl);
+    active_.resize(nl);
+    work_.resize(dot_work_size(ne_, nc, sg_) * 8);
+    std::vector<double> one(nl, 1.0);
     ones_.upload(one, 0);
     EMF_HIP(hipDeviceSynchronize());
     if (a32_) a32_->ensure(nc);
     if (a64_) a64_->ensure(nc);
-    h_bb_.resize(nc);
-    h_rr_.resize(nc);
-    h_active_.resize(nc);
+    h_bb_.resize(nl);
+    h_rr_.resize(nl);
+    h_active_.resize(nl);
   }
 
   void precondition(hipStream_t s, int nc, const double *r, double *z, SolveStats &st) {
     ++st.vcycles;
     if (!fp32_) {
-      a64_->apply(s, nc, r, z, &smooth_clock_, o_.use_graph);
+      a64_->apply(s, nc, r, z, o_.use_graph);
     } else {
-      a32_->apply(s, nc, r, zt_.get(), &smooth_clock_, o_.use_graph);
+      a32_->apply(s, nc, r, zt_.get(), o_.use_graph);
       axpby<float, double>(s, ne_, nc, 1.0, zt_.get(), nc, 0.0, z, nc);
     }
   }
 
+  // Flexible CG per logical column (segment, column): each stops on its own
+  // unpreconditioned residual; frozen columns keep alpha = beta = 0.
   void solve(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) {
     double *r = r_.get(), *z = z_.get(), *p = p_.get(), *q = q_.get();
     double *w = work_.get();
+    const int nl = sg_.nseg * nc;
     ++st.pc_applies;
     // x = 0, r = b
     EMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * (size_t)ne_ * nc, s));
     EMF_HIP(hipMemcpyAsync(r, b, sizeof(double) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
-    coldot<double>(s, ne_, nc, b, nc, b, nc, rr_.get(), w);
-    EMF_HIP(hipMemcpyAsync(h_bb_.data(), rr_.get(), sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    coldot<double>(s, ne_, nc, b, nc, b, nc, rr_.get(), w, sg_);
+    EMF_HIP(hipMemcpyAsync(h_bb_.data(), rr_.get(), sizeof(double) * nl, hipMemcpyDeviceToHost, s));
     EMF_HIP(hipStreamSynchronize(s));
     bool any = false;
-    for (int c = 0; c < nc; ++c) {
+    for (int c = 0; c < nl; ++c) {
       h_active_[c] = h_bb_[c] > 0.0;
       any |= h_active_[c] != 0;
     }
     if (!any) return;
-    active_.upload(h_active_.data(), nc, s);
+    active_.upload(h_active_.data(), nl, s);
     const double tol2 = o_.inner_rtol * o_.inner_rtol;
     // z = M r ; p = z ; q = B p
     precondition(s, nc, r, z, st);
     EMF_HIP(hipMemcpyAsync(p, z, sizeof(double) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
     bspmm(s, nc, p, q);
-    coldot<double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w);
-    coldot<double>(s, ne_, nc, r, nc, z, nc, rz_.get(), w);
+    coldot<double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
+    coldot<double>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
     for (int it = 1; it <= o_.inner_max_it; ++it) {
-      fcg_alpha(s, nc, rz_.get(), pq_.get(), active_.get(), alpha_.get());
-      fcg_xr<double>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w);
+      fcg_alpha(s, nl, rz_.get(), pq_.get(), active_.get(), alpha_.get());
+      fcg_xr<double>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w, sg_);
       ++st.inner_its;
-      EMF_HIP(hipMemcpyAsync(h_rr_.data(), rr_.get(), sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+      EMF_HIP(hipMemcpyAsync(h_rr_.data(), rr_.get(), sizeof(double) * nl, hipMemcpyDeviceToHost, s));
       EMF_HIP(hipStreamSynchronize(s));
       any = false;
-      for (int c = 0; c < nc; ++c) {
+      for (int c = 0; c < nl; ++c) {
         if (h_active_[c] && h_rr_[c] <= tol2 * h_bb_[c]) h_active_[c] = 0;
         any |= h_active_[c] != 0;
       }
       if (!any || it == o_.inner_max_it) break;
-      active_.upload(h_active_.data(), nc, s);
+      active_.upload(h_active_.data(), nl, s);
       precondition(s, nc, r, z, st);
-      coldot<double>(s, ne_, nc, z, nc, q, nc, zq_.get(), w);
-      coldot<double>(s, ne_, nc, r, nc, z, nc, rz_.get(), w);
-      fcg_beta(s, nc, zq_.get(), pq_.get(), active_.get(), beta_.get());
-      axpby_cols<double, double>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc);
+      coldot<double>(s, ne_, nc, z, nc, q, nc, zq_.get(), w, sg_);
+      coldot<double>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
+      fcg_beta(s, nl, zq_.get(), pq_.get(), active_.get(), beta_.get());
+      axpby_cols<double, double>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc, sg_);
       bspmm(s, nc, p, q);
-      coldot<double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w);
+      coldot<double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
     }
   }
 
   void bspmm(hipStream_t s, int nc, const double *x, double *y) {
-    const bool t = fcg_clock_.armed();
-    if (t) fcg_clock_.start(s);
     spmm<double>(s, B64(), nc, x, nc, y, nc, 1.0, nullptr, 0.0, nullptr, 0, 0.0, nullptr, 0);
-    if (t) fcg_clock_.stop(s);
   }
 
   const AmsHost &H_;
+  Seg sg_;
   int ne_ = 0, nnz_ = 0, nc_ = 0;
   const int *rp_, *ci_;
   const double2 *K_;
@@ -489,7 +497,6 @@ class InnerPcImpl : public InnerPc {
   DevBuf<uint8_t> active_;
   std::vector<double> h_bb_, h_rr_;
   std::vector<uint8_t> h_active_;
-  KernelClock smooth_clock_, fcg_clock_;
 };
 
 }  // namespace
@@ -508,26 +515,27 @@ namespace {
 // Convergence is decided on the true fp64 residual at every restart: a right-hand
 // side whose Arnoldi estimate converged inside a cycle is frozen for the rest of
 // that cycle and re-checked (and, if needed, resumed) at the next restart.
-void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, InnerPc &pc, int nrhs,
-                 const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o, SolveStats &st,
-                 KernelClock *clk) {
-  const int nc = 2 * nrhs;
+void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, InnerPc &pc,
+                 int nrhs_seg, const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o,
+                 SolveStats &st, const Seg &sg) {
+  // nc columns per row; nrhs logical right-hand sides = segments x column pairs, their
+  // scalars at [2 r], [2 r + 1] of nl-long arrays
+  const int nc = 2 * nrhs_seg;
+  const int nrhs = sg.nseg * nrhs_seg;
+  const int nl = 2 * nrhs;
   const int m = std::max(1, o.restart);
   const size_t blk = (size_t)ne * nc;
-  DevBuf<double> V((m + 1) * blk), w(blk), dots((size_t)(m + 1) * nc), coef((size_t)(m + 1) * nc), scal(nc);
+  DevBuf<double> V((m + 1) * blk), w(blk), dots((size_t)(m + 1) * nl), coef((size_t)(m + 1) * nl), scal(nl);
   DevBuf<double> Z(m * blk);
   std::vector<std::vector<double>> hist(nrhs);
-  DevBuf<double> work(dot_work_size(ne, nc) * 8);
-  std::vector<double> h_dots((size_t)(m + 1) * nc), h_coef((size_t)(m + 1) * nc), h_scal(nc);
+  DevBuf<double> work(dot_work_size(ne, nc, sg) * 8);
+  std::vector<double> h_dots((size_t)(m + 1) * nl), h_coef((size_t)(m + 1) * nl), h_scal(nl);
   auto op = [&](const auto *in, double *out, double alpha, double beta, const double *z) {
-    const bool t = clk && clk->armed();
-    if (t) clk->start(s);
-    op_spmm(s, ne, nnz, rp, ci, K, nrhs, in, nc, out, nc, alpha, beta, z, nc);
-    if (t) clk->stop(s);
+    op_spmm(s, ne, nnz, rp, ci, K, nrhs_seg, in, nc, out, nc, alpha, beta, z, nc);
   };
   auto rhs_norms = [&](const double *v, std::vector<double> &out) {
-    coldot<double>(s, ne, nc, v, nc, v, nc, dots.get(), work.get());
-    EMF_HIP(hipMemcpyAsync(h_dots.data(), dots.get(), sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    coldot<double>(s, ne, nc, v, nc, v, nc, dots.get(), work.get(), sg);
+    EMF_HIP(hipMemcpyAsync(h_dots.data(), dots.get(), sizeof(double) * nl, hipMemcpyDeviceToHost, s));
     EMF_HIP(hipStreamSynchronize(s));
     out.resize(nrhs);
     for (int r = 0; r < nrhs; ++r) out[r] = std::sqrt(h_dots[2 * r] + h_dots[2 * r + 1]);
@@ -535,8 +543,8 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
   // dst = fac .* src per right-hand side (in place allowed)
   auto scale_cols = [&](const double *src, double *dst, const std::vector<double> &fac) {
     for (int r = 0; r < nrhs; ++r) h_scal[2 * r] = h_scal[2 * r + 1] = fac[r];
-    scal.upload(h_scal.data(), nc, s);
-    axpby_cols<double, double>(s, ne, nc, scal.get(), src, nc, nullptr, dst, nc);
+    scal.upload(h_scal.data(), nl, s);
+    axpby_cols<double, double>(s, ne, nc, scal.get(), src, nc, nullptr, dst, nc, sg);
   };
 
   std::vector<double> bnorm;
@@ -544,6 +552,8 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
   st.its.assign(nrhs, 0);
   st.relres.assign(nrhs, 0.0);
   st.converged.assign(nrhs, 0);
+  st.done_ms.assign(nrhs, -1.0);
+  const double t_start = now_ms();
   std::vector<uint8_t> done(nrhs, 0), est_ok(nrhs, 0);
   std::vector<double> last_true(nrhs, 0.0);
   for (int r = 0; r < nrhs; ++r) done[r] = bnorm[r] == 0.0;
@@ -573,6 +583,9 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
       }
       done[r] = true_ok || floor;
       if (done[r]) st.converged[r] = 1;
+      // accepted: keep the time its Arnoldi process stopped; rejected: running again
+      if (done[r] && st.done_ms[r] < 0) st.done_ms[r] = now_ms() - t_start;
+      if (!done[r]) st.done_ms[r] = -1.0;
       active[r] = !done[r];
       any |= active[r] != 0;
     }
@@ -594,17 +607,17 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
       pc.apply(s, nc, Vj, Zj, st);
       op(Zj, w.get(), 1.0, 0.0, nullptr);
       // classical Gram-Schmidt against V_0..V_j
-      mdot(s, ne, nc, V.get(), blk, j + 1, w.get(), dots.get(), work.get());
-      EMF_HIP(hipMemcpyAsync(h_dots.data(), dots.get(), sizeof(double) * (j + 1) * nc, hipMemcpyDeviceToHost, s));
+      mdot(s, ne, nc, V.get(), blk, j + 1, w.get(), dots.get(), work.get(), sg);
+      EMF_HIP(hipMemcpyAsync(h_dots.data(), dots.get(), sizeof(double) * (j + 1) * nl, hipMemcpyDeviceToHost, s));
       EMF_HIP(hipStreamSynchronize(s));
       for (int i = 0; i <= j; ++i)
         for (int r = 0; r < nrhs; ++r) {
-          const double hij = active[r] ? h_dots[(size_t)i * nc + 2 * r] + h_dots[(size_t)i * nc + 2 * r + 1] : 0.0;
+          const double hij = active[r] ? h_dots[(size_t)i * nl + 2 * r] + h_dots[(size_t)i * nl + 2 * r + 1] : 0.0;
           H[((size_t)r * (m + 1) + i) * m + j] = hij;
-          h_coef[(size_t)i * nc + 2 * r] = h_coef[(size_t)i * nc + 2 * r + 1] = hij;
+          h_coef[(size_t)i * nl + 2 * r] = h_coef[(size_t)i * nl + 2 * r + 1] = hij;
         }
-      coef.upload(h_coef.data(), (size_t)(j + 1) * nc, s);
-      mupdate<double>(s, ne, nc, V.get(), blk, j + 1, coef.get(), nc, w.get(), 1.0);
+      coef.upload(h_coef.data(), (size_t)(j + 1) * nl, s);
+      mupdate<double>(s, ne, nc, V.get(), blk, j + 1, coef.get(), nl, w.get(), 1.0, sg);
       std::vector<double> hn;
       rhs_norms(w.get(), hn);
       std::vector<double> fac(nrhs);
@@ -644,6 +657,7 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
           active[r] = 0;  // converged in this cycle: freeze its basis
           done[r] = 1;
           est_ok[r] = 1;
+          st.done_ms[r] = now_ms() - t_start;
         } else {
           still = true;
         }
@@ -669,11 +683,11 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
         for (int l = i + 1; l < k; ++l) t -= Hr[(size_t)i * m + l] * y[l];
         y[i] = Hr[(size_t)i * m + i] != 0.0 ? t / Hr[(size_t)i * m + i] : 0.0;
       }
-      for (int i = 0; i < k; ++i) h_coef[(size_t)i * nc + 2 * r] = h_coef[(size_t)i * nc + 2 * r + 1] = y[i];
+      for (int i = 0; i < k; ++i) h_coef[(size_t)i * nl + 2 * r] = h_coef[(size_t)i * nl + 2 * r + 1] = y[i];
     }
     if (kmax > 0) {
-      coef.upload(h_coef.data(), (size_t)kmax * nc, s);
-      mupdate<double>(s, ne, nc, Z.get(), blk, kmax, coef.get(), nc, x, -1.0);
+      coef.upload(h_coef.data(), (size_t)kmax * nl, s);
+      mupdate<double>(s, ne, nc, Z.get(), blk, kmax, coef.get(), nl, x, -1.0, sg);
     }
   }
   // true residual
@@ -682,6 +696,7 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
   rhs_norms(w.get(), rn);
   st.outer_its = 0;
   for (int r = 0; r < nrhs; ++r) {
+    if (st.done_ms[r] < 0) st.done_ms[r] = now_ms() - t_start;
     st.relres[r] = bnorm[r] > 0 ? rn[r] / bnorm[r] : 0.0;
     st.outer_its = std::max(st.outer_its, st.its[r]);
   }
@@ -691,8 +706,9 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
 
 void fgmres(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, InnerPc &pc, int nrhs,
             const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o, SolveStats &st,
-            KernelClock *clk) {
-  fgmres_impl(s, ne, nnz, rp, ci, K, pc, nrhs, b, x, rtol, o, st, clk);
+            const Seg &sg) {
+  pc.set_segments(sg);
+  fgmres_impl(s, ne, nnz, rp, ci, K, pc, nrhs, b, x, rtol, o, st, sg);
 }
 
 }  // namespace emfem

@@ -486,20 +486,32 @@ namespace {
 
 constexpr int kDotBlocks = 512;
 
-// Threads of a block are assigned to columns c = tid % nc (block size is a multiple
-// of nc) and stride over rows; the per-column block sums go to work[block][c].
+// Rows [r0, r1) of reduction block b: the segment chunks, or an even split of all rows.
+__device__ __forceinline__ void chunk_rows(const Seg &sg, int rows, int b, int nb, int &r0, int &r1) {
+  if (sg.chunk_row) {
+    r0 = sg.chunk_row[b];
+    r1 = sg.chunk_row[b + 1];
+  } else {
+    r0 = (int)((int64_t)rows * b / nb);
+    r1 = (int)((int64_t)rows * (b + 1) / nb);
+  }
+}
+
+// Threads of a block are assigned to columns c = tid % nc and stride over the
+// block's rows; the per-column block sums go to work[block][c].
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_coldot(int rows, int nc, int rows_per_pass, const T *__restrict__ x,
-                                                   int ldx, const T *__restrict__ y, int ldy, double *work) {
+__global__ __launch_bounds__(kBlock) void k_coldot(int rows, int nc, const T *__restrict__ x, int ldx,
+                                                   const T *__restrict__ y, int ldy, double *work, Seg sg) {
   __shared__ double sh[kBlock];
-  const int per_block = (kBlock / nc) * nc;
+  const int rpp = kBlock / nc;
+  const int per_block = rpp * nc;
   const int t = threadIdx.x;
+  int r0, r1;
+  chunk_rows(sg, rows, blockIdx.x, gridDim.x, r0, r1);
   double acc = 0.0;
   if (t < per_block) {
     const int c = t % nc;
-    const int r0 = t / nc;
-    for (int r = blockIdx.x * rows_per_pass + r0; r < rows; r += gridDim.x * rows_per_pass)
-      acc += (double)x[(size_t)r * ldx + c] * (double)y[(size_t)r * ldy + c];
+    for (int r = r0 + t / nc; r < r1; r += rpp) acc += (double)x[(size_t)r * ldx + c] * (double)y[(size_t)r * ldy + c];
   }
   sh[t] = acc;
   __syncthreads();
@@ -521,55 +533,74 @@ __device__ __forceinline__ void block_tree_sum(double *sh) {
   __syncthreads();
 }
 
-// One block per column: the per-block partials are summed in a fixed order (strided
-// per thread, then a tree), so results do not depend on scheduling.
-__global__ __launch_bounds__(kBlock) void k_finish_dot(int nblocks, int nc, const double *__restrict__ work,
-                                                       double *out) {
+// One block per output (basis vector i, segment s, column c): the partials of the
+// segment's reduction blocks are summed in a fixed order (strided per thread, then a
+// tree), so results do not depend on scheduling.  work is [i][block][c]; out is
+// [i][s * nc + c].
+__global__ __launch_bounds__(kBlock) void k_finish(int nblocks, int nc, const double *__restrict__ work, double *out,
+                                                   Seg sg) {
   __shared__ double sh[kBlock];
-  const int c = blockIdx.x;
+  const int nl = sg.nseg * nc;
+  const int idx = blockIdx.x;
+  const int i = idx / nl, rem = idx % nl, sgi = rem / nc, c = rem % nc;
+  const int b0 = sg.seg_chunk ? sg.seg_chunk[sgi] : 0, b1 = sg.seg_chunk ? sg.seg_chunk[sgi + 1] : nblocks;
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += kBlock) s += work[(size_t)b * nc + c];
+  for (int b = b0 + threadIdx.x; b < b1; b += kBlock) s += work[((size_t)i * nblocks + b) * nc + c];
   sh[threadIdx.x] = s;
   block_tree_sum(sh);
-  if (threadIdx.x == 0) out[c] = sh[0];
+  if (threadIdx.x == 0) out[idx] = sh[0];
+}
+
+int reduce_blocks(int rows, int nc, const Seg &sg) {
+  if (sg.chunk_row) return sg.nchunks;
+  const int rpp = kBlock / nc;
+  return std::max(1, std::min(kDotBlocks, (rows + rpp - 1) / rpp));
 }
 
 }  // namespace
 
-size_t dot_work_size(int rows, int nc) { return (size_t)kDotBlocks * std::max(nc, 1) + 64; }
+size_t dot_work_size(int rows, int nc, const Seg &sg) {
+  (void)rows;
+  return (size_t)std::max(kDotBlocks, sg.nchunks) * std::max(nc, 1) + 64;
+}
 
 template <typename T>
-void coldot(hipStream_t s, int rows, int nc, const T *x, int ldx, const T *y, int ldy, double *out, double *work) {
+void coldot(hipStream_t s, int rows, int nc, const T *x, int ldx, const T *y, int ldy, double *out, double *work,
+            const Seg &sg) {
   if (nc > kBlock) {
+    EMF_REQUIRE(sg.nseg == 1, kErrArg, "segmented reductions support at most 256 columns");
     for (int c0 = 0; c0 < nc; c0 += kBlock)
-      coldot<T>(s, rows, std::min(kBlock, nc - c0), x + c0, ldx, y + c0, ldy, out + c0, work);
+      coldot<T>(s, rows, std::min(kBlock, nc - c0), x + c0, ldx, y + c0, ldy, out + c0, work, sg);
     return;
   }
-  const int rows_per_pass = kBlock / nc;
-  const int nb = std::max(1, std::min(kDotBlocks, (rows + rows_per_pass - 1) / rows_per_pass));
-  hipLaunchKernelGGL((k_coldot<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, rows_per_pass, x, ldx, y, ldy, work);
-  hipLaunchKernelGGL(k_finish_dot, dim3(nc), dim3(kBlock), 0, s, nb, nc, work, out);
+  const int nb = reduce_blocks(rows, nc, sg);
+  hipLaunchKernelGGL((k_coldot<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, x, ldx, y, ldy, work, sg);
+  hipLaunchKernelGGL(k_finish, dim3(sg.nseg * nc), dim3(kBlock), 0, s, nb, nc, work, out, sg);
   EMF_HIP(hipGetLastError());
 }
-template void coldot<float>(hipStream_t, int, int, const float *, int, const float *, int, double *, double *);
-template void coldot<double>(hipStream_t, int, int, const double *, int, const double *, int, double *, double *);
+template void coldot<float>(hipStream_t, int, int, const float *, int, const float *, int, double *, double *,
+                            const Seg &);
+template void coldot<double>(hipStream_t, int, int, const double *, int, const double *, int, double *, double *,
+                             const Seg &);
 
 namespace {
 
 // Up to 8 basis vectors per pass; w is read once per pass.
 template <int M>
-__global__ __launch_bounds__(kBlock) void k_mdot(int rows, int nc, int rows_per_pass, const double *__restrict__ V,
-                                                 size_t vstride, const double *__restrict__ w, double *work) {
+__global__ __launch_bounds__(kBlock) void k_mdot(int rows, int nc, const double *__restrict__ V, size_t vstride,
+                                                 const double *__restrict__ w, double *work, Seg sg) {
   __shared__ double sh[M][kBlock];
-  const int per_block = (kBlock / nc) * nc;
+  const int rpp = kBlock / nc;
+  const int per_block = rpp * nc;
   const int t = threadIdx.x;
+  int r0, r1;
+  chunk_rows(sg, rows, blockIdx.x, gridDim.x, r0, r1);
   double acc[M];
 #pragma unroll
   for (int i = 0; i < M; ++i) acc[i] = 0.0;
   if (t < per_block) {
     const int c = t % nc;
-    const int r0 = t / nc;
-    for (int r = blockIdx.x * rows_per_pass + r0; r < rows; r += gridDim.x * rows_per_pass) {
+    for (int r = r0 + t / nc; r < r1; r += rpp) {
       const size_t o = (size_t)r * nc + c;
       const double wv = w[o];
 #pragma unroll
@@ -589,39 +620,25 @@ __global__ __launch_bounds__(kBlock) void k_mdot(int rows, int nc, int rows_per_
   }
 }
 
-// One block per (basis vector i, column c), fixed-order sum as in k_finish_dot.
-__global__ __launch_bounds__(kBlock) void k_finish_mdot(int nblocks, int nc, int m, const double *__restrict__ work,
-                                                        double *out) {
-  __shared__ double sh[kBlock];
-  const int idx = blockIdx.x;
-  const int i = idx / nc, c = idx % nc;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += kBlock) s += work[((size_t)i * nblocks + b) * nc + c];
-  sh[threadIdx.x] = s;
-  block_tree_sum(sh);
-  if (threadIdx.x == 0) out[idx] = sh[0];
-}
-
 }  // namespace
 
 void mdot(hipStream_t s, int rows, int nc, const double *V, size_t vstride, int m, const double *w, double *out,
-          double *work) {
+          double *work, const Seg &sg) {
   EMF_REQUIRE(nc <= kBlock, kErrArg, "mdot supports at most 256 columns per call");
-  const int rows_per_pass = kBlock / nc;
-  const int nb = std::max(1, std::min(kDotBlocks, (rows + rows_per_pass - 1) / rows_per_pass));
+  const int nb = reduce_blocks(rows, nc, sg);
+  const int nl = sg.nseg * nc;
   for (int i0 = 0; i0 < m; i0 += 8) {
     const int mm = std::min(8, m - i0);
     const double *Vi = V + (size_t)i0 * vstride;
     switch (mm) {
-#define EMF_MDOT(K)                                                                                           \
-  case K:                                                                                                     \
-    hipLaunchKernelGGL((k_mdot<K>), dim3(nb), dim3(kBlock), 0, s, rows, nc, rows_per_pass, Vi, vstride, w, work); \
+#define EMF_MDOT(K)                                                                                    \
+  case K:                                                                                              \
+    hipLaunchKernelGGL((k_mdot<K>), dim3(nb), dim3(kBlock), 0, s, rows, nc, Vi, vstride, w, work, sg); \
     break;
       EMF_MDOT(1) EMF_MDOT(2) EMF_MDOT(3) EMF_MDOT(4) EMF_MDOT(5) EMF_MDOT(6) EMF_MDOT(7) EMF_MDOT(8)
 #undef EMF_MDOT
     }
-    hipLaunchKernelGGL(k_finish_mdot, dim3(nc * mm), dim3(kBlock), 0, s, nb, nc, mm, work,
-                       out + (size_t)i0 * nc);
+    hipLaunchKernelGGL(k_finish, dim3(nl * mm), dim3(kBlock), 0, s, nb, nc, work, out + (size_t)i0 * nl, sg);
   }
   EMF_HIP(hipGetLastError());
 }
@@ -632,12 +649,17 @@ void mdot(hipStream_t s, int rows, int nc, const double *V, size_t vstride, int 
 
 namespace {
 
+// Per-column scalars of row r, column c sit at [segment(r) * nc + c].
+__device__ __forceinline__ int scalar_index(const int *rseg, int r, int nc, int c) {
+  return (rseg ? rseg[r] * nc : 0) + c;
+}
+
 template <typename TB>
 __global__ void k_mupdate(int64_t n, int nc, const TB *__restrict__ V, size_t vstride, int m,
-                          const double *__restrict__ h, int hld, double *w, double sign) {
+                          const double *__restrict__ h, int hld, double *w, double sign, const int *rseg) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const int c = (int)(i % nc);
+  const int c = scalar_index(rseg, (int)(i / nc), nc, (int)(i % nc));
   double acc = 0.0;
   for (int k = 0; k < m; ++k) acc += h[(size_t)k * hld + c] * (double)V[k * vstride + i];
   w[i] -= sign * acc;
@@ -645,12 +667,13 @@ __global__ void k_mupdate(int64_t n, int nc, const TB *__restrict__ V, size_t vs
 
 template <typename TX, typename TY>
 __global__ void k_axpby_cols(int rows, int nc, const double *__restrict__ a, const TX *__restrict__ x, int ldx,
-                             const double *__restrict__ b, TY *y, int ldy) {
+                             const double *__restrict__ b, TY *y, int ldy, const int *rseg) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= (int64_t)rows * nc) return;
   const int r = (int)(i / nc), c = (int)(i % nc);
-  double v = a[c] * (double)x[(size_t)r * ldx + c];
-  if (b) v += b[c] * (double)y[(size_t)r * ldy + c];
+  const int k = scalar_index(rseg, r, nc, c);
+  double v = a[k] * (double)x[(size_t)r * ldx + c];
+  if (b) v += b[k] * (double)y[(size_t)r * ldy + c];
   y[(size_t)r * ldy + c] = (TY)v;
 }
 
@@ -666,18 +689,20 @@ __global__ void k_axpby(int rows, int nc, double alpha, const TX *__restrict__ x
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, int rows_per_pass, const double *__restrict__ alpha,
+__global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, const double *__restrict__ alpha,
                                                    const T *__restrict__ p, const T *__restrict__ q, T *x, T *r,
-                                                   double *work) {
+                                                   double *work, Seg sg) {
   __shared__ double sh[kBlock];
-  const int per_block = (kBlock / nc) * nc;
+  const int rpp = kBlock / nc;
+  const int per_block = rpp * nc;
   const int t = threadIdx.x;
+  int r0, r1;
+  chunk_rows(sg, rows, blockIdx.x, gridDim.x, r0, r1);
   double acc = 0.0;
-  if (t < per_block) {
+  if (t < per_block && r0 < r1) {
     const int c = t % nc;
-    const int r0 = t / nc;
-    const double a = alpha[c];
-    for (int row = blockIdx.x * rows_per_pass + r0; row < rows; row += gridDim.x * rows_per_pass) {
+    const double a = alpha[scalar_index(sg.rseg, r0, nc, c)];  // a chunk lies in one segment
+    for (int row = r0 + t / nc; row < r1; row += rpp) {
       const size_t o = (size_t)row * nc + c;
       x[o] = (T)((double)x[o] + a * (double)p[o]);
       const T rv = (T)((double)r[o] - a * (double)q[o]);
@@ -756,26 +781,30 @@ __global__ void k_fcg_beta(int nc, const double *zq, const double *pq, const uin
 
 template <typename TB>
 void mupdate(hipStream_t s, int rows, int nc, const TB *V, size_t vstride, int m, const double *h, int hld, double *w,
-             double sign) {
+             double sign, const Seg &sg) {
   if (m <= 0) return;
   const int64_t n = (int64_t)rows * nc;
-  hipLaunchKernelGGL((k_mupdate<TB>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, V, vstride, m, h, hld, w, sign);
+  hipLaunchKernelGGL((k_mupdate<TB>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, V, vstride, m, h, hld, w, sign,
+                     sg.rseg);
   EMF_HIP(hipGetLastError());
 }
-template void mupdate<float>(hipStream_t, int, int, const float *, size_t, int, const double *, int, double *, double);
-template void mupdate<double>(hipStream_t, int, int, const double *, size_t, int, const double *, int, double *, double);
+template void mupdate<float>(hipStream_t, int, int, const float *, size_t, int, const double *, int, double *, double,
+                             const Seg &);
+template void mupdate<double>(hipStream_t, int, int, const double *, size_t, int, const double *, int, double *,
+                              double, const Seg &);
 
 template <typename TX, typename TY>
 void axpby_cols(hipStream_t s, int rows, int nc, const double *a, const TX *x, int ldx, const double *b, TY *y,
-                int ldy) {
+                int ldy, const Seg &sg) {
   const int64_t n = (int64_t)rows * nc;
-  hipLaunchKernelGGL((k_axpby_cols<TX, TY>), dim3(grid_for(n)), dim3(kBlock), 0, s, rows, nc, a, x, ldx, b, y, ldy);
+  hipLaunchKernelGGL((k_axpby_cols<TX, TY>), dim3(grid_for(n)), dim3(kBlock), 0, s, rows, nc, a, x, ldx, b, y, ldy,
+                     sg.rseg);
   EMF_HIP(hipGetLastError());
 }
 template void axpby_cols<float, float>(hipStream_t, int, int, const double *, const float *, int, const double *,
-                                       float *, int);
+                                       float *, int, const Seg &);
 template void axpby_cols<double, double>(hipStream_t, int, int, const double *, const double *, int, const double *,
-                                         double *, int);
+                                         double *, int, const Seg &);
 
 template <typename TX, typename TY>
 void axpby(hipStream_t s, int rows, int nc, double alpha, const TX *x, int ldx, double beta, TY *y, int ldy) {
@@ -791,18 +820,17 @@ template void axpby<float, double>(hipStream_t, int, int, double, const float *,
 
 template <typename T>
 void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const T *p, const T *q, T *x, T *r, double *rr,
-            double *work) {
+            double *work, const Seg &sg) {
   EMF_REQUIRE(nc <= kBlock, kErrArg, "inner solve supports at most 256 columns");
-  const int rows_per_pass = kBlock / nc;
-  const int nb = std::max(1, std::min(kDotBlocks, (rows + rows_per_pass - 1) / rows_per_pass));
-  hipLaunchKernelGGL((k_fcg_xr<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, rows_per_pass, alpha, p, q, x, r, work);
-  hipLaunchKernelGGL(k_finish_dot, dim3(nc), dim3(kBlock), 0, s, nb, nc, work, rr);
+  const int nb = reduce_blocks(rows, nc, sg);
+  hipLaunchKernelGGL((k_fcg_xr<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, alpha, p, q, x, r, work, sg);
+  hipLaunchKernelGGL(k_finish, dim3(sg.nseg * nc), dim3(kBlock), 0, s, nb, nc, work, rr, sg);
   EMF_HIP(hipGetLastError());
 }
 template void fcg_xr<float>(hipStream_t, int, int, const double *, const float *, const float *, float *, float *,
-                            double *, double *);
+                            double *, double *, const Seg &);
 template void fcg_xr<double>(hipStream_t, int, int, const double *, const double *, const double *, double *, double *,
-                             double *, double *);
+                             double *, double *, const Seg &);
 
 template <typename T>
 void cheb_init(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *rres, T *d, T *x, double inv_theta) {

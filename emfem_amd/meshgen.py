@@ -399,3 +399,38 @@ def marine_csem(cells=(48, 48, 40), n_tx=64, freqs=(0.25, 0.75, 1.25), with_air=
                     obs.append((o, f, t, r))
     return Model(pts, tets, attr, 0, rho, np.asarray(freqs, dtype=np.float64), tx, rx,
                  np.asarray(obs, dtype=np.int32), name="marine_csem")
+
+
+def uneven_mt_batch(n_models=256, seed=4, e_min=2.0e4, e_max=2.0e5):
+    """Uneven batch of independent MT models: edge counts log-uniform in
+    [e_min, e_max], a half-space of 10-1000 ohm-m (log-uniform) with 1-4 random blocks
+    (1-1000 ohm-m), one random frequency per model, log-uniform in 1e-2-10 Hz, and a
+    3x3 receiver grid.  The grid size n per model is the one whose edge count (about
+    7 (n+1)^2 (n+1+n_pad) on the padded Kuhn grid) is nearest the drawn target."""
+    rng = np.random.default_rng(seed)
+    models = []
+    for k in range(n_models):
+        target = float(np.exp(rng.uniform(np.log(e_min), np.log(e_max))))
+        best = None
+        for n in range(4, 64):
+            nx = n + 2 * max(2, n // 2)
+            nz = n + max(2, n // 2) + max(3, n // 2)
+            e = 7.0 * (nx + 1) * (nx + 1) * (nz + 1)
+            if best is None or abs(np.log(e / target)) < abs(np.log(best[1] / target)):
+                best = (n, e)
+        n = best[0]
+        rho_hs = float(np.exp(rng.uniform(np.log(10.0), np.log(1000.0))))
+        freq = float(np.exp(rng.uniform(np.log(1e-2), np.log(10.0))))
+        m = halfspace_mt(n=n, rho=rho_hs, freq=freq, n_rx=3)
+        c = centroids(m.points, m.tets)
+        rho = [[AIR_RHO], [rho_hs]]
+        for b in range(int(rng.integers(1, 5))):
+            lo = rng.uniform([-1500.0, -1500.0, 100.0], [1000.0, 1000.0, 2500.0])
+            size = rng.uniform([300.0, 300.0, 200.0], [1500.0, 1500.0, 1200.0])
+            inside = np.all((c > lo) & (c < lo + size), axis=1)
+            m.attr[inside] = len(rho)
+            rho.append([float(np.exp(rng.uniform(np.log(1.0), np.log(1000.0))))])
+        m.rho = np.array(rho)
+        m.name = "uneven_mt_%03d" % k
+        models.append(m)
+    return models

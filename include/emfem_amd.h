@@ -167,6 +167,21 @@ int emfem_destroy_pc(emfem_context *ctx);
 int emfem_solve_linear_system(emfem_context *ctx, int first, int n, int max_it, double rtol,
                               const double *rtols, int *iters, double *relres);
 
+/* Multi-model batch: solve_linear_system (src/em_la.cc:35-63) for slots
+ * [first, first+n) of n_ctx contexts at once.  The reference runs its models one at a
+ * time or in separate MPI task groups (src/em_ctx.cc:17-43); here the contexts'
+ * assembled systems (all on this device) are stacked block-diagonally, the
+ * auxiliary-space set-up is built for the stack, and one FGMRES runs every launch over
+ * all models while each (model, slot) keeps its own Arnoldi process, inner FCG and
+ * stopping test at rtol.  Solutions are written to each context's slots.  Outputs
+ * (any may be NULL) are indexed k * n + j for model k, slot first + j: outer
+ * iterations, true FP64 relative residual, and the wall time (ms) from the start of
+ * the solve until that right-hand side was accepted.  Options (precision, restart,
+ * cycle) are taken from ctxs[0].  Returns EMFEM_ERR_NOT_CONVERGED like
+ * emfem_solve_linear_system. */
+int emfem_solve_batch(emfem_context **ctxs, int n_ctx, int first, int n, int max_it, double rtol, int *iters,
+                      double *relres, double *solve_ms);
+
 /* matshell_mult_a (src/em_la.cc:65-87) on device arrays: y = A x for n_rhs
  * interleaved right-hand sides ([n_edges][2*n_rhs]).  Also MATOP_MULT_TRANSPOSE
  * (A is symmetric, src/em_la.cc:169). */
@@ -271,8 +286,12 @@ int emfem_copy_to_host(emfem_context *ctx, void *dst, const void *src, unsigned 
  * "outer_its", "inner_its", "vcycles"); returns EMFEM_ERR_ARG for an unknown name. */
 int emfem_get_stat(const emfem_context *ctx, const char *name, double *value);
 
-/* Average device time (ms) of the hot kernels of the last solve, measured with HIP
- * events on the context's stream: which = "assemble", "op_spmm", "b_spmm". */
+/* Device time (ms per launch) and algorithmic bytes of the hot kernels:
+ * which = "assemble" (all colours of the last assembly), "estimate", and, after a
+ * solve run with -use_graph false, "op_spmm" (complex operator), "b_spmm" (smoother
+ * B SpMM in the cycle precision) and "fcg_spmm" (fp64 B SpMM of the FCG), each from
+ * HIP events on the context's stream around 20 back-to-back launches on that solve's
+ * operands. */
 int emfem_kernel_time(const emfem_context *ctx, const char *which, double *ms, double *bytes);
 
 #ifdef __cplusplus

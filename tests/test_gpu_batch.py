@@ -1,0 +1,60 @@
+"""Multi-model batches (emfem_solve_batch): models of different sizes, conductivities
+and frequencies stacked block-diagonally, each solved by its own Krylov process."""
+import numpy as np
+import pytest
+
+from emfem_amd import api
+from emfem_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _models():
+    return [mg.halfspace_mt(n=3, rho=100.0, freq=1.0, n_rx=2),
+            mg.tiny_mt(n=3, aniso_form=3, seed=5, freq=0.7),
+            mg.halfspace_mt(n=4, rho=10.0, freq=0.05, n_rx=2)]
+
+
+def _ready(m, opts):
+    c = api.Context(opts)
+    c.load_model(m)
+    c.create_linear_system(2)
+    c.assemble_matrix(0)
+    c.assemble_rhs_mt(0, api.XY_POLAR, 0)
+    c.assemble_rhs_mt(0, api.YX_POLAR, 1)
+    return c
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_matches_direct_solutions(precision, oracle_mod):
+    models = _models()
+    ctxs = [_ready(m, "-inner_precision %s" % precision) for m in models]
+    rtol = 1e-10
+    its, rr, ms = api.solve_batch(ctxs, 0, 2, 300, rtol)
+    print(precision, "its", its.tolist(), "relres", rr.tolist(), "ms", ms.tolist())
+    assert np.all(rr <= rtol * 1.0001), rr
+    assert np.all(ms > 0)
+    for c in ctxs:
+        rp, ci, C, M = c.get_matrix()
+        for slot in range(2):
+            sr, si = c.get_rhs(slot)
+            e = oracle_mod.solve_direct(rp, ci, C, M, sr, si)
+            # solution error <= condition number x residual; these low-frequency
+            # systems have condition numbers up to ~1e4
+            assert np.linalg.norm(c.get_solution(slot) - e) <= 1e4 * rtol * np.linalg.norm(e)
+
+
+def test_batch_agrees_with_single_model_solves():
+    """The batch gives each model the solution its own solve gives (both to 1e-10),
+    and each model keeps its own iteration count."""
+    models = _models()
+    ctxs = [_ready(m, "-inner_precision fp32") for m in models]
+    its_b, rr_b, _ = api.solve_batch(ctxs, 0, 2, 300, 1e-10)
+    batch = [[c.get_solution(s) for s in range(2)] for c in ctxs]
+    for k, c in enumerate(ctxs):
+        c.create_pc()
+        its, rr = c.solve_linear_system(0, 2, 300, 1e-10)
+        for s in range(2):
+            e = c.get_solution(s)
+            assert np.linalg.norm(batch[k][s] - e) <= 1e-6 * np.linalg.norm(e)
+    assert len(set(its_b[:, 0].tolist())) > 1 or len(set(its_b[:, 1].tolist())) > 1
