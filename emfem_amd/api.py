@@ -81,6 +81,12 @@ SIGNATURES = {
     "emfem_get_responses": (I, [P, P]),
     "emfem_locate_points": (I, [P, I, P, P]),
     "emfem_solve_batch": (I, [P, I, I, I, I, D, P, P, P]),
+    "emfem_surrogate_init": (I, [P, ctypes.c_uint]),
+    "emfem_surrogate_params": (I, [P, P, P, P]),
+    "emfem_surrogate_features": (I, [P, I, I, I, P]),
+    "emfem_surrogate_train": (I, [P, I, P, P, P, I, D, P]),
+    "emfem_surrogate_predict": (I, [P, I, I, I, P]),
+    "emfem_surrogate_eval": (I, [P, I, P, P, P, P, P, P]),
     "emfem_save_mesh": (I, [P, S, I, P, P]),
     "emfem_forward": (I, [P]),
     "emfem_refine_mesh": (I, [P, P]),
@@ -375,6 +381,58 @@ class Context:
         e1 = _c(error1, np.float64) if error1 is not None else None
         self._check(self._L.emfem_save_mesh(self._h, str(path).encode(), int(tidx), _p(e0),
                                             _p(e1) if e1 is not None else None))
+
+    # -- learned indicator surrogate ----------------------------------------------
+    def surrogate_init(self, seed=0):
+        self._check(self._L.emfem_surrogate_init(self._h, int(seed)))
+
+    def surrogate_params(self, set_params=None):
+        """Returns the flat parameter vector, or installs set_params."""
+        n = ctypes.c_int(0)
+        self._check(self._L.emfem_surrogate_params(self._h, None, None, ctypes.byref(n)))
+        if set_params is not None:
+            sp = _c(set_params, np.float32)
+            assert sp.size == n.value
+            self._check(self._L.emfem_surrogate_params(self._h, None, _p(sp), ctypes.byref(n)))
+            return sp
+        out = np.empty(n.value, dtype=np.float32)
+        self._check(self._L.emfem_surrogate_params(self._h, _p(out), None, ctypes.byref(n)))
+        return out
+
+    def surrogate_features(self, fidx, slot_xy, slot_yx):
+        f = np.empty((self.mesh_info()["n_tets"], 16), dtype=np.float32)
+        self._check(self._L.emfem_surrogate_features(self._h, int(fidx), int(slot_xy), int(slot_yx), _p(f)))
+        return f
+
+    def surrogate_train(self, feats, neighbors, indicator, epochs, lr=1e-3):
+        f = _c(feats, np.float32)
+        nb = _c(neighbors, np.int32)
+        ind = _c(indicator, np.float64)
+        loss = np.zeros(int(epochs))
+        self._check(self._L.emfem_surrogate_train(self._h, f.shape[0], _p(f), _p(nb), _p(ind), int(epochs),
+                                                  float(lr), _p(loss)))
+        return loss
+
+    def surrogate_predict(self, fidx, slot_xy, slot_yx):
+        out = np.empty(self.mesh_info()["n_tets"])
+        self._check(self._L.emfem_surrogate_predict(self._h, int(fidx), int(slot_xy), int(slot_yx), _p(out)))
+        return out
+
+    def surrogate_eval(self, feats, neighbors, target=None):
+        """(y, loss, grad) of the raw network on standardized features."""
+        f = _c(feats, np.float32)
+        nb = _c(neighbors, np.int32)
+        n = f.shape[0]
+        y = np.empty(n, dtype=np.float32)
+        loss = np.zeros(1)
+        npar = ctypes.c_int(0)
+        self._check(self._L.emfem_surrogate_params(self._h, None, None, ctypes.byref(npar)))
+        grad = np.empty(npar.value - 34, dtype=np.float32)
+        tg = _c(target, np.float32) if target is not None else None
+        self._check(self._L.emfem_surrogate_eval(self._h, n, _p(f), _p(nb), _p(tg), _p(y),
+                                                 _p(loss) if tg is not None else None,
+                                                 _p(grad) if tg is not None else None))
+        return (y, float(loss[0]), grad) if tg is not None else (y, None, None)
 
     def responses(self, n_obs=None):
         n = self._nobs if n_obs is None else n_obs

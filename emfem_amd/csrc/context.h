@@ -16,6 +16,7 @@
 #include "locate.h"
 #include "model.h"
 #include "solver.h"
+#include "surrogate.h"
 #include "topology.h"
 #include "fe.h"
 #include "kernels.h"
@@ -60,6 +61,13 @@ struct emfem_context {
 
   std::map<std::string, double> stats;
   double assemble_kernel_ms = 0.0;
+
+  // learned indicator surrogate: network, feature/target standardization
+  // (mean[16], std[16], then target mean, std), and whether the last rank-correlation
+  // check rejected it
+  std::unique_ptr<SurrogateNet> sur;
+  std::vector<float> sur_norm;
+  bool sur_rejected = false;
 
   std::string opt(const std::string &k) const {
     auto it = opts.find(k);
@@ -198,5 +206,10 @@ inline PcOptions pc_options(const emfem_context *c) {
   o.use_graph = c->opt("use_graph") == "true";
   return o;
 }
+
+// Learned indicator for the current mesh and solutions (surrogate_ctx.hip).
+std::vector<double> surrogate_indicator(emfem_context *c, int fidx, int sxy, int syx);
+// Spearman rank correlation of two equally long arrays.
+double rank_correlation(const std::vector<double> &a, const std::vector<double> &b);
 
 }  // namespace emfem_ctx

@@ -70,6 +70,8 @@ const std::map<std::string, std::string> &option_defaults() {
       {"ams_cheb_ratio", "30"},
       {"ams_theta", "0.08"},
       {"ams_cycle", "additive"},
+      {"surrogate_cycles", "false"},
+      {"surrogate_min_corr", "0.9"},
   };
   return d;
 }
@@ -85,6 +87,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"fp64_fallback", {"true", "false"}},
       {"use_graph", {"true", "false"}},
       {"ams_cycle", {"multiplicative", "additive"}},
+      {"surrogate_cycles", {"true", "false"}},
       {"save_mesh", {"true", "false"}},
   };
   static const std::set<std::string> none;

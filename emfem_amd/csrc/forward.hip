@@ -256,13 +256,26 @@ void forward_mt(emfem_context *c, int fidx) {
   }
   const int max_cycles = c->opti("max_adaptive_refinements");
   for (int cycle = 0; cycle <= max_cycles; ++cycle) {
+    const bool last = cycle == max_cycles || c->T.ne >= c->opti("max_dofs");
+    // learned indicator in intermediate cycles: no dual solve, no estimator
+    const bool sur = c->opt("surrogate_cycles") == "true" && c->sur && !c->sur_rejected;
     check(c, emfem_create_linear_system(c, 3));
     check(c, emfem_assemble_matrix(c, fidx));
     check(c, emfem_create_pc(c));
     check(c, emfem_assemble_rhs_mt(c, fidx, EMFEM_XY_POLAR, 0));
     check(c, emfem_assemble_rhs_mt(c, fidx, EMFEM_YX_POLAR, 1));
-    check(c, emfem_assemble_rhs_dual(c, 2));
     const double e_rtol = c->optd("e_rtol");
+    if (sur && !last) {
+      solve_slots(c, 0, 2, {e_rtol, e_rtol});
+      check(c, emfem_destroy_pc(c));
+      const std::vector<double> ind = surrogate_indicator(c, fidx, 0, 1);
+      c->stats["surrogate_cycles_used"] += 1;
+      if (saving_mesh(c)) write_vtk(c, vtk_name(c, fidx, -3, cycle), -3, ind.data(), ind.data());
+      check(c, emfem_destroy_linear_system(c));
+      refine_by_error(c, {ind});
+      continue;
+    }
+    check(c, emfem_assemble_rhs_dual(c, 2));
     solve_slots(c, 0, 2, {e_rtol, e_rtol});
     solve_dual(c, 2);
     check(c, emfem_destroy_pc(c));
@@ -276,7 +289,16 @@ void forward_mt(emfem_context *c, int fidx) {
       eyx[t] = std::sqrt(err[(size_t)nt + t] * err[(size_t)2 * nt + t]);
     }
     if (saving_mesh(c)) write_vtk(c, vtk_name(c, fidx, -3, cycle), -3, exy.data(), eyx.data());
-    if (cycle == max_cycles || c->T.ne >= c->opti("max_dofs")) {
+    if (sur && last) {
+      // the plain estimator judges the surrogate on the final mesh
+      std::vector<double> plain(nt);
+      for (int t = 0; t < nt; ++t) plain[t] = exy[t] + eyx[t];
+      const double corr = rank_correlation(surrogate_indicator(c, fidx, 0, 1), plain);
+      c->stats["surrogate_rank_corr"] = corr;
+      if (corr < c->optd("surrogate_min_corr")) c->sur_rejected = true;
+      c->stats["surrogate_rejected"] = c->sur_rejected ? 1.0 : 0.0;
+    }
+    if (last) {
       std::vector<double> rsp(c->obs4.size() / 2, NAN);
       check(c, emfem_calculate_response_mt(c, fidx, 0, 1, (int)c->obs4.size() / 4, c->obs4.data(), rsp.data()));
       for (size_t o = 0; o < c->obs4.size() / 4; ++o)

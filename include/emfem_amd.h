@@ -270,6 +270,43 @@ int emfem_refine_uniform(emfem_context *ctx);
 /* The current mesh: points [n_vertices*3], tets [n_tets*4], attributes [n_tets]. */
 int emfem_get_mesh(const emfem_context *ctx, double *points, int *tets, int *attributes);
 
+/* ---- learned indicator surrogate (no counterpart in the reference) ----------
+ * A per-element message-passing network over the face-neighbour graph (input layer
+ * of 16 padded features, 3 message-passing layers of width 64, bf16 matrix-core
+ * GEMMs with fp32 accumulation) that predicts the goal-oriented indicator of
+ * estimate_error (src/em_fwd.cc:1102-1180) from the primal XY / YX solutions alone.
+ * emfem_forward uses it, when -surrogate_cycles is true, in the intermediate adaptive
+ * MT cycles in place of the dual solve and the estimator; the last cycle always runs
+ * the plain estimator, compares its indicator with the surrogate's by Spearman rank
+ * correlation (stat "surrogate_rank_corr") and rejects the surrogate for the rest of
+ * the context's life below -surrogate_min_corr (0.9; stat "surrogate_rejected"). */
+
+/* New network with Glorot-uniform weights from seed. */
+int emfem_surrogate_init(emfem_context *ctx, unsigned seed);
+
+/* *n = parameter count (network weights, then 16 feature means, 16 feature standard
+ * deviations, target mean and standard deviation of log10 indicator); copies them out
+ * to get and/or in from set (either may be NULL). */
+int emfem_surrogate_params(emfem_context *ctx, float *get, const float *set, int *n);
+
+/* Raw features [n_tets][16] of the current mesh for the XY / YX solution slots. */
+int emfem_surrogate_features(emfem_context *ctx, int fidx, int slot_xy, int slot_yx, float *feats);
+
+/* Full-batch Adam training on n elements: raw features [n][16], face neighbours
+ * [n][4] (-1 for none; symmetric), indicator [n] (> 0; fitted in log10 after
+ * standardization).  loss[e] (may be NULL) receives the mean squared error of epoch e. */
+int emfem_surrogate_train(emfem_context *ctx, int n, const float *feats, const int *neighbors,
+                          const double *indicator, int epochs, double lr, double *loss);
+
+/* Predicted indicator [n_tets] for the current mesh and solution slots. */
+int emfem_surrogate_predict(emfem_context *ctx, int fidx, int slot_xy, int slot_yx, double *indicator);
+
+/* The raw network on n elements of standardized features [n][16] and neighbours
+ * [n][4]: outputs y [n], and with target [n] the mean squared error and its gradient
+ * with respect to the network weights (any output may be NULL). */
+int emfem_surrogate_eval(emfem_context *ctx, int n, const float *feats, const int *neighbors, const float *target,
+                         float *y, double *loss, float *grad);
+
 /* ---- device buffers -------------------------------------------------------- */
 
 /* Device memory on the context's GPU for callers that have no GPU framework of
