@@ -184,15 +184,21 @@ def batch_leg(api, mg, n_models, n_single=8):
     return out
 
 
-def time_to_1pct_leg(api, mg, cells=10, max_cycles=5, fraction=0.2):
-    """Time-to-1% receiver error of the goal-oriented adaptive MT loop: the adaptive-MT
-    model on a cells^3 grid at two frequencies (1e-3 and 1 Hz class), XY + YX + dual per
-    cycle.  The target is a fine-mesh FP64 forward on the same model refined uniformly
-    twice; the adaptive loop (em_forward with k refinement cycles, each run timed from
-    scratch) is run for k = 0.. until every Z_xy and Z_yx is within 1% of the target,
-    with the mixed and with the plain FP64 inner solve."""
+def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
+    """Accuracy against wall time of the goal-oriented adaptive MT loop (em_forward):
+    the adaptive-MT model (topography, anisotropic blocks) on a cells^3 grid at two
+    frequencies with 36 receivers 1 m below the surface, Z_xy and Z_yx.  Target: the
+    fine-mesh FP64 forward on the same model refined uniformly twice.  The loop runs
+    with k = 0..max_cycles refinement cycles (each run timed from scratch) in three
+    modes: plain (fp64 inner solve), mixed (fp32 inner solve), and mixed with the
+    learned indicator in the intermediate cycles (trained beforehand on plain cycles of
+    16 uneven-batch models; its data generation and training are timed separately).
+    Reports, per mode, the time to reach a median / max receiver error and the
+    surrogate's final-cycle rank correlation with the plain estimator."""
+    from emfem_amd import surrogate as sur
     m = mg.adaptive_mt(cells=(cells, cells, cells), n_freqs=8, n_rx_side=6)
-    m.freqs = m.freqs[[2, 6]]
+    m.freqs = m.freqs[[0, 3]]
+    m.rx = m.rx + np.array([0.0, 0.0, 1.0])
     m.obs = mg.mt_obs(len(m.freqs), len(m.rx), types=(mg.Z_XY_RI, mg.Z_YX_RI))
     ref = api.Context("-n_uniform_refinements 2 -max_adaptive_refinements 0 -inner_precision fp64 "
                       "-e_rtol 1e-8 -save_mesh false")
@@ -204,28 +210,53 @@ def time_to_1pct_leg(api, mg, cells=10, max_cycles=5, fraction=0.2):
     target = ref.responses()
     fine_edges = ref.mesh_info()["n_edges"]
     ref.close()
-    out = {"workload": "adaptive-MT %d^3 cells, 2 frequencies, %d receivers; target: 2x uniform refinement "
-                       "(%d edges), fp64, %.1f s" % (cells, len(m.rx), fine_edges, t_ref)}
-    for prec in ("fp32", "fp64"):
+    # surrogate: plain cycles of uneven-batch models, then training
+    t0 = time.perf_counter()
+    F, NB, Y = sur.training_data(mg.uneven_mt_batch(n_models=16, seed=11, e_min=8e3, e_max=4e4), cycles=5,
+                                 fraction=fraction)
+    t1 = time.perf_counter()
+    trainer = api.Context()
+    trainer.surrogate_init(1)
+    loss = trainer.surrogate_train(F, NB, Y, epochs=600, lr=3e-3)
+    t2 = time.perf_counter()
+    params = trainer.surrogate_params()
+    trainer.close()
+    out = {"workload": "adaptive-MT %d^3 cells, 2 frequencies, %d receivers, Z_xy and Z_yx; target: 2x uniform "
+                       "refinement (%d edges), fp64, %.1f s" % (cells, len(m.rx), fine_edges, t_ref),
+           "surrogate_training": {"elements": int(F.shape[0]), "data_s": round(t1 - t0, 2),
+                                  "train_s": round(t2 - t1, 2), "loss_first": round(float(loss[0]), 4),
+                                  "loss_last": round(float(loss[-1]), 4)}}
+    modes = (("plain_fp64", "-inner_precision fp64", False), ("mixed_fp32", "-inner_precision fp32", False),
+             ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true", True))
+    for name, opts, use_sur in modes:
         trace = []
         for k in range(max_cycles + 1):
-            c = api.Context("-max_adaptive_refinements %d -refine_fraction %g -inner_precision %s -e_rtol 1e-8 "
-                            "-max_dofs 100000000 -save_mesh false" % (k, fraction, prec))
+            c = api.Context("%s -max_adaptive_refinements %d -refine_strategy fixed_number -refine_fraction %g "
+                            "-e_rtol 1e-8 -max_dofs 100000000 -save_mesh false" % (opts, k, fraction))
             c.load_model(m)
             c.set_observations(m.obs)
+            if use_sur:
+                c.surrogate_init(0)
+                c.surrogate_params(params)
             t0 = time.perf_counter()
             c.forward()
             dt = time.perf_counter() - t0
-            err = float(np.max(np.abs(c.responses() - target) / np.abs(target)))
-            trace.append({"cycles": k, "edges": c.mesh_info()["n_edges"], "s": round(dt, 3), "max_rel_err": err})
+            err = np.abs(c.responses() - target) / np.abs(target)
+            row = {"cycles": k, "edges": c.mesh_info()["n_edges"], "s": round(dt, 3),
+                   "median_err": round(float(np.median(err)), 5), "max_err": round(float(err.max()), 5)}
+            if use_sur and k > 0:
+                row["rank_corr"] = round(c.stat("surrogate_rank_corr"), 3)
+            trace.append(row)
             c.close()
-            log("time-to-1%% %s: %d cycles, %d edges, %.2fs, max rel err %.2e" % (prec, k, trace[-1]["edges"], dt, err))
-            if err <= 0.01:
-                break
-        hit = [t for t in trace if t["max_rel_err"] <= 0.01]
-        out[prec] = {"time_to_1pct_s": hit[0]["s"] if hit else None, "trace": trace}
-    a, b = out["fp32"]["time_to_1pct_s"], out["fp64"]["time_to_1pct_s"]
-    out["mixed_vs_plain"] = round(a / b, 3) if a and b else None
+            log("adaptive %s: %s" % (name, row))
+        res = {"trace": trace}
+        for key, thr in (("median_err", 0.05), ("max_err", 0.01)):
+            hit = [t["s"] for t in trace if t[key] <= thr]
+            res["time_to_%s_%g" % (key.split("_")[0], thr)] = hit[0] if hit else None
+        out[name] = res
+    for name in ("mixed_fp32", "mixed_surrogate"):
+        a, b = out[name]["time_to_median_0.05"], out["plain_fp64"]["time_to_median_0.05"]
+        out[name + "_vs_plain_time_to_median_5pct"] = round(a / b, 3) if a and b else None
     return out
 
 
@@ -239,7 +270,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-csem", action="store_true", help="skip the batched marine-CSEM comparison")
     ap.add_argument("--csem-cells", type=str, default="48,48,40", help="marine-CSEM grid cells x,y,z")
-    ap.add_argument("--no-adaptive", action="store_true", help="skip the time-to-1%% adaptive comparison")
+    ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive accuracy-versus-time comparison")
     ap.add_argument("--batch-models", type=int, default=64,
                     help="models of the uneven multi-model batch (0 skips it)")
     ap.add_argument("--no-graph", action="store_true",
@@ -322,7 +353,7 @@ def main():
     if not args.no_csem:
         extra["csem_batched_64tx"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")))
     if not args.no_adaptive:
-        extra["time_to_1pct"] = time_to_1pct_leg(api, mg)
+        extra["adaptive"] = adaptive_leg(api, mg)
     if args.batch_models > 0:
         extra["uneven_batch"] = batch_leg(api, mg, args.batch_models)
     cpu = None

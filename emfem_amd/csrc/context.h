@@ -204,6 +204,7 @@ inline PcOptions pc_options(const emfem_context *c) {
   o.amg.theta = c->optd("ams_theta");
   o.amg.additive = c->opt("ams_cycle") == "additive";
   o.use_graph = c->opt("use_graph") == "true";
+  o.spmm_rows = c->opt("spmm_rows") == "lanes" ? 1 : (c->opt("spmm_rows") == "rowblock" ? 2 : 0);
   return o;
 }
 

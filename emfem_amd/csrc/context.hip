@@ -66,6 +66,7 @@ const std::map<std::string, std::string> &option_defaults() {
       {"dual_precision", "fp64"},
       {"fp64_fallback", "true"},
       {"use_graph", "true"},
+      {"spmm_rows", "auto"},
       {"ams_cheb_degree", "3"},
       {"ams_cheb_ratio", "30"},
       {"ams_theta", "0.08"},
@@ -86,6 +87,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"dual_precision", {"fp32", "fp64"}},
       {"fp64_fallback", {"true", "false"}},
       {"use_graph", {"true", "false"}},
+      {"spmm_rows", {"auto", "lanes", "rowblock"}},
       {"ams_cycle", {"multiplicative", "additive"}},
       {"surrogate_cycles", {"true", "false"}},
       {"save_mesh", {"true", "false"}},

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace emfem {
 
@@ -21,11 +22,28 @@ struct Csr {
   const T *v = nullptr;
   int lpr = 0;  // lanes per row of the narrow SpMM kernel; 0 = chosen from the row length
   int role = 0; // 1 = edge-space smoother operator (a separate kernel instance in traces)
+  // lpr = -1: LDS-staged row blocks; blk_row [nblk + 1] are the first rows of the
+  // blocks (row_blocks), for the column count and precision the table was built for
+  const int *blk_row = nullptr;
+  int nblk = 0;
 };
 
-// Picks the fastest lanes-per-row (2, 4, 8, 16) of the narrow SpMM for A on nc
-// columns by timing each with HIP events on stream s; x and y are scratch blocks
-// of [A.ncols][nc] and [A.nrows][nc].  Returns 0 when nc uses the wide kernel.
+// Nonzeros staged per workgroup by the row-block SpMM: 16 KB of products.
+template <typename T, int NC>
+constexpr int stream_tile() {
+  return (16384 / (NC * (int)sizeof(T))) / 256 * 256 > 2048 ? 2048 : (16384 / (NC * (int)sizeof(T))) / 256 * 256;
+}
+int stream_tile_for(int nc, bool fp64);
+
+// Row blocks for the row-block SpMM: consecutive rows while their nonzeros fit in
+// `tile`; returns the first row of every block and nrows (empty when a row is longer
+// than the tile).
+std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
+
+// Picks the fastest row mapping of the narrow SpMM for A on nc columns by timing
+// each with HIP events on stream s: lanes per row 2, 4, 8, 16, and -1 (LDS-staged
+// row blocks) when A carries a block table for nc; x and y are scratch blocks of
+// [A.ncols][nc] and [A.nrows][nc].  Returns 0 when nc uses the wide kernel.
 template <typename TA, typename TX, typename T>
 int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y);
 

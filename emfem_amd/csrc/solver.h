@@ -42,6 +42,8 @@ struct PcOptions {
   int inner_max_it = 100;
   AmgParams amg;
   bool use_graph = true;  // replay each auxiliary-space cycle as one captured hipGraph
+  // smoother SpMM row mapping: 0 = tuned, 1 = lanes per row only, 2 = row blocks when possible
+  int spmm_rows = 0;
 };
 
 // Preconditioner for B on nc real columns: flexible CG in fp64 with the

@@ -176,6 +176,8 @@ class DevAms {
       c.v = static_cast<const T *>(v);
       return c;
     };
+    hrp_.resize(ne + 1);
+    EMF_HIP(hipMemcpy(hrp_.data(), rp, sizeof(int) * (ne + 1), hipMemcpyDeviceToHost));
     bvals_.resize(nnz);
     make_b<T>(s, nnz, K, bvals_.get());
     B_ = view(bvals_.get());
@@ -203,6 +205,7 @@ class DevAms {
     amgPi_.upload(H.amgPi, s);
     nv0_ = H.nv0;
     nblk_ = H.nblk;
+    spmm_rows_ = o.spmm_rows;
     deg_ = std::max(1, o.cheb_degree);
     const double lmax = 1.0;  // Gershgorin: the l1-scaled operator has spectrum in (0, 1]
     const double lmin = lmax / o.cheb_ratio;
@@ -226,13 +229,31 @@ class DevAms {
     amgPi_.ensure(nc);
   }
 
-  // Lanes per row of the smoother SpMM for nc columns, timed once per nc (outside
-  // any graph capture) on scratch blocks.
+  // Row mapping of the smoother SpMM for nc columns (lanes per row or LDS-staged
+  // row blocks), timed once per nc (outside any graph capture) on scratch blocks.
   void tune(hipStream_t s, int nc) {
     ensure(nc);
     if (lpr_.count(nc)) return;
     d_.zero(s);
-    lpr_[nc] = tune_lpr<T, T, T>(s, B_, nc, d_.get(), rres_.get());
+    Csr<T> B = B_;
+    const std::vector<int> blk =
+        spmm_rows_ == 1 ? std::vector<int>() : row_blocks(hrp_, stream_tile_for(nc, sizeof(T) == 8));
+    if (!blk.empty()) {
+      blk_[nc].upload(blk, s);
+      B.blk_row = blk_[nc].get();
+      B.nblk = (int)blk.size() - 1;
+    }
+    lpr_[nc] = spmm_rows_ == 2 && !blk.empty() ? -1 : tune_lpr<T, T, T>(s, B, nc, d_.get(), rres_.get());
+  }
+
+  Csr<T> smoother(int nc) {
+    Csr<T> B = B_;
+    B.lpr = lpr_[nc];
+    if (B.lpr < 0) {
+      B.blk_row = blk_[nc].get();
+      B.nblk = (int)(blk_[nc].size() - 1);
+    }
+    return B;
   }
 
   ~DevAms() {
@@ -302,9 +323,7 @@ class DevAms {
 
   void bspmm(hipStream_t s, int nc, const T *x, T *y, T alpha, const T *rs, T gamma, const T *w, T beta,
              const T *z) {
-    Csr<T> B = B_;
-    B.lpr = lpr_[nc];
-    spmm<T>(s, B, nc, x, nc, y, nc, alpha, rs, gamma, w, nc, beta, z, nc);
+    spmm<T>(s, smoother(nc), nc, x, nc, y, nc, alpha, rs, gamma, w, nc, beta, z, nc);
   }
 
   // r64 = b - B x in fp64 (the residual the auxiliary spaces see)
@@ -331,7 +350,7 @@ class DevAms {
     spmm<T>(s, G0_.view(), nc, vx_.get(), nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), x, nc);
   }
 
-  int ne_ = 0, nv0_ = 0, nc_ = 0, deg_ = 3, nblk_ = 3;
+  int ne_ = 0, nv0_ = 0, nc_ = 0, deg_ = 3, nblk_ = 3, spmm_rows_ = 0;
   bool additive_ = false;
   double theta_ = 0.5, delta_ = 0.5;
   Csr<T> B_;
@@ -342,6 +361,8 @@ class DevAms {
   DevAmg<T> amgG_, amgPi_;
   std::map<std::tuple<int, const void *, const void *>, hipGraphExec_t> graphs_;
   std::map<int, int> lpr_;
+  std::map<int, DevBuf<int>> blk_;
+  std::vector<int> hrp_;  // host row pointers of B (row-block tables)
 };
 
 // ---------------------------------------------------------------------------------

@@ -158,15 +158,33 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
 #pragma unroll
   for (int j = 0; j < CPL; ++j) acc[j] = T(0);
   const int b = rp[row], e = rp[row + 1];
-  for (int p = b; p < e; ++p) {
-    const int c0 = ci[p];
-    const T a = xs ? (T)av[p] * xs[c0] : (T)av[p];
-    const TX *xr = x + (size_t)c0 * ldx;
+  // the row index is wave-uniform, so column and value loads are scalar; four
+  // nonzeros per trip keep four row gathers per lane in flight
+  constexpr int kU = 4;
+  for (int p0 = b; p0 < e; p0 += kU) {
+    int cu[kU];
+    T au[kU];
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      const int c = lane + 64 * j;
-      if (c < nc) acc[j] += a * (T)xr[c];
+    for (int u = 0; u < kU; ++u) {
+      const bool ok = p0 + u < e;
+      cu[u] = ok ? ci[p0 + u] : ci[b];
+      au[u] = ok ? (T)av[p0 + u] : T(0);
+      if (xs && ok) au[u] *= xs[cu[u]];
     }
+    TX xv[kU][CPL];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const TX *xr = x + (size_t)cu[u] * ldx;
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        const int c = lane + 64 * j;
+        xv[u][j] = c < nc ? xr[c] : TX(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) acc[j] += au[u] * (T)xv[u][j];
   }
   const T scale = rs ? rs[row] : T(1);
   const T zscale = zs ? beta * zs[row] : beta;
@@ -183,10 +201,68 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
   }
 }
 
+// LDS-staged row blocks: a workgroup takes the rows [blk_row[b], blk_row[b+1]), whose
+// nonzeros (at most TILE) are streamed with fully coalesced column/value loads, and
+// their products a * x[col][:] are staged in LDS; each (row, column) is then summed
+// from LDS by one thread, in nonzero order.
+template <typename TA, typename TX, typename T, int NC, int TILE>
+__global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                        const TA *__restrict__ av, const TX *__restrict__ x, int ldx,
+                                                        T *y, int ldy, T alpha, const T *__restrict__ rs, T gamma,
+                                                        const T *w, int ldw, T beta, const T *z, int ldz,
+                                                        const int *__restrict__ blk_row) {
+  __shared__ T prod[TILE * NC];
+  __shared__ int srp[TILE + 1];
+  const int b = xcd_block();
+  const int r0 = blk_row[b], r1 = blk_row[b + 1];
+  const int p0 = rp[r0];
+  const int nnz = rp[r1] - p0;
+  for (int i = threadIdx.x; i <= r1 - r0; i += kBlock) srp[i] = rp[r0 + i] - p0;
+  constexpr int kTrips = TILE / kBlock;
+  int c[kTrips];
+  T a[kTrips];
+#pragma unroll
+  for (int u = 0; u < kTrips; ++u) {
+    const int idx = threadIdx.x + u * kBlock;
+    const bool ok = idx < nnz;
+    c[u] = ok ? ci[p0 + idx] : 0;
+    a[u] = ok ? (T)av[p0 + idx] : T(0);
+  }
+#pragma unroll
+  for (int u = 0; u < kTrips; ++u) {
+    const int idx = threadIdx.x + u * kBlock;
+    if (idx < nnz) {
+      const TX *xr = x + (size_t)c[u] * ldx;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) prod[idx * NC + j] = a[u] * (T)xr[j];
+    }
+  }
+  __syncthreads();
+  const int nr = r1 - r0;
+  for (int q = threadIdx.x; q < nr * NC; q += kBlock) {
+    const int lr = q / NC, j = q % NC, row = r0 + lr;
+    T acc = T(0);
+    for (int k = srp[lr]; k < srp[lr + 1]; ++k) acc += prod[k * NC + j];
+    T v = alpha * acc;
+    if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
+    if (rs) v *= rs[row];
+    if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
+    y[(size_t)row * ldy + j] = v;
+  }
+}
+
 template <typename TA, typename TX, typename T, int NC>
 void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
                    T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs) {
   const double avg = A.nrows ? double(A.nnz) / A.nrows : 0.0;
+  if (A.lpr < 0) {
+    // LDS-staged row blocks (tables built by row_blocks for this NC and T)
+    EMF_REQUIRE(A.blk_row != nullptr && A.nblk > 0, kErrArg, "row-block SpMM without its block table");
+    EMF_REQUIRE(xs == nullptr && zs == nullptr, kErrArg, "row-block SpMM has no scaled operands");
+    hipLaunchKernelGGL((k_spmm_stream<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(A.nblk), dim3(kBlock), 0, s, A.rp,
+                       A.ci, A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, A.blk_row);
+    return;
+  }
   // one trip of 4 nonzeros per lane covers a row of average length
   const int lpr = A.lpr > 0 ? A.lpr : (avg <= 8 ? 2 : (avg <= 16 ? 4 : (avg <= 32 ? 8 : (avg <= 64 ? 16 : 32))));
   const int64_t threads = (int64_t)A.nrows * lpr;
@@ -210,10 +286,36 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
 
 }  // namespace
 
+int stream_tile_for(int nc, bool fp64) {
+  switch (nc) {
+#define EMF_TILE(K) \
+  case K: return fp64 ? stream_tile<double, K>() : stream_tile<float, K>();
+    EMF_TILE(1) EMF_TILE(2) EMF_TILE(3) EMF_TILE(4) EMF_TILE(6) EMF_TILE(8)
+#undef EMF_TILE
+    default: return 0;
+  }
+}
+
+std::vector<int> row_blocks(const std::vector<int> &rp, int tile) {
+  const int n = (int)rp.size() - 1;
+  std::vector<int> blk{0};
+  if (tile <= 0) return {};
+  for (int r = 0; r < n;) {
+    if (rp[r + 1] - rp[r] > tile) return {};
+    int e = r;
+    while (e < n && rp[e + 1] - rp[r] <= tile && e - r < tile) ++e;
+    blk.push_back(e);
+    r = e;
+  }
+  return blk;
+}
+
 template <typename TA, typename TX, typename T>
 void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
              T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs) {
   if (A.nrows == 0 || nc == 0) return;
+  EMF_REQUIRE(A.lpr >= 0 || (nc <= 8 && nc != 5 && nc != 7), kErrArg,
+              "row-block SpMM supports 1, 2, 3, 4, 6 or 8 columns");
   if (nc >= 16) {
     const int g = (A.nrows + kBlock / 64 - 1) / (kBlock / 64);
     if (nc <= 64)
@@ -274,7 +376,9 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
   EMF_HIP(hipEventCreate(&e1));
   int best = 0;
   float best_ms = 1e30f;
-  for (int lpr : {2, 4, 8, 16}) {
+  std::vector<int> cand{2, 4, 8, 16};
+  if (A.blk_row && A.nblk > 0) cand.push_back(-1);  // LDS-staged row blocks
+  for (int lpr : cand) {
     Csr<TA> B = A;
     B.lpr = lpr;
     spmm_mixed<TA, TX, T>(s, B, nc, x, nc, y, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
@@ -371,18 +475,34 @@ __global__ __launch_bounds__(kBlock) void k_op_wide(int nrows, const int *__rest
 #pragma unroll
   for (int j = 0; j < PPL; ++j) ar[j] = ai[j] = 0.0;
   const int b = rp[row], e = rp[row + 1];
-  for (int p = b; p < e; ++p) {
-    const double2 cm = K[p];
-    const TX *xr = x + (size_t)ci[p] * ldx;
+  constexpr int kU = 4;  // four nonzeros per trip, all gathers in flight together
+  for (int p0 = b; p0 < e; p0 += kU) {
+    double2 cm[kU];
+    int cu[kU];
 #pragma unroll
-    for (int j = 0; j < PPL; ++j) {
-      const int q = lane + 64 * j;
-      if (q < np) {
-        const double re = (double)xr[2 * q], im = (double)xr[2 * q + 1];
-        ar[j] += cm.x * re - cm.y * im;
-        ai[j] += -cm.y * re - cm.x * im;
+    for (int u = 0; u < kU; ++u) {
+      const bool ok = p0 + u < e;
+      cu[u] = ok ? ci[p0 + u] : ci[b];
+      cm[u] = ok ? K[p0 + u] : make_double2(0.0, 0.0);
+    }
+    double xr_[kU][PPL], xi_[kU][PPL];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const TX *xr = x + (size_t)cu[u] * ldx;
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        const int q = lane + 64 * j;
+        xr_[u][j] = q < np ? (double)xr[2 * q] : 0.0;
+        xi_[u][j] = q < np ? (double)xr[2 * q + 1] : 0.0;
       }
     }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        ar[j] += cm[u].x * xr_[u][j] - cm[u].y * xi_[u][j];
+        ai[j] += -cm[u].y * xr_[u][j] - cm[u].x * xi_[u][j];
+      }
   }
 #pragma unroll
   for (int j = 0; j < PPL; ++j) {

@@ -340,7 +340,7 @@ namespace {
 __global__ void k_features(int nt, const int *__restrict__ tets, const int *__restrict__ t2e,
                            const double *__restrict__ pts, const int *__restrict__ attr, const double *__restrict__ s6,
                            double freq, const double *__restrict__ exy, const double *__restrict__ eyx, int ld,
-                           float *F) {
+                           const double *__restrict__ rx, int nrx, float *F) {
   const int t = xcd_block() * kBlock + threadIdx.x;
   if (t >= nt) return;
   int vid[4];
@@ -413,7 +413,20 @@ __global__ void k_features(int nt, const int *__restrict__ tets, const int *__re
   }
   f[9] = (float)(0.5 * log10(e2 + 1e-300));
   f[10] = (float)(0.5 * log10(c2 + 1e-300));
-  for (int k = 11; k < kSurFeat; ++k) f[k] = 0.f;
+  // distance from the centroid to the nearest receiver, in units of the tet size:
+  // the dual (goal) solution the indicator weights by is concentrated there
+  double cx[3] = {0, 0, 0};
+  for (int k = 0; k < 4; ++k)
+    for (int d = 0; d < 3; ++d) cx[d] += 0.25 * x[k][d];
+  double dmin = 1e300;
+  for (int r = 0; r < nrx; ++r) {
+    const double dx = rx[r * 3] - cx[0], dy = rx[r * 3 + 1] - cx[1], dz = rx[r * 3 + 2] - cx[2];
+    dmin = fmin(dmin, dx * dx + dy * dy + dz * dz);
+  }
+  const double h = cbrt(fmax(vol, 1e-30));
+  f[11] = nrx > 0 ? (float)log10(sqrt(dmin) / h + 1.0) : 0.f;
+  f[12] = nrx > 0 ? (float)log10(sqrt(dmin) + 1.0) : 0.f;
+  for (int k = 13; k < kSurFeat; ++k) f[k] = 0.f;
 }
 
 __global__ void k_standardize(int n, const float *__restrict__ mean, const float *__restrict__ stdv, float *F) {
@@ -427,10 +440,10 @@ __global__ void k_standardize(int n, const float *__restrict__ mean, const float
 
 void surrogate_features(hipStream_t s, int nt, const int *tets, const int *tet2edge, const double *pts,
                         const int *attr, const double *sig6, double freq, const double *e_xy, const double *e_yx,
-                        int ld, float *F) {
+                        int ld, const double *rx, int nrx, float *F) {
   if (nt == 0) return;
   hipLaunchKernelGGL(k_features, dim3(grid_for(nt)), dim3(kBlock), 0, s, nt, tets, tet2edge, pts, attr, sig6, freq,
-                     e_xy, e_yx, ld, F);
+                     e_xy, e_yx, ld, rx, nrx, F);
   EMF_HIP(hipGetLastError());
 }
 

@@ -27,8 +27,12 @@ void device_features(emfem_context *c, int fidx, int sxy, int syx, DevBuf<float>
   require_freq(c, fidx);
   F.resize((size_t)c->T.nt * kSurFeat);
   const int ld = 2 * c->nslots;
+  DevBuf<double> rx;
+  if (!c->rx.empty()) rx.upload(c->rx, c->stream);
   surrogate_features(c->stream, c->T.nt, c->d_tets.get(), c->d_t2e.get(), c->d_pts.get(), c->d_attr.get(),
-                     c->d_s6.get(), c->freqs[fidx], c->d_X.get() + 2 * sxy, c->d_X.get() + 2 * syx, ld, F.get());
+                     c->d_s6.get(), c->freqs[fidx], c->d_X.get() + 2 * sxy, c->d_X.get() + 2 * syx, ld, rx.get(),
+                     (int)(c->rx.size() / 3), F.get());
+  EMF_HIP(hipStreamSynchronize(c->stream));
 }
 
 }  // namespace

@@ -268,3 +268,25 @@ def test_device_point_location_matches_oracle(oracle_mod):
     assert np.array_equal(got, want), np.nonzero(got != want)
     assert np.all(got[:-3] >= 0) and np.all(got[-3:] == -1)
     assert all(ctx.find_cell(p) == w for p, w in zip(q[:50], want[:50]))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_row_block_spmm_matches_lane_spmm(precision, oracle_mod):
+    """The LDS-staged row-block smoother SpMM and the lanes-per-row one give the same
+    solve (both to the exact discrete solution)."""
+    m = mg.halfspace_mt(n=4, n_rx=3)
+    sols = {}
+    for rows in ("lanes", "rowblock"):
+        ctx = _ctx(m, "-inner_precision %s -spmm_rows %s" % (precision, rows))
+        ctx.create_linear_system(2)
+        ctx.assemble_matrix(0)
+        ctx.assemble_rhs_mt(0, api.XY_POLAR, 0)
+        ctx.assemble_rhs_mt(0, api.YX_POLAR, 1)
+        ctx.create_pc()
+        its, rr = ctx.solve_linear_system(0, 2, 300, 1e-10)
+        assert np.all(rr <= 1e-10 * 1.0001)
+        sols[rows] = [ctx.get_solution(s) for s in range(2)]
+        rp, ci, C, M = ctx.get_matrix()
+        for s in range(2):
+            e = oracle_mod.solve_direct(rp, ci, C, M, *ctx.get_rhs(s))
+            assert np.linalg.norm(sols[rows][s] - e) <= 1e4 * 1e-10 * np.linalg.norm(e)
