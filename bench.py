@@ -83,7 +83,7 @@ def cpu_baseline(budget_s=20.0):
                        "estimator, %.1f s" % (om.n_edges, m.freqs[fidx], len(sols), dt))
 
 
-def csem_leg(api, mg, cells, n_tx=64, n_plain=4, fidx=1):
+def csem_leg(api, mg, cells, n_tx=64, n_plain=4, fidx=1, graph=True):
     """Marine-CSEM batched multi-RHS solve (one frequency, n_tx x-directed dipoles as
     one block of right-hand sides, fp32 inner cycle in fp64 FGMRES) against the plain
     fp64 single-RHS path on the first n_plain transmitters of the same operator.
@@ -94,7 +94,7 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=4, fidx=1):
     fields = {}
     for name, opts, batched in (("mixed_batched", "-inner_precision fp32", True),
                                 ("plain_fp64_single_rhs", "-inner_precision fp64", False)):
-        ctx = api.Context(opts + " -e_rtol 1e-8")
+        ctx = api.Context(opts + " -e_rtol 1e-8 -use_graph %s" % ("true" if graph else "false"))
         ctx.load_model(m)
         nt = n_tx if batched else n_plain
         ctx.create_linear_system(nt)
@@ -192,7 +192,8 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     with k = 0..max_cycles refinement cycles (each run timed from scratch) in three
     modes: plain (fp64 inner solve), mixed (fp32 inner solve), and mixed with the
     learned indicator in the intermediate cycles (trained beforehand on plain cycles of
-    16 uneven-batch models; its data generation and training are timed separately).
+    16 uneven-batch models at 1e-3..10 Hz; data generation and training are timed
+    separately).
     Reports, per mode, the time to reach a median / max receiver error and the
     surrogate's final-cycle rank correlation with the plain estimator."""
     from emfem_amd import surrogate as sur
@@ -212,7 +213,7 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     ref.close()
     # surrogate: plain cycles of uneven-batch models, then training
     t0 = time.perf_counter()
-    F, NB, Y = sur.training_data(mg.uneven_mt_batch(n_models=16, seed=11, e_min=8e3, e_max=4e4), cycles=5,
+    F, NB, Y = sur.training_data(mg.uneven_mt_batch(n_models=16, seed=11, e_min=8e3, e_max=4e4, f_min=1e-3), cycles=5,
                                  fraction=fraction)
     t1 = time.perf_counter()
     trainer = api.Context()
@@ -257,6 +258,12 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     for name in ("mixed_fp32", "mixed_surrogate"):
         a, b = out[name]["time_to_median_0.05"], out["plain_fp64"]["time_to_median_0.05"]
         out[name + "_vs_plain_time_to_median_5pct"] = round(a / b, 3) if a and b else None
+        # same number of cycles: wall time and median error relative to the plain loop
+        pt, mt = out["plain_fp64"]["trace"], out[name]["trace"]
+        out[name + "_vs_plain_same_cycles"] = {
+            "time": round(sum(t["s"] for t in mt[1:]) / sum(t["s"] for t in pt[1:]), 3),
+            "median_err": round(float(np.mean([a["median_err"] / b["median_err"] for a, b in zip(mt[1:], pt[1:])])),
+                                3)}
     return out
 
 
@@ -273,6 +280,8 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive accuracy-versus-time comparison")
     ap.add_argument("--batch-models", type=int, default=64,
                     help="models of the uneven multi-model batch (0 skips it)")
+    ap.add_argument("--spmm-rows", default="auto", choices=["auto", "lanes", "rowblock", "2", "4", "8", "16"],
+                    help="row mapping of the smoother SpMM (auto: autotuned)")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of hipGraph-captured cycles (for kernel tracers)")
     args = ap.parse_args()
@@ -286,8 +295,8 @@ def main():
 
     t0 = time.perf_counter()
     m = mg.adaptive_mt(cells=(args.cells, args.cells, args.cells))
-    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6 -use_graph %s"
-                      % ("false" if args.no_graph else "true"))
+    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6 -use_graph %s -spmm_rows %s"
+                      % ("false" if args.no_graph else "true", args.spmm_rows))
     ctx.load_model(m)
     ctx.create_linear_system(3)
     info = ctx.mesh_info()
@@ -313,7 +322,8 @@ def main():
 
     # roofline of the dominant kernel: the fp32 B SpMM of the inner solve.  The timed
     # steps replay captured cycles, so the per-launch times come from one extra,
-    # untimed solve with plain launches bracketed by HIP events on the solver stream.
+    # untimed solve with plain launches, each smoother SpMM bracketed by HIP events
+    # on the solver stream.
     ctx.process_options("-use_graph false")
     ctx.solve_linear_system(0, 2, 100, 1e-8)
     ctx.process_options("-use_graph %s" % ("false" if args.no_graph else "true"))
@@ -323,10 +333,25 @@ def main():
     as_ms, as_bytes = ctx.kernel_time("assemble")
     peak = 8000.0
     bw = (b_bytes / (b_ms * 1e-3) / 1e9) if b_ms > 0 else 0.0
+    rows = int(ctx.stat("b_spmm_rows"))
+    kname = ("%s: Chebyshev sweep rres -= Dinv .* (B d), B = C + M fp32, XY+YX = 4 real columns"
+             % ("k_spmm_stream (LDS-staged row blocks)" if rows < 0 else
+                "k_spmm_narrow role 1 (%d lanes per row)" % rows))
+    # HBM traffic of the same kernel and workload from the committed PMC summary
+    # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on k_make_b), when it exists
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "%s_smoother_adaptive_mt_%d.md"
+                        % ("k_spmm_stream" if rows < 0 else "k_spmm_narrow", args.cells))
+    if os.path.exists(prof):
+        for line in open(prof):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = float(line.split(":")[1].split()[0])
     roof = {"bound": "hbm", "achieved": round(bw, 1), "peak": peak, "unit": "GB/s", "frac": round(bw / peak, 4),
-            "traffic": None, "kernel": "k_spmm_narrow role 1: Chebyshev sweep rres -= Dinv .* (B d), B = C + M fp32, XY+YX = 4 real columns", "launch_ms": round(b_ms, 5),
+            "traffic": traffic, "kernel": kname, "launch_ms": round(b_ms, 5),
             "algorithmic_bytes": b_bytes}
+    b2b = ctx.stat("b_spmm_back_to_back_ms")
     extra = {
+        "b_spmm_back_to_back_GBps": round(b_bytes / (b2b * 1e-3) / 1e9, 1) if b2b > 0 else None,
         "op_spmm_GBps": round(op_bytes / (op_ms * 1e-3) / 1e9, 1) if op_ms > 0 else None,
         "fcg_spmm_fp64_GBps": round(fcg_bytes / (fcg_ms * 1e-3) / 1e9, 1) if fcg_ms > 0 else None,
         "assemble_GBps": round(as_bytes / (as_ms * 1e-3) / 1e9, 1) if as_ms > 0 else None,
@@ -335,6 +360,13 @@ def main():
                     for k in ("assemble_rhs", "pc", "solve", "estimate")},
         "outer_its": [s["its"] for s in steps],
     }
+    pcb = {}
+    for k in ("download", "aux_build", "galerkin", "amg", "setup_host", "setup"):
+        try:
+            pcb[k] = round(ctx.stat("pc_%s_ms" % k), 1)
+        except Exception:
+            pass
+    extra["pc_last_ms"] = pcb
     if not args.no_plain:
         ctxp = api.Context("-inner_precision fp64")
         ctxp.load_model(m)
@@ -351,7 +383,8 @@ def main():
         extra["speedup_vs_plain_solve"] = round(p_solve / mixed_solve, 3)
         ctxp.close()
     if not args.no_csem:
-        extra["csem_batched_64tx"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")))
+        extra["csem_batched_64tx"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")),
+                                              graph=not args.no_graph)
     if not args.no_adaptive:
         extra["adaptive"] = adaptive_leg(api, mg)
     if args.batch_models > 0:

@@ -400,15 +400,14 @@ void build_amg(const HostCsr &A0, const AmgParams &prm, AmgHost &H) {
   tick("amg_coarse_ms", t0);
 }
 
-void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, AmsHost &H) {
-  AmgParams prm = prm_in;
-  prm.timing = &H.timing;
+std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive) {
+  auto X = std::make_shared<AmsAux>();
   // interior vertex numbering
   std::vector<int> vmap(T.nv, -1);
   int nv0 = 0;
   for (int v = 0; v < T.nv; ++v)
     if (!T.vertex_bdr[v]) vmap[v] = nv0++;
-  H.nv0 = nv0;
+  X->nv0 = nv0;
   EMF_REQUIRE(nv0 > 0, kErrUser, "mesh has no interior vertex");
   auto make = [&](int comp) {
     HostCsr G;
@@ -447,61 +446,62 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, Ams
     });
     return G;
   };
-  double t0 = now_ms();
-  H.G0 = make(-1);
-  H.G0T = csr_transpose(H.G0);
-  H.timing["aux_build_ms"] += now_ms() - t0;
-  t0 = now_ms();
-  HostCsr AG = csr_triple(H.G0T, B, H.G0);
-  H.timing["galerkin_ms"] += now_ms() - t0;
-  std::vector<HostCsr> P(3), A(3);
+  X->G0 = make(-1);
+  X->G0T = csr_transpose(X->G0);
   for (int d = 0; d < 3; ++d) {
-    t0 = now_ms();
-    P[d] = make(d);
-    const HostCsr PT = csr_transpose(P[d]);
-    H.timing["aux_build_ms"] += now_ms() - t0;
-    t0 = now_ms();
-    A[d] = csr_triple(PT, B, P[d]);
-    H.timing["galerkin_ms"] += now_ms() - t0;
+    X->P[d] = make(d);
+    X->PT[d] = csr_transpose(X->P[d]);
   }
-  H.additive = prm.additive;
-  std::vector<const HostCsr *> pb, ab;
-  if (prm.additive) {
-    pb.push_back(&H.G0);
-    ab.push_back(&AG);
-  } else {
-    t0 = now_ms();
-    build_amg(AG, prm, H.amgG);
-    H.timing["amg_ms"] += now_ms() - t0;
-  }
-  for (int d = 0; d < 3; ++d) {
-    pb.push_back(&P[d]);
-    ab.push_back(&A[d]);
-  }
-  H.nblk = (int)pb.size();
-  t0 = now_ms();
-  const int nb = H.nblk;
-  H.Pi.n = T.ne;
-  H.Pi.m = nb * nv0;
-  H.Pi.rp.assign(T.ne + 1, 0);
+  X->additive = additive;
+  std::vector<const HostCsr *> pb;
+  if (additive) pb.push_back(&X->G0);
+  for (int d = 0; d < 3; ++d) pb.push_back(&X->P[d]);
+  const int nb = (int)pb.size();
+  X->nblk = nb;
+  HostCsr &Pi = X->Pi;
+  Pi.n = T.ne;
+  Pi.m = nb * nv0;
+  Pi.rp.assign(T.ne + 1, 0);
   for (int e = 0; e < T.ne; ++e) {
     int k = 0;
     for (const HostCsr *q : pb) k += q->rp[e + 1] - q->rp[e];
-    H.Pi.rp[e + 1] = H.Pi.rp[e] + k;
+    Pi.rp[e + 1] = Pi.rp[e] + k;
   }
-  H.Pi.ci.resize(H.Pi.rp[T.ne]);
-  H.Pi.v.resize(H.Pi.rp[T.ne]);
+  Pi.ci.resize(Pi.rp[T.ne]);
+  Pi.v.resize(Pi.rp[T.ne]);
   parallel_ranges(T.ne, [&](int64_t eb, int64_t ee, int) {
     for (int64_t e = eb; e < ee; ++e) {
-      int q = H.Pi.rp[e];
+      int q = Pi.rp[e];
       for (int d = 0; d < nb; ++d)
         for (int k = pb[d]->rp[e]; k < pb[d]->rp[e + 1]; ++k) {
-          H.Pi.ci[q] = d * nv0 + pb[d]->ci[k];
-          H.Pi.v[q++] = pb[d]->v[k];
+          Pi.ci[q] = d * nv0 + pb[d]->ci[k];
+          Pi.v[q++] = pb[d]->v[k];
         }
     }
   });
-  H.PiT = csr_transpose(H.Pi);
+  X->PiT = csr_transpose(Pi);
+  return X;
+}
+
+void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, AmsHost &H,
+               std::shared_ptr<const AmsAux> aux) {
+  AmgParams prm = prm_in;
+  prm.timing = &H.timing;
+  double t0 = now_ms();
+  if (!aux || aux->additive != prm.additive || aux->G0.n != T.ne) aux = build_ams_aux(T, prm.additive);
+  H.aux = aux;
+  const int nv0 = aux->nv0;
+  H.timing["aux_build_ms"] += now_ms() - t0;
+  // Galerkin products of the auxiliary blocks
+  t0 = now_ms();
+  std::vector<HostCsr> A(4);
+  const bool additive = aux->additive;
+  const HostCsr *R[4] = {&aux->G0T, &aux->PT[0], &aux->PT[1], &aux->PT[2]};
+  const HostCsr *P[4] = {&aux->G0, &aux->P[0], &aux->P[1], &aux->P[2]};
+  for (int k = 0; k < 4; ++k) A[k] = csr_triple(*R[k], B, *P[k]);
+  std::vector<const HostCsr *> ab;
+  for (int k = additive ? 0 : 1; k < 4; ++k) ab.push_back(&A[k]);
+  const int nb = (int)ab.size();
   HostCsr AP;
   AP.n = AP.m = nb * nv0;
   AP.rp.assign(AP.n + 1, 0);
@@ -511,7 +511,12 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, Ams
     AP.v.insert(AP.v.end(), ab[d]->v.begin(), ab[d]->v.end());
     for (size_t k = base; k < AP.ci.size(); ++k) AP.ci[k] += d * nv0;
   }
-  H.timing["aux_build_ms"] += now_ms() - t0;
+  H.timing["galerkin_ms"] += now_ms() - t0;
+  if (!additive) {
+    t0 = now_ms();
+    build_amg(A[0], prm, H.amgG);
+    H.timing["amg_ms"] += now_ms() - t0;
+  }
   t0 = now_ms();
   build_amg(AP, prm, H.amgPi);
   H.timing["amg_ms"] += now_ms() - t0;

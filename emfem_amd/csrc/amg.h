@@ -14,6 +14,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -54,11 +55,13 @@ struct AmgParams {
 
 void build_amg(const HostCsr &A, const AmgParams &p, AmgHost &H);
 
-struct AmsHost {
-  // E x V0 gradient restricted to interior vertices, its transpose and the AMG of
-  // G0^T B G0 (multiplicative cycle only).
+// Mesh-only parts of the auxiliary spaces: they depend on the topology and vertex
+// coordinates, not on B, so one set serves every frequency on a mesh.
+struct AmsAux {
+  // E x V0 gradient restricted to interior vertices and its transpose
   HostCsr G0, G0T;
-  AmgHost amgG;
+  // nodal-vector components Pi_d (E x V0) and their transposes
+  HostCsr P[3], PT[3];
   // Auxiliary blocks side by side: E x (nblk V0), column k * V0 + v for block k of
   // interior vertex v.  Blocks are the three nodal-vector components Pi_d
   // (multiplicative cycle) or G0 followed by Pi_x, Pi_y, Pi_z (additive cycle).
@@ -66,15 +69,24 @@ struct AmsHost {
   // form one block-diagonal matrix diag(P_k^T B P_k) with one AMG hierarchy whose
   // aggregates never cross blocks.
   HostCsr Pi, PiT;
-  AmgHost amgPi;
   int nblk = 3;
   bool additive = false;
-  std::vector<double> l1inv;  // edge-space l1 row norms of B, inverted
   int nv0 = 0;
+};
+
+std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive);
+
+struct AmsHost {
+  std::shared_ptr<const AmsAux> aux;
+  AmgHost amgG;   // AMG of G0^T B G0 (multiplicative cycle only)
+  AmgHost amgPi;  // AMG of diag(P_k^T B P_k)
+  std::vector<double> l1inv;  // edge-space l1 row norms of B, inverted
   std::map<std::string, double> timing;  // set-up phase times (ms)
 };
 
-// B given on the edge pattern (fp64 values).
-void build_ams(const Topology &T, const HostCsr &B, const AmgParams &p, AmsHost &H);
+// B given on the edge pattern (fp64 values).  `aux` from an earlier call on the same
+// mesh (same additive flag) is reused; otherwise it is built.
+void build_ams(const Topology &T, const HostCsr &B, const AmgParams &p, AmsHost &H,
+               std::shared_ptr<const AmsAux> aux = nullptr);
 
 }  // namespace emfem

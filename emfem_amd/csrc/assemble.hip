@@ -171,48 +171,55 @@ __device__ void field_curl(const TetData &D, const double *sol, int ld, int col,
 
 constexpr int kMaxSol = 8;
 
+// NS solutions at compile time keep the per-solution sums in registers; the two sides
+// of a face are loaded into their own structs (no run-time choice between structs,
+// which would force them into scratch memory).
+template <int NS>
 __global__ __launch_bounds__(128) void k_estimate(int nt, const int *__restrict__ tets, const int *__restrict__ tet2edge,
                                                   const int *__restrict__ neighbor, const int *__restrict__ non,
                                                   const double *__restrict__ pts, const int *__restrict__ attr,
-                                                  const double *__restrict__ sig6, double omega, int ns,
+                                                  const double *__restrict__ sig6, double omega,
                                                   const int *__restrict__ slots, const double *__restrict__ sol, int ld,
                                                   double *errors) {
   const int t = xcd_block() * 128 + threadIdx.x;
   if (t >= nt) return;
-  TetData D;
-  load_full(t, tets, tet2edge, pts, attr, sig6, D);
-  // volume term: h^2 sum_q |i w sigma E(q)|^2 jxw, 4-point rule (src/em_fe.cc:225-250)
-  const double qa = 0.585410196624969, qb = 0.138196601125011;
-  const double jxw = D.G.det / 24.0;
-  const double h = tet_circumradius(D);
-  double err[kMaxSol];
-  for (int s = 0; s < ns; ++s) {
-    const int col = 2 * slots[s];
-    double res = 0.0;
+  double err[NS];
+  {
+    TetData D;
+    load_full(t, tets, tet2edge, pts, attr, sig6, D);
+    // volume term: h^2 sum_q |i w sigma E(q)|^2 jxw, 4-point rule (src/em_fe.cc:225-250)
+    const double qa = 0.585410196624969, qb = 0.138196601125011;
+    const double jxw = D.G.det / 24.0;
+    const double h = tet_circumradius(D);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      double lam[4] = {qb, qb, qb, qb};
-      // reference point q has natural coordinate (1 - p0 - p1 - p2) on vertex 0
-      lam[0] = q == 3 ? 1.0 - 3.0 * qb : 1.0 - qa - 2.0 * qb;
-      if (q < 3) lam[q + 1] = qa;
-      double er[3], ei[3], sr[3], si[3];
-      field_value(D, lam, sol, ld, col, er, ei);
-      sig_apply(D.sig, er, sr);
-      sig_apply(D.sig, ei, si);
-      res += omega * omega * (dot3(sr, sr) + dot3(si, si)) * jxw;
+    for (int s = 0; s < NS; ++s) {
+      const int col = 2 * slots[s];
+      double res = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double lam[4] = {qb, qb, qb, qb};
+        // reference point q has natural coordinate (1 - p0 - p1 - p2) on vertex 0
+        lam[0] = q == 3 ? 1.0 - 3.0 * qb : 1.0 - qa - 2.0 * qb;
+        if (q < 3) lam[q + 1] = qa;
+        double er[3], ei[3], sr[3], si[3];
+        field_value(D, lam, sol, ld, col, er, ei);
+        sig_apply(D.sig, er, sr);
+        sig_apply(D.sig, ei, si);
+        res += omega * omega * (dot3(sr, sr) + dot3(si, si)) * jxw;
+      }
+      err[s] = res * h * h;
     }
-    err[s] = res * h * h;
   }
   // face terms
   for (int f = 0; f < 4; ++f) {
     const int n = neighbor[t * 4 + f];
     if (n < 0) continue;
-    // evaluate from the lower-index side
+    // evaluate from the lower-index side: P is the side whose face rule and normal
+    // are used
     const bool own = t < n;
-    TetData O;
-    load_full(n, tets, tet2edge, pts, attr, sig6, O);
-    const TetData &P = own ? D : O;   // side whose face rule and normal are used
-    const TetData &Q = own ? O : D;
+    TetData P, Q;
+    load_full(own ? t : n, tets, tet2edge, pts, attr, sig6, P);
+    load_full(own ? n : t, tets, tet2edge, pts, attr, sig6, Q);
     const int pf = own ? f : non[t * 4 + f];
     double nrm[3];
     {
@@ -225,7 +232,8 @@ __global__ __launch_bounds__(128) void k_estimate(int nt, const int *__restrict_
     // 3-point face rule (src/em_fe.cc:134-208): weights |grad lambda_f| det 6 / 6
     const double fj = sqrt(dot3(P.G.g[pf], P.G.g[pf])) * fabs(P.G.det);
     const double hf = face_circumradius(P, pf);
-    for (int s = 0; s < ns; ++s) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
       const int col = 2 * slots[s];
       double cpr[3], cpi[3], cqr[3], cqi[3];
       field_curl(P, sol, ld, col, cpr, cpi);
@@ -261,7 +269,8 @@ __global__ __launch_bounds__(128) void k_estimate(int nt, const int *__restrict_
       err[s] += (dotj + 3.0 * cross2 * fj) * hf / 2.0;
     }
   }
-  for (int s = 0; s < ns; ++s) errors[(size_t)s * nt + t] = err[s];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) errors[(size_t)s * nt + t] = err[s];
 }
 
 }  // namespace
@@ -280,8 +289,16 @@ void estimate(hipStream_t s, int nt, const int *tets, const int *tet2edge, const
               const double *sol, int ld, double *errors) {
   EMF_REQUIRE(ns <= kMaxSol, kErrArg, "estimator supports at most 8 solutions per call");
   if (nt == 0) return;
-  hipLaunchKernelGGL(k_estimate, dim3((nt + 127) / 128), dim3(128), 0, s, nt, tets, tet2edge, neighbor, non, pts, attr,
-                     sig6, omega, ns, slots, sol, ld, errors);
+  switch (ns) {
+#define EMF_EST(K)                                                                                                 \
+  case K:                                                                                                          \
+    hipLaunchKernelGGL((k_estimate<K>), dim3((nt + 127) / 128), dim3(128), 0, s, nt, tets, tet2edge, neighbor, non, \
+                       pts, attr, sig6, omega, slots, sol, ld, errors);                                            \
+    break;
+    EMF_EST(1) EMF_EST(2) EMF_EST(3) EMF_EST(4) EMF_EST(5) EMF_EST(6) EMF_EST(7) EMF_EST(8)
+#undef EMF_EST
+    default: EMF_REQUIRE(false, kErrArg, "estimator supports 1 to 8 solutions per call");
+  }
   EMF_HIP(hipGetLastError());
 }
 

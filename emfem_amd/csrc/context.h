@@ -57,6 +57,11 @@ struct emfem_context {
 
   // preconditioner
   std::unique_ptr<AmsHost> ams_host;
+  // host copy of the matrix pattern (values B = C + M filled per preconditioner
+  // set-up) and the mesh-only auxiliary matrices, both kept while the linear system
+  // lives
+  HostCsr h_B;
+  std::shared_ptr<const AmsAux> ams_aux;
   std::unique_ptr<InnerPc> pc;
 
   std::map<std::string, double> stats;
@@ -204,7 +209,14 @@ inline PcOptions pc_options(const emfem_context *c) {
   o.amg.theta = c->optd("ams_theta");
   o.amg.additive = c->opt("ams_cycle") == "additive";
   o.use_graph = c->opt("use_graph") == "true";
-  o.spmm_rows = c->opt("spmm_rows") == "lanes" ? 1 : (c->opt("spmm_rows") == "rowblock" ? 2 : 0);
+  // spmm_rows: 0 tuned, 1 lanes per row tuned, 2 row blocks, 3 x-blocks (16 or more
+  // columns), 10 + n fixed n lanes per row
+  const std::string sr = c->opt("spmm_rows");
+  o.spmm_rows = sr == "lanes" ? 1
+                : sr == "rowblock" ? 2
+                : sr == "xblock"   ? 3
+                : sr == "auto"     ? 0
+                                   : 10 + c->opti("spmm_rows");
   return o;
 }
 

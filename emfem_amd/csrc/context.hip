@@ -87,7 +87,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"dual_precision", {"fp32", "fp64"}},
       {"fp64_fallback", {"true", "false"}},
       {"use_graph", {"true", "false"}},
-      {"spmm_rows", {"auto", "lanes", "rowblock"}},
+      {"spmm_rows", {"auto", "lanes", "rowblock", "xblock", "2", "4", "8", "16"}},
       {"ams_cycle", {"multiplicative", "additive"}},
       {"surrogate_cycles", {"true", "false"}},
       {"save_mesh", {"true", "false"}},
@@ -314,8 +314,11 @@ int emfem_create_linear_system(emfem_context *c, int n_slots) {
     EMF_REQUIRE(n_slots > 0 && n_slots <= 512, kErrArg, "n_slots must be in 1..512");
     const double t0 = now_ms();
     c->pc.reset();
-    std::vector<int> rp, ci;
+    c->ams_aux.reset();
+    c->h_B = HostCsr();
+    std::vector<int> &rp = c->h_B.rp, &ci = c->h_B.ci;
     build_pattern(c->T, rp, ci);
+    c->h_B.n = c->h_B.m = c->T.ne;
     c->nnz = (int)ci.size();
     c->d_rp.upload(rp, c->stream);
     c->d_ci.upload(ci, c->stream);
@@ -335,6 +338,8 @@ int emfem_create_linear_system(emfem_context *c, int n_slots) {
 int emfem_destroy_linear_system(emfem_context *c) {
   return guarded(c, [&]() {
     c->pc.reset();
+    c->ams_aux.reset();
+    c->h_B = HostCsr();
     c->d_rp.release();
     c->d_ci.release();
     c->d_K.release();
@@ -463,22 +468,22 @@ int emfem_create_pc(emfem_context *c) {
     EMF_REQUIRE(c->assembled_fidx >= 0, kErrState, "matrix not assembled");
     const double t0 = now_ms();
     hipStream_t s = c->stream;
-    // B = C + M on the host for the auxiliary-space set-up
-    std::vector<double2> K(c->nnz);
-    c->d_K.download(K.data(), c->nnz, s);
-    HostCsr B;
-    B.n = B.m = c->T.ne;
-    B.rp.resize(c->T.ne + 1);
-    B.ci.resize(c->nnz);
-    c->d_rp.download(B.rp.data(), c->T.ne + 1, s);
-    c->d_ci.download(B.ci.data(), c->nnz, s);
-    EMF_HIP(hipStreamSynchronize(s));
-    B.v.resize(c->nnz);
-    for (int i = 0; i < c->nnz; ++i) B.v[i] = K[i].x + K[i].y;
+    // B = C + M (formed on the device) on the host pattern for the auxiliary-space
+    // set-up
+    HostCsr &B = c->h_B;
+    {
+      DevBuf<double> bsum(c->nnz);
+      make_b<double>(s, c->nnz, c->d_K.get(), bsum.get());
+      B.v.resize(c->nnz);
+      bsum.download(B.v.data(), c->nnz, s);
+      EMF_HIP(hipStreamSynchronize(s));
+    }
     PcOptions o = pc_options(c);
+    c->pc.reset();
     c->ams_host.reset(new AmsHost());
     const double tb = now_ms();
-    build_ams(c->T, B, o.amg, *c->ams_host);
+    build_ams(c->T, B, o.amg, *c->ams_host, c->ams_aux);
+    c->ams_aux = c->ams_host->aux;
     const double t1 = now_ms();
     c->stats["pc_download_ms"] = tb - t0;
     for (auto &kv : c->ams_host->timing) c->stats["pc_" + kv.first] = kv.second;
@@ -519,11 +524,14 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
     c->pc->set_fp32(c->opt("inner_precision") == "fp32", s);
     c->pc->set_use_graph(c->opt("use_graph") == "true");
     SolveStats st;
+    const bool plain = c->opt("use_graph") == "false";
+    c->pc->smoother_clock(plain, s);
     const double t0 = now_ms();
     fgmres(s, ne, c->nnz, c->d_rp.get(), c->d_ci.get(), c->d_K.get(), *c->pc, n, b.get(), x.get(), tol, oo, st);
     axpby<double, double>(s, ne, nc, 1.0, x.get(), nc, 0.0, c->d_X.get() + 2 * first, 2 * c->nslots);
     EMF_HIP(hipStreamSynchronize(s));
     const double ms = now_ms() - t0;
+    const double in_cycle = c->pc->smoother_clock(false, s);
     c->stats["solve_ms"] = ms;
     c->stats["outer_its"] = st.outer_its;
     c->stats["inner_its"] = (double)st.inner_its;
@@ -537,7 +545,11 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
       const int reps = 20;
       c->stats["op_spmm_avg_ms"] = time_operator(s, ne, c->nnz, c->d_rp.get(), c->d_ci.get(), c->d_K.get(), n, x.get(),
                                                  b.get(), reps);
-      c->stats["b_spmm_avg_ms"] = c->pc->time_smoother(s, nc, reps);
+      // the smoother SpMM as launched in the cycles (events around each launch) and
+      // back to back on resident operands
+      c->stats["b_spmm_avg_ms"] = in_cycle;
+      c->stats["b_spmm_back_to_back_ms"] = c->pc->time_smoother(s, nc, reps);
+      c->stats["b_spmm_rows"] = c->pc->smoother_rows(s, nc);
       c->stats["fcg_spmm_avg_ms"] = c->pc->time_fcg(s, nc, reps);
     }
     c->stats["op_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * 20 + double(ne) * n * 32;

@@ -266,13 +266,23 @@ void forward_mt(emfem_context *c, int fidx) {
     check(c, emfem_assemble_rhs_mt(c, fidx, EMFEM_YX_POLAR, 1));
     const double e_rtol = c->optd("e_rtol");
     if (sur && !last) {
+      // primal indicators exactly; the dual solve's indicator from the network
       solve_slots(c, 0, 2, {e_rtol, e_rtol});
       check(c, emfem_destroy_pc(c));
-      const std::vector<double> ind = surrogate_indicator(c, fidx, 0, 1);
+      const int nt = c->T.nt;
+      const int slots[2] = {0, 1};
+      std::vector<double> err((size_t)2 * nt);
+      check(c, emfem_estimate_error(c, fidx, 2, slots, err.data()));
+      const std::vector<double> dual = surrogate_indicator(c, fidx, 0, 1);
+      std::vector<double> exy(nt), eyx(nt);
+      for (int t = 0; t < nt; ++t) {
+        exy[t] = std::sqrt(err[t] * dual[t]);
+        eyx[t] = std::sqrt(err[(size_t)nt + t] * dual[t]);
+      }
       c->stats["surrogate_cycles_used"] += 1;
-      if (saving_mesh(c)) write_vtk(c, vtk_name(c, fidx, -3, cycle), -3, ind.data(), ind.data());
+      if (saving_mesh(c)) write_vtk(c, vtk_name(c, fidx, -3, cycle), -3, exy.data(), eyx.data());
       check(c, emfem_destroy_linear_system(c));
-      refine_by_error(c, {ind});
+      refine_by_error(c, {exy, eyx});
       continue;
     }
     check(c, emfem_assemble_rhs_dual(c, 2));
@@ -290,10 +300,15 @@ void forward_mt(emfem_context *c, int fidx) {
     }
     if (saving_mesh(c)) write_vtk(c, vtk_name(c, fidx, -3, cycle), -3, exy.data(), eyx.data());
     if (sur && last) {
-      // the plain estimator judges the surrogate on the final mesh
-      std::vector<double> plain(nt);
-      for (int t = 0; t < nt; ++t) plain[t] = exy[t] + eyx[t];
-      const double corr = rank_correlation(surrogate_indicator(c, fidx, 0, 1), plain);
+      // the plain estimator judges the surrogate on the final mesh: the indicators
+      // with the predicted and with the solved dual
+      const std::vector<double> dual = surrogate_indicator(c, fidx, 0, 1);
+      std::vector<double> plain(nt), pred(nt);
+      for (int t = 0; t < nt; ++t) {
+        plain[t] = exy[t] + eyx[t];
+        pred[t] = std::sqrt(err[t] * dual[t]) + std::sqrt(err[(size_t)nt + t] * dual[t]);
+      }
+      const double corr = rank_correlation(pred, plain);
       c->stats["surrogate_rank_corr"] = corr;
       if (corr < c->optd("surrogate_min_corr")) c->sur_rejected = true;
       c->stats["surrogate_rejected"] = c->sur_rejected ? 1.0 : 0.0;

@@ -26,7 +26,27 @@ struct Csr {
   // blocks (row_blocks), for the column count and precision the table was built for
   const int *blk_row = nullptr;
   int nblk = 0;
+  // lpr = -2 (16 or more columns): LDS x-blocks of 64-column slabs.
+  // Block b takes the rows [xb_row[b], xb_row[b+1]); its distinct columns
+  // xb_ucol[xb_uptr[b] .. xb_uptr[b+1]) (at most xb_umax, ascending) are staged in LDS
+  // and xb_lci [nnz] holds each nonzero's slot among them.
+  const int *xb_row = nullptr, *xb_uptr = nullptr, *xb_ucol = nullptr;
+  const int *xb_lci = nullptr;
+  int xb_nblk = 0, xb_umax = 0;
 };
+
+// Host tables of the LDS x-block SpMM for a pattern (see Csr::xb_*): consecutive rows
+// while their distinct columns number at most umax and the rows at most rmax.  Empty
+// when a single row has more than umax distinct columns.
+struct XBlocks {
+  std::vector<int> row, uptr, ucol;
+  std::vector<int> lci;
+  int umax = 0;
+  bool empty() const { return row.size() < 2; }
+};
+XBlocks x_blocks(const std::vector<int> &rp, const std::vector<int> &ci, int ncols, int umax, int rmax);
+constexpr int kXBlockCols = 64;  // distinct columns per x-block
+constexpr int kXBlockRows = 64;
 
 // Nonzeros staged per workgroup by the row-block SpMM: 16 KB of products.
 template <typename T, int NC>
@@ -42,8 +62,9 @@ std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
 
 // Picks the fastest row mapping of the narrow SpMM for A on nc columns by timing
 // each with HIP events on stream s: lanes per row 2, 4, 8, 16, and -1 (LDS-staged
-// row blocks) when A carries a block table for nc; x and y are scratch blocks of
-// [A.ncols][nc] and [A.nrows][nc].  Returns 0 when nc uses the wide kernel.
+// row blocks) when A carries a block table for nc; for 16 or more columns the slab
+// width of the wide kernel (16 .. 256) and, when A carries x-block tables, -2 and -3
+// (LDS x-blocks).  x and y are scratch blocks of [A.ncols][nc] and [A.nrows][nc].
 template <typename TA, typename TX, typename T>
 int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y);
 

@@ -42,7 +42,9 @@ struct PcOptions {
   int inner_max_it = 100;
   AmgParams amg;
   bool use_graph = true;  // replay each auxiliary-space cycle as one captured hipGraph
-  // smoother SpMM row mapping: 0 = tuned, 1 = lanes per row only, 2 = row blocks when possible
+  // smoother SpMM row mapping: 0 = tuned, 1 = lanes per row only (tuned; lanes on
+  // columns at 16 or more columns), 2 = row blocks when possible, 3 = LDS x-blocks at
+  // 16 or more columns when possible, 10 + n = n lanes per row
   int spmm_rows = 0;
 };
 
@@ -64,7 +66,13 @@ class InnerPc {
   // ms per launch of the B SpMM of the Chebyshev smoother (cycle precision) and of
   // the fp64 FCG on nc columns, from HIP events around `reps` back-to-back launches
   virtual double time_smoother(hipStream_t s, int nc, int reps) = 0;
+  // in-cycle timing of the smoother SpMM with HIP events around each launch (graph
+  // replays are not timed): on = start; off = stop and return ms per launch
+  virtual double smoother_clock(bool on, hipStream_t s) = 0;
   virtual double time_fcg(hipStream_t s, int nc, int reps) = 0;
+  // row mapping the smoother SpMM uses on nc columns: lanes per row, -1 = row blocks,
+  // 0 = many-column kernel
+  virtual int smoother_rows(hipStream_t s, int nc) = 0;
 };
 
 std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp,

@@ -194,17 +194,18 @@ class DevAms {
       B64_.v = b64vals_.get();
     }
     upload_vec(l1inv_, H.l1inv, s);
-    additive_ = H.additive;
+    const AmsAux &X = *H.aux;
+    additive_ = X.additive;
     if (!additive_) {
-      G0_.upload(H.G0, s);
-      G0T_.upload(H.G0T, s);
+      G0_.upload(X.G0, s);
+      G0T_.upload(X.G0T, s);
       amgG_.upload(H.amgG, s);
     }
-    Pi_.upload(H.Pi, s);
-    PiT_.upload(H.PiT, s);
+    Pi_.upload(X.Pi, s);
+    PiT_.upload(X.PiT, s);
     amgPi_.upload(H.amgPi, s);
-    nv0_ = H.nv0;
-    nblk_ = H.nblk;
+    nv0_ = X.nv0;
+    nblk_ = X.nblk;
     spmm_rows_ = o.spmm_rows;
     deg_ = std::max(1, o.cheb_degree);
     const double lmax = 1.0;  // Gershgorin: the l1-scaled operator has spectrum in (0, 1]
@@ -214,7 +215,14 @@ class DevAms {
     EMF_HIP(hipStreamSynchronize(s));
   }
 
-  const Csr<double> &B64() const { return B64_; }
+  // B in fp64 with the row mapping tuned for nc columns (x-blocks or wide slabs at 16
+  // or more columns once tune(nc) has run)
+  Csr<double> B64(int nc) const {
+    Csr<double> B = B64_;
+    auto it = lpr64_.find(nc);
+    if (it != lpr64_.end()) B.lpr = it->second;
+    return B;
+  }
 
   void ensure(int nc) {
     if (nc <= nc_) return;
@@ -237,13 +245,67 @@ class DevAms {
     d_.zero(s);
     Csr<T> B = B_;
     const std::vector<int> blk =
-        spmm_rows_ == 1 ? std::vector<int>() : row_blocks(hrp_, stream_tile_for(nc, sizeof(T) == 8));
+        spmm_rows_ == 1 || spmm_rows_ >= 10 ? std::vector<int>() : row_blocks(hrp_, stream_tile_for(nc, sizeof(T) == 8));
     if (!blk.empty()) {
       blk_[nc].upload(blk, s);
       B.blk_row = blk_[nc].get();
       B.nblk = (int)blk.size() - 1;
     }
-    lpr_[nc] = spmm_rows_ == 2 && !blk.empty() ? -1 : tune_lpr<T, T, T>(s, B, nc, d_.get(), rres_.get());
+    if (nc >= 16) {
+      if (spmm_rows_ != 1) x_blocks_table(s);  // "lanes": lanes on columns only
+      B.xb_row = B_.xb_row;
+      B.xb_uptr = B_.xb_uptr;
+      B.xb_ucol = B_.xb_ucol;
+      B.xb_lci = B_.xb_lci;
+      B.xb_nblk = B_.xb_nblk;
+      B.xb_umax = B_.xb_umax;
+      // the fp64 B (fp64 residuals here, the FCG operator of the inner solve)
+      DevBuf<double> tmp((size_t)ne_ * nc);
+      r64_.zero(s);
+      const bool xb = B_.xb_nblk > 0;
+      if (spmm_rows_ == 3 && xb) {
+        // forced x-blocks
+        lpr_[nc] = -2;
+        lpr64_[nc] = -2;
+        return;
+      }
+      lpr64_[nc] = tune_lpr<double, double, double>(s, B64_, nc, r64_.get(), tmp.get());
+      EMF_HIP(hipStreamSynchronize(s));
+    }
+    if (spmm_rows_ >= 10 && nc < 16) lpr_[nc] = spmm_rows_ - 10;  // fixed lanes per row
+    else if (spmm_rows_ == 2 && !blk.empty()) lpr_[nc] = -1;
+    else lpr_[nc] = tune_lpr<T, T, T>(s, B, nc, d_.get(), rres_.get());
+  }
+
+  // LDS x-block tables of the pattern of B (shared by B and B64), built once on first
+  // use with 16 or more columns
+  void x_blocks_table(hipStream_t s) {
+    if (xb_tried_) return;
+    xb_tried_ = true;
+    std::vector<int> hci(B_.nnz);
+    EMF_HIP(hipMemcpy(hci.data(), B_.ci, sizeof(int) * hci.size(), hipMemcpyDeviceToHost));
+    const XBlocks X = x_blocks(hrp_, hci, ne_, kXBlockCols, kXBlockRows);
+    if (X.empty()) return;
+    xb_row_.upload(X.row, s);
+    xb_uptr_.upload(X.uptr, s);
+    xb_ucol_.upload(X.ucol, s);
+    xb_lci_.upload(X.lci, s);
+    EMF_HIP(hipStreamSynchronize(s));  // X is host memory of this scope
+    auto set = [&](auto &B) {
+      B.xb_row = xb_row_.get();
+      B.xb_uptr = xb_uptr_.get();
+      B.xb_ucol = xb_ucol_.get();
+      B.xb_lci = xb_lci_.get();
+      B.xb_nblk = (int)X.row.size() - 1;
+      B.xb_umax = X.umax;
+    };
+    set(B_);
+    set(B64_);
+  }
+
+  int rows(hipStream_t s, int nc) {
+    tune(s, nc);
+    return lpr_[nc];
   }
 
   Csr<T> smoother(int nc) {
@@ -258,6 +320,7 @@ class DevAms {
 
   ~DevAms() {
     for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    for (auto e : clock_) (void)hipEventDestroy(e);
   }
 
   // out = M_AMS b: b fp64 compact [E][nc], out in T; out must not alias b.  With
@@ -323,12 +386,48 @@ class DevAms {
 
   void bspmm(hipStream_t s, int nc, const T *x, T *y, T alpha, const T *rs, T gamma, const T *w, T beta,
              const T *z) {
+    // in-cycle timing (plain launches only): HIP events around the first launches
+    const bool timed = clock_on_ && clock_.size() < 2 * kClockPairs;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+      EMF_HIP(hipEventCreate(&e0));
+      EMF_HIP(hipEventCreate(&e1));
+      EMF_HIP(hipEventRecord(e0, s));
+    }
     spmm<T>(s, smoother(nc), nc, x, nc, y, nc, alpha, rs, gamma, w, nc, beta, z, nc);
+    if (timed) {
+      EMF_HIP(hipEventRecord(e1, s));
+      clock_.push_back(e0);
+      clock_.push_back(e1);
+    }
   }
+
+ public:
+  // Starts (on) or ends (off) the in-cycle timing of the smoother SpMM; ending it
+  // returns the average ms per timed launch (0 if none).
+  double in_cycle_clock(bool on, hipStream_t s) {
+    double avg = 0.0;
+    if (!on && !clock_.empty()) {
+      EMF_HIP(hipStreamSynchronize(s));
+      double tot = 0.0;
+      for (size_t i = 0; i + 1 < clock_.size(); i += 2) {
+        float ms = 0.f;
+        EMF_HIP(hipEventElapsedTime(&ms, clock_[i], clock_[i + 1]));
+        tot += ms;
+      }
+      avg = tot / (clock_.size() / 2);
+    }
+    for (auto e : clock_) (void)hipEventDestroy(e);
+    clock_.clear();
+    clock_on_ = on;
+    return avg;
+  }
+
+ private:
 
   // r64 = b - B x in fp64 (the residual the auxiliary spaces see)
   void residual64(hipStream_t s, int nc, const double *b, const T *x) {
-    spmm_mixed<double, T, double>(s, B64_, nc, x, nc, r64_.get(), nc, -1.0, nullptr, 1.0, b, nc, 0.0, nullptr, 0);
+    spmm_mixed<double, T, double>(s, B64(nc), nc, x, nc, r64_.get(), nc, -1.0, nullptr, 1.0, b, nc, 0.0, nullptr, 0);
   }
 
   void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x) {
@@ -360,8 +459,14 @@ class DevAms {
   DevCsr<T> G0_, G0T_, Pi_, PiT_;
   DevAmg<T> amgG_, amgPi_;
   std::map<std::tuple<int, const void *, const void *>, hipGraphExec_t> graphs_;
-  std::map<int, int> lpr_;
+  std::map<int, int> lpr_, lpr64_;
   std::map<int, DevBuf<int>> blk_;
+  bool xb_tried_ = false;
+  DevBuf<int> xb_row_, xb_uptr_, xb_ucol_;
+  DevBuf<int> xb_lci_;
+  static constexpr size_t kClockPairs = 512;
+  bool clock_on_ = false;
+  std::vector<hipEvent_t> clock_;
   std::vector<int> hrp_;  // host row pointers of B (row-block tables)
 };
 
@@ -383,6 +488,13 @@ class InnerPcImpl : public InnerPc {
   double time_smoother(hipStream_t s, int nc, int reps) override {
     ensure(nc);
     return fp32_ ? a32_->time_b(s, nc, reps) : a64_->time_b(s, nc, reps);
+  }
+  double smoother_clock(bool on, hipStream_t s) override {
+    return fp32_ ? (a32_ ? a32_->in_cycle_clock(on, s) : 0.0) : (a64_ ? a64_->in_cycle_clock(on, s) : 0.0);
+  }
+  int smoother_rows(hipStream_t s, int nc) override {
+    ensure(nc);
+    return fp32_ ? a32_->rows(s, nc) : a64_->rows(s, nc);
   }
   double time_fcg(hipStream_t s, int nc, int reps) override {
     ensure(nc);
@@ -414,7 +526,14 @@ class InnerPcImpl : public InnerPc {
   }
 
  private:
-  const Csr<double> &B64() const { return a64_ ? a64_->B64() : a32_->B64(); }
+  Csr<double> B64(hipStream_t s, int nc) {
+    if (a64_) {
+      a64_->tune(s, nc);
+      return a64_->B64(nc);
+    }
+    a32_->tune(s, nc);
+    return a32_->B64(nc);
+  }
 
   void ensure(int nc) {
     if (nc <= nc_) return;
@@ -500,7 +619,7 @@ class InnerPcImpl : public InnerPc {
   }
 
   void bspmm(hipStream_t s, int nc, const double *x, double *y) {
-    spmm<double>(s, B64(), nc, x, nc, y, nc, 1.0, nullptr, 0.0, nullptr, 0, 0.0, nullptr, 0);
+    spmm<double>(s, B64(s, nc), nc, x, nc, y, nc, 1.0, nullptr, 0.0, nullptr, 0, 0.0, nullptr, 0);
   }
 
   const AmsHost &H_;

@@ -143,24 +143,30 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
   }
 }
 
-template <typename TA, typename TX, typename T, int CPL>
-__global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__restrict__ rp,
+// Many columns (nc >= 16, batched right-hand sides): LW lanes per row, lane l owning
+// columns l, l + LW, ..., l + (CPL-1) LW (each load instruction reads LW consecutive
+// columns of a row: coalesced), 64 / LW rows per wavefront; kU nonzeros per trip with
+// all column/value loads, then all row gathers, in flight together.
+template <typename TA, typename TX, typename T, int LW, int CPL>
+__global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, int rblocks, const int *__restrict__ rp,
                                                       const int *__restrict__ ci, const TA *__restrict__ av,
                                                       const TX *__restrict__ x, int ldx, T *y, int ldy, int nc,
                                                       T alpha, const T *__restrict__ rs, T gamma, const T *w,
                                                       int ldw, T beta, const T *z, int ldz, const T *__restrict__ xs,
                                                       const T *__restrict__ zs) {
-  int row = xcd_block() * (kBlock / 64) + threadIdx.x / 64;
-  row = __builtin_amdgcn_readfirstlane(row);
-  const int lane = threadIdx.x % 64;
+  // slab-major block order: with the XCD-aware remap each XCD sweeps all rows of one
+  // slab (or a contiguous share of one), so its L2 holds only that slab's window of x
+  const int bid = xcd_block();
+  const int slab = bid / rblocks;
+  const int gid = (bid - slab * rblocks) * kBlock + threadIdx.x;
+  const int row = gid / LW;
+  const int c0 = slab * LW * CPL + gid % LW;  // first column of this lane
   if (row >= nrows) return;
   T acc[CPL];
 #pragma unroll
   for (int j = 0; j < CPL; ++j) acc[j] = T(0);
   const int b = rp[row], e = rp[row + 1];
-  // the row index is wave-uniform, so column and value loads are scalar; four
-  // nonzeros per trip keep four row gathers per lane in flight
-  constexpr int kU = 4;
+  constexpr int kU = (sizeof(T) == 8 || sizeof(TX) == 8) ? 4 : 8;  // registers of the gathers
   for (int p0 = b; p0 < e; p0 += kU) {
     int cu[kU];
     T au[kU];
@@ -174,12 +180,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
     TX xv[kU][CPL];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-      const TX *xr = x + (size_t)cu[u] * ldx;
+      const TX *xr = x + (size_t)cu[u] * ldx + c0;
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) {
-        const int c = lane + 64 * j;
-        xv[u][j] = c < nc ? xr[c] : TX(0);
-      }
+      for (int j = 0; j < CPL; ++j) xv[u][j] = c0 + j * LW < nc ? xr[j * LW] : TX(0);
     }
 #pragma unroll
     for (int u = 0; u < kU; ++u)
@@ -190,12 +193,152 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, const int *__re
   const T zscale = zs ? beta * zs[row] : beta;
 #pragma unroll
   for (int j = 0; j < CPL; ++j) {
-    const int c = lane + 64 * j;
+    const int c = c0 + j * LW;
     if (c < nc) {
       T v = alpha * acc[j];
       if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
       v *= scale;
       if (beta != T(0)) v += zscale * z[(size_t)row * ldz + c];
+      y[(size_t)row * ldy + c] = v;
+    }
+  }
+}
+
+// Wide SpMM with 16-byte loads: LW lanes per row, each lane owns VEC = 16 / sizeof(TX)
+// consecutive columns of a slab of LW * VEC columns and gathers them with one vector
+// load per nonzero (nc a multiple of VEC, x rows 16-byte aligned).  Slab-major block
+// order as in k_spmm_wide.
+template <typename TX>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+};
+
+template <typename TA, typename TX, typename T, int LW>
+__global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci, const TA *__restrict__ av,
+                                                      const TX *__restrict__ x, int ldx, T *y, int ldy, int nc,
+                                                      T alpha, const T *__restrict__ rs, T gamma, const T *w,
+                                                      int ldw, T beta, const T *z, int ldz,
+                                                      const T *__restrict__ xs, const T *__restrict__ zs) {
+  using V = typename Vec16<TX>::type;
+  constexpr int VEC = Vec16<TX>::n;
+  const int bid = xcd_block();
+  const int slab = bid / rblocks;
+  const int gid = (bid - slab * rblocks) * kBlock + threadIdx.x;
+  const int row = gid / LW;
+  const int c0 = slab * LW * VEC + (gid % LW) * VEC;
+  if (row >= nrows || c0 >= nc) return;
+  T acc[VEC];
+#pragma unroll
+  for (int j = 0; j < VEC; ++j) acc[j] = T(0);
+  const int b = rp[row], e = rp[row + 1];
+  constexpr int kU = 4;
+  for (int p0 = b; p0 < e; p0 += kU) {
+    int cu[kU];
+    T au[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const bool ok = p0 + u < e;
+      cu[u] = ok ? ci[p0 + u] : ci[b];
+      au[u] = ok ? (T)av[p0 + u] : T(0);
+      if (xs && ok) au[u] *= xs[cu[u]];
+    }
+    V xv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) xv[u] = *reinterpret_cast<const V *>(x + (size_t)cu[u] * ldx + c0);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const TX *xe = reinterpret_cast<const TX *>(&xv[u]);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) acc[j] += au[u] * (T)xe[j];
+    }
+  }
+  const T scale = rs ? rs[row] : T(1);
+  const T zscale = zs ? beta * zs[row] : beta;
+#pragma unroll
+  for (int j = 0; j < VEC; ++j) {
+    const int c = c0 + j;
+    T v = alpha * acc[j];
+    if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
+    v *= scale;
+    if (beta != T(0)) v += zscale * z[(size_t)row * ldz + c];
+    y[(size_t)row * ldy + c] = v;
+  }
+}
+
+// LDS x-blocks for many columns: a workgroup takes one row block of one 64-column slab,
+// stages the block's distinct x rows for the slab in LDS once (consecutive rows of an
+// edge matrix share most of their columns), and each wavefront then sums its rows
+// from LDS with lanes on columns.  Rows are wavefront-uniform, so a nonzero's value
+// and LDS slot are scalar loads; each row is summed in nonzero order, as in the other
+// SpMM kernels.
+template <typename TA, typename TX, typename T>
+__global__ __launch_bounds__(kBlock) void k_spmm_xblk(int nblk, const int *__restrict__ rp,
+                                                      const TA *__restrict__ av, const int *__restrict__ xb_row,
+                                                      const int *__restrict__ xb_uptr,
+                                                      const int *__restrict__ xb_ucol,
+                                                      const int *__restrict__ lci, const TX *__restrict__ x,
+                                                      int ldx, T *y, int ldy, int nc, T alpha,
+                                                      const T *__restrict__ rs, T gamma, const T *w, int ldw,
+                                                      T beta, const T *z, int ldz, int vec_ok) {
+  extern __shared__ __align__(16) unsigned char xblk_lds[];
+  TX *xl = reinterpret_cast<TX *>(xblk_lds);
+  using V = typename Vec16<TX>::type;
+  constexpr int VEC = Vec16<TX>::n;
+  constexpr int W = 64;
+  constexpr int kPer = W / VEC;  // threads per staged x row
+  const int bid = xcd_block();
+  const int slab = bid / nblk;
+  const int b = bid - slab * nblk;
+  const int cb = slab * W;
+  const int u0 = xb_uptr[b], nu = xb_uptr[b + 1] - u0;
+  if (vec_ok) {
+    // 16-byte loads, all of a thread's loads issued before the LDS stores
+    constexpr int kRowsPerPass = kBlock / kPer;
+    constexpr int kPass = kXBlockCols / kRowsPerPass;
+    const int k0 = threadIdx.x / kPer, cc = (threadIdx.x % kPer) * VEC;
+    V v[kPass];
+#pragma unroll
+    for (int q = 0; q < kPass; ++q) {
+      const int k = k0 + q * kRowsPerPass;
+      const int c = cb + cc;
+      if (k < nu && c < nc) v[q] = *reinterpret_cast<const V *>(x + (size_t)xb_ucol[u0 + k] * ldx + c);
+      else v[q] = V{};
+    }
+#pragma unroll
+    for (int q = 0; q < kPass; ++q) {
+      const int k = k0 + q * kRowsPerPass;
+      if (k < nu) *reinterpret_cast<V *>(xl + k * W + cc) = v[q];
+    }
+  } else {
+    for (int i = threadIdx.x; i < nu * W; i += kBlock) {
+      const int k = i / W, c = cb + (i % W);
+      xl[i] = c < nc ? x[(size_t)xb_ucol[u0 + k] * ldx + c] : TX(0);
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r1 = xb_row[b + 1];
+  for (int row = xb_row[b] + wave; row < r1; row += kBlock / 64) {
+    T acc = T(0);
+    const int pb = rp[row], pe = rp[row + 1];
+#pragma unroll 4
+    for (int p = pb; p < pe; ++p) acc += (T)av[p] * (T)xl[lci[p] * W + lane];
+    const int c = cb + lane;
+    if (c < nc) {
+      T v = alpha * acc;
+      if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
+      if (rs) v *= rs[row];
+      if (beta != T(0)) v += beta * z[(size_t)row * ldz + c];
       y[(size_t)row * ldy + c] = v;
     }
   }
@@ -310,29 +453,119 @@ std::vector<int> row_blocks(const std::vector<int> &rp, int tile) {
   return blk;
 }
 
+XBlocks x_blocks(const std::vector<int> &rp, const std::vector<int> &ci, int ncols, int umax, int rmax) {
+  XBlocks X;
+  X.umax = umax;
+  const int n = (int)rp.size() - 1;
+  if (umax <= 0 || umax > 65536) return X;
+  std::vector<int> mark(ncols, -1), slot(ncols, 0);
+  std::vector<int> cols;
+  X.row.push_back(0);
+  X.uptr.push_back(0);
+  X.lci.resize(ci.size());
+  int blk = 0;
+  for (int r = 0; r < n;) {
+    cols.clear();
+    int e = r;
+    while (e < n && e - r < rmax) {
+      int fresh = 0;
+      for (int p = rp[e]; p < rp[e + 1]; ++p)
+        if (mark[ci[p]] != blk) {
+          mark[ci[p]] = blk;
+          slot[ci[p]] = -1;  // provisional
+          ++fresh;
+        }
+      if ((int)cols.size() + fresh > umax) {
+        // roll back this row's provisional marks
+        for (int p = rp[e]; p < rp[e + 1]; ++p)
+          if (slot[ci[p]] == -1) mark[ci[p]] = blk - 1 < 0 ? -2 : blk - 1;
+        break;
+      }
+      for (int p = rp[e]; p < rp[e + 1]; ++p)
+        if (slot[ci[p]] == -1) {
+          slot[ci[p]] = 0;
+          cols.push_back(ci[p]);
+        }
+      ++e;
+    }
+    if (e == r) return XBlocks();  // one row has more than umax distinct columns
+    std::sort(cols.begin(), cols.end());
+    for (int k = 0; k < (int)cols.size(); ++k) slot[cols[k]] = k;
+    for (int p = rp[r]; p < rp[e]; ++p) X.lci[p] = slot[ci[p]];
+    X.ucol.insert(X.ucol.end(), cols.begin(), cols.end());
+    X.row.push_back(e);
+    X.uptr.push_back((int)X.ucol.size());
+    ++blk;
+    r = e;
+  }
+  return X;
+}
+
 template <typename TA, typename TX, typename T>
 void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
              T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs) {
   if (A.nrows == 0 || nc == 0) return;
-  EMF_REQUIRE(A.lpr >= 0 || (nc <= 8 && nc != 5 && nc != 7), kErrArg,
+  EMF_REQUIRE(A.lpr != -1 || (nc <= 8 && nc != 5 && nc != 7), kErrArg,
               "row-block SpMM supports 1, 2, 3, 4, 6 or 8 columns");
-  if (nc >= 16) {
-    const int g = (A.nrows + kBlock / 64 - 1) / (kBlock / 64);
-    if (nc <= 64)
-      hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, 1>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y,
-                         ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs);
-    else if (nc <= 128)
-      hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, 2>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x, ldx, y,
-                         ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs);
-    else {
-      // 256-column slabs
-      for (int c0 = 0; c0 < nc; c0 += 256) {
-        const int n = std::min(256, nc - c0);
-        hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, 4>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, A.v, x + c0,
-                           ldx, y + c0, ldy, n, alpha, rs, gamma, w ? w + c0 : nullptr, ldw, beta,
-                           z ? z + c0 : nullptr, ldz, xs, zs);
-      }
+  EMF_REQUIRE(A.lpr >= -1 || nc >= 16, kErrArg, "x-block SpMM needs 16 or more columns");
+  if (nc >= 16 && A.lpr == -2) {
+    EMF_REQUIRE(A.xb_row != nullptr && A.xb_nblk > 0 && A.xb_umax <= kXBlockCols, kErrArg,
+                "x-block SpMM without its block tables");
+    EMF_REQUIRE(xs == nullptr && zs == nullptr, kErrArg, "x-block SpMM has no scaled operands");
+    const int nslab = (nc + 63) / 64;
+    const size_t lds = (size_t)kXBlockCols * 64 * sizeof(TX);
+    const int vec_ok = ((uintptr_t)x % 16 == 0) && ((size_t)ldx * sizeof(TX)) % 16 == 0 && nc % 4 == 0;
+    hipLaunchKernelGGL((k_spmm_xblk<TA, TX, T>), dim3(A.xb_nblk * nslab), dim3(kBlock), lds, s, A.xb_nblk, A.rp, A.v,
+                       A.xb_row, A.xb_uptr, A.xb_ucol, A.xb_lci, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta,
+                       z, ldz, vec_ok);
+    EMF_HIP(hipGetLastError());
+    return;
+  }
+  const int vec = 16 / (int)sizeof(TX);
+  const bool vec_ok = nc % vec == 0 && (uintptr_t)x % 16 == 0 && ((size_t)ldx * sizeof(TX)) % 16 == 0;
+  // untuned: 16-byte gathers with 16 lanes per row where the operands allow
+  const int code = A.lpr != 0 || nc < 16 ? A.lpr : (vec_ok ? 16 * vec + 1 : 0);
+  if (nc >= 16 && code >= 16 && (code & 1) && vec_ok) {
+    // 16-byte gathers: slab of W = code - 1 columns, 16 / sizeof(TX) per lane (other
+    // operands take the strided kernel with the same slab)
+    const int W = code - 1;
+    const int lw = W / vec;
+    const int rb = grid_for((int64_t)A.nrows * lw);
+    const int nslab = (nc + W - 1) / W;
+#define EMF_WVEC(LW)                                                                                             \
+  hipLaunchKernelGGL((k_spmm_wvec<TA, TX, T, LW>), dim3(rb * nslab), dim3(kBlock), 0, s, A.nrows, rb, A.rp, A.ci, \
+                     A.v, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs)
+    switch (lw) {
+      case 4: EMF_WVEC(4); break;
+      case 8: EMF_WVEC(8); break;
+      case 16: EMF_WVEC(16); break;
+      case 32: EMF_WVEC(32); break;
+      case 64: EMF_WVEC(64); break;
+      default: EMF_REQUIRE(false, kErrArg, "vector wide SpMM slab width out of range");
     }
+#undef EMF_WVEC
+    EMF_HIP(hipGetLastError());
+    return;
+  }
+  if (nc >= 16) {
+    // column slabs of W = LW * CPL columns (A.lpr = W when tuned), CPL = 4 columns per
+    // lane; 16 lanes per row up to 64 columns, 32 up to 128, 64 beyond
+    const int wdef = nc <= 64 ? 64 : (nc <= 128 ? 128 : 256);
+    const int W = code >= 16 ? (code & ~1) : wdef;
+    const int rb = grid_for((int64_t)A.nrows * (W / 4));
+    const int nslab = (nc + W - 1) / W;
+#define EMF_WIDE(LW)                                                                                            \
+  hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, LW, 4>), dim3(rb * nslab), dim3(kBlock), 0, s, A.nrows, rb, A.rp, \
+                     A.ci, A.v, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs)
+    switch (W) {
+      case 16: EMF_WIDE(4); break;
+      case 32: EMF_WIDE(8); break;
+      case 64: EMF_WIDE(16); break;
+      case 128: EMF_WIDE(32); break;
+      case 256: EMF_WIDE(64); break;
+      default: EMF_REQUIRE(false, kErrArg, "wide SpMM slab width must be 16, 32, 64, 128 or 256");
+    }
+#undef EMF_WIDE
     EMF_HIP(hipGetLastError());
     return;
   }
@@ -370,14 +603,32 @@ void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T
 
 template <typename TA, typename TX, typename T>
 int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
-  if (nc >= 16 || A.nrows == 0) return 0;
+  if (A.nrows == 0) return 0;
   hipEvent_t e0, e1;
   EMF_HIP(hipEventCreate(&e0));
   EMF_HIP(hipEventCreate(&e1));
   int best = 0;
   float best_ms = 1e30f;
   std::vector<int> cand{2, 4, 8, 16};
-  if (A.blk_row && A.nblk > 0) cand.push_back(-1);  // LDS-staged row blocks
+  if (nc >= 16) {
+    // wide: column slab widths; slabs narrower than nc trade matrix re-reads for L2
+    // reuse of x
+    cand.clear();
+    for (int W = 16; W <= 256; W *= 2) {
+      cand.push_back(W);
+      if (W >= nc) break;
+    }
+    // 16-byte gathers (slab W, code W + 1) where nc and the row stride allow
+    const int vec = 16 / (int)sizeof(TX);
+    if (nc % vec == 0 && (uintptr_t)x % 16 == 0 && ((size_t)nc * sizeof(TX)) % 16 == 0)
+      for (int W = 16; W <= 256; W *= 2) {
+        if (W / vec >= 4 && W / vec <= 64) cand.push_back(W + 1);
+        if (W >= nc) break;
+      }
+    if (A.xb_row && A.xb_nblk > 0) cand.push_back(-2);
+  } else if (A.blk_row && A.nblk > 0) {
+    cand.push_back(-1);  // LDS-staged row blocks
+  }
   for (int lpr : cand) {
     Csr<TA> B = A;
     B.lpr = lpr;
@@ -389,6 +640,9 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
     EMF_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     EMF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    // the row-block variant must win by 5% (timing noise would otherwise flip a
+    // near tie from run to run)
+    if (lpr < 0) ms *= 1.05f;
     if (ms < best_ms) {
       best_ms = ms;
       best = lpr;

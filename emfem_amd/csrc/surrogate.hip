@@ -418,15 +418,18 @@ __global__ void k_features(int nt, const int *__restrict__ tets, const int *__re
   double cx[3] = {0, 0, 0};
   for (int k = 0; k < 4; ++k)
     for (int d = 0; d < 3; ++d) cx[d] += 0.25 * x[k][d];
-  double dmin = 1e300;
+  const double h = cbrt(fmax(vol, 1e-30));
+  double dmin = 1e300, dens = 0.0;
   for (int r = 0; r < nrx; ++r) {
     const double dx = rx[r * 3] - cx[0], dy = rx[r * 3 + 1] - cx[1], dz = rx[r * 3 + 2] - cx[2];
-    dmin = fmin(dmin, dx * dx + dy * dy + dz * dz);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    dmin = fmin(dmin, d2);
+    dens += h / (sqrt(d2) + h);  // superposed point sources, relative to the tet size
   }
-  const double h = cbrt(fmax(vol, 1e-30));
   f[11] = nrx > 0 ? (float)log10(sqrt(dmin) / h + 1.0) : 0.f;
   f[12] = nrx > 0 ? (float)log10(sqrt(dmin) + 1.0) : 0.f;
-  for (int k = 13; k < kSurFeat; ++k) f[k] = 0.f;
+  f[13] = nrx > 0 ? (float)log10(dens) : 0.f;
+  for (int k = 14; k < kSurFeat; ++k) f[k] = 0.f;
 }
 
 __global__ void k_standardize(int n, const float *__restrict__ mean, const float *__restrict__ stdv, float *F) {

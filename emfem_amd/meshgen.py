@@ -244,7 +244,7 @@ def _subdivide(nodes, s):
 
 
 def halfspace_mt(n=8, rho=100.0, freq=1.0, extent=20000.0, depth=20000.0, air=50000.0,
-                 core=4000.0, n_rx=3):
+                 core=4000.0, n_rx=3, rx_span=None):
     """Homogeneous half-space under air: the closed-form MT test case."""
     xn = graded(core / 2, core / n, extent / 2, max(2, n // 2))
     earth = np.concatenate([np.linspace(0, core, n + 1), _geometric(core, depth, core / n, max(2, n // 2))])
@@ -254,7 +254,8 @@ def halfspace_mt(n=8, rho=100.0, freq=1.0, extent=20000.0, depth=20000.0, air=50
     pts = _round(pts)
     c = centroids(pts, tets)
     attr = np.where(c[:, 2] < 0, 0, 1).astype(np.int32)
-    rx = surface_grid(-core / 4, core / 4, n_rx)
+    span = core / 4 if rx_span is None else rx_span
+    rx = surface_grid(-span, span, n_rx)
     return Model(pts, tets, attr, 0, np.array([[AIR_RHO], [rho]]), np.array([freq]),
                  np.zeros((0, 7)), rx, mt_obs(1, rx.shape[0]), name="halfspace_mt")
 
@@ -401,11 +402,11 @@ def marine_csem(cells=(48, 48, 40), n_tx=64, freqs=(0.25, 0.75, 1.25), with_air=
                  np.asarray(obs, dtype=np.int32), name="marine_csem")
 
 
-def uneven_mt_batch(n_models=256, seed=4, e_min=2.0e4, e_max=2.0e5):
+def uneven_mt_batch(n_models=256, seed=4, e_min=2.0e4, e_max=2.0e5, f_min=1e-2, f_max=10.0):
     """Uneven batch of independent MT models: edge counts log-uniform in
     [e_min, e_max], a half-space of 10-1000 ohm-m (log-uniform) with 1-4 random blocks
-    (1-1000 ohm-m), one random frequency per model, log-uniform in 1e-2-10 Hz, and a
-    3x3 receiver grid.  The grid size n per model is the one whose edge count (about
+    (1-1000 ohm-m), one random frequency per model, log-uniform in [f_min, f_max], and a
+    square surface receiver grid of 2-7 per side spanning +-0.5..4 km.  The grid size n per model is the one whose edge count (about
     7 (n+1)^2 (n+1+n_pad) on the padded Kuhn grid) is nearest the drawn target."""
     rng = np.random.default_rng(seed)
     models = []
@@ -420,8 +421,9 @@ def uneven_mt_batch(n_models=256, seed=4, e_min=2.0e4, e_max=2.0e5):
                 best = (n, e)
         n = best[0]
         rho_hs = float(np.exp(rng.uniform(np.log(10.0), np.log(1000.0))))
-        freq = float(np.exp(rng.uniform(np.log(1e-2), np.log(10.0))))
-        m = halfspace_mt(n=n, rho=rho_hs, freq=freq, n_rx=3)
+        freq = float(np.exp(rng.uniform(np.log(f_min), np.log(f_max))))
+        m = halfspace_mt(n=n, rho=rho_hs, freq=freq, n_rx=int(rng.integers(2, 8)),
+                         rx_span=float(rng.uniform(500.0, 4000.0)))
         c = centroids(m.points, m.tets)
         rho = [[AIR_RHO], [rho_hs]]
         for b in range(int(rng.integers(1, 5))):

@@ -1,9 +1,12 @@
 """Host-side helpers for the learned indicator surrogate: training data from plain
 goal-oriented adaptive cycles, training, and the rank correlation used to judge it.
 
-A training sample is one adaptive cycle of one model: the raw per-tet features of the
-primal XY / YX solutions (emfem_surrogate_features), the face-neighbour graph, and the
-plain indicator exy + eyx with exy = sqrt(eta2_xy * eta2_dual) (src/em_fwd.cc:1172-1177).
+A training sample is one adaptive cycle of one model: the raw per-tet features
+(emfem_surrogate_features), the face-neighbour graph, and the dual solution's squared
+indicator eta2_dual, the factor of the goal-oriented indicator
+exy = sqrt(eta2_xy * eta2_dual) (src/em_fwd.cc:1172-1177) that needs the dual solve.
+The primal factors are computed exactly in every cycle, so the network only stands in
+for the dual.
 The cycle follows the reference's MT driver (src/em_fwd.cc:1320-1380): assemble, solve
 XY / YX / dual, estimate, mark, refine.
 """
@@ -14,7 +17,7 @@ from . import api
 
 def cycle_sample(ctx, fidx=0):
     """Solve XY, YX and the dual on the context's current mesh and return
-    (features, neighbours, indicator)."""
+    (features, neighbours, eta2_dual, indicator exy + eyx)."""
     ctx.create_linear_system(3)
     ctx.assemble_matrix(fidx)
     ctx.assemble_rhs_mt(fidx, api.XY_POLAR, 0)
@@ -29,7 +32,7 @@ def cycle_sample(ctx, fidx=0):
     feats = ctx.surrogate_features(fidx, 0, 1)
     nb = ctx.topology()[2]
     ctx.destroy_linear_system()
-    return feats, nb, ind
+    return feats, nb, e[2], ind
 
 
 def training_data(models, cycles=3, fraction=0.2, options="-inner_precision fp32"):
@@ -41,10 +44,10 @@ def training_data(models, cycles=3, fraction=0.2, options="-inner_precision fp32
         c = api.Context(options)
         c.load_model(m)
         for k in range(cycles):
-            f, nb, ind = cycle_sample(c)
+            f, nb, dual, ind = cycle_sample(c)
             F.append(f)
             NB.append(np.where(nb >= 0, nb + off, -1))
-            Y.append(ind)
+            Y.append(dual)
             off += f.shape[0]
             if k + 1 < cycles:
                 c.refine_mesh(api.refine_fixed_number(ind, fraction))

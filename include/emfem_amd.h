@@ -273,13 +273,16 @@ int emfem_get_mesh(const emfem_context *ctx, double *points, int *tets, int *att
 /* ---- learned indicator surrogate (no counterpart in the reference) ----------
  * A per-element message-passing network over the face-neighbour graph (input layer
  * of 16 padded features, 3 message-passing layers of width 64, bf16 matrix-core
- * GEMMs with fp32 accumulation) that predicts the goal-oriented indicator of
- * estimate_error (src/em_fwd.cc:1102-1180) from the primal XY / YX solutions alone.
- * emfem_forward uses it, when -surrogate_cycles is true, in the intermediate adaptive
- * MT cycles in place of the dual solve and the estimator; the last cycle always runs
- * the plain estimator, compares its indicator with the surrogate's by Spearman rank
- * correlation (stat "surrogate_rank_corr") and rejects the surrogate for the rest of
- * the context's life below -surrogate_min_corr (0.9; stat "surrogate_rejected"). */
+ * GEMMs with fp32 accumulation) that predicts, from per-tet features of the mesh,
+ * the conductivity, the frequency, the receivers and the primal XY / YX solutions, the
+ * dual solution's squared indicator eta2_dual of estimate_error
+ * (src/em_fwd.cc:1102-1180).  emfem_forward uses it, when -surrogate_cycles is true, in
+ * the intermediate adaptive MT cycles in place of the dual solve: the indicators
+ * sqrt(eta2_xy * eta2_dual) (src/em_fwd.cc:1172-1177) take the exact primal factors
+ * and the predicted dual.  The last cycle always solves the dual, compares the
+ * indicator with the predicted and with the solved dual by Spearman rank correlation
+ * (stat "surrogate_rank_corr") and rejects the surrogate for the rest of the
+ * context's life below -surrogate_min_corr (0.9; stat "surrogate_rejected"). */
 
 /* New network with Glorot-uniform weights from seed. */
 int emfem_surrogate_init(emfem_context *ctx, unsigned seed);
@@ -293,12 +296,12 @@ int emfem_surrogate_params(emfem_context *ctx, float *get, const float *set, int
 int emfem_surrogate_features(emfem_context *ctx, int fidx, int slot_xy, int slot_yx, float *feats);
 
 /* Full-batch Adam training on n elements: raw features [n][16], face neighbours
- * [n][4] (-1 for none; symmetric), indicator [n] (> 0; fitted in log10 after
+ * [n][4] (-1 for none; symmetric), target eta2_dual [n] (> 0; fitted in log10 after
  * standardization).  loss[e] (may be NULL) receives the mean squared error of epoch e. */
 int emfem_surrogate_train(emfem_context *ctx, int n, const float *feats, const int *neighbors,
                           const double *indicator, int epochs, double lr, double *loss);
 
-/* Predicted indicator [n_tets] for the current mesh and solution slots. */
+/* Predicted eta2_dual [n_tets] for the current mesh and solution slots. */
 int emfem_surrogate_predict(emfem_context *ctx, int fidx, int slot_xy, int slot_yx, double *indicator);
 
 /* The raw network on n elements of standardized features [n][16] and neighbours
@@ -325,10 +328,11 @@ int emfem_get_stat(const emfem_context *ctx, const char *name, double *value);
 
 /* Device time (ms per launch) and algorithmic bytes of the hot kernels:
  * which = "assemble" (all colours of the last assembly), "estimate", and, after a
- * solve run with -use_graph false, "op_spmm" (complex operator), "b_spmm" (smoother
- * B SpMM in the cycle precision) and "fcg_spmm" (fp64 B SpMM of the FCG), each from
- * HIP events on the context's stream around 20 back-to-back launches on that solve's
- * operands. */
+ * solve run with -use_graph false, "b_spmm" (the smoother B SpMM in the cycle
+ * precision, averaged over HIP events around each of its first 512 launches in that
+ * solve's cycles), "op_spmm" (complex operator) and "fcg_spmm" (fp64 B SpMM of the
+ * FCG), the last two from HIP events around 20 back-to-back launches on that solve's
+ * operands; all on the context's stream. */
 int emfem_kernel_time(const emfem_context *ctx, const char *which, double *ms, double *bytes);
 
 #ifdef __cplusplus

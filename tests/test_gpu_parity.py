@@ -290,3 +290,33 @@ def test_row_block_spmm_matches_lane_spmm(precision, oracle_mod):
         for s in range(2):
             e = oracle_mod.solve_direct(rp, ci, C, M, *ctx.get_rhs(s))
             assert np.linalg.norm(sols[rows][s] - e) <= 1e4 * 1e-10 * np.linalg.norm(e)
+
+
+@pytest.mark.parametrize("rows", ["auto", "lanes", "xblock"])
+@pytest.mark.parametrize("n_tx", [10, 40])
+def test_many_rhs_csem_block_solve(n_tx, rows, oracle_mod):
+    """CSEM right-hand sides as one block (nc = 2 n_tx >= 16 real columns: the
+    many-column SpMM kernels, tuned, lanes on columns only, or LDS x-blocks) reach
+    the exact discrete solutions."""
+    m = mg.tiny_mt(n=3, aniso_form=0, seed=1, freq=1.25)
+    rng = np.random.default_rng(n_tx)
+    lo, hi = m.points.min(0), m.points.max(0)
+    tx = np.zeros((n_tx, 7))
+    tx[:, 0] = rng.uniform(-2000.0, 2000.0, n_tx)
+    tx[:, 1] = rng.uniform(-2000.0, 2000.0, n_tx)
+    tx[:, 2] = rng.uniform(100.0, 1500.0, n_tx)
+    tx[:, 3] = rng.uniform(0.0, 360.0, n_tx)
+    tx[:, 5] = 1.0
+    m.tx = tx
+    ctx = _ctx(m, "-inner_precision fp32 -spmm_rows %s" % rows)
+    ctx.create_linear_system(n_tx)
+    ctx.assemble_matrix(0)
+    for t in range(n_tx):
+        ctx.assemble_rhs_csem(0, t, t)
+    ctx.create_pc()
+    its, rr = ctx.solve_linear_system(0, n_tx, 300, 1e-10)
+    assert np.all(rr <= 1e-10 * 1.0001), rr
+    rp, ci, C, M = ctx.get_matrix()
+    for t in range(0, n_tx, 7):
+        e = oracle_mod.solve_direct(rp, ci, C, M, *ctx.get_rhs(t))
+        assert np.linalg.norm(ctx.get_solution(t) - e) <= 1e4 * 1e-10 * np.linalg.norm(e)
