@@ -41,11 +41,11 @@ def run_step(ctx, m, fidx, batched=True):
     rtols = np.array([1e-8, 1e-8, 1e-6])
     its = []
     if batched:
-        # XY and YX as one block in the mixed path; the dual in the fp64 cycle
-        i, r = ctx.solve_linear_system(0, 2, 100, 1e-8)
-        i2, r2 = ctx.solve_linear_system(2, 1, 100, 1e-6, precision="fp64")
-        its = i.tolist() + i2.tolist()
-        rr = r.tolist() + r2.tolist()
+        # XY, YX and the dual as one block of three in the mixed path (per-column
+        # tolerances), as the forward driver solves them
+        i, r = ctx.solve_linear_system(0, 3, 100, 1e-8, rtols=rtols)
+        its = i.tolist()
+        rr = r.tolist()
     else:
         rr = []
         for s in range(3):

@@ -63,7 +63,7 @@ const std::map<std::string, std::string> &option_defaults() {
       {"B_ksp_max_it", "100"},
       // this implementation
       {"inner_precision", "fp32"},
-      {"dual_precision", "fp64"},
+      {"dual_precision", "same"},
       {"fp64_fallback", "true"},
       {"use_graph", "true"},
       {"spmm_rows", "auto"},
@@ -84,7 +84,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"direct_solver_type", {"mumps", "superlu_dist"}},
       {"refine_strategy", {"fixed_number", "fixed_fraction"}},
       {"inner_precision", {"fp32", "fp64"}},
-      {"dual_precision", {"fp32", "fp64"}},
+      {"dual_precision", {"same", "fp32", "fp64"}},
       {"fp64_fallback", {"true", "false"}},
       {"use_graph", {"true", "false"}},
       {"spmm_rows", {"auto", "lanes", "rowblock", "xblock", "2", "4", "8", "16"}},

@@ -220,10 +220,10 @@ void solve_slots(emfem_context *c, int first, int n, const std::vector<double> &
 
 // The dual problem (point sources at receivers on the air-earth interface) drives
 // gradient fields ~ 1/(omega sigma_air) in the air; it is solved with the cycle
-// precision of -dual_precision (fp64 by default).
+// precision of -dual_precision ("same": that of the primal solves).
 void solve_dual(emfem_context *c, int slot) {
   const std::string keep = c->opts["inner_precision"];
-  c->opts["inner_precision"] = c->opt("dual_precision");
+  if (c->opt("dual_precision") != "same") c->opts["inner_precision"] = c->opt("dual_precision");
   try {
     solve_slots(c, slot, 1, {c->optd("dual_rtol")});
   } catch (...) {
@@ -231,6 +231,22 @@ void solve_dual(emfem_context *c, int slot) {
     throw;
   }
   c->opts["inner_precision"] = keep;
+}
+
+// Primal slots [first, first + n) at -e_rtol and the dual at slot first + n at
+// -dual_rtol.  With the dual in the primal's cycle precision they are one block of
+// n + 1 right-hand sides (one set of cycles for all) when the block keeps the
+// many-column SpMMs on 16-byte rows (n + 1 <= 7 or n + 1 even); otherwise two solves.
+void solve_primal_dual(emfem_context *c, int first, int n) {
+  const std::string dp = c->opt("dual_precision");
+  std::vector<double> tol(n, c->optd("e_rtol"));
+  if ((dp == "same" || dp == c->opt("inner_precision")) && (n + 1 <= 7 || (n + 1) % 2 == 0)) {
+    tol.push_back(c->optd("dual_rtol"));
+    solve_slots(c, first, n + 1, tol);
+    return;
+  }
+  solve_slots(c, first, n, tol);
+  solve_dual(c, first + n);
 }
 
 void solve_slots(emfem_context *c, int first, int n, const std::vector<double> &rtols) {
@@ -286,8 +302,7 @@ void forward_mt(emfem_context *c, int fidx) {
       continue;
     }
     check(c, emfem_assemble_rhs_dual(c, 2));
-    solve_slots(c, 0, 2, {e_rtol, e_rtol});
-    solve_dual(c, 2);
+    solve_primal_dual(c, 0, 2);
     check(c, emfem_destroy_pc(c));
     const int nt = c->T.nt;
     const int slots[3] = {0, 1, 2};
@@ -355,8 +370,7 @@ void forward_csem(emfem_context *c, int fidx, const std::vector<int> &tidxs) {
       check(c, emfem_create_pc(c));
       for (int k = 0; k < ng; ++k) check(c, emfem_assemble_rhs_csem(c, fidx, grp[k], k));
       check(c, emfem_assemble_rhs_dual(c, ng));
-      solve_slots(c, 0, ng, std::vector<double>(ng, c->optd("e_rtol")));
-      solve_dual(c, ng);
+      solve_primal_dual(c, 0, ng);
       check(c, emfem_destroy_pc(c));
       const bool last = cycle == max_cycles || c->T.ne >= c->opti("max_dofs");
       if (saving_mesh(c)) {
