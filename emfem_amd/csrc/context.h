@@ -207,6 +207,7 @@ inline PcOptions pc_options(const emfem_context *c) {
   o.inner_rtol = c->optd("B_ksp_rtol");
   o.inner_max_it = c->opti("B_ksp_max_it");
   o.amg.theta = c->optd("ams_theta");
+  o.amg.smooth_levels = c->opti("ams_smooth_levels");
   o.amg.additive = c->opt("ams_cycle") == "additive";
   o.use_graph = c->opt("use_graph") == "true";
   // spmm_rows: 0 tuned, 1 lanes per row tuned, 2 row blocks, 3 x-blocks (16 or more

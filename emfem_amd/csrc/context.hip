@@ -67,9 +67,10 @@ const std::map<std::string, std::string> &option_defaults() {
       {"fp64_fallback", "true"},
       {"use_graph", "true"},
       {"spmm_rows", "auto"},
-      {"ams_cheb_degree", "3"},
-      {"ams_cheb_ratio", "30"},
+      {"ams_cheb_degree", "5"},
+      {"ams_cheb_ratio", "60"},
       {"ams_theta", "0.08"},
+      {"ams_smooth_levels", "0"},
       {"ams_cycle", "additive"},
       {"surrogate_cycles", "false"},
       {"surrogate_min_corr", "0.9"},

@@ -36,8 +36,8 @@ double time_operator(hipStream_t s, int ne, int nnz, const int *rp, const int *c
                      const double *x, double *y, int reps);
 
 struct PcOptions {
-  int cheb_degree = 3;
-  double cheb_ratio = 30.0;
+  int cheb_degree = 5;
+  double cheb_ratio = 60.0;
   double inner_rtol = 1e-2;
   int inner_max_it = 100;
   AmgParams amg;

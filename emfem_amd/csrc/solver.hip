@@ -259,16 +259,18 @@ class DevAms {
       B.xb_lci = B_.xb_lci;
       B.xb_nblk = B_.xb_nblk;
       B.xb_umax = B_.xb_umax;
-      // the fp64 B (fp64 residuals here, the FCG operator of the inner solve)
-      DevBuf<double> tmp((size_t)ne_ * nc);
-      r64_.zero(s);
-      const bool xb = B_.xb_nblk > 0;
-      if (spmm_rows_ == 3 && xb) {
+      if (spmm_rows_ == 3 && B_.xb_nblk > 0) {
         // forced x-blocks
         lpr_[nc] = -2;
         lpr64_[nc] = -2;
         return;
       }
+    }
+    {
+      // the fp64 B (fp64 residuals of the cycle, the FCG operator of the inner solve):
+      // its own lanes per row (slab width at 16 or more columns)
+      DevBuf<double> tmp((size_t)ne_ * nc);
+      r64_.zero(s);
       lpr64_[nc] = tune_lpr<double, double, double>(s, B64_, nc, r64_.get(), tmp.get());
       EMF_HIP(hipStreamSynchronize(s));
     }

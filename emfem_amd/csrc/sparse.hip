@@ -34,7 +34,8 @@ __device__ __forceinline__ T shfl_xor(T v, int mask, int width) {
 // Sums the NC per-lane partials over the LPR lanes of a row.  For power-of-two NC
 // the sum is reduce-scattered (each exchange halves the values a lane keeps, so
 // NC/2 + NC/4 + ... shuffles instead of NC per step); the lane then owns columns
-// [col0, col0 + KEND).  Other NC use a full butterfly and every lane owns all.
+// [col0, col0 + KEND).  NC = 1 uses a full butterfly.  (The SpMM pads 3 and 6
+// columns to 4 and 8 zero-initialised partials to take the reduce-scatter.)
 template <typename T, int NC, int LPR>
 __device__ __forceinline__ int row_reduce(T (&acc)[NC], int lane) {
   constexpr bool kScatter = (NC & (NC - 1)) == 0 && NC > 1;
@@ -83,9 +84,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
   const int row = gid / LPR;
   const int lane = gid % LPR;
   if (row >= nrows) return;
-  T acc[NC];
+  // partials padded to a power of two (the padding stays zero)
+  constexpr int NCP = NC <= 1 ? 1 : (NC <= 2 ? 2 : (NC <= 4 ? 4 : (NC <= 8 ? 8 : NC)));
+  T acc[NCP];
 #pragma unroll
-  for (int j = 0; j < NC; ++j) acc[j] = T(0);
+  for (int j = 0; j < NCP; ++j) acc[j] = T(0);
   const int e = rp[row + 1];
   // kU nonzeros per lane per trip: all column/value loads, then all row gathers,
   // are in flight together (a typical 16-nonzero edge row is one trip at LPR 4)
@@ -111,18 +114,19 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
       for (int j = 0; j < NC; ++j) acc[j] += a[u] * (T)xr[j];
     }
   }
-  const int col0 = row_reduce<T, NC, LPR>(acc, lane);
+  const int col0 = row_reduce<T, NCP, LPR>(acc, lane);
   const T scale = rs ? rs[row] : T(1);
   const T zscale = zs ? beta * zs[row] : beta;
-  constexpr bool kScatter = (NC & (NC - 1)) == 0 && NC > 1;
+  constexpr bool kScatter = (NCP & (NCP - 1)) == 0 && NCP > 1;
   if constexpr (kScatter) {
     // lanes that own a column after the reduce-scatter
-    constexpr int kOwn = NC >= LPR ? NC / LPR : 1;
-    constexpr int kRep = NC >= LPR ? 1 : LPR / NC;
+    constexpr int kOwn = NCP >= LPR ? NCP / LPR : 1;
+    constexpr int kRep = NCP >= LPR ? 1 : LPR / NCP;
     if ((lane & (kRep - 1)) != 0) return;
 #pragma unroll
     for (int i = 0; i < kOwn; ++i) {
       const int j = col0 + i;
+      if (NCP != NC && j >= NC) break;  // padding columns
       T v = alpha * acc[i];
       if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
       v *= scale;

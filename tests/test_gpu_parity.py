@@ -182,8 +182,10 @@ def test_estimator_matches_oracle(oracle_mod):
     assert _rel(ixy, oxy) < 1e-12
 
 
-def test_pc_apply_matches_fcg_tolerance(tiny, oracle_mod):
-    """pc_apply_b gives x with ||b - B x|| <= rtol ||b|| per column (src/em_la.cc:208-228)."""
+@pytest.mark.parametrize("n_rhs", [1, 2, 3])
+def test_pc_apply_matches_fcg_tolerance(tiny, n_rhs, oracle_mod):
+    """pc_apply_b gives x with ||b - B x|| <= rtol ||b|| per column (src/em_la.cc:208-228),
+    on 2, 4 and 6 real columns (6: the padded reduce-scatter of the SpMMs)."""
     m, om = tiny
     for prec, tol in (("fp64", 1e-2), ("fp32", 1e-2), ("fp64", 1e-6)):
         ctx = _ctx(m, "-inner_precision %s -B_ksp_rtol %g" % (prec, tol))
@@ -193,15 +195,15 @@ def test_pc_apply_matches_fcg_tolerance(tiny, oracle_mod):
         rp, ci, C, M = ctx.get_matrix()
         B = oracle_mod.csr(rp, ci, C + M, rp.size - 1)
         rng = np.random.default_rng(9)
-        b = rng.standard_normal((rp.size - 1, 4))
+        b = rng.standard_normal((rp.size - 1, 2 * n_rhs))
         bd = ctx.device_array(b)
         xd = ctx.device_array(np.zeros_like(b))
-        ctx.pc_apply_b(2, bd.ptr, xd.ptr)
+        ctx.pc_apply_b(n_rhs, bd.ptr, xd.ptr)
         x = xd.numpy()
         # the per-column stopping test ends the FCG once the tolerance is met: far
         # fewer than B_ksp_max_it iterations, and not far below the tolerance
         assert ctx.stat("inner_its") < (30 if tol >= 1e-2 else 100), ctx.stat("inner_its")
-        for c in range(4):
+        for c in range(2 * n_rhs):
             r = b[:, c] - B @ x[:, c]
             assert np.linalg.norm(r) <= tol * np.linalg.norm(b[:, c]) * 1.01
             assert np.linalg.norm(r) >= 1e-3 * tol * np.linalg.norm(b[:, c])
