@@ -335,13 +335,15 @@ def main():
     bw = (b_bytes / (b_ms * 1e-3) / 1e9) if b_ms > 0 else 0.0
     rows = int(ctx.stat("b_spmm_rows"))
     kname = ("%s: Chebyshev sweep rres -= Dinv .* (B d), B = C + M fp32, XY+YX+dual = 6 real columns"
-             % ("k_spmm_stream (LDS-staged row blocks)" if rows < 0 else
+             % ("k_spmm_stream (LDS-staged row blocks)" if rows == -1 else
+                "k_spmm_xnarrow (LDS x-blocks, %d lanes per row)" % -rows if rows < -1 else
                 "k_spmm_narrow role 1 (%d lanes per row)" % rows))
     # HBM traffic of the same kernel and workload from the committed PMC summary
     # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on k_make_b), when it exists
     traffic = None
     prof = os.path.join(ROOT, "profiles", "%s_smoother_adaptive_mt_%d.md"
-                        % ("k_spmm_stream" if rows < 0 else "k_spmm_narrow", args.cells))
+                        % ("k_spmm_stream" if rows == -1 else "k_spmm_xnarrow" if rows < -1 else "k_spmm_narrow",
+                           args.cells))
     if os.path.exists(prof):
         for line in open(prof):
             if line.startswith("traffic_bytes_per_launch:"):

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include <complex>
 #include <map>
 #include <memory>
@@ -167,6 +169,27 @@ inline void check_attributes(emfem_context *c) {
 // one contiguous transfer to a device staging buffer, then a strided on-device copy
 // (a host<->device 2-D copy with 16-byte rows is row-by-row and very slow).
 inline void put_slot(emfem_context *c, DevBuf<double> &blk, int slot, const double *re, const double *im) {
+  // mostly-zero right-hand sides (the MT lifting, point sources) go up as their
+  // nonzero rows only
+  std::vector<int> idx;
+  std::vector<double2> v;
+  const int ne = c->T.ne;
+  for (int e = 0; e < ne && (int64_t)idx.size() * 4 < ne; ++e) {
+    const double r = re ? re[e] : 0.0, i = im ? im[e] : 0.0;
+    if (r != 0.0 || i != 0.0 || std::signbit(r) || std::signbit(i)) {  // -0.0 kept as written
+      idx.push_back(e);
+      v.push_back(make_double2(r, i));
+    }
+  }
+  if ((int64_t)idx.size() * 4 < ne) {
+    DevBuf<int> didx;
+    DevBuf<double2> dv;
+    didx.upload(idx, c->stream);
+    dv.upload(v, c->stream);
+    put_rows(c->stream, ne, (int)idx.size(), didx.get(), dv.get(), blk.get() + 2 * slot, 2 * c->nslots);
+    EMF_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
   std::vector<double> tmp((size_t)c->T.ne * 2);
   for (int e = 0; e < c->T.ne; ++e) {
     tmp[(size_t)e * 2] = re ? re[e] : 0.0;

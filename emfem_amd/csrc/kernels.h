@@ -33,6 +33,10 @@ struct Csr {
   const int *xb_row = nullptr, *xb_uptr = nullptr, *xb_ucol = nullptr;
   const int *xb_lci = nullptr;
   int xb_nblk = 0, xb_umax = 0;
+  // lpr = -2, -4, -8 (fewer than 16 columns): narrow x-blocks with -lpr lanes per row;
+  // tables as above with at most kXnCols distinct columns per block
+  const int *xn_row = nullptr, *xn_uptr = nullptr, *xn_ucol = nullptr, *xn_lci = nullptr;
+  int xn_nblk = 0;
 };
 
 // Host tables of the LDS x-block SpMM for a pattern (see Csr::xb_*): consecutive rows
@@ -47,6 +51,8 @@ struct XBlocks {
 XBlocks x_blocks(const std::vector<int> &rp, const std::vector<int> &ci, int ncols, int umax, int rmax);
 constexpr int kXBlockCols = 64;  // distinct columns per x-block
 constexpr int kXBlockRows = 64;
+constexpr int kXnCols = 256;  // distinct columns per narrow x-block
+constexpr int kXnRows = 64;
 
 // Nonzeros staged per workgroup by the row-block SpMM: 16 KB of products.
 template <typename T, int NC>
@@ -147,6 +153,10 @@ void axpby_cols(hipStream_t s, int rows, int nc, const double *a, const TX *x, i
 // y[r][c] = alpha * x[r][c] + beta * y[r][c] with host scalars (beta = 0 ignores y).
 template <typename TX, typename TY>
 void axpby(hipStream_t s, int rows, int nc, double alpha, const TX *x, int ldx, double beta, TY *y, int ldy);
+
+// Two columns of y [rows][ldy]: zero every row, then y[idx[i]][0..1] = (v[i].x, v[i].y)
+// for i < n (distinct idx).
+void put_rows(hipStream_t s, int rows, int n, const int *idx, const double2 *v, double *y, int ldy);
 
 // Flexible CG updates on nc columns with device scalars:
 //   x += alpha[c] p ; r -= alpha[c] q ; rr[c] = r . r (fused reduction)

@@ -239,14 +239,30 @@ void rhs_mt(const Topology &T, const std::vector<double> &s6, const Background &
   PlaneWave pw[4];
   for (int k = 0; k < 4; ++k) pw[k].init(bg.col[k], freq);
   std::vector<cplx> bval(T.ne, cplx(0));
-  std::vector<uint8_t> have(T.ne, 0);
   // Dirichlet values: the first bdr cell holding an edge defines it (src/em_fwd.cc:424-482).
-  for (int t : T.bdr_cells) {
-    TetLocal L;
-    load_tet(T, t, L);
-    for (int i = 0; i < 6; ++i) {
+  // The owner (cell, local edge) of every boundary edge is fixed by one sequential
+  // pass; the values are then computed in parallel.
+  const int nb = (int)T.bdr_cells.size();
+  std::vector<int> own_cell, own_local;
+  {
+    std::vector<uint8_t> have(T.ne, 0);
+    for (int q = 0; q < nb; ++q) {
+      const int t = T.bdr_cells[q];
+      for (int i = 0; i < 6; ++i) {
+        const int g = T.tet2edge[(size_t)t * 6 + i];
+        if (!T.edge_bdr[g] || have[g]) continue;
+        have[g] = 1;
+        own_cell.push_back(t);
+        own_local.push_back(i);
+      }
+    }
+  }
+  parallel_ranges((int64_t)own_cell.size(), [&](int64_t qb, int64_t qe, int) {
+    for (int64_t q = qb; q < qe; ++q) {
+      TetLocal L;
+      load_tet(T, own_cell[q], L);
+      const int i = own_local[q];
       const int g = L.edge[i];
-      if (!T.edge_bdr[g] || have[g]) continue;
       int a, b;
       edge_ends(L.vid, i, a, b);
       double ec[3], ed[3];
@@ -267,33 +283,43 @@ void rhs_mt(const Topology &T, const std::vector<double> &s6, const Background &
       whitney_value(L.G, lam, a, b, N);
       cplx v = (ed[0] * ep[0] + ed[1] * ep[1] + ed[2] * ep[2]) / dot3(ed, N);
       bval[g] = std::abs(v) > 1E-13 ? v : cplx(0.0);
-      have[g] = 1;
     }
-  }
-  // Lift the Dirichlet data with the raw element matrices (src/em_fwd.cc:516-572).
+  });
+  // Lift the Dirichlet data with the raw element matrices (src/em_fwd.cc:516-572):
+  // per-cell contributions in parallel, summed in cell order.
   s_re.assign(T.ne, 0.0);
   s_im.assign(T.ne, 0.0);
   const double omega = 2 * kPi * freq;
-  for (int t : T.bdr_cells) {
-    TetLocal L;
-    load_tet(T, t, L);
-    double C[6][6], M[6][6];
-    element_matrices(L.G, L.vid, &s6[(size_t)T.attr[t] * 6], omega, C, M);
-    double br[6] = {0, 0, 0, 0, 0, 0}, bi[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < 6; ++i) {
-      if (!T.edge_bdr[L.edge[i]]) continue;
-      const double vr = bval[L.edge[i]].real(), vi = bval[L.edge[i]].imag();
-      br[i] = vr * C[i][i];
-      bi[i] = -vi * C[i][i];
-      for (int j = 0; j < 6; ++j) {
-        if (T.edge_bdr[L.edge[j]]) continue;
-        br[j] += -C[j][i] * vr + M[j][i] * vi;
-        bi[j] += M[j][i] * vr + C[j][i] * vi;
+  std::vector<double> contrib((size_t)nb * 12);
+  parallel_ranges(nb, [&](int64_t qb, int64_t qe, int) {
+    for (int64_t q = qb; q < qe; ++q) {
+      const int t = T.bdr_cells[q];
+      TetLocal L;
+      load_tet(T, t, L);
+      double C[6][6], M[6][6];
+      element_matrices(L.G, L.vid, &s6[(size_t)T.attr[t] * 6], omega, C, M);
+      double *br = &contrib[(size_t)q * 12], *bi = br + 6;
+      for (int i = 0; i < 6; ++i) br[i] = bi[i] = 0.0;
+      for (int i = 0; i < 6; ++i) {
+        if (!T.edge_bdr[L.edge[i]]) continue;
+        const double vr = bval[L.edge[i]].real(), vi = bval[L.edge[i]].imag();
+        br[i] = vr * C[i][i];
+        bi[i] = -vi * C[i][i];
+        for (int j = 0; j < 6; ++j) {
+          if (T.edge_bdr[L.edge[j]]) continue;
+          br[j] += -C[j][i] * vr + M[j][i] * vi;
+          bi[j] += M[j][i] * vr + C[j][i] * vi;
+        }
       }
     }
+  });
+  for (int q = 0; q < nb; ++q) {
+    const int t = T.bdr_cells[q];
+    const double *br = &contrib[(size_t)q * 12], *bi = br + 6;
     for (int i = 0; i < 6; ++i) {
-      s_re[L.edge[i]] += br[i];
-      s_im[L.edge[i]] += bi[i];
+      const int g = T.tet2edge[(size_t)t * 6 + i];
+      s_re[g] += br[i];
+      s_im[g] += bi[i];
     }
   }
   if (bval_out) bval_out->swap(bval);

@@ -243,6 +243,7 @@ class DevAms {
     ensure(nc);
     if (lpr_.count(nc)) return;
     d_.zero(s);
+    if (nc < 16 && (spmm_rows_ == 0 || spmm_rows_ == 3)) x_narrow_table(s);
     Csr<T> B = B_;
     const std::vector<int> blk =
         spmm_rows_ == 1 || spmm_rows_ >= 10 ? std::vector<int>() : row_blocks(hrp_, stream_tile_for(nc, sizeof(T) == 8));
@@ -276,7 +277,39 @@ class DevAms {
     }
     if (spmm_rows_ >= 10 && nc < 16) lpr_[nc] = spmm_rows_ - 10;  // fixed lanes per row
     else if (spmm_rows_ == 2 && !blk.empty()) lpr_[nc] = -1;
+    else if (spmm_rows_ == 3 && nc < 16 && B_.xn_nblk > 0) lpr_[nc] = -4;  // forced x-blocks
     else lpr_[nc] = tune_lpr<T, T, T>(s, B, nc, d_.get(), rres_.get());
+  }
+
+  const std::vector<int> &host_ci() {
+    if (hci_.empty() && B_.nnz > 0) {
+      hci_.resize(B_.nnz);
+      EMF_HIP(hipMemcpy(hci_.data(), B_.ci, sizeof(int) * hci_.size(), hipMemcpyDeviceToHost));
+    }
+    return hci_;
+  }
+
+  // narrow x-block tables of the pattern of B (shared by B and B64), built once on
+  // first use with fewer than 16 columns
+  void x_narrow_table(hipStream_t s) {
+    if (xn_tried_) return;
+    xn_tried_ = true;
+    const XBlocks X = x_blocks(hrp_, host_ci(), ne_, kXnCols, kXnRows);
+    if (X.empty()) return;
+    xn_row_.upload(X.row, s);
+    xn_uptr_.upload(X.uptr, s);
+    xn_ucol_.upload(X.ucol, s);
+    xn_lci_.upload(X.lci, s);
+    EMF_HIP(hipStreamSynchronize(s));  // X is host memory of this scope
+    auto set = [&](auto &B) {
+      B.xn_row = xn_row_.get();
+      B.xn_uptr = xn_uptr_.get();
+      B.xn_ucol = xn_ucol_.get();
+      B.xn_lci = xn_lci_.get();
+      B.xn_nblk = (int)X.row.size() - 1;
+    };
+    set(B_);
+    set(B64_);
   }
 
   // LDS x-block tables of the pattern of B (shared by B and B64), built once on first
@@ -284,9 +317,7 @@ class DevAms {
   void x_blocks_table(hipStream_t s) {
     if (xb_tried_) return;
     xb_tried_ = true;
-    std::vector<int> hci(B_.nnz);
-    EMF_HIP(hipMemcpy(hci.data(), B_.ci, sizeof(int) * hci.size(), hipMemcpyDeviceToHost));
-    const XBlocks X = x_blocks(hrp_, hci, ne_, kXBlockCols, kXBlockRows);
+    const XBlocks X = x_blocks(hrp_, host_ci(), ne_, kXBlockCols, kXBlockRows);
     if (X.empty()) return;
     xb_row_.upload(X.row, s);
     xb_uptr_.upload(X.uptr, s);
@@ -463,7 +494,9 @@ class DevAms {
   std::map<std::tuple<int, const void *, const void *>, hipGraphExec_t> graphs_;
   std::map<int, int> lpr_, lpr64_;
   std::map<int, DevBuf<int>> blk_;
-  bool xb_tried_ = false;
+  bool xb_tried_ = false, xn_tried_ = false;
+  std::vector<int> hci_;  // host column indices of B (x-block tables)
+  DevBuf<int> xn_row_, xn_uptr_, xn_ucol_, xn_lci_;
   DevBuf<int> xb_row_, xb_uptr_, xb_ucol_;
   DevBuf<int> xb_lci_;
   static constexpr size_t kClockPairs = 512;

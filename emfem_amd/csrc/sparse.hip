@@ -147,6 +147,89 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
   }
 }
 
+// Narrow x-blocks: a workgroup takes one row block (at most kXnCols distinct
+// columns), stages the distinct x rows (NC values each) in LDS once, then sums its
+// rows with LPR lanes per row as k_spmm_narrow does, gathering from LDS.  Consecutive
+// rows of an edge matrix share most columns, so the global gathers fall to the
+// distinct rows of the block.
+template <typename TA, typename TX, typename T, int NC, int LPR>
+__global__ __launch_bounds__(kBlock) void k_spmm_xnarrow(const int *__restrict__ rp, const TA *__restrict__ av,
+                                                         const int *__restrict__ xb_row,
+                                                         const int *__restrict__ xb_uptr,
+                                                         const int *__restrict__ xb_ucol,
+                                                         const int *__restrict__ lci, const TX *__restrict__ x,
+                                                         int ldx, T *y, int ldy, T alpha, const T *__restrict__ rs,
+                                                         T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
+  __shared__ TX xl[kXnCols * NC];
+  const int b = xcd_block();
+  const int u0 = xb_uptr[b], nu = xb_uptr[b + 1] - u0;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nu * NC; i += kBlock) {
+    const int k = i / NC, j = i - k * NC;
+    xl[i] = x[(size_t)xb_ucol[u0 + k] * ldx + j];
+  }
+  __syncthreads();
+  constexpr int NCP = NC <= 1 ? 1 : (NC <= 2 ? 2 : (NC <= 4 ? 4 : (NC <= 8 ? 8 : NC)));
+  const int r0 = xb_row[b], r1 = xb_row[b + 1];
+  const int lane = threadIdx.x % LPR;
+  // the LPR lanes of a row group share the row, so a group enters and leaves the
+  // loop together (the reduction's shuffles stay within the group)
+  for (int row = r0 + threadIdx.x / LPR; row < r1; row += kBlock / LPR) {
+    T acc[NCP];
+#pragma unroll
+    for (int j = 0; j < NCP; ++j) acc[j] = T(0);
+    const int e = rp[row + 1];
+    constexpr int kU = 4;
+    for (int p0 = rp[row] + lane; p0 < e; p0 += kU * LPR) {
+      int c[kU];
+      T a[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int p = p0 + u * LPR;
+        const bool ok = p < e;
+        c[u] = ok ? lci[p] : 0;
+        a[u] = ok ? (T)av[p] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const TX *xr = xl + c[u] * NC;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[j] += a[u] * (T)xr[j];
+      }
+    }
+    const int col0 = row_reduce<T, NCP, LPR>(acc, lane);
+    const T scale = rs ? rs[row] : T(1);
+    constexpr bool kScatter = (NCP & (NCP - 1)) == 0 && NCP > 1;
+    if constexpr (kScatter) {
+      constexpr int kOwn = NCP >= LPR ? NCP / LPR : 1;
+      constexpr int kRep = NCP >= LPR ? 1 : LPR / NCP;
+      if ((lane & (kRep - 1)) == 0) {
+#pragma unroll
+        for (int i = 0; i < kOwn; ++i) {
+          const int j = col0 + i;
+          if (NCP != NC && j >= NC) break;
+          T v = alpha * acc[i];
+          if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
+          v *= scale;
+          if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
+          y[(size_t)row * ldy + j] = v;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        if (lane == (j % LPR)) {
+          T v = alpha * acc[j];
+          if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
+          v *= scale;
+          if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
+          y[(size_t)row * ldy + j] = v;
+        }
+      }
+    }
+  }
+}
+
 // Many columns (nc >= 16, batched right-hand sides): LW lanes per row, lane l owning
 // columns l, l + LW, ..., l + (CPL-1) LW (each load instruction reads LW consecutive
 // columns of a row: coalesced), 64 / LW rows per wavefront; kU nonzeros per trip with
@@ -402,6 +485,22 @@ template <typename TA, typename TX, typename T, int NC>
 void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
                    T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs) {
   const double avg = A.nrows ? double(A.nnz) / A.nrows : 0.0;
+  if (A.lpr <= -2) {
+    // narrow x-blocks with -lpr lanes per row
+    EMF_REQUIRE(A.xn_row != nullptr && A.xn_nblk > 0, kErrArg, "x-block SpMM without its block tables");
+    EMF_REQUIRE(xs == nullptr && zs == nullptr, kErrArg, "x-block SpMM has no scaled operands");
+#define EMF_XN(L)                                                                                                 \
+  hipLaunchKernelGGL((k_spmm_xnarrow<TA, TX, T, NC, L>), dim3(A.xn_nblk), dim3(kBlock), 0, s, A.rp, A.v, A.xn_row, \
+                     A.xn_uptr, A.xn_ucol, A.xn_lci, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz)
+    switch (-A.lpr) {
+      case 2: EMF_XN(2); break;
+      case 4: EMF_XN(4); break;
+      case 8: EMF_XN(8); break;
+      default: EMF_REQUIRE(false, kErrArg, "x-block SpMM takes 2, 4 or 8 lanes per row");
+    }
+#undef EMF_XN
+    return;
+  }
   if (A.lpr < 0) {
     // LDS-staged row blocks (tables built by row_blocks for this NC and T)
     EMF_REQUIRE(A.blk_row != nullptr && A.nblk > 0, kErrArg, "row-block SpMM without its block table");
@@ -511,7 +610,7 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
   if (A.nrows == 0 || nc == 0) return;
   EMF_REQUIRE(A.lpr != -1 || (nc <= 8 && nc != 5 && nc != 7), kErrArg,
               "row-block SpMM supports 1, 2, 3, 4, 6 or 8 columns");
-  EMF_REQUIRE(A.lpr >= -1 || nc >= 16, kErrArg, "x-block SpMM needs 16 or more columns");
+  EMF_REQUIRE(A.lpr >= -1 || nc >= 16 || A.xn_nblk > 0, kErrArg, "narrow x-block SpMM without its tables");
   if (nc >= 16 && A.lpr == -2) {
     EMF_REQUIRE(A.xb_row != nullptr && A.xb_nblk > 0 && A.xb_umax <= kXBlockCols, kErrArg,
                 "x-block SpMM without its block tables");
@@ -630,8 +729,10 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
         if (W >= nc) break;
       }
     if (A.xb_row && A.xb_nblk > 0) cand.push_back(-2);
-  } else if (A.blk_row && A.nblk > 0) {
-    cand.push_back(-1);  // LDS-staged row blocks
+  } else {
+    if (A.blk_row && A.nblk > 0) cand.push_back(-1);  // LDS-staged row blocks
+    if (A.xn_row && A.xn_nblk > 0)
+      for (int l : {-2, -4, -8}) cand.push_back(l);  // x-blocks, 2 / 4 / 8 lanes per row
   }
   for (int lpr : cand) {
     Csr<TA> B = A;
@@ -644,8 +745,8 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
     EMF_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     EMF_HIP(hipEventElapsedTime(&ms, e0, e1));
-    // the row-block variant must win by 5% (timing noise would otherwise flip a
-    // near tie from run to run)
+    // the block variants must win by 5% (timing noise would otherwise flip a near
+    // tie from run to run)
     if (lpr < 0) ms *= 1.05f;
     if (ms < best_ms) {
       best_ms = ms;
@@ -855,6 +956,29 @@ void make_b(hipStream_t s, int nnz, const double2 *K, T *B) {
 }
 template void make_b<float>(hipStream_t, int, const double2 *, float *);
 template void make_b<double>(hipStream_t, int, const double2 *, double *);
+
+namespace {
+__global__ void k_put_rows(int rows, int n, const int *__restrict__ idx, const double2 *__restrict__ v,
+                           double *y, int ldy, int phase) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (phase == 0) {
+    // zero the two columns of every row
+    if (i < rows) {
+      y[(size_t)i * ldy] = 0.0;
+      y[(size_t)i * ldy + 1] = 0.0;
+    }
+  } else if (i < n) {
+    y[(size_t)idx[i] * ldy] = v[i].x;
+    y[(size_t)idx[i] * ldy + 1] = v[i].y;
+  }
+}
+}  // namespace
+
+void put_rows(hipStream_t s, int rows, int n, const int *idx, const double2 *v, double *y, int ldy) {
+  hipLaunchKernelGGL(k_put_rows, dim3(grid_for(rows)), dim3(kBlock), 0, s, rows, n, idx, v, y, ldy, 0);
+  if (n > 0) hipLaunchKernelGGL(k_put_rows, dim3(grid_for(n)), dim3(kBlock), 0, s, rows, n, idx, v, y, ldy, 1);
+  EMF_HIP(hipGetLastError());
+}
 
 // ------------------------------------------------------------------------------
 // reductions
