@@ -363,7 +363,8 @@ def main():
         "outer_its": [s["its"] for s in steps],
     }
     pcb = {}
-    for k in ("download", "aux_build", "galerkin", "amg", "setup_host", "setup"):
+    for k in ("download", "aux_build", "galerkin", "amg", "amg_aggregate", "amg_prolongator", "amg_transpose",
+              "amg_rap", "amg_coarse", "setup_host", "setup"):
         try:
             pcb[k] = round(ctx.stat("pc_%s_ms" % k), 1)
         except Exception:

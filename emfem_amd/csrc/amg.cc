@@ -2,6 +2,7 @@
 #include "amg.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <numeric>
 #include <string>
@@ -179,11 +180,13 @@ std::vector<double> diagonal(const HostCsr &A) {
 
 std::vector<double> l1_inverse(const HostCsr &A) {
   std::vector<double> l(A.n, 0.0);
-  for (int r = 0; r < A.n; ++r) {
-    double s = 0.0;
-    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) s += std::abs(A.v[p]);
-    l[r] = s > 0.0 ? 1.0 / s : 0.0;
-  }
+  parallel_ranges(A.n, [&](int64_t b, int64_t e, int) {
+    for (int64_t r = b; r < e; ++r) {
+      double s = 0.0;
+      for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) s += std::abs(A.v[p]);
+      l[r] = s > 0.0 ? 1.0 / s : 0.0;
+    }
+  });
   return l;
 }
 
@@ -192,13 +195,20 @@ std::vector<double> l1_inverse(const HostCsr &A) {
 // their own.
 int aggregate(const HostCsr &A, const std::vector<double> &d, double theta, std::vector<int> &agg) {
   const int n = A.n;
+  // strength graph: flags per nonzero in parallel, then compaction in row order
+  std::vector<uint8_t> strong(A.ci.size());
+  parallel_ranges(n, [&](int64_t b, int64_t e, int) {
+    for (int64_t r = b; r < e; ++r)
+      for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
+        const int c = A.ci[p];
+        strong[p] = c != r && std::abs(A.v[p]) >= theta * std::sqrt(std::abs(d[r] * d[c]));
+      }
+  });
   std::vector<int> sp(n + 1, 0), sc;
   sc.reserve(A.ci.size());
   for (int r = 0; r < n; ++r) {
-    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
-      const int c = A.ci[p];
-      if (c != r && std::abs(A.v[p]) >= theta * std::sqrt(std::abs(d[r] * d[c]))) sc.push_back(c);
-    }
+    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p)
+      if (strong[p]) sc.push_back(A.ci[p]);
     sp[r + 1] = (int)sc.size();
   }
   agg.assign(n, -1);
@@ -350,9 +360,8 @@ std::vector<double> dense_inverse(const HostCsr &A) {
 
 }  // namespace
 
-void build_amg(const HostCsr &A0, const AmgParams &prm, AmgHost &H) {
+void build_amg(HostCsr A, const AmgParams &prm, AmgHost &H) {
   H.levels.clear();
-  HostCsr A = A0;
   auto tick = [&](const char *k, double t0) {
     if (prm.timing) (*prm.timing)[k] += now_ms() - t0;
   };
@@ -401,6 +410,9 @@ void build_amg(const HostCsr &A0, const AmgParams &prm, AmgHost &H) {
 }
 
 std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive) {
+  EMF_REQUIRE(T.nt > 0 && T.tets.size() == (size_t)T.nt * 4 && T.pts.size() == (size_t)T.nv * 3 &&
+                  T.edgelist.size() == (size_t)T.ne * 2 && T.vertex_bdr.size() == (size_t)T.nv,
+              kErrState, "auxiliary spaces: incomplete topology");
   auto X = std::make_shared<AmsAux>();
   // interior vertex numbering
   std::vector<int> vmap(T.nv, -1);
@@ -480,7 +492,85 @@ std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive) {
     }
   });
   X->PiT = csr_transpose(Pi);
+  // Galerkin pattern: interior vertices u, v of a common tet (B couples the edges of a
+  // tet, and P maps an edge to its interior endpoints)
+  {
+    HostCsr &A = X->Apat;
+    A.n = A.m = nv0;
+    std::vector<std::vector<int>> adj(nv0);
+    for (int t = 0; t < T.nt; ++t)
+      for (int i = 0; i < 4; ++i) {
+        const int u = vmap[T.tets[(size_t)t * 4 + i]];
+        if (u < 0) continue;
+        for (int j = 0; j < 4; ++j) {
+          const int v = vmap[T.tets[(size_t)t * 4 + j]];
+          if (v >= 0) adj[u].push_back(v);
+        }
+      }
+    A.rp.assign(nv0 + 1, 0);
+    parallel_ranges(nv0, [&](int64_t b, int64_t e, int) {
+      for (int64_t u = b; u < e; ++u) {
+        std::sort(adj[u].begin(), adj[u].end());
+        adj[u].erase(std::unique(adj[u].begin(), adj[u].end()), adj[u].end());
+      }
+    });
+    for (int u = 0; u < nv0; ++u) A.rp[u + 1] = A.rp[u] + (int)adj[u].size();
+    A.ci.resize(A.rp[nv0]);
+    for (int u = 0; u < nv0; ++u) std::copy(adj[u].begin(), adj[u].end(), A.ci.begin() + A.rp[u]);
+    // block-diagonal stack of the nblk Galerkin blocks
+    HostCsr &S = X->APpat;
+    const int64_t z = A.rp[nv0];
+    S.n = S.m = nb * nv0;
+    S.rp.resize((size_t)nb * nv0 + 1);
+    S.ci.resize((size_t)nb * z);
+    S.rp[0] = 0;
+    for (int d = 0; d < nb; ++d) {
+      for (int u = 0; u < nv0; ++u) S.rp[(size_t)d * nv0 + u + 1] = (int)(d * z + A.rp[u + 1]);
+      for (int64_t k = 0; k < z; ++k) S.ci[d * z + k] = A.ci[k] + d * nv0;
+    }
+  }
   return X;
+}
+
+// The four Galerkin products P_k^T B P_k (k = G0, Pi_x, Pi_y, Pi_z) in one pass over
+// B: they share the pattern Apat, so each row u accumulates the four value sets at
+// positions found through a dense column map.  Each value is summed in the order of
+// csr_triple (rows of P_k^T, then of B, then of P_k).
+void galerkin_blocks(const AmsAux &X, const HostCsr &B, std::vector<double> (&vals)[4]) {
+  const HostCsr &A = X.Apat;
+  const int64_t z = A.rp[A.n];
+  const HostCsr *R[4] = {&X.G0T, &X.PT[0], &X.PT[1], &X.PT[2]};
+  const HostCsr *P[4] = {&X.G0, &X.P[0], &X.P[1], &X.P[2]};
+  for (int k = 0; k < 4; ++k) vals[k].assign(z, 0.0);
+  EMF_REQUIRE(B.n == X.G0.n && A.n == X.nv0, kErrArg, "Galerkin blocks: B does not match the auxiliary spaces");
+  std::atomic<int64_t> missing{0};
+  parallel_ranges(A.n, [&](int64_t b, int64_t e, int) {
+    std::vector<int> pos(A.m, -1);
+    for (int64_t u = b; u < e; ++u) {
+      const int r0 = A.rp[u];
+      for (int t = r0; t < A.rp[u + 1]; ++t) pos[A.ci[t]] = t;
+      for (int p = R[0]->rp[u]; p < R[0]->rp[u + 1]; ++p) {
+        const int ed = R[0]->ci[p];
+        double r[4];
+        for (int k = 0; k < 4; ++k) r[k] = R[k]->v[p];
+        for (int q = B.rp[ed]; q < B.rp[ed + 1]; ++q) {
+          const int f = B.ci[q];
+          const double bv = B.v[q];
+          double ra[4];
+          for (int k = 0; k < 4; ++k) ra[k] = r[k] * bv;
+          for (int s2 = P[0]->rp[f]; s2 < P[0]->rp[f + 1]; ++s2) {
+            const int t = pos[P[0]->ci[s2]];
+            if (t < r0 || t >= A.rp[u + 1]) {  // B couples edges outside a common tet
+              ++missing;
+              continue;
+            }
+            for (int k = 0; k < 4; ++k) vals[k][t] += ra[k] * P[k]->v[s2];
+          }
+        }
+      }
+    }
+  });
+  EMF_REQUIRE(missing.load() == 0, kErrArg, "Galerkin blocks: B has couplings outside the tet pattern");
 }
 
 void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, AmsHost &H,
@@ -492,33 +582,26 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, Ams
   H.aux = aux;
   const int nv0 = aux->nv0;
   H.timing["aux_build_ms"] += now_ms() - t0;
-  // Galerkin products of the auxiliary blocks
+  // Galerkin products of the auxiliary blocks (one fused pass, shared pattern)
   t0 = now_ms();
-  std::vector<HostCsr> A(4);
+  (void)nv0;
   const bool additive = aux->additive;
-  const HostCsr *R[4] = {&aux->G0T, &aux->PT[0], &aux->PT[1], &aux->PT[2]};
-  const HostCsr *P[4] = {&aux->G0, &aux->P[0], &aux->P[1], &aux->P[2]};
-  for (int k = 0; k < 4; ++k) A[k] = csr_triple(*R[k], B, *P[k]);
-  std::vector<const HostCsr *> ab;
-  for (int k = additive ? 0 : 1; k < 4; ++k) ab.push_back(&A[k]);
-  const int nb = (int)ab.size();
-  HostCsr AP;
-  AP.n = AP.m = nb * nv0;
-  AP.rp.assign(AP.n + 1, 0);
-  for (int d = 0, base = 0; d < nb; base += ab[d]->rp[nv0], ++d) {
-    for (int r = 0; r < nv0; ++r) AP.rp[d * nv0 + r + 1] = base + ab[d]->rp[r + 1];
-    AP.ci.insert(AP.ci.end(), ab[d]->ci.begin(), ab[d]->ci.end());
-    AP.v.insert(AP.v.end(), ab[d]->v.begin(), ab[d]->v.end());
-    for (size_t k = base; k < AP.ci.size(); ++k) AP.ci[k] += d * nv0;
-  }
+  std::vector<double> vals[4];
+  galerkin_blocks(*aux, B, vals);
+  HostCsr AP = aux->APpat;
+  const int64_t z = aux->Apat.rp[aux->Apat.n];
+  AP.v.resize(AP.ci.size());
+  for (int k = additive ? 0 : 1, d = 0; k < 4; ++k, ++d) std::copy(vals[k].begin(), vals[k].end(), AP.v.begin() + d * z);
   H.timing["galerkin_ms"] += now_ms() - t0;
   if (!additive) {
     t0 = now_ms();
-    build_amg(A[0], prm, H.amgG);
+    HostCsr AG = aux->Apat;
+    AG.v = std::move(vals[0]);
+    build_amg(std::move(AG), prm, H.amgG);
     H.timing["amg_ms"] += now_ms() - t0;
   }
   t0 = now_ms();
-  build_amg(AP, prm, H.amgPi);
+  build_amg(std::move(AP), prm, H.amgPi);
   H.timing["amg_ms"] += now_ms() - t0;
   H.l1inv = l1_inverse(B);
 }

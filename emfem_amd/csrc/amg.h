@@ -53,7 +53,8 @@ struct AmgParams {
   std::map<std::string, double> *timing = nullptr;  // optional phase times (ms)
 };
 
-void build_amg(const HostCsr &A, const AmgParams &p, AmgHost &H);
+// A is taken by value: callers that no longer need it move it in.
+void build_amg(HostCsr A, const AmgParams &p, AmgHost &H);
 
 // Mesh-only parts of the auxiliary spaces: they depend on the topology and vertex
 // coordinates, not on B, so one set serves every frequency on a mesh.
@@ -72,9 +73,15 @@ struct AmsAux {
   int nblk = 3;
   bool additive = false;
   int nv0 = 0;
+  // pattern of every P_k^T B P_k (interior vertices sharing a tet, columns ascending)
+  // and of their block-diagonal stack (values left empty)
+  HostCsr Apat, APpat;
 };
 
 std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive);
+
+// vals[k] = values of P_k^T B P_k on X.Apat, k = G0, Pi_x, Pi_y, Pi_z.
+void galerkin_blocks(const AmsAux &X, const HostCsr &B, std::vector<double> (&vals)[4]);
 
 struct AmsHost {
   std::shared_ptr<const AmsAux> aux;

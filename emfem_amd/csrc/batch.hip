@@ -95,6 +95,9 @@ extern "C" int emfem_solve_batch(emfem_context **ctxs, int n_ctx, int first, int
       U.pts.insert(U.pts.end(), c->T.pts.begin(), c->T.pts.end());
       for (int v : c->T.edgelist) U.edgelist.push_back(v + v0);
       U.vertex_bdr.insert(U.vertex_bdr.end(), c->T.vertex_bdr.begin(), c->T.vertex_bdr.end());
+      // tets (the pattern of the auxiliary Galerkin products)
+      for (int v : c->T.tets) U.tets.push_back(v + v0);
+      U.nt += c->T.nt;
     }
     rp.upload(B.rp, s);
     ci.upload(B.ci, s);
