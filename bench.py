@@ -334,7 +334,8 @@ def main():
     peak = 8000.0
     bw = (b_bytes / (b_ms * 1e-3) / 1e9) if b_ms > 0 else 0.0
     rows = int(ctx.stat("b_spmm_rows"))
-    kname = ("%s: Chebyshev sweep rres -= Dinv .* (B d), B = C + M fp32, XY+YX+dual = 6 real columns"
+    kname = ("%s: Chebyshev step rres -= Dinv .* (B d) with d' = c1 d + c2 rres, x += d' in its epilogue, "
+             "B = C + M fp32, XY+YX+dual = 6 real columns"
              % ("k_spmm_stream (LDS-staged row blocks)" if rows == -1 else
                 "k_spmm_xnarrow (LDS x-blocks, %d lanes per row)" % -rows if rows < -1 else
                 "k_spmm_narrow role 1 (%d lanes per row)" % rows))

@@ -83,12 +83,25 @@ template <typename TA, typename TX, typename T>
 void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
                 T gamma, const T *w, int ldw, T beta, const T *z, int ldz);
 
+// A Chebyshev step fused into an SpMM epilogue (off when dn is null): with every
+// output entry y(row, j) also dn(row, j) = c1 d(row, j) + c2 y(row, j) (in double,
+// rounded to T, as cheb_step) and x(row, j) += dn(row, j); d, dn and x are
+// [rows][ld].  d may be the SpMM's input block; dn must not alias it.
+template <typename T>
+struct ChebFuse {
+  const T *d = nullptr;
+  T *dn = nullptr, *x = nullptr;
+  int ld = 0;
+  double c1 = 0.0, c2 = 0.0;
+};
+
 // spmm_mixed with two optional per-row scalings: the input is read as
 // xs[col] * x[col][:] (the operator A diag(xs)) and z as zs[row] * z[row][:]; either
-// may be null (= 1).
+// may be null (= 1); and an optional fused Chebyshev step.
 template <typename TA, typename TX, typename T>
 void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
-             T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs);
+             T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs,
+             const ChebFuse<T> &cf = ChebFuse<T>());
 
 template <typename T>
 inline void spmm(hipStream_t s, const Csr<T> &A, int nc, const T *x, int ldx, T *y, int ldy, T alpha, const T *rs,

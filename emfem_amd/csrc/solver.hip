@@ -230,6 +230,7 @@ class DevAms {
     r64_.resize((size_t)ne_ * nc);
     rres_.resize((size_t)ne_ * nc);
     d_.resize((size_t)ne_ * nc);
+    d2_.resize((size_t)ne_ * nc);
     if (!std::is_same<T, double>::value) bt_.resize((size_t)ne_ * nc);
     vb_.resize((size_t)nblk_ * nv0_ * nc);
     vx_.resize((size_t)nblk_ * nv0_ * nc);
@@ -382,10 +383,28 @@ class DevAms {
 
   // ms per launch of the smoother SpMM in its Chebyshev form, rres -= Dinv .* (B d),
   // on nc columns (lanes per row as tuned), on the cycle's own buffers
+  // (below 16 columns the fused Chebyshev form of the cycle's iterations: also
+  // d' = c1 d + c2 rres and x += d', here into scratch blocks)
   double time_b(hipStream_t s, int nc, int reps) {
     tune(s, nc);
-    return time_launches(s, reps,
-                         [&]() { bspmm(s, nc, d_.get(), rres_.get(), T(-1), l1inv_.get(), T(0), nullptr, T(1), rres_.get()); });
+    DevBuf<T> xs((size_t)ne_ * nc);
+    xs.zero(s);
+    d_.zero(s);
+    rres_.zero(s);
+    ChebFuse<T> cf;
+    if (nc < 16) {
+      cf.d = d_.get();
+      cf.dn = d2_.get();
+      cf.x = xs.get();
+      cf.ld = nc;
+      cf.c1 = 0.5;
+      cf.c2 = 0.5;
+    }
+    const double ms = time_launches(s, reps, [&]() {
+      bspmm(s, nc, d_.get(), rres_.get(), T(-1), l1inv_.get(), T(0), nullptr, T(1), rres_.get(), cf);
+    });
+    EMF_HIP(hipStreamSynchronize(s));
+    return ms;
   }
 
  private:
@@ -418,16 +437,17 @@ class DevAms {
   }
 
   void bspmm(hipStream_t s, int nc, const T *x, T *y, T alpha, const T *rs, T gamma, const T *w, T beta,
-             const T *z) {
-    // in-cycle timing (plain launches only): HIP events around the first launches
-    const bool timed = clock_on_ && clock_.size() < 2 * kClockPairs;
+             const T *z, const ChebFuse<T> &cf = ChebFuse<T>()) {
+    // in-cycle timing (plain launches only): HIP events around the first launches of
+    // the fused Chebyshev form
+    const bool timed = clock_on_ && (cf.dn != nullptr) == (nc < 16) && clock_.size() < 2 * kClockPairs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) {
       EMF_HIP(hipEventCreate(&e0));
       EMF_HIP(hipEventCreate(&e1));
       EMF_HIP(hipEventRecord(e0, s));
     }
-    spmm<T>(s, smoother(nc), nc, x, nc, y, nc, alpha, rs, gamma, w, nc, beta, z, nc);
+    spmm_ex<T, T, T>(s, smoother(nc), nc, x, nc, y, nc, alpha, rs, gamma, w, nc, beta, z, nc, nullptr, nullptr, cf);
     if (timed) {
       EMF_HIP(hipEventRecord(e1, s));
       clock_.push_back(e0);
@@ -463,13 +483,34 @@ class DevAms {
     spmm_mixed<double, T, double>(s, B64(nc), nc, x, nc, r64_.get(), nc, -1.0, nullptr, 1.0, b, nc, 0.0, nullptr, 0);
   }
 
+  // The Chebyshev steps: rres -= l1inv .* (B d), then d = c1 d + c2 rres and x += d.
+  // Below 16 columns both are one SpMM pass, d alternating between d and a second
+  // block (the SpMM gathers d, so the new d cannot overwrite it; d's last value is not
+  // used afterwards).  Wider blocks keep the separate streaming update, which their
+  // SpMM epilogues (one element per store) would do more slowly.
   void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x) {
     double sigma = theta_ / delta_, rho = 1.0 / sigma;
+    if (nc >= 16) {
+      for (int k = 1; k < deg_; ++k) {
+        bspmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres);
+        const double rho_new = 1.0 / (2.0 * sigma - rho);
+        cheb_step<T>(s, ne_, nc, d, rres, x, rho_new * rho, 2.0 * rho_new / delta_);
+        rho = rho_new;
+      }
+      return;
+    }
+    T *dc = d, *dn = d2_.get();
     for (int k = 1; k < deg_; ++k) {
-      // rres -= l1inv .* (B d)
-      bspmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres);
       const double rho_new = 1.0 / (2.0 * sigma - rho);
-      cheb_step<T>(s, ne_, nc, d, rres, x, rho_new * rho, 2.0 * rho_new / delta_);
+      ChebFuse<T> cf;
+      cf.d = dc;
+      cf.dn = dn;
+      cf.x = x;
+      cf.ld = nc;
+      cf.c1 = rho_new * rho;
+      cf.c2 = 2.0 * rho_new / delta_;
+      bspmm(s, nc, dc, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres, cf);
+      std::swap(dc, dn);
       rho = rho_new;
     }
   }
@@ -487,7 +528,7 @@ class DevAms {
   double theta_ = 0.5, delta_ = 0.5;
   Csr<T> B_;
   Csr<double> B64_;
-  DevBuf<T> bvals_, l1inv_, rres_, d_, bt_, vb_, vx_;
+  DevBuf<T> bvals_, l1inv_, rres_, d_, d2_, bt_, vb_, vx_;
   DevBuf<double> b64vals_, r64_;
   DevCsr<T> G0_, G0T_, Pi_, PiT_;
   DevAmg<T> amgG_, amgPi_;

@@ -31,6 +31,20 @@ __device__ __forceinline__ T shfl_xor(T v, int mask, int width) {
 // real CSR x block
 // ------------------------------------------------------------------------------
 
+// Optional Chebyshev update in an SpMM epilogue (ChebFuse in kernels.h): with each
+// output y(row, j) also dn(row, j) = c1 d(row, j) + c2 y(row, j) and x(row, j) += dn,
+// in the arithmetic of cheb_step.
+template <typename T>
+__device__ __forceinline__ void store_y(T *y, size_t iy, T v, const ChebFuse<T> &cf, size_t ic) {
+  y[iy] = v;
+  if (cf.dn) {
+    const T dv = (T)(cf.c1 * (double)cf.d[ic] + cf.c2 * (double)v);
+    cf.dn[ic] = dv;
+    cf.x[ic] += dv;
+  }
+}
+
+
 // Sums the NC per-lane partials over the LPR lanes of a row.  For power-of-two NC
 // the sum is reduce-scattered (each exchange halves the values a lane keeps, so
 // NC/2 + NC/4 + ... shuffles instead of NC per step); the lane then owns columns
@@ -79,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
                                                         const TX *__restrict__ x, int ldx, T *y, int ldy, T alpha,
                                                         const T *__restrict__ rs, T gamma, const T *w, int ldw,
                                                         T beta, const T *z, int ldz, const T *__restrict__ xs,
-                                                        const T *__restrict__ zs) {
+                                                        const T *__restrict__ zs, ChebFuse<T> cf) {
   const int gid = xcd_block() * kBlock + threadIdx.x;
   const int row = gid / LPR;
   const int lane = gid % LPR;
@@ -131,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
       if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
       v *= scale;
       if (beta != T(0)) v += zscale * z[(size_t)row * ldz + j];
-      y[(size_t)row * ldy + j] = v;
+      store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
     }
   } else {
 #pragma unroll
@@ -141,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
         if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
         v *= scale;
         if (beta != T(0)) v += zscale * z[(size_t)row * ldz + j];
-        y[(size_t)row * ldy + j] = v;
+        store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
       }
     }
   }
@@ -159,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xnarrow(const int *__restrict__
                                                          const int *__restrict__ xb_ucol,
                                                          const int *__restrict__ lci, const TX *__restrict__ x,
                                                          int ldx, T *y, int ldy, T alpha, const T *__restrict__ rs,
-                                                         T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
+                                                         T gamma, const T *w, int ldw, T beta, const T *z, int ldz, ChebFuse<T> cf) {
   __shared__ TX xl[kXnCols * NC];
   const int b = xcd_block();
   const int u0 = xb_uptr[b], nu = xb_uptr[b + 1] - u0;
@@ -212,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xnarrow(const int *__restrict__
           if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
           v *= scale;
           if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
-          y[(size_t)row * ldy + j] = v;
+          store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
         }
       }
     } else {
@@ -223,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xnarrow(const int *__restrict__
           if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
           v *= scale;
           if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
-          y[(size_t)row * ldy + j] = v;
+          store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
         }
       }
     }
@@ -240,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, int rblocks, co
                                                       const TX *__restrict__ x, int ldx, T *y, int ldy, int nc,
                                                       T alpha, const T *__restrict__ rs, T gamma, const T *w,
                                                       int ldw, T beta, const T *z, int ldz, const T *__restrict__ xs,
-                                                      const T *__restrict__ zs) {
+                                                      const T *__restrict__ zs, ChebFuse<T> cf) {
   // slab-major block order: with the XCD-aware remap each XCD sweeps all rows of one
   // slab (or a contiguous share of one), so its L2 holds only that slab's window of x
   const int bid = xcd_block();
@@ -286,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, int rblocks, co
       if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
       v *= scale;
       if (beta != T(0)) v += zscale * z[(size_t)row * ldz + c];
-      y[(size_t)row * ldy + c] = v;
+      store_y(y, (size_t)row * ldy + c, v, cf, (size_t)row * cf.ld + c);
     }
   }
 }
@@ -314,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
                                                       const TX *__restrict__ x, int ldx, T *y, int ldy, int nc,
                                                       T alpha, const T *__restrict__ rs, T gamma, const T *w,
                                                       int ldw, T beta, const T *z, int ldz,
-                                                      const T *__restrict__ xs, const T *__restrict__ zs) {
+                                                      const T *__restrict__ xs, const T *__restrict__ zs, ChebFuse<T> cf) {
   using V = typename Vec16<TX>::type;
   constexpr int VEC = Vec16<TX>::n;
   const int bid = xcd_block();
@@ -357,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
     if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
     v *= scale;
     if (beta != T(0)) v += zscale * z[(size_t)row * ldz + c];
-    y[(size_t)row * ldy + c] = v;
+    store_y(y, (size_t)row * ldy + c, v, cf, (size_t)row * cf.ld + c);
   }
 }
 
@@ -375,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xblk(int nblk, const int *__res
                                                       const int *__restrict__ lci, const TX *__restrict__ x,
                                                       int ldx, T *y, int ldy, int nc, T alpha,
                                                       const T *__restrict__ rs, T gamma, const T *w, int ldw,
-                                                      T beta, const T *z, int ldz, int vec_ok) {
+                                                      T beta, const T *z, int ldz, int vec_ok, ChebFuse<T> cf) {
   extern __shared__ __align__(16) unsigned char xblk_lds[];
   TX *xl = reinterpret_cast<TX *>(xblk_lds);
   using V = typename Vec16<TX>::type;
@@ -426,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xblk(int nblk, const int *__res
       if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
       if (rs) v *= rs[row];
       if (beta != T(0)) v += beta * z[(size_t)row * ldz + c];
-      y[(size_t)row * ldy + c] = v;
+      store_y(y, (size_t)row * ldy + c, v, cf, (size_t)row * cf.ld + c);
     }
   }
 }
@@ -440,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ 
                                                         const TA *__restrict__ av, const TX *__restrict__ x, int ldx,
                                                         T *y, int ldy, T alpha, const T *__restrict__ rs, T gamma,
                                                         const T *w, int ldw, T beta, const T *z, int ldz,
-                                                        const int *__restrict__ blk_row) {
+                                                        const int *__restrict__ blk_row, ChebFuse<T> cf) {
   __shared__ T prod[TILE * NC];
   __shared__ int srp[TILE + 1];
   const int b = xcd_block();
@@ -477,13 +491,14 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ 
     if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
     if (rs) v *= rs[row];
     if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
-    y[(size_t)row * ldy + j] = v;
+    store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
   }
 }
 
 template <typename TA, typename TX, typename T, int NC>
 void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
-                   T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs) {
+                   T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs,
+                   const ChebFuse<T> &cf) {
   const double avg = A.nrows ? double(A.nnz) / A.nrows : 0.0;
   if (A.lpr <= -2) {
     // narrow x-blocks with -lpr lanes per row
@@ -491,7 +506,7 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
     EMF_REQUIRE(xs == nullptr && zs == nullptr, kErrArg, "x-block SpMM has no scaled operands");
 #define EMF_XN(L)                                                                                                 \
   hipLaunchKernelGGL((k_spmm_xnarrow<TA, TX, T, NC, L>), dim3(A.xn_nblk), dim3(kBlock), 0, s, A.rp, A.v, A.xn_row, \
-                     A.xn_uptr, A.xn_ucol, A.xn_lci, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz)
+                     A.xn_uptr, A.xn_ucol, A.xn_lci, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, cf)
     switch (-A.lpr) {
       case 2: EMF_XN(2); break;
       case 4: EMF_XN(4); break;
@@ -506,7 +521,7 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
     EMF_REQUIRE(A.blk_row != nullptr && A.nblk > 0, kErrArg, "row-block SpMM without its block table");
     EMF_REQUIRE(xs == nullptr && zs == nullptr, kErrArg, "row-block SpMM has no scaled operands");
     hipLaunchKernelGGL((k_spmm_stream<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(A.nblk), dim3(kBlock), 0, s, A.rp,
-                       A.ci, A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, A.blk_row);
+                       A.ci, A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, A.blk_row, cf);
     return;
   }
   // one trip of 4 nonzeros per lane covers a row of average length
@@ -516,10 +531,10 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
 #define EMF_NARROW(L)                                                                                        \
   if (A.role == 1)                                                                                           \
     hipLaunchKernelGGL((k_spmm_narrow<TA, TX, T, NC, L, 1>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, \
-                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs);                 \
+                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs, cf);             \
   else                                                                                                       \
     hipLaunchKernelGGL((k_spmm_narrow<TA, TX, T, NC, L, 0>), dim3(g), dim3(kBlock), 0, s, A.nrows, A.rp, A.ci, \
-                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs)
+                       A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs, cf)
   switch (lpr) {
     case 2: EMF_NARROW(2); break;
     case 4: EMF_NARROW(4); break;
@@ -606,7 +621,8 @@ XBlocks x_blocks(const std::vector<int> &rp, const std::vector<int> &ci, int nco
 
 template <typename TA, typename TX, typename T>
 void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
-             T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs) {
+             T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs,
+             const ChebFuse<T> &cf) {
   if (A.nrows == 0 || nc == 0) return;
   EMF_REQUIRE(A.lpr != -1 || (nc <= 8 && nc != 5 && nc != 7), kErrArg,
               "row-block SpMM supports 1, 2, 3, 4, 6 or 8 columns");
@@ -620,7 +636,7 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     const int vec_ok = ((uintptr_t)x % 16 == 0) && ((size_t)ldx * sizeof(TX)) % 16 == 0 && nc % 4 == 0;
     hipLaunchKernelGGL((k_spmm_xblk<TA, TX, T>), dim3(A.xb_nblk * nslab), dim3(kBlock), lds, s, A.xb_nblk, A.rp, A.v,
                        A.xb_row, A.xb_uptr, A.xb_ucol, A.xb_lci, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta,
-                       z, ldz, vec_ok);
+                       z, ldz, vec_ok, cf);
     EMF_HIP(hipGetLastError());
     return;
   }
@@ -637,7 +653,7 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     const int nslab = (nc + W - 1) / W;
 #define EMF_WVEC(LW)                                                                                             \
   hipLaunchKernelGGL((k_spmm_wvec<TA, TX, T, LW>), dim3(rb * nslab), dim3(kBlock), 0, s, A.nrows, rb, A.rp, A.ci, \
-                     A.v, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs)
+                     A.v, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs, cf)
     switch (lw) {
       case 4: EMF_WVEC(4); break;
       case 8: EMF_WVEC(8); break;
@@ -659,7 +675,7 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     const int nslab = (nc + W - 1) / W;
 #define EMF_WIDE(LW)                                                                                            \
   hipLaunchKernelGGL((k_spmm_wide<TA, TX, T, LW, 4>), dim3(rb * nslab), dim3(kBlock), 0, s, A.nrows, rb, A.rp, \
-                     A.ci, A.v, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs)
+                     A.ci, A.v, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs, cf)
     switch (W) {
       case 16: EMF_WIDE(4); break;
       case 32: EMF_WIDE(8); break;
@@ -678,9 +694,16 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     const int n = left >= 8 ? 8 : (left >= 6 ? 6 : (left >= 4 ? 4 : (left >= 3 ? 3 : (left >= 2 ? 2 : 1))));
     const T *wc = w ? w + c0 : nullptr;
     const T *zc = z ? z + c0 : nullptr;
-#define EMF_CASE(K)                                                                                                 \
-  case K:                                                                                                           \
-    launch_narrow<TA, TX, T, K>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz, xs, zs); \
+    ChebFuse<T> cfc = cf;
+    if (cf.dn) {
+      cfc.d += c0;
+      cfc.dn += c0;
+      cfc.x += c0;
+    }
+#define EMF_CASE(K)                                                                                             \
+  case K:                                                                                                       \
+    launch_narrow<TA, TX, T, K>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz, xs, zs, \
+                                cfc);                                                                           \
     break;
     switch (n) {
       EMF_CASE(8)
@@ -689,7 +712,8 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
       EMF_CASE(3)
       EMF_CASE(2)
       default:
-        launch_narrow<TA, TX, T, 1>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz, xs, zs);
+        launch_narrow<TA, TX, T, 1>(s, A, x + c0, ldx, y + c0, ldy, alpha, rs, gamma, wc, ldw, beta, zc, ldz, xs, zs,
+                                    cfc);
         break;
     }
 #undef EMF_CASE
@@ -701,7 +725,8 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
 template <typename TA, typename TX, typename T>
 void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
                 T gamma, const T *w, int ldw, T beta, const T *z, int ldz) {
-  spmm_ex<TA, TX, T>(s, A, nc, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, nullptr, nullptr);
+  spmm_ex<TA, TX, T>(s, A, nc, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, nullptr, nullptr,
+                     ChebFuse<T>());
 }
 
 template <typename TA, typename TX, typename T>
@@ -762,7 +787,8 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
   template void spmm_mixed<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, int, T *, int, T, const T *, T, \
                                       const T *, int, T, const T *, int);                                           \
   template void spmm_ex<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, int, T *, int, T, const T *, T,    \
-                                   const T *, int, T, const T *, int, const T *, const T *);                        \
+                                   const T *, int, T, const T *, int, const T *, const T *,                         \
+                                   const ChebFuse<T> &);                                                            \
   template int tune_lpr<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, T *);
 EMF_SPMM_INST(float, float, float)
 EMF_SPMM_INST(double, double, double)
