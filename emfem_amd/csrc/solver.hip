@@ -342,6 +342,18 @@ class DevAms {
     return lpr_[nc];
   }
 
+  // Whether the Chebyshev update rides in the smoother SpMM's epilogue: always below
+  // 16 columns; at 16 or more only with the 16-byte-gather kernel (odd code or the
+  // untuned default), whose epilogue moves whole vectors (the others store one element
+  // at a time, slower than the separate streaming update).
+  bool fused(int nc) const {
+    if (nc < 16) return true;
+    auto it = lpr_.find(nc);
+    const int code = it == lpr_.end() ? 0 : it->second;
+    const bool vec_cols = nc % (16 / (int)sizeof(T)) == 0;
+    return vec_cols && (code == 0 || (code >= 17 && (code & 1)));
+  }
+
   Csr<T> smoother(int nc) {
     Csr<T> B = B_;
     B.lpr = lpr_[nc];
@@ -383,7 +395,7 @@ class DevAms {
 
   // ms per launch of the smoother SpMM in its Chebyshev form, rres -= Dinv .* (B d),
   // on nc columns (lanes per row as tuned), on the cycle's own buffers
-  // (below 16 columns the fused Chebyshev form of the cycle's iterations: also
+  // (where fused(nc), the fused Chebyshev form of the cycle's iterations: also
   // d' = c1 d + c2 rres and x += d', here into scratch blocks)
   double time_b(hipStream_t s, int nc, int reps) {
     tune(s, nc);
@@ -392,7 +404,7 @@ class DevAms {
     d_.zero(s);
     rres_.zero(s);
     ChebFuse<T> cf;
-    if (nc < 16) {
+    if (fused(nc)) {
       cf.d = d_.get();
       cf.dn = d2_.get();
       cf.x = xs.get();
@@ -440,7 +452,7 @@ class DevAms {
              const T *z, const ChebFuse<T> &cf = ChebFuse<T>()) {
     // in-cycle timing (plain launches only): HIP events around the first launches of
     // the fused Chebyshev form
-    const bool timed = clock_on_ && (cf.dn != nullptr) == (nc < 16) && clock_.size() < 2 * kClockPairs;
+    const bool timed = clock_on_ && (cf.dn != nullptr) == fused(nc) && clock_.size() < 2 * kClockPairs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) {
       EMF_HIP(hipEventCreate(&e0));
@@ -483,14 +495,13 @@ class DevAms {
     spmm_mixed<double, T, double>(s, B64(nc), nc, x, nc, r64_.get(), nc, -1.0, nullptr, 1.0, b, nc, 0.0, nullptr, 0);
   }
 
-  // The Chebyshev steps: rres -= l1inv .* (B d), then d = c1 d + c2 rres and x += d.
-  // Below 16 columns both are one SpMM pass, d alternating between d and a second
-  // block (the SpMM gathers d, so the new d cannot overwrite it; d's last value is not
-  // used afterwards).  Wider blocks keep the separate streaming update, which their
-  // SpMM epilogues (one element per store) would do more slowly.
+  // The Chebyshev steps: rres -= l1inv .* (B d), then d = c1 d + c2 rres and x += d,
+  // both in one SpMM pass where fused(nc), d alternating between d and a second block
+  // (the SpMM gathers d, so the new d cannot overwrite it; d's last value is not used
+  // afterwards).  Otherwise the separate streaming update.
   void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x) {
     double sigma = theta_ / delta_, rho = 1.0 / sigma;
-    if (nc >= 16) {
+    if (!fused(nc)) {
       for (int k = 1; k < deg_; ++k) {
         bspmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres);
         const double rho_new = 1.0 / (2.0 * sigma - rho);
@@ -571,6 +582,15 @@ class InnerPcImpl : public InnerPc {
   int smoother_rows(hipStream_t s, int nc) override {
     ensure(nc);
     return fp32_ ? a32_->rows(s, nc) : a64_->rows(s, nc);
+  }
+  bool smoother_fused(hipStream_t s, int nc) override {
+    ensure(nc);
+    if (fp32_) {
+      a32_->tune(s, nc);
+      return a32_->fused(nc);
+    }
+    a64_->tune(s, nc);
+    return a64_->fused(nc);
   }
   double time_fcg(hipStream_t s, int nc, int reps) override {
     ensure(nc);

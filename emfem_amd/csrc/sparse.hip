@@ -322,13 +322,19 @@ struct Vec16<double> {
   static constexpr int n = 2;
 };
 
+// VEC consecutive values of T moved as one aligned vector (two for 32 bytes)
+template <typename T, int VEC>
+struct alignas(sizeof(T) * VEC > 16 ? 16 : sizeof(T) * VEC) VecT {
+  T v[VEC];
+};
+
 template <typename TA, typename TX, typename T, int LW>
 __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, const int *__restrict__ rp,
                                                       const int *__restrict__ ci, const TA *__restrict__ av,
                                                       const TX *__restrict__ x, int ldx, T *y, int ldy, int nc,
                                                       T alpha, const T *__restrict__ rs, T gamma, const T *w,
                                                       int ldw, T beta, const T *z, int ldz,
-                                                      const T *__restrict__ xs, const T *__restrict__ zs, ChebFuse<T> cf) {
+                                                      const T *__restrict__ xs, const T *__restrict__ zs, int vepi, ChebFuse<T> cf) {
   using V = typename Vec16<TX>::type;
   constexpr int VEC = Vec16<TX>::n;
   const int bid = xcd_block();
@@ -364,6 +370,39 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
   }
   const T scale = rs ? rs[row] : T(1);
   const T zscale = zs ? beta * zs[row] : beta;
+  if (vepi) {
+    // the lane's VEC outputs (and the fused Chebyshev operands) as aligned vectors
+    using TV = VecT<T, VEC>;
+    TV v;
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) v.v[j] = alpha * acc[j];
+    if (gamma != T(0)) {
+      const TV wv = *reinterpret_cast<const TV *>(w + (size_t)row * ldw + c0);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) v.v[j] += gamma * wv.v[j];
+    }
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) v.v[j] *= scale;
+    if (beta != T(0)) {
+      const TV zv = *reinterpret_cast<const TV *>(z + (size_t)row * ldz + c0);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) v.v[j] += zscale * zv.v[j];
+    }
+    *reinterpret_cast<TV *>(y + (size_t)row * ldy + c0) = v;
+    if (cf.dn) {
+      const size_t ic = (size_t)row * cf.ld + c0;
+      const TV dv = *reinterpret_cast<const TV *>(cf.d + ic);
+      TV xv = *reinterpret_cast<const TV *>(cf.x + ic), dn;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        dn.v[j] = (T)(cf.c1 * (double)dv.v[j] + cf.c2 * (double)v.v[j]);
+        xv.v[j] += dn.v[j];
+      }
+      *reinterpret_cast<TV *>(cf.dn + ic) = dn;
+      *reinterpret_cast<TV *>(cf.x + ic) = xv;
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < VEC; ++j) {
     const int c = c0 + j;
@@ -649,11 +688,16 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     // operands take the strided kernel with the same slab)
     const int W = code - 1;
     const int lw = W / vec;
+    // vector epilogue when every row segment it moves is aligned to its vector size
+    const size_t va = std::min<size_t>(16, sizeof(T) * vec);
+    auto al = [&](const void *p, int ld) { return (uintptr_t)p % va == 0 && ((size_t)ld * sizeof(T)) % va == 0; };
+    const int vepi = al(y, ldy) && (gamma == T(0) || al(w, ldw)) && (beta == T(0) || al(z, ldz)) &&
+                     (cf.dn == nullptr || (al(cf.d, cf.ld) && al(cf.dn, cf.ld) && al(cf.x, cf.ld)));
     const int rb = grid_for((int64_t)A.nrows * lw);
     const int nslab = (nc + W - 1) / W;
 #define EMF_WVEC(LW)                                                                                             \
   hipLaunchKernelGGL((k_spmm_wvec<TA, TX, T, LW>), dim3(rb * nslab), dim3(kBlock), 0, s, A.nrows, rb, A.rp, A.ci, \
-                     A.v, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs, cf)
+                     A.v, x, ldx, y, ldy, nc, alpha, rs, gamma, w, ldw, beta, z, ldz, xs, zs, vepi, cf)
     switch (lw) {
       case 4: EMF_WVEC(4); break;
       case 8: EMF_WVEC(8); break;
