@@ -280,7 +280,7 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive accuracy-versus-time comparison")
     ap.add_argument("--batch-models", type=int, default=64,
                     help="models of the uneven multi-model batch (0 skips it)")
-    ap.add_argument("--spmm-rows", default="auto", choices=["auto", "lanes", "rowblock", "xblock", "2", "4", "8", "16"],
+    ap.add_argument("--spmm-rows", default="auto", choices=["auto", "lanes", "rowblock", "pipelined", "xblock", "2", "4", "8", "16"],
                     help="row mapping of the smoother SpMM (auto: autotuned)")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of hipGraph-captured cycles (for kernel tracers)")
@@ -337,13 +337,15 @@ def main():
     kname = ("%s: Chebyshev step rres -= Dinv .* (B d) with d' = c1 d + c2 rres, x += d' in its epilogue, "
              "B = C + M fp32, XY+YX+dual = 6 real columns"
              % ("k_spmm_stream (LDS-staged row blocks)" if rows == -1 else
+                "k_spmm_pstream (persistent pipelined LDS-staged row blocks)" if rows == -3 else
                 "k_spmm_xnarrow (LDS x-blocks, %d lanes per row)" % -rows if rows < -1 else
                 "k_spmm_narrow role 1 (%d lanes per row)" % rows))
     # HBM traffic of the same kernel and workload from the committed PMC summary
     # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on k_make_b), when it exists
     traffic = None
     prof = os.path.join(ROOT, "profiles", "%s_smoother_adaptive_mt_%d.md"
-                        % ("k_spmm_stream" if rows == -1 else "k_spmm_xnarrow" if rows < -1 else "k_spmm_narrow",
+                        % ("k_spmm_stream" if rows == -1 else "k_spmm_pstream" if rows == -3 else
+                           "k_spmm_xnarrow" if rows < -1 else "k_spmm_narrow",
                            args.cells))
     if os.path.exists(prof):
         for line in open(prof):

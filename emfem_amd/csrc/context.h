@@ -233,12 +233,13 @@ inline PcOptions pc_options(const emfem_context *c) {
   o.amg.smooth_levels = c->opti("ams_smooth_levels");
   o.amg.additive = c->opt("ams_cycle") == "additive";
   o.use_graph = c->opt("use_graph") == "true";
-  // spmm_rows: 0 tuned, 1 lanes per row tuned, 2 row blocks, 3 x-blocks (16 or more
-  // columns), 10 + n fixed n lanes per row
+  // spmm_rows: 0 tuned, 1 lanes per row tuned, 2 row blocks, 3 x-blocks, 4 persistent
+  // pipelined row blocks, 10 + n fixed n lanes per row
   const std::string sr = c->opt("spmm_rows");
   o.spmm_rows = sr == "lanes" ? 1
                 : sr == "rowblock" ? 2
                 : sr == "xblock"   ? 3
+                : sr == "pipelined" ? 4
                 : sr == "auto"     ? 0
                                    : 10 + c->opti("spmm_rows");
   return o;

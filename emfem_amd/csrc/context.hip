@@ -88,7 +88,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"dual_precision", {"same", "fp32", "fp64"}},
       {"fp64_fallback", {"true", "false"}},
       {"use_graph", {"true", "false"}},
-      {"spmm_rows", {"auto", "lanes", "rowblock", "xblock", "2", "4", "8", "16"}},
+      {"spmm_rows", {"auto", "lanes", "rowblock", "pipelined", "xblock", "2", "4", "8", "16"}},
       {"ams_cycle", {"multiplicative", "additive"}},
       {"surrogate_cycles", {"true", "false"}},
       {"save_mesh", {"true", "false"}},

@@ -22,7 +22,8 @@ struct Csr {
   const T *v = nullptr;
   int lpr = 0;  // lanes per row of the narrow SpMM kernel; 0 = chosen from the row length
   int role = 0; // 1 = edge-space smoother operator (a separate kernel instance in traces)
-  // lpr = -1: LDS-staged row blocks; blk_row [nblk + 1] are the first rows of the
+  // lpr = -1: LDS-staged row blocks (-3: the same, persistent and software-pipelined);
+  // blk_row [nblk + 1] are the first rows of the
   // blocks (row_blocks), for the column count and precision the table was built for
   const int *blk_row = nullptr;
   int nblk = 0;
@@ -84,8 +85,8 @@ void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T
                 T gamma, const T *w, int ldw, T beta, const T *z, int ldz);
 
 // A Chebyshev step fused into an SpMM epilogue (off when dn is null): with every
-// output entry y(row, j) also dn(row, j) = c1 d(row, j) + c2 y(row, j) (in double,
-// rounded to T, as cheb_step) and x(row, j) += dn(row, j); d, dn and x are
+// output entry y(row, j) also dn(row, j) = c1 d(row, j) + c2 y(row, j) (in T with the
+// coefficients rounded to T, as cheb_step) and x(row, j) += dn(row, j); d, dn and x are
 // [rows][ld].  d may be the SpMM's input block; dn must not alias it.
 template <typename T>
 struct ChebFuse {
@@ -142,8 +143,8 @@ struct Seg {
 // out[seg * nc + c] on the device.  `work` must hold dot_work_size(rows, nc, sg)
 // doubles.  Results are bitwise reproducible (fixed summation order).
 size_t dot_work_size(int rows, int nc, const Seg &sg = Seg());
-template <typename T>
-void coldot(hipStream_t s, int rows, int nc, const T *x, int ldx, const T *y, int ldy, double *out, double *work,
+template <typename TX, typename TY = TX>
+void coldot(hipStream_t s, int rows, int nc, const TX *x, int ldx, const TY *y, int ldy, double *out, double *work,
             const Seg &sg = Seg());
 
 // Multi-dot for Gram-Schmidt: out[i*nl + seg*nc + c] = sum_r V_i[r][c] w[r][c] for
@@ -173,8 +174,9 @@ void put_rows(hipStream_t s, int rows, int n, const int *idx, const double2 *v, 
 
 // Flexible CG updates on nc columns with device scalars:
 //   x += alpha[c] p ; r -= alpha[c] q ; rr[c] = r . r (fused reduction)
-template <typename T>
-void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const T *p, const T *q, T *x, T *r, double *rr,
+// (x, r in T; the search directions p, q in TP; arithmetic in double)
+template <typename T, typename TP = T>
+void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const TP *p, const TP *q, T *x, T *r, double *rr,
             double *work, const Seg &sg = Seg());
 
 // Chebyshev smoother helpers (Dinv = 1 / l1 row norms):

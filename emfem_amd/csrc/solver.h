@@ -43,8 +43,9 @@ struct PcOptions {
   AmgParams amg;
   bool use_graph = true;  // replay each auxiliary-space cycle as one captured hipGraph
   // smoother SpMM row mapping: 0 = tuned, 1 = lanes per row only (tuned; lanes on
-  // columns at 16 or more columns), 2 = row blocks when possible, 3 = LDS x-blocks at
-  // 16 or more columns when possible, 10 + n = n lanes per row
+  // columns at 16 or more columns), 2 = row blocks when possible, 3 = LDS x-blocks
+  // when possible, 4 = persistent pipelined row blocks when possible, 10 + n = n
+  // lanes per row
   int spmm_rows = 0;
 };
 

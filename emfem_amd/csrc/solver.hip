@@ -244,7 +244,9 @@ class DevAms {
     ensure(nc);
     if (lpr_.count(nc)) return;
     d_.zero(s);
-    if (nc < 16 && (spmm_rows_ == 0 || spmm_rows_ == 3)) x_narrow_table(s);
+    // narrow x-blocks only on request: tuned in isolation they can win for the fp64 B
+    // and then lose inside the solve
+    if (nc < 16 && spmm_rows_ == 3) x_narrow_table(s);
     Csr<T> B = B_;
     const std::vector<int> blk =
         spmm_rows_ == 1 || spmm_rows_ >= 10 ? std::vector<int>() : row_blocks(hrp_, stream_tile_for(nc, sizeof(T) == 8));
@@ -254,7 +256,7 @@ class DevAms {
       B.nblk = (int)blk.size() - 1;
     }
     if (nc >= 16) {
-      if (spmm_rows_ != 1) x_blocks_table(s);  // "lanes": lanes on columns only
+      if (spmm_rows_ == 3) x_blocks_table(s);  // x-blocks only on request
       B.xb_row = B_.xb_row;
       B.xb_uptr = B_.xb_uptr;
       B.xb_ucol = B_.xb_ucol;
@@ -278,6 +280,7 @@ class DevAms {
     }
     if (spmm_rows_ >= 10 && nc < 16) lpr_[nc] = spmm_rows_ - 10;  // fixed lanes per row
     else if (spmm_rows_ == 2 && !blk.empty()) lpr_[nc] = -1;
+    else if (spmm_rows_ == 4 && !blk.empty()) lpr_[nc] = -3;
     else if (spmm_rows_ == 3 && nc < 16 && B_.xn_nblk > 0) lpr_[nc] = -4;  // forced x-blocks
     else lpr_[nc] = tune_lpr<T, T, T>(s, B, nc, d_.get(), rres_.get());
   }
@@ -654,6 +657,8 @@ class InnerPcImpl : public InnerPc {
     h_active_.resize(nl);
   }
 
+  // z = M r: the cycle in its precision; z in fp64 (conversion from the fp32 cycle)
+  // or in fp32 (the cycle's own output block)
   void precondition(hipStream_t s, int nc, const double *r, double *z, SolveStats &st) {
     ++st.vcycles;
     if (!fp32_) {
@@ -663,11 +668,18 @@ class InnerPcImpl : public InnerPc {
       axpby<float, double>(s, ne_, nc, 1.0, zt_.get(), nc, 0.0, z, nc);
     }
   }
+  void solve(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) {
+    fcg<double>(s, nc, b, x, p_.get(), q_.get(), z_.get(), st);
+  }
 
   // Flexible CG per logical column (segment, column): each stops on its own
-  // unpreconditioned residual; frozen columns keep alpha = beta = 0.
-  void solve(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) {
-    double *r = r_.get(), *z = z_.get(), *p = p_.get(), *q = q_.get();
+  // unpreconditioned residual; frozen columns keep alpha = beta = 0.  All vectors are
+  // fp64: q = B p for a gradient-like p in the air is a small result of large
+  // cancelling curl terms, which an fp32 product turns into rounding noise (fp32
+  // search vectors with an fp32 B stalled the outer solve at 1e-10).
+  template <typename F>
+  void fcg(hipStream_t s, int nc, const double *b, double *x, F *p, F *q, F *z, SolveStats &st) {
+    double *r = r_.get();
     double *w = work_.get();
     const int nl = sg_.nseg * nc;
     ++st.pc_applies;
@@ -687,13 +699,13 @@ class InnerPcImpl : public InnerPc {
     const double tol2 = o_.inner_rtol * o_.inner_rtol;
     // z = M r ; p = z ; q = B p
     precondition(s, nc, r, z, st);
-    EMF_HIP(hipMemcpyAsync(p, z, sizeof(double) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
+    EMF_HIP(hipMemcpyAsync(p, z, sizeof(F) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
     bspmm(s, nc, p, q);
-    coldot<double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
-    coldot<double>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
+    coldot<F, F>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
+    coldot<double, F>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
     for (int it = 1; it <= o_.inner_max_it; ++it) {
       fcg_alpha(s, nl, rz_.get(), pq_.get(), active_.get(), alpha_.get());
-      fcg_xr<double>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w, sg_);
+      fcg_xr<double, F>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w, sg_);
       ++st.inner_its;
       EMF_HIP(hipMemcpyAsync(h_rr_.data(), rr_.get(), sizeof(double) * nl, hipMemcpyDeviceToHost, s));
       EMF_HIP(hipStreamSynchronize(s));
@@ -705,12 +717,12 @@ class InnerPcImpl : public InnerPc {
       if (!any || it == o_.inner_max_it) break;
       active_.upload(h_active_.data(), nl, s);
       precondition(s, nc, r, z, st);
-      coldot<double>(s, ne_, nc, z, nc, q, nc, zq_.get(), w, sg_);
-      coldot<double>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
+      coldot<F, F>(s, ne_, nc, z, nc, q, nc, zq_.get(), w, sg_);
+      coldot<double, F>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
       fcg_beta(s, nl, zq_.get(), pq_.get(), active_.get(), beta_.get());
-      axpby_cols<double, double>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc, sg_);
+      axpby_cols<F, F>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc, sg_);
       bspmm(s, nc, p, q);
-      coldot<double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
+      coldot<F, F>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
     }
   }
 

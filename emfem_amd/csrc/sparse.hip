@@ -31,19 +31,36 @@ __device__ __forceinline__ T shfl_xor(T v, int mask, int width) {
 // real CSR x block
 // ------------------------------------------------------------------------------
 
-// Optional Chebyshev update in an SpMM epilogue (ChebFuse in kernels.h): with each
-// output y(row, j) also dn(row, j) = c1 d(row, j) + c2 y(row, j) and x(row, j) += dn,
-// in the arithmetic of cheb_step.
+// SpMM epilogue of one output entry in two steps: epi_load issues every operand load
+// (w, z, the row scale and the fused Chebyshev d and x) before any arithmetic, so they
+// are in flight together; epi_store writes y and, with a fused Chebyshev step
+// (ChebFuse in kernels.h), dn = c1 d + c2 y and x + dn in T, as cheb_step does.
 template <typename T>
-__device__ __forceinline__ void store_y(T *y, size_t iy, T v, const ChebFuse<T> &cf, size_t ic) {
-  y[iy] = v;
-  if (cf.dn) {
-    const T dv = (T)(cf.c1 * (double)cf.d[ic] + cf.c2 * (double)v);
-    cf.dn[ic] = dv;
-    cf.x[ic] += dv;
-  }
+struct EpiIn {
+  T w, z, s, d, x;
+};
+
+template <typename T>
+__device__ __forceinline__ EpiIn<T> epi_load(const T *w, size_t iw, bool gw, const T *z, size_t iz, bool gz,
+                                             const T *rs, int row, const ChebFuse<T> &cf, size_t ic) {
+  EpiIn<T> e;
+  e.w = gw ? w[iw] : T(0);
+  e.z = gz ? z[iz] : T(0);
+  e.s = rs ? rs[row] : T(1);
+  e.d = cf.dn ? cf.d[ic] : T(0);
+  e.x = cf.dn ? cf.x[ic] : T(0);
+  return e;
 }
 
+template <typename T>
+__device__ __forceinline__ void epi_store(T *y, size_t iy, T v, const EpiIn<T> &e, const ChebFuse<T> &cf, size_t ic) {
+  y[iy] = v;
+  if (cf.dn) {
+    const T dv = T(cf.c1) * e.d + T(cf.c2) * v;
+    cf.dn[ic] = dv;
+    cf.x[ic] = e.x + dv;
+  }
+}
 
 // Sums the NC per-lane partials over the LPR lanes of a row.  For power-of-two NC
 // the sum is reduce-scattered (each exchange halves the values a lane keeps, so
@@ -110,12 +127,15 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
   for (int p0 = rp[row] + lane; p0 < e; p0 += kU * LPR) {
     int c[kU];
     T a[kU];
+    // unconditional loads from clamped indices (e - 1 is in this row), masked
+    // afterwards: no branch and no wait per load
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int p = p0 + u * LPR;
-      const bool ok = p < e;
-      c[u] = ok ? ci[p] : 0;
-      a[u] = ok ? (T)av[p] : T(0);
+      const int pc = min(p, e - 1);
+      c[u] = ci[pc];
+      const T av_ = (T)av[pc];
+      a[u] = p < e ? av_ : T(0);
     }
     if (xs) {
 #pragma unroll
@@ -137,25 +157,36 @@ __global__ __launch_bounds__(kBlock) void k_spmm_narrow(int nrows, const int *__
     constexpr int kOwn = NCP >= LPR ? NCP / LPR : 1;
     constexpr int kRep = NCP >= LPR ? 1 : LPR / NCP;
     if ((lane & (kRep - 1)) != 0) return;
+    // the operands of all owned columns first, then the arithmetic and stores
+    EpiIn<T> e_[kOwn];
+#pragma unroll
+    for (int i = 0; i < kOwn; ++i) {
+      const int j = col0 + i;
+      if (NCP == NC || j < NC)  // not a padding column
+        e_[i] = epi_load<T>(w, (size_t)row * ldw + j, gamma != T(0), z, (size_t)row * ldz + j, beta != T(0),
+                            nullptr, row, cf, (size_t)row * cf.ld + j);
+    }
 #pragma unroll
     for (int i = 0; i < kOwn; ++i) {
       const int j = col0 + i;
       if (NCP != NC && j >= NC) break;  // padding columns
       T v = alpha * acc[i];
-      if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
+      if (gamma != T(0)) v += gamma * e_[i].w;
       v *= scale;
-      if (beta != T(0)) v += zscale * z[(size_t)row * ldz + j];
-      store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
+      if (beta != T(0)) v += zscale * e_[i].z;
+      epi_store(y, (size_t)row * ldy + j, v, e_[i], cf, (size_t)row * cf.ld + j);
     }
   } else {
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
       if (lane == (j % LPR)) {
+        const EpiIn<T> e_ = epi_load<T>(w, (size_t)row * ldw + j, gamma != T(0), z, (size_t)row * ldz + j,
+                                        beta != T(0), nullptr, row, cf, (size_t)row * cf.ld + j);
         T v = alpha * acc[j];
-        if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
+        if (gamma != T(0)) v += gamma * e_.w;
         v *= scale;
-        if (beta != T(0)) v += zscale * z[(size_t)row * ldz + j];
-        store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
+        if (beta != T(0)) v += zscale * e_.z;
+        epi_store(y, (size_t)row * ldy + j, v, e_, cf, (size_t)row * cf.ld + j);
       }
     }
   }
@@ -200,9 +231,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xnarrow(const int *__restrict__
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         const int p = p0 + u * LPR;
-        const bool ok = p < e;
-        c[u] = ok ? lci[p] : 0;
-        a[u] = ok ? (T)av[p] : T(0);
+        const int pc = min(p, e - 1);  // unconditional loads, masked
+        c[u] = lci[pc];
+        const T av_ = (T)av[pc];
+        a[u] = p < e ? av_ : T(0);
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
@@ -218,26 +250,36 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xnarrow(const int *__restrict__
       constexpr int kOwn = NCP >= LPR ? NCP / LPR : 1;
       constexpr int kRep = NCP >= LPR ? 1 : LPR / NCP;
       if ((lane & (kRep - 1)) == 0) {
+        EpiIn<T> e_[kOwn];
+#pragma unroll
+        for (int i = 0; i < kOwn; ++i) {
+          const int j = col0 + i;
+          if (NCP == NC || j < NC)
+            e_[i] = epi_load<T>(w, (size_t)row * ldw + j, gamma != T(0), z, (size_t)row * ldz + j, beta != T(0),
+                                nullptr, row, cf, (size_t)row * cf.ld + j);
+        }
 #pragma unroll
         for (int i = 0; i < kOwn; ++i) {
           const int j = col0 + i;
           if (NCP != NC && j >= NC) break;
           T v = alpha * acc[i];
-          if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
+          if (gamma != T(0)) v += gamma * e_[i].w;
           v *= scale;
-          if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
-          store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
+          if (beta != T(0)) v += beta * e_[i].z;
+          epi_store(y, (size_t)row * ldy + j, v, e_[i], cf, (size_t)row * cf.ld + j);
         }
       }
     } else {
 #pragma unroll
       for (int j = 0; j < NC; ++j) {
         if (lane == (j % LPR)) {
+          const EpiIn<T> e_ = epi_load<T>(w, (size_t)row * ldw + j, gamma != T(0), z, (size_t)row * ldz + j,
+                                          beta != T(0), nullptr, row, cf, (size_t)row * cf.ld + j);
           T v = alpha * acc[j];
-          if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
+          if (gamma != T(0)) v += gamma * e_.w;
           v *= scale;
-          if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
-          store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
+          if (beta != T(0)) v += beta * e_.z;
+          epi_store(y, (size_t)row * ldy + j, v, e_, cf, (size_t)row * cf.ld + j);
         }
       }
     }
@@ -274,8 +316,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, int rblocks, co
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const bool ok = p0 + u < e;
-      cu[u] = ok ? ci[p0 + u] : ci[b];
-      au[u] = ok ? (T)av[p0 + u] : T(0);
+      const int pc = min(p0 + u, e - 1);
+      cu[u] = ci[pc];
+      const T av_ = (T)av[pc];
+      au[u] = ok ? av_ : T(0);
       if (xs && ok) au[u] *= xs[cu[u]];
     }
     TX xv[kU][CPL];
@@ -292,15 +336,23 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wide(int nrows, int rblocks, co
   }
   const T scale = rs ? rs[row] : T(1);
   const T zscale = zs ? beta * zs[row] : beta;
+  EpiIn<T> e_[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    const int c = c0 + j * LW;
+    if (c < nc)
+      e_[j] = epi_load<T>(w, (size_t)row * ldw + c, gamma != T(0), z, (size_t)row * ldz + c, beta != T(0), nullptr,
+                          row, cf, (size_t)row * cf.ld + c);
+  }
 #pragma unroll
   for (int j = 0; j < CPL; ++j) {
     const int c = c0 + j * LW;
     if (c < nc) {
       T v = alpha * acc[j];
-      if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
+      if (gamma != T(0)) v += gamma * e_[j].w;
       v *= scale;
-      if (beta != T(0)) v += zscale * z[(size_t)row * ldz + c];
-      store_y(y, (size_t)row * ldy + c, v, cf, (size_t)row * cf.ld + c);
+      if (beta != T(0)) v += zscale * e_[j].z;
+      epi_store(y, (size_t)row * ldy + c, v, e_[j], cf, (size_t)row * cf.ld + c);
     }
   }
 }
@@ -354,8 +406,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const bool ok = p0 + u < e;
-      cu[u] = ok ? ci[p0 + u] : ci[b];
-      au[u] = ok ? (T)av[p0 + u] : T(0);
+      const int pc = min(p0 + u, e - 1);
+      cu[u] = ci[pc];
+      const T av_ = (T)av[pc];
+      au[u] = ok ? av_ : T(0);
       if (xs && ok) au[u] *= xs[cu[u]];
     }
     V xv[kU];
@@ -373,29 +427,33 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
   if (vepi) {
     // the lane's VEC outputs (and the fused Chebyshev operands) as aligned vectors
     using TV = VecT<T, VEC>;
+    // all operand vectors first, then the arithmetic
+    const size_t ic = (size_t)row * cf.ld + c0;
+    TV wv{}, zv{}, dv{}, xv{}, dn;
+    if (gamma != T(0)) wv = *reinterpret_cast<const TV *>(w + (size_t)row * ldw + c0);
+    if (beta != T(0)) zv = *reinterpret_cast<const TV *>(z + (size_t)row * ldz + c0);
+    if (cf.dn) {
+      dv = *reinterpret_cast<const TV *>(cf.d + ic);
+      xv = *reinterpret_cast<const TV *>(cf.x + ic);
+    }
     TV v;
 #pragma unroll
     for (int j = 0; j < VEC; ++j) v.v[j] = alpha * acc[j];
     if (gamma != T(0)) {
-      const TV wv = *reinterpret_cast<const TV *>(w + (size_t)row * ldw + c0);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) v.v[j] += gamma * wv.v[j];
     }
 #pragma unroll
     for (int j = 0; j < VEC; ++j) v.v[j] *= scale;
     if (beta != T(0)) {
-      const TV zv = *reinterpret_cast<const TV *>(z + (size_t)row * ldz + c0);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) v.v[j] += zscale * zv.v[j];
     }
     *reinterpret_cast<TV *>(y + (size_t)row * ldy + c0) = v;
     if (cf.dn) {
-      const size_t ic = (size_t)row * cf.ld + c0;
-      const TV dv = *reinterpret_cast<const TV *>(cf.d + ic);
-      TV xv = *reinterpret_cast<const TV *>(cf.x + ic), dn;
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
-        dn.v[j] = (T)(cf.c1 * (double)dv.v[j] + cf.c2 * (double)v.v[j]);
+        dn.v[j] = T(cf.c1) * dv.v[j] + T(cf.c2) * v.v[j];
         xv.v[j] += dn.v[j];
       }
       *reinterpret_cast<TV *>(cf.dn + ic) = dn;
@@ -403,14 +461,21 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
     }
     return;
   }
+  EpiIn<T> e_[VEC];
+#pragma unroll
+  for (int j = 0; j < VEC; ++j) {
+    const int c = c0 + j;
+    e_[j] = epi_load<T>(w, (size_t)row * ldw + c, gamma != T(0), z, (size_t)row * ldz + c, beta != T(0), nullptr,
+                        row, cf, (size_t)row * cf.ld + c);
+  }
 #pragma unroll
   for (int j = 0; j < VEC; ++j) {
     const int c = c0 + j;
     T v = alpha * acc[j];
-    if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
+    if (gamma != T(0)) v += gamma * e_[j].w;
     v *= scale;
-    if (beta != T(0)) v += zscale * z[(size_t)row * ldz + c];
-    store_y(y, (size_t)row * ldy + c, v, cf, (size_t)row * cf.ld + c);
+    if (beta != T(0)) v += zscale * e_[j].z;
+    epi_store(y, (size_t)row * ldy + c, v, e_[j], cf, (size_t)row * cf.ld + c);
   }
 }
 
@@ -475,11 +540,13 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xblk(int nblk, const int *__res
     for (int p = pb; p < pe; ++p) acc += (T)av[p] * (T)xl[lci[p] * W + lane];
     const int c = cb + lane;
     if (c < nc) {
+      const EpiIn<T> e_ = epi_load<T>(w, (size_t)row * ldw + c, gamma != T(0), z, (size_t)row * ldz + c,
+                                      beta != T(0), rs, row, cf, (size_t)row * cf.ld + c);
       T v = alpha * acc;
-      if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + c];
-      if (rs) v *= rs[row];
-      if (beta != T(0)) v += beta * z[(size_t)row * ldz + c];
-      store_y(y, (size_t)row * ldy + c, v, cf, (size_t)row * cf.ld + c);
+      if (gamma != T(0)) v += gamma * e_.w;
+      v *= e_.s;
+      if (beta != T(0)) v += beta * e_.z;
+      epi_store(y, (size_t)row * ldy + c, v, e_, cf, (size_t)row * cf.ld + c);
     }
   }
 }
@@ -488,6 +555,112 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xblk(int nblk, const int *__res
 // nonzeros (at most TILE) are streamed with fully coalesced column/value loads, and
 // their products a * x[col][:] are staged in LDS; each (row, column) is then summed
 // from LDS by one thread, in nonzero order.
+// Persistent row blocks: workgroup g takes the consecutive tiles [g per, (g+1) per)
+// of the row-block table and runs them as a three-stage software pipeline: while
+// tile t is summed from LDS (and its epilogue loads are in flight), the x gathers of
+// tile t+1 and the column/value/row-pointer loads of tile t+2 are already issued, so
+// a tile costs about one memory latency instead of three.  Products and summation
+// order are those of k_spmm_stream.
+template <typename TA, typename TX, typename T, int NC, int TILE>
+__global__ __launch_bounds__(kBlock) void k_spmm_pstream(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                         const TA *__restrict__ av, const TX *__restrict__ x,
+                                                         int ldx, T *y, int ldy, T alpha, const T *__restrict__ rs,
+                                                         T gamma, const T *w, int ldw, T beta, const T *z, int ldz,
+                                                         const int *__restrict__ blk_row, int nblk, int per,
+                                                         ChebFuse<T> cf) {
+  __shared__ T prod[TILE * NC];
+  __shared__ int srp[TILE + 1];
+  constexpr int kTrips = TILE / kBlock;
+  constexpr int kRp = (TILE + 1 + kBlock - 1) / kBlock;
+  struct Tile {
+    int r0, r1, p0, nnz;
+    int c[kTrips];
+    T a[kTrips];
+    int rpv[kRp];
+  };
+  const int t0 = xcd_block() * per;
+  const int t1 = min(nblk, t0 + per);
+  if (t0 >= t1) return;
+  // all loads are unconditional from clamped (valid) indices and masked afterwards, so
+  // the compiler keeps them in flight instead of branching and waiting per load
+  auto load_idx = [&](int t, Tile &I) {
+    I.r0 = blk_row[t];
+    I.r1 = blk_row[t + 1];
+    I.p0 = rp[I.r0];
+    I.nnz = rp[I.r1] - I.p0;
+#pragma unroll
+    for (int k = 0; k < kRp; ++k) {
+      const int i = threadIdx.x + k * kBlock;
+      I.rpv[k] = rp[I.r0 + min(i, I.r1 - I.r0)] - I.p0;
+    }
+    const int last = max(I.nnz - 1, 0);
+#pragma unroll
+    for (int u = 0; u < kTrips; ++u) {
+      const int idx = threadIdx.x + u * kBlock;
+      const int pc = I.p0 + min(idx, last);
+      const int cv = ci[pc];
+      const T aval = (T)av[pc];
+      I.c[u] = cv;
+      I.a[u] = idx < I.nnz ? aval : T(0);
+    }
+  };
+  auto gather = [&](const Tile &I, TX (&xv)[kTrips][NC]) {
+#pragma unroll
+    for (int u = 0; u < kTrips; ++u) {
+      const TX *xr = x + (size_t)I.c[u] * ldx;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) xv[u][j] = xr[j];
+    }
+  };
+  auto stage = [&](const Tile &I, const TX (&xv)[kTrips][NC]) {
+#pragma unroll
+    for (int k = 0; k < kRp; ++k) {
+      const int i = threadIdx.x + k * kBlock;
+      if (i <= I.r1 - I.r0) srp[i] = I.rpv[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kTrips; ++u) {
+      const int idx = threadIdx.x + u * kBlock;
+      if (idx < I.nnz) {
+#pragma unroll
+        for (int j = 0; j < NC; ++j) prod[idx * NC + j] = I.a[u] * (T)xv[u][j];
+      }
+    }
+  };
+  Tile cur, nxt, nn;
+  TX xv[kTrips][NC];
+  load_idx(t0, cur);
+  gather(cur, xv);
+  stage(cur, xv);
+  if (t0 + 1 < t1) load_idx(t0 + 1, nxt);
+  __syncthreads();
+  for (int t = t0; t < t1; ++t) {
+    const bool has1 = t + 1 < t1, has2 = t + 2 < t1;
+    if (has1) gather(nxt, xv);
+    if (has2) load_idx(t + 2, nn);
+    const int nr = cur.r1 - cur.r0;
+    for (int q = threadIdx.x; q < nr * NC; q += kBlock) {
+      const int lr = q / NC, j = q % NC, row = cur.r0 + lr;
+      const EpiIn<T> e_ = epi_load<T>(w, (size_t)row * ldw + j, gamma != T(0), z, (size_t)row * ldz + j,
+                                      beta != T(0), rs, row, cf, (size_t)row * cf.ld + j);
+      T acc = T(0);
+      for (int k = srp[lr]; k < srp[lr + 1]; ++k) acc += prod[k * NC + j];
+      T v = alpha * acc;
+      if (gamma != T(0)) v += gamma * e_.w;
+      v *= e_.s;
+      if (beta != T(0)) v += beta * e_.z;
+      epi_store(y, (size_t)row * ldy + j, v, e_, cf, (size_t)row * cf.ld + j);
+    }
+    __syncthreads();
+    if (has1) {
+      stage(nxt, xv);
+      cur = nxt;
+      nxt = nn;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename TA, typename TX, typename T, int NC, int TILE>
 __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ rp, const int *__restrict__ ci,
                                                         const TA *__restrict__ av, const TX *__restrict__ x, int ldx,
@@ -502,35 +675,45 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ 
   const int nnz = rp[r1] - p0;
   for (int i = threadIdx.x; i <= r1 - r0; i += kBlock) srp[i] = rp[r0 + i] - p0;
   constexpr int kTrips = TILE / kBlock;
+  // unconditional loads from clamped indices (masked at the LDS store), so all of a
+  // thread's loads are in flight together
   int c[kTrips];
   T a[kTrips];
+  const int last = max(nnz - 1, 0);
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
-    const int idx = threadIdx.x + u * kBlock;
-    const bool ok = idx < nnz;
-    c[u] = ok ? ci[p0 + idx] : 0;
-    a[u] = ok ? (T)av[p0 + idx] : T(0);
+    const int pc = p0 + min((int)threadIdx.x + u * kBlock, last);
+    c[u] = ci[pc];
+    a[u] = (T)av[pc];
+  }
+  TX xv[kTrips][NC];
+#pragma unroll
+  for (int u = 0; u < kTrips; ++u) {
+    const TX *xr = x + (size_t)c[u] * ldx;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) xv[u][j] = xr[j];
   }
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
     const int idx = threadIdx.x + u * kBlock;
     if (idx < nnz) {
-      const TX *xr = x + (size_t)c[u] * ldx;
 #pragma unroll
-      for (int j = 0; j < NC; ++j) prod[idx * NC + j] = a[u] * (T)xr[j];
+      for (int j = 0; j < NC; ++j) prod[idx * NC + j] = a[u] * (T)xv[u][j];
     }
   }
   __syncthreads();
   const int nr = r1 - r0;
   for (int q = threadIdx.x; q < nr * NC; q += kBlock) {
     const int lr = q / NC, j = q % NC, row = r0 + lr;
+    const EpiIn<T> e_ = epi_load<T>(w, (size_t)row * ldw + j, gamma != T(0), z, (size_t)row * ldz + j,
+                                    beta != T(0), rs, row, cf, (size_t)row * cf.ld + j);
     T acc = T(0);
     for (int k = srp[lr]; k < srp[lr + 1]; ++k) acc += prod[k * NC + j];
     T v = alpha * acc;
-    if (gamma != T(0)) v += gamma * w[(size_t)row * ldw + j];
-    if (rs) v *= rs[row];
-    if (beta != T(0)) v += beta * z[(size_t)row * ldz + j];
-    store_y(y, (size_t)row * ldy + j, v, cf, (size_t)row * cf.ld + j);
+    if (gamma != T(0)) v += gamma * e_.w;
+    v *= e_.s;
+    if (beta != T(0)) v += beta * e_.z;
+    epi_store(y, (size_t)row * ldy + j, v, e_, cf, (size_t)row * cf.ld + j);
   }
 }
 
@@ -539,6 +722,24 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
                    T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs,
                    const ChebFuse<T> &cf) {
   const double avg = A.nrows ? double(A.nnz) / A.nrows : 0.0;
+  if (A.lpr == -3) {
+    // persistent pipelined row blocks: about 8 workgroups per CU
+    EMF_REQUIRE(A.blk_row != nullptr && A.nblk > 0, kErrArg, "row-block SpMM without its block table");
+    EMF_REQUIRE(xs == nullptr && zs == nullptr, kErrArg, "row-block SpMM has no scaled operands");
+    static int n_cu = 0;
+    if (n_cu == 0) {
+      int dev = 0;
+      EMF_HIP(hipGetDevice(&dev));
+      EMF_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+      n_cu = std::max(1, n_cu);
+    }
+    const int groups = std::min(A.nblk, 8 * n_cu);
+    const int per = (A.nblk + groups - 1) / groups;
+    const int used = (A.nblk + per - 1) / per;
+    hipLaunchKernelGGL((k_spmm_pstream<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(used), dim3(kBlock), 0, s, A.rp,
+                       A.ci, A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, A.blk_row, A.nblk, per, cf);
+    return;
+  }
   if (A.lpr <= -2) {
     // narrow x-blocks with -lpr lanes per row
     EMF_REQUIRE(A.xn_row != nullptr && A.xn_nblk > 0, kErrArg, "x-block SpMM without its block tables");
@@ -663,9 +864,10 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
              T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs,
              const ChebFuse<T> &cf) {
   if (A.nrows == 0 || nc == 0) return;
-  EMF_REQUIRE(A.lpr != -1 || (nc <= 8 && nc != 5 && nc != 7), kErrArg,
+  EMF_REQUIRE((A.lpr != -1 && A.lpr != -3) || (nc <= 8 && nc != 5 && nc != 7), kErrArg,
               "row-block SpMM supports 1, 2, 3, 4, 6 or 8 columns");
-  EMF_REQUIRE(A.lpr >= -1 || nc >= 16 || A.xn_nblk > 0, kErrArg, "narrow x-block SpMM without its tables");
+  EMF_REQUIRE(A.lpr >= -1 || A.lpr == -3 || nc >= 16 || A.xn_nblk > 0, kErrArg,
+              "narrow x-block SpMM without its tables");
   if (nc >= 16 && A.lpr == -2) {
     EMF_REQUIRE(A.xb_row != nullptr && A.xb_nblk > 0 && A.xb_umax <= kXBlockCols, kErrArg,
                 "x-block SpMM without its block tables");
@@ -799,7 +1001,10 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
       }
     if (A.xb_row && A.xb_nblk > 0) cand.push_back(-2);
   } else {
-    if (A.blk_row && A.nblk > 0) cand.push_back(-1);  // LDS-staged row blocks
+    if (A.blk_row && A.nblk > 0) {
+      cand.push_back(-1);  // LDS-staged row blocks
+      cand.push_back(-3);  // the same, persistent and pipelined
+    }
     if (A.xn_row && A.xn_nblk > 0)
       for (int l : {-2, -4, -8}) cand.push_back(l);  // x-blocks, 2 / 4 / 8 lanes per row
   }
@@ -911,8 +1116,10 @@ __global__ __launch_bounds__(kBlock) void k_op_wide(int nrows, const int *__rest
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const bool ok = p0 + u < e;
-      cu[u] = ok ? ci[p0 + u] : ci[b];
-      cm[u] = ok ? K[p0 + u] : make_double2(0.0, 0.0);
+      const int pc = min(p0 + u, e - 1);  // unconditional loads, masked
+      cu[u] = ci[pc];
+      const double2 k_ = K[pc];
+      cm[u] = ok ? k_ : make_double2(0.0, 0.0);
     }
     double xr_[kU][PPL], xi_[kU][PPL];
 #pragma unroll
@@ -1071,9 +1278,9 @@ __device__ __forceinline__ void chunk_rows(const Seg &sg, int rows, int b, int n
 
 // Threads of a block are assigned to columns c = tid % nc and stride over the
 // block's rows; the per-column block sums go to work[block][c].
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_coldot(int rows, int nc, const T *__restrict__ x, int ldx,
-                                                   const T *__restrict__ y, int ldy, double *work, Seg sg) {
+template <typename TX, typename TY>
+__global__ __launch_bounds__(kBlock) void k_coldot(int rows, int nc, const TX *__restrict__ x, int ldx,
+                                                   const TY *__restrict__ y, int ldy, double *work, Seg sg) {
   __shared__ double sh[kBlock];
   const int rpp = kBlock / nc;
   const int per_block = rpp * nc;
@@ -1136,24 +1343,26 @@ size_t dot_work_size(int rows, int nc, const Seg &sg) {
   return (size_t)std::max(kDotBlocks, sg.nchunks) * std::max(nc, 1) + 64;
 }
 
-template <typename T>
-void coldot(hipStream_t s, int rows, int nc, const T *x, int ldx, const T *y, int ldy, double *out, double *work,
+template <typename TX, typename TY>
+void coldot(hipStream_t s, int rows, int nc, const TX *x, int ldx, const TY *y, int ldy, double *out, double *work,
             const Seg &sg) {
   if (nc > kBlock) {
     EMF_REQUIRE(sg.nseg == 1, kErrArg, "segmented reductions support at most 256 columns");
     for (int c0 = 0; c0 < nc; c0 += kBlock)
-      coldot<T>(s, rows, std::min(kBlock, nc - c0), x + c0, ldx, y + c0, ldy, out + c0, work, sg);
+      coldot<TX, TY>(s, rows, std::min(kBlock, nc - c0), x + c0, ldx, y + c0, ldy, out + c0, work, sg);
     return;
   }
   const int nb = reduce_blocks(rows, nc, sg);
-  hipLaunchKernelGGL((k_coldot<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, x, ldx, y, ldy, work, sg);
+  hipLaunchKernelGGL((k_coldot<TX, TY>), dim3(nb), dim3(kBlock), 0, s, rows, nc, x, ldx, y, ldy, work, sg);
   hipLaunchKernelGGL(k_finish, dim3(sg.nseg * nc), dim3(kBlock), 0, s, nb, nc, work, out, sg);
   EMF_HIP(hipGetLastError());
 }
-template void coldot<float>(hipStream_t, int, int, const float *, int, const float *, int, double *, double *,
-                            const Seg &);
-template void coldot<double>(hipStream_t, int, int, const double *, int, const double *, int, double *, double *,
-                             const Seg &);
+template void coldot<float, float>(hipStream_t, int, int, const float *, int, const float *, int, double *, double *,
+                                   const Seg &);
+template void coldot<double, double>(hipStream_t, int, int, const double *, int, const double *, int, double *,
+                                     double *, const Seg &);
+template void coldot<double, float>(hipStream_t, int, int, const double *, int, const float *, int, double *,
+                                    double *, const Seg &);
 
 namespace {
 
@@ -1260,9 +1469,9 @@ __global__ void k_axpby(int rows, int nc, double alpha, const TX *__restrict__ x
   y[(size_t)r * ldy + c] = (TY)v;
 }
 
-template <typename T>
+template <typename T, typename TP>
 __global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, const double *__restrict__ alpha,
-                                                   const T *__restrict__ p, const T *__restrict__ q, T *x, T *r,
+                                                   const TP *__restrict__ p, const TP *__restrict__ q, T *x, T *r,
                                                    double *work, Seg sg) {
   __shared__ double sh[kBlock];
   const int rpp = kBlock / nc;
@@ -1307,7 +1516,7 @@ template <typename T>
 __global__ void k_cheb_step(int64_t n, T *d, const T *__restrict__ rres, T *x, double c1, double c2) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const T dv = (T)(c1 * (double)d[i] + c2 * (double)rres[i]);
+  const T dv = T(c1) * d[i] + T(c2) * rres[i];
   d[i] = dv;
   x[i] += dv;
 }
@@ -1390,19 +1599,21 @@ template void axpby<double, double>(hipStream_t, int, int, double, const double 
 template void axpby<double, float>(hipStream_t, int, int, double, const double *, int, double, float *, int);
 template void axpby<float, double>(hipStream_t, int, int, double, const float *, int, double, double *, int);
 
-template <typename T>
-void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const T *p, const T *q, T *x, T *r, double *rr,
+template <typename T, typename TP>
+void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const TP *p, const TP *q, T *x, T *r, double *rr,
             double *work, const Seg &sg) {
   EMF_REQUIRE(nc <= kBlock, kErrArg, "inner solve supports at most 256 columns");
   const int nb = reduce_blocks(rows, nc, sg);
-  hipLaunchKernelGGL((k_fcg_xr<T>), dim3(nb), dim3(kBlock), 0, s, rows, nc, alpha, p, q, x, r, work, sg);
+  hipLaunchKernelGGL((k_fcg_xr<T, TP>), dim3(nb), dim3(kBlock), 0, s, rows, nc, alpha, p, q, x, r, work, sg);
   hipLaunchKernelGGL(k_finish, dim3(sg.nseg * nc), dim3(kBlock), 0, s, nb, nc, work, rr, sg);
   EMF_HIP(hipGetLastError());
 }
-template void fcg_xr<float>(hipStream_t, int, int, const double *, const float *, const float *, float *, float *,
-                            double *, double *, const Seg &);
-template void fcg_xr<double>(hipStream_t, int, int, const double *, const double *, const double *, double *, double *,
-                             double *, double *, const Seg &);
+template void fcg_xr<float, float>(hipStream_t, int, int, const double *, const float *, const float *, float *,
+                                   float *, double *, double *, const Seg &);
+template void fcg_xr<double, double>(hipStream_t, int, int, const double *, const double *, const double *, double *,
+                                     double *, double *, double *, const Seg &);
+template void fcg_xr<double, float>(hipStream_t, int, int, const double *, const float *, const float *, double *,
+                                    double *, double *, double *, const Seg &);
 
 template <typename T>
 void cheb_init(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *rres, T *d, T *x, double inv_theta) {

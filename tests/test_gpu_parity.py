@@ -274,11 +274,12 @@ def test_device_point_location_matches_oracle(oracle_mod):
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_row_block_spmm_matches_lane_spmm(precision, oracle_mod):
-    """The LDS-staged row-block smoother SpMM, the LDS x-block one and the
-    lanes-per-row one give the same solve (all to the exact discrete solution)."""
+    """The LDS-staged row-block smoother SpMM (plain and persistent-pipelined), the LDS
+    x-block one and the lanes-per-row one give the same solve (all to the exact
+    discrete solution)."""
     m = mg.halfspace_mt(n=4, n_rx=3)
     sols = {}
-    for rows in ("lanes", "rowblock", "xblock"):
+    for rows in ("lanes", "rowblock", "pipelined", "xblock"):
         ctx = _ctx(m, "-inner_precision %s -spmm_rows %s" % (precision, rows))
         ctx.create_linear_system(2)
         ctx.assemble_matrix(0)
