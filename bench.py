@@ -272,7 +272,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=48, help="grid cells per axis")
+    ap.add_argument("--cells", type=int, default=60, help="grid cells per axis")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain fp64 single-RHS comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-csem", action="store_true", help="skip the batched marine-CSEM comparison")
@@ -412,7 +412,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (outer FGMRES, assembly, estimator) + f32 (inner FCG/AMS)",
+        "dtype": "f64 (outer FGMRES, inner FCG, assembly, estimator) + f32 (AMS cycle)",
         "data": "synthetic",
         "config": {"workload": "adaptive-MT model, final-cycle mesh (%d^3 cells, %d edges, %d tets), "
                                "3 RHS (XY, YX, dual), cycling over 8 frequencies 1e-3..100 Hz"

@@ -67,23 +67,6 @@ int stream_tile_for(int nc, bool fp64);
 // than the tile).
 std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
 
-// Picks the fastest row mapping of the narrow SpMM for A on nc columns by timing
-// each with HIP events on stream s: lanes per row 2, 4, 8, 16, and -1 (LDS-staged
-// row blocks) when A carries a block table for nc; for 16 or more columns the slab
-// width of the wide kernel (16 .. 256) and, when A carries x-block tables, -2 and -3
-// (LDS x-blocks).  x and y are scratch blocks of [A.ncols][nc] and [A.nrows][nc].
-template <typename TA, typename TX, typename T>
-int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y);
-
-// y = rs .* (alpha * (A x) + gamma * w) + beta * z   (rows of A; rs may be null = 1,
-// w / z may be null when their coefficient is 0).  y may alias z (same row only) but
-// must not alias x.
-// Matrix values TA, input TX, output and accumulation T; instantiated for
-// (f32,f32,f32), (f64,f64,f64), (f64,f32,f64) and (f32,f64,f32).
-template <typename TA, typename TX, typename T>
-void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
-                T gamma, const T *w, int ldw, T beta, const T *z, int ldz);
-
 // A Chebyshev step fused into an SpMM epilogue (off when dn is null): with every
 // output entry y(row, j) also dn(row, j) = c1 d(row, j) + c2 y(row, j) (in T with the
 // coefficients rounded to T, as cheb_step) and x(row, j) += dn(row, j); d, dn and x are
@@ -95,6 +78,25 @@ struct ChebFuse {
   int ld = 0;
   double c1 = 0.0, c2 = 0.0;
 };
+
+// Picks the fastest row mapping of the narrow SpMM for A on nc columns by timing
+// each with HIP events on stream s: lanes per row 2, 4, 8, 16, and -1 (LDS-staged
+// row blocks) when A carries a block table for nc; for 16 or more columns the slab
+// width of the wide kernel (16 .. 256) and, when A carries x-block tables, -2 and -3
+// (LDS x-blocks).  x and y are scratch blocks of [A.ncols][nc] and [A.nrows][nc].
+template <typename TA, typename TX, typename T>
+int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y, const T *rs = nullptr,
+             const ChebFuse<T> &cf = ChebFuse<T>());
+
+// y = rs .* (alpha * (A x) + gamma * w) + beta * z   (rows of A; rs may be null = 1,
+// w / z may be null when their coefficient is 0).  y may alias z (same row only) but
+// must not alias x.
+// Matrix values TA, input TX, output and accumulation T; instantiated for
+// (f32,f32,f32), (f64,f64,f64), (f64,f32,f64) and (f32,f64,f32).
+template <typename TA, typename TX, typename T>
+void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
+                T gamma, const T *w, int ldw, T beta, const T *z, int ldz);
+
 
 // spmm_mixed with two optional per-row scalings: the input is read as
 // xs[col] * x[col][:] (the operator A diag(xs)) and z as zs[row] * z[row][:]; either

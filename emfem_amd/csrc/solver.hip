@@ -282,7 +282,24 @@ class DevAms {
     else if (spmm_rows_ == 2 && !blk.empty()) lpr_[nc] = -1;
     else if (spmm_rows_ == 4 && !blk.empty()) lpr_[nc] = -3;
     else if (spmm_rows_ == 3 && nc < 16 && B_.xn_nblk > 0) lpr_[nc] = -4;  // forced x-blocks
-    else lpr_[nc] = tune_lpr<T, T, T>(s, B, nc, d_.get(), rres_.get());
+    else {
+      // timed in the cycle's own form: the l1-scaled residual update with, below 16
+      // columns (or with the 16-byte-gather kernel), the fused Chebyshev step
+      DevBuf<T> xs((size_t)ne_ * nc);
+      xs.zero(s);
+      rres_.zero(s);
+      ChebFuse<T> cf;
+      if (nc < 16) {
+        cf.d = d_.get();
+        cf.dn = d2_.get();
+        cf.x = xs.get();
+        cf.ld = nc;
+        cf.c1 = 0.5;
+        cf.c2 = 0.5;
+      }
+      lpr_[nc] = tune_lpr<T, T, T>(s, B, nc, d_.get(), rres_.get(), l1inv_.get(), cf);
+      EMF_HIP(hipStreamSynchronize(s));
+    }
   }
 
   const std::vector<int> &host_ci() {

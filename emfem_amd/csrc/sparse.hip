@@ -976,7 +976,7 @@ void spmm_mixed(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T
 }
 
 template <typename TA, typename TX, typename T>
-int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
+int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y, const T *rs, const ChebFuse<T> &cf) {
   if (A.nrows == 0) return 0;
   hipEvent_t e0, e1;
   EMF_HIP(hipEventCreate(&e0));
@@ -1001,20 +1001,26 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
       }
     if (A.xb_row && A.xb_nblk > 0) cand.push_back(-2);
   } else {
-    if (A.blk_row && A.nblk > 0) {
-      cand.push_back(-1);  // LDS-staged row blocks
-      cand.push_back(-3);  // the same, persistent and pipelined
-    }
+    // LDS-staged row blocks (the persistent pipelined variant, -3, only on request: it
+    // can win an isolated timing yet move more bytes and lose inside the solve)
+    if (A.blk_row && A.nblk > 0) cand.push_back(-1);
     if (A.xn_row && A.xn_nblk > 0)
       for (int l : {-2, -4, -8}) cand.push_back(l);  // x-blocks, 2 / 4 / 8 lanes per row
   }
   for (int lpr : cand) {
     Csr<TA> B = A;
     B.lpr = lpr;
-    spmm_mixed<TA, TX, T>(s, B, nc, x, nc, y, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+    // the form the caller runs: y = rs .* (-A x) + y with the optional fused step when
+    // rs is given, else y = A x
+    auto run = [&]() {
+      if (rs)
+        spmm_ex<TA, TX, T>(s, B, nc, x, nc, y, nc, T(-1), rs, T(0), nullptr, 0, T(1), y, nc, nullptr, nullptr, cf);
+      else
+        spmm_mixed<TA, TX, T>(s, B, nc, x, nc, y, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+    };
+    run();
     EMF_HIP(hipEventRecord(e0, s));
-    for (int r = 0; r < 5; ++r)
-      spmm_mixed<TA, TX, T>(s, B, nc, x, nc, y, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
+    for (int r = 0; r < 5; ++r) run();
     EMF_HIP(hipEventRecord(e1, s));
     EMF_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1038,7 +1044,8 @@ int tune_lpr(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, T *y) {
   template void spmm_ex<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, int, T *, int, T, const T *, T,    \
                                    const T *, int, T, const T *, int, const T *, const T *,                         \
                                    const ChebFuse<T> &);                                                            \
-  template int tune_lpr<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, T *);
+  template int tune_lpr<TA, TX, T>(hipStream_t, const Csr<TA> &, int, const TX *, T *, const T *,                   \
+                                   const ChebFuse<T> &);
 EMF_SPMM_INST(float, float, float)
 EMF_SPMM_INST(double, double, double)
 EMF_SPMM_INST(double, float, double)
