@@ -221,6 +221,12 @@ class DevAms {
     Csr<double> B = B64_;
     auto it = lpr64_.find(nc);
     if (it != lpr64_.end()) B.lpr = it->second;
+    if (B.lpr == -1) {
+      // fp64 row-block tables for this column count
+      const DevBuf<int> &t = blk64_.at(nc);
+      B.blk_row = t.get();
+      B.nblk = (int)t.size() - 1;
+    }
     return B;
   }
 
@@ -272,10 +278,19 @@ class DevAms {
     }
     {
       // the fp64 B (fp64 residuals of the cycle, the FCG operator of the inner solve):
-      // its own lanes per row (slab width at 16 or more columns)
+      // its own lanes per row, row blocks (fp64 tiles) or slab width
       DevBuf<double> tmp((size_t)ne_ * nc);
       r64_.zero(s);
-      lpr64_[nc] = tune_lpr<double, double, double>(s, B64_, nc, r64_.get(), tmp.get());
+      Csr<double> B64t = B64_;
+      if (nc < 16 && spmm_rows_ != 1 && spmm_rows_ < 10) {
+        const std::vector<int> b64 = row_blocks(hrp_, stream_tile_for(nc, true));
+        if (!b64.empty()) {
+          blk64_[nc].upload(b64, s);
+          B64t.blk_row = blk64_[nc].get();
+          B64t.nblk = (int)b64.size() - 1;
+        }
+      }
+      lpr64_[nc] = tune_lpr<double, double, double>(s, B64t, nc, r64_.get(), tmp.get());
       EMF_HIP(hipStreamSynchronize(s));
     }
     if (spmm_rows_ >= 10 && nc < 16) lpr_[nc] = spmm_rows_ - 10;  // fixed lanes per row
@@ -565,7 +580,7 @@ class DevAms {
   DevAmg<T> amgG_, amgPi_;
   std::map<std::tuple<int, const void *, const void *>, hipGraphExec_t> graphs_;
   std::map<int, int> lpr_, lpr64_;
-  std::map<int, DevBuf<int>> blk_;
+  std::map<int, DevBuf<int>> blk_, blk64_;
   bool xb_tried_ = false, xn_tried_ = false;
   std::vector<int> hci_;  // host column indices of B (x-block tables)
   DevBuf<int> xn_row_, xn_uptr_, xn_ucol_, xn_lci_;
