@@ -142,29 +142,37 @@ HostCsr csr_triple(const HostCsr &R, const HostCsr &A, const HostCsr &P) {
       }
     }
   };
-  parallel_ranges(R.n, [&](int64_t b, int64_t e, int) {
-    std::vector<double> acc(P.m);
-    std::vector<int> mark(P.m, -1), cols;
-    for (int64_t i = b; i < e; ++i) {
-      row(i, acc, mark, cols);
-      C.rp[i + 1] = (int)cols.size();
-    }
-  });
+  // one pass: each thread keeps its rows (sorted) in its own buffers, which are then
+  // copied to their offsets; both passes use the same row partition
+  const int nth = host_threads();
+  std::vector<std::vector<int>> tci(nth);
+  std::vector<std::vector<double>> tv(nth);
+  parallel_ranges(
+      R.n,
+      [&](int64_t b, int64_t e, int t) {
+        std::vector<double> acc(P.m);
+        std::vector<int> mark(P.m, -1), cols;
+        for (int64_t i = b; i < e; ++i) {
+          row(i, acc, mark, cols);
+          std::sort(cols.begin(), cols.end());
+          C.rp[i + 1] = (int)cols.size();
+          for (const int c : cols) {
+            tci[t].push_back(c);
+            tv[t].push_back(acc[c]);
+          }
+        }
+      },
+      nth);
   for (int i = 0; i < R.n; ++i) C.rp[i + 1] += C.rp[i];
   C.ci.resize(C.rp[R.n]);
   C.v.resize(C.rp[R.n]);
-  parallel_ranges(R.n, [&](int64_t b, int64_t e, int) {
-    std::vector<double> acc(P.m);
-    std::vector<int> mark(P.m, -1), cols;
-    for (int64_t i = b; i < e; ++i) {
-      row(i, acc, mark, cols);
-      std::sort(cols.begin(), cols.end());
-      for (size_t t = 0; t < cols.size(); ++t) {
-        C.ci[C.rp[i] + t] = cols[t];
-        C.v[C.rp[i] + t] = acc[cols[t]];
-      }
-    }
-  });
+  parallel_ranges(
+      R.n,
+      [&](int64_t b, int64_t, int t) {
+        std::copy(tci[t].begin(), tci[t].end(), C.ci.begin() + C.rp[b]);
+        std::copy(tv[t].begin(), tv[t].end(), C.v.begin() + C.rp[b]);
+      },
+      nth);
   return C;
 }
 
@@ -172,9 +180,11 @@ namespace {
 
 std::vector<double> diagonal(const HostCsr &A) {
   std::vector<double> d(A.n, 0.0);
-  for (int r = 0; r < A.n; ++r)
-    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p)
-      if (A.ci[p] == r) d[r] = A.v[p];
+  parallel_ranges(A.n, [&](int64_t b, int64_t e, int) {
+    for (int64_t r = b; r < e; ++r)
+      for (int p = A.rp[r]; p < A.rp[r + 1]; ++p)
+        if (A.ci[p] == r) d[r] = A.v[p];
+  });
   return d;
 }
 
@@ -195,22 +205,30 @@ std::vector<double> l1_inverse(const HostCsr &A) {
 // their own.
 int aggregate(const HostCsr &A, const std::vector<double> &d, double theta, std::vector<int> &agg) {
   const int n = A.n;
-  // strength graph: flags per nonzero in parallel, then compaction in row order
+  // strength graph: flags and row counts per nonzero in parallel, then compaction in
+  // row order (also in parallel, at the prefix-summed offsets)
   std::vector<uint8_t> strong(A.ci.size());
+  std::vector<int> sp(n + 1, 0), sc;
   parallel_ranges(n, [&](int64_t b, int64_t e, int) {
-    for (int64_t r = b; r < e; ++r)
+    for (int64_t r = b; r < e; ++r) {
+      int cnt = 0;
       for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
         const int c = A.ci[p];
         strong[p] = c != r && std::abs(A.v[p]) >= theta * std::sqrt(std::abs(d[r] * d[c]));
+        cnt += strong[p];
       }
+      sp[r + 1] = cnt;
+    }
   });
-  std::vector<int> sp(n + 1, 0), sc;
-  sc.reserve(A.ci.size());
-  for (int r = 0; r < n; ++r) {
-    for (int p = A.rp[r]; p < A.rp[r + 1]; ++p)
-      if (strong[p]) sc.push_back(A.ci[p]);
-    sp[r + 1] = (int)sc.size();
-  }
+  for (int r = 0; r < n; ++r) sp[r + 1] += sp[r];
+  sc.resize(sp[n]);
+  parallel_ranges(n, [&](int64_t b, int64_t e, int) {
+    for (int64_t r = b; r < e; ++r) {
+      int o = sp[r];
+      for (int p = A.rp[r]; p < A.rp[r + 1]; ++p)
+        if (strong[p]) sc[o++] = A.ci[p];
+    }
+  });
   agg.assign(n, -1);
   int na = 0;
   for (int i = 0; i < n; ++i) {
@@ -534,18 +552,18 @@ std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive) {
 
 // The four Galerkin products P_k^T B P_k (k = G0, Pi_x, Pi_y, Pi_z) in one pass over
 // B: they share the pattern Apat, so each row u accumulates the four value sets at
-// positions found through a dense column map.  Each value is summed in the order of
-// csr_triple (rows of P_k^T, then of B, then of P_k).
-void galerkin_blocks(const AmsAux &X, const HostCsr &B, std::vector<double> (&vals)[4]) {
+// positions found through a dense column map, into vals[k] (Apat layout; each thread
+// zeroes its own rows first, so the pages are first touched in parallel).  Each value
+// is summed in the order of csr_triple (rows of P_k^T, then of B, then of P_k).
+void galerkin_blocks(const AmsAux &X, const HostCsr &B, double *const vals[4]) {
   const HostCsr &A = X.Apat;
-  const int64_t z = A.rp[A.n];
   const HostCsr *R[4] = {&X.G0T, &X.PT[0], &X.PT[1], &X.PT[2]};
   const HostCsr *P[4] = {&X.G0, &X.P[0], &X.P[1], &X.P[2]};
-  for (int k = 0; k < 4; ++k) vals[k].assign(z, 0.0);
   EMF_REQUIRE(B.n == X.G0.n && A.n == X.nv0, kErrArg, "Galerkin blocks: B does not match the auxiliary spaces");
   std::atomic<int64_t> missing{0};
   parallel_ranges(A.n, [&](int64_t b, int64_t e, int) {
     std::vector<int> pos(A.m, -1);
+    for (int k = 0; k < 4; ++k) std::fill(vals[k] + A.rp[b], vals[k] + A.rp[e], 0.0);
     for (int64_t u = b; u < e; ++u) {
       const int r0 = A.rp[u];
       for (int t = r0; t < A.rp[u + 1]; ++t) pos[A.ci[t]] = t;
@@ -573,6 +591,16 @@ void galerkin_blocks(const AmsAux &X, const HostCsr &B, std::vector<double> (&va
   EMF_REQUIRE(missing.load() == 0, kErrArg, "Galerkin blocks: B has couplings outside the tet pattern");
 }
 
+void galerkin_blocks(const AmsAux &X, const HostCsr &B, std::vector<double> (&vals)[4]) {
+  const int64_t z = X.Apat.rp[X.Apat.n];
+  double *out[4];
+  for (int k = 0; k < 4; ++k) {
+    vals[k].resize(z);
+    out[k] = vals[k].data();
+  }
+  galerkin_blocks(X, B, out);
+}
+
 void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, AmsHost &H,
                std::shared_ptr<const AmsAux> aux) {
   AmgParams prm = prm_in;
@@ -582,21 +610,35 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, Ams
   H.aux = aux;
   const int nv0 = aux->nv0;
   H.timing["aux_build_ms"] += now_ms() - t0;
-  // Galerkin products of the auxiliary blocks (one fused pass, shared pattern)
+  // Galerkin products of the auxiliary blocks (one fused pass, shared pattern), written
+  // straight into the block-diagonal stack; a recycled stack from an earlier set-up on
+  // this mesh keeps its pages
   t0 = now_ms();
   (void)nv0;
   const bool additive = aux->additive;
-  std::vector<double> vals[4];
-  galerkin_blocks(*aux, B, vals);
-  HostCsr AP = aux->APpat;
   const int64_t z = aux->Apat.rp[aux->Apat.n];
+  HostCsr AP = std::move(H.recycle);
+  AP.n = aux->APpat.n;
+  AP.m = aux->APpat.m;
+  AP.rp.assign(aux->APpat.rp.begin(), aux->APpat.rp.end());
+  AP.ci.assign(aux->APpat.ci.begin(), aux->APpat.ci.end());
   AP.v.resize(AP.ci.size());
-  for (int k = additive ? 0 : 1, d = 0; k < 4; ++k, ++d) std::copy(vals[k].begin(), vals[k].end(), AP.v.begin() + d * z);
+  std::vector<double> vg;
+  double *out[4];
+  for (int k = 0; k < 4; ++k) {
+    const int d = additive ? k : k - 1;
+    if (d >= 0) out[k] = AP.v.data() + d * z;
+  }
+  if (!additive) {
+    vg.resize(z);
+    out[0] = vg.data();
+  }
+  galerkin_blocks(*aux, B, out);
   H.timing["galerkin_ms"] += now_ms() - t0;
   if (!additive) {
     t0 = now_ms();
     HostCsr AG = aux->Apat;
-    AG.v = std::move(vals[0]);
+    AG.v = std::move(vg);
     build_amg(std::move(AG), prm, H.amgG);
     H.timing["amg_ms"] += now_ms() - t0;
   }

@@ -80,11 +80,15 @@ struct AmsAux {
 
 std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive);
 
-// vals[k] = values of P_k^T B P_k on X.Apat, k = G0, Pi_x, Pi_y, Pi_z.
+// vals[k] = values of P_k^T B P_k on X.Apat, k = G0, Pi_x, Pi_y, Pi_z (the first form
+// writes into caller arrays of Apat's nonzero count).
+void galerkin_blocks(const AmsAux &X, const HostCsr &B, double *const vals[4]);
 void galerkin_blocks(const AmsAux &X, const HostCsr &B, std::vector<double> (&vals)[4]);
 
 struct AmsHost {
   std::shared_ptr<const AmsAux> aux;
+  // storage of an earlier set-up's block-diagonal Galerkin stack, reused by build_ams
+  HostCsr recycle;
   AmgHost amgG;   // AMG of G0^T B G0 (multiplicative cycle only)
   AmgHost amgPi;  // AMG of diag(P_k^T B P_k)
   std::vector<double> l1inv;  // edge-space l1 row norms of B, inverted

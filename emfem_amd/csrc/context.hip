@@ -481,7 +481,10 @@ int emfem_create_pc(emfem_context *c) {
     }
     PcOptions o = pc_options(c);
     c->pc.reset();
+    HostCsr recycle;
+    if (c->ams_host && !c->ams_host->amgPi.levels.empty()) recycle = std::move(c->ams_host->amgPi.levels[0].A);
     c->ams_host.reset(new AmsHost());
+    c->ams_host->recycle = std::move(recycle);
     const double tb = now_ms();
     build_ams(c->T, B, o.amg, *c->ams_host, c->ams_aux);
     c->ams_aux = c->ams_host->aux;
