@@ -334,7 +334,7 @@ def main():
     peak = 8000.0
     bw = (b_bytes / (b_ms * 1e-3) / 1e9) if b_ms > 0 else 0.0
     rows = int(ctx.stat("b_spmm_rows"))
-    kname = ("%s: Chebyshev step rres -= Dinv .* (B d) with d' = c1 d + c2 rres, x += d' in its epilogue, "
+    kname = ("%s: Chebyshev step x' = x + c1 (x - x_prev) + c2 Dinv .* (b - B x) in one pass, "
              "B = C + M fp32, XY+YX+dual = 6 real columns"
              % ("k_spmm_stream (LDS-staged row blocks)" if rows == -1 else
                 "k_spmm_pstream (persistent pipelined LDS-staged row blocks)" if rows == -3 else

@@ -67,16 +67,24 @@ int stream_tile_for(int nc, bool fp64);
 // than the tile).
 std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
 
-// A Chebyshev step fused into an SpMM epilogue (off when dn is null): with every
-// output entry y(row, j) also dn(row, j) = c1 d(row, j) + c2 y(row, j) (in T with the
-// coefficients rounded to T, as cheb_step) and x(row, j) += dn(row, j); d, dn and x are
-// [rows][ld].  d may be the SpMM's input block; dn must not alias it.
+// A Chebyshev step fused into an SpMM epilogue, in one of two forms (off when dn is
+// null and xform is false):
+//  - residual form (dn set): with every output entry y(row, j) also dn(row, j) =
+//    c1 d(row, j) + c2 y(row, j) (in T with the coefficients rounded to T, as
+//    cheb_step) and x(row, j) += dn(row, j).  d may be the SpMM's input block; dn must
+//    not alias it.
+//  - iterate form (xform): the SpMM's value v is not stored; y(row, j) receives
+//    x + c1 (x - d) + c2 v at (row, j) instead (d null = 0).  x is normally the SpMM's
+//    input block and d the output block (each read only at its own row).
+// d, dn and x are [rows][ld].
 template <typename T>
 struct ChebFuse {
   const T *d = nullptr;
   T *dn = nullptr, *x = nullptr;
   int ld = 0;
   double c1 = 0.0, c2 = 0.0;
+  bool xform = false;
+  __host__ __device__ bool on() const { return dn != nullptr || xform; }
 };
 
 // Picks the fastest row mapping of the narrow SpMM for A on nc columns by timing
@@ -188,6 +196,9 @@ template <typename T>
 void cheb_init(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *rres, T *d, T *x, double inv_theta);
 template <typename T>
 void cheb_step(hipStream_t s, int rows, int nc, T *d, const T *rres, T *x, double c1, double c2);
+// x = inv_theta * dinv[row] .* b  (the first Chebyshev iterate from zero)
+template <typename T>
+void cheb_first(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x, double inv_theta);
 
 // x = Dinv .* b (one l1-Jacobi sweep from a zero guess)
 template <typename T>

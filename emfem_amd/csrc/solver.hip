@@ -298,16 +298,16 @@ class DevAms {
     else if (spmm_rows_ == 4 && !blk.empty()) lpr_[nc] = -3;
     else if (spmm_rows_ == 3 && nc < 16 && B_.xn_nblk > 0) lpr_[nc] = -4;  // forced x-blocks
     else {
-      // timed in the cycle's own form: the l1-scaled residual update with, below 16
-      // columns (or with the 16-byte-gather kernel), the fused Chebyshev step
-      DevBuf<T> xs((size_t)ne_ * nc);
-      xs.zero(s);
+      // timed in the cycle's own form: below 16 columns the iterate-form Chebyshev
+      // step (gather x_k, one right-hand-side block, x_{k-1} and x_k at the own row,
+      // x_{k+1} over x_{k-1}), else the l1-scaled residual update
+      d_.zero(s);
       rres_.zero(s);
       ChebFuse<T> cf;
       if (nc < 16) {
-        cf.d = d_.get();
-        cf.dn = d2_.get();
-        cf.x = xs.get();
+        cf.xform = true;
+        cf.x = d_.get();
+        cf.d = rres_.get();
         cf.ld = nc;
         cf.c1 = 0.5;
         cf.c2 = 0.5;
@@ -428,27 +428,30 @@ class DevAms {
     EMF_HIP(hipGraphLaunch(it->second, s));
   }
 
-  // ms per launch of the smoother SpMM in its Chebyshev form, rres -= Dinv .* (B d),
-  // on nc columns (lanes per row as tuned), on the cycle's own buffers
-  // (where fused(nc), the fused Chebyshev form of the cycle's iterations: also
-  // d' = c1 d + c2 rres and x += d', here into scratch blocks)
+  // ms per launch of the smoother SpMM in the cycle's form on nc columns (lanes per
+  // row as tuned), on the cycle's own buffers: where fused(nc) the iterate-form
+  // Chebyshev step x_{k+1} = x_k + c1 (x_k - x_{k-1}) + c2 Dinv .* (w - B x_k) (x_k in
+  // d, x_{k-1} and x_{k+1} in d2, w = rres), else the residual update
+  // rres -= Dinv .* (B d)
   double time_b(hipStream_t s, int nc, int reps) {
     tune(s, nc);
-    DevBuf<T> xs((size_t)ne_ * nc);
-    xs.zero(s);
     d_.zero(s);
+    d2_.zero(s);
     rres_.zero(s);
     ChebFuse<T> cf;
     if (fused(nc)) {
-      cf.d = d_.get();
-      cf.dn = d2_.get();
-      cf.x = xs.get();
+      cf.xform = true;
+      cf.x = d_.get();
+      cf.d = d2_.get();
       cf.ld = nc;
       cf.c1 = 0.5;
       cf.c2 = 0.5;
     }
     const double ms = time_launches(s, reps, [&]() {
-      bspmm(s, nc, d_.get(), rres_.get(), T(-1), l1inv_.get(), T(0), nullptr, T(1), rres_.get(), cf);
+      if (fused(nc))
+        bspmm(s, nc, d_.get(), d2_.get(), T(-1), l1inv_.get(), T(1), rres_.get(), T(0), nullptr, cf);
+      else
+        bspmm(s, nc, d_.get(), rres_.get(), T(-1), l1inv_.get(), T(0), nullptr, T(1), rres_.get(), cf);
     });
     EMF_HIP(hipStreamSynchronize(s));
     return ms;
@@ -463,6 +466,33 @@ class DevAms {
     } else {
       axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, bt_.get(), nc);
       bt = bt_.get();
+    }
+    if (fused(nc)) {
+      // iterate-form Chebyshev in two blocks, d and out: pre-smoothing from zero starts
+      // in d; the buffers are chosen so that post-smoothing (deg writes, alternating)
+      // ends in out: the middle iterate lives in d for odd deg, in out for even deg
+      T *mid = (deg_ & 1) ? d : out, *other = (deg_ & 1) ? out : d;
+      cheb_first<T>(s, ne_, nc, l1inv_.get(), bt, d, 1.0 / theta_);
+      T *xm = cheb_x(s, nc, bt, out, d, true);
+      EMF_REQUIRE(xm == mid, kErrState, "Chebyshev buffer plan");
+      if (!additive_) aux_gradient(s, nc, b, mid);
+      residual64(s, nc, b, mid);
+      spmm_mixed<T, double, T>(s, PiT_.view(), nc, r64_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0,
+                               T(0), nullptr, 0);
+      amgPi_.vcycle(s, nc, vb_.get(), vx_.get());
+      spmm<T>(s, Pi_.view(), nc, vx_.get(), nc, mid, nc, T(1), nullptr, T(0), nullptr, 0, T(1), mid, nc);
+      if (!additive_) aux_gradient(s, nc, b, mid);
+      // post-smoothing from the current iterate: x_1 = x_0 + Dinv .* (bt - B x_0) / theta
+      ChebFuse<T> cf;
+      cf.xform = true;
+      cf.x = mid;
+      cf.ld = nc;
+      cf.c1 = 0.0;
+      cf.c2 = 1.0 / theta_;
+      bspmm(s, nc, mid, other, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr, cf);
+      xm = cheb_x(s, nc, bt, mid, other, false);
+      EMF_REQUIRE(xm == out, kErrState, "Chebyshev buffer plan");
+      return;
     }
     // pre-smoothing (Chebyshev from zero)
     cheb_init<T>(s, ne_, nc, l1inv_.get(), bt, rres, d, out, 1.0 / theta_);
@@ -487,7 +517,7 @@ class DevAms {
              const T *z, const ChebFuse<T> &cf = ChebFuse<T>()) {
     // in-cycle timing (plain launches only): HIP events around the first launches of
     // the fused Chebyshev form
-    const bool timed = clock_on_ && (cf.dn != nullptr) == fused(nc) && clock_.size() < 2 * kClockPairs;
+    const bool timed = clock_on_ && cf.on() == fused(nc) && clock_.size() < 2 * kClockPairs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) {
       EMF_HIP(hipEventCreate(&e0));
@@ -530,35 +560,39 @@ class DevAms {
     spmm_mixed<double, T, double>(s, B64(nc), nc, x, nc, r64_.get(), nc, -1.0, nullptr, 1.0, b, nc, 0.0, nullptr, 0);
   }
 
-  // The Chebyshev steps: rres -= l1inv .* (B d), then d = c1 d + c2 rres and x += d,
-  // both in one SpMM pass where fused(nc), d alternating between d and a second block
-  // (the SpMM gathers d, so the new d cannot overwrite it; d's last value is not used
-  // afterwards).  Otherwise the separate streaming update.
+  // The Chebyshev steps in the residual form (cycles whose smoother SpMM does not take
+  // the fused step): rres -= l1inv .* (B d), then the streaming update d = c1 d + c2 rres
+  // and x += d.
   void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x) {
     double sigma = theta_ / delta_, rho = 1.0 / sigma;
-    if (!fused(nc)) {
-      for (int k = 1; k < deg_; ++k) {
-        bspmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres);
-        const double rho_new = 1.0 / (2.0 * sigma - rho);
-        cheb_step<T>(s, ne_, nc, d, rres, x, rho_new * rho, 2.0 * rho_new / delta_);
-        rho = rho_new;
-      }
-      return;
+    for (int k = 1; k < deg_; ++k) {
+      bspmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres);
+      const double rho_new = 1.0 / (2.0 * sigma - rho);
+      cheb_step<T>(s, ne_, nc, d, rres, x, rho_new * rho, 2.0 * rho_new / delta_);
+      rho = rho_new;
     }
-    T *dc = d, *dn = d2_.get();
+  }
+
+  // Chebyshev steps k = 1 .. deg-1 in the iterate form, one SpMM pass each:
+  // x_{k+1} = x_k + c1 (x_k - x_{k-1}) + c2 Dinv .* (bt - B x_k), written over x_{k-1}
+  // (the SpMM gathers x_k; x_{k-1} is read only at its own row).  x_{k-1} starts in
+  // prev (zero when prev_zero), x_k in cur; returns the block holding x_deg.
+  T *cheb_x(hipStream_t s, int nc, const T *bt, T *prev, T *cur, bool prev_zero) {
+    double sigma = theta_ / delta_, rho = 1.0 / sigma;
     for (int k = 1; k < deg_; ++k) {
       const double rho_new = 1.0 / (2.0 * sigma - rho);
       ChebFuse<T> cf;
-      cf.d = dc;
-      cf.dn = dn;
-      cf.x = x;
+      cf.xform = true;
+      cf.x = cur;
+      cf.d = prev_zero && k == 1 ? nullptr : prev;
       cf.ld = nc;
       cf.c1 = rho_new * rho;
       cf.c2 = 2.0 * rho_new / delta_;
-      bspmm(s, nc, dc, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres, cf);
-      std::swap(dc, dn);
+      bspmm(s, nc, cur, prev, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr, cf);
+      std::swap(prev, cur);
       rho = rho_new;
     }
+    return cur;
   }
 
   void aux_gradient(hipStream_t s, int nc, const double *b, T *x) {

@@ -34,7 +34,8 @@ __device__ __forceinline__ T shfl_xor(T v, int mask, int width) {
 // SpMM epilogue of one output entry in two steps: epi_load issues every operand load
 // (w, z, the row scale and the fused Chebyshev d and x) before any arithmetic, so they
 // are in flight together; epi_store writes y and, with a fused Chebyshev step
-// (ChebFuse in kernels.h), dn = c1 d + c2 y and x + dn in T, as cheb_step does.
+// (ChebFuse in kernels.h), dn = c1 d + c2 y and x + dn in T, as cheb_step does, or in
+// the iterate form y = x + c1 (x - d) + c2 v in place of v.
 template <typename T>
 struct EpiIn {
   T w, z, s, d, x;
@@ -47,13 +48,17 @@ __device__ __forceinline__ EpiIn<T> epi_load(const T *w, size_t iw, bool gw, con
   e.w = gw ? w[iw] : T(0);
   e.z = gz ? z[iz] : T(0);
   e.s = rs ? rs[row] : T(1);
-  e.d = cf.dn ? cf.d[ic] : T(0);
-  e.x = cf.dn ? cf.x[ic] : T(0);
+  e.d = cf.on() && cf.d ? cf.d[ic] : T(0);
+  e.x = cf.on() ? cf.x[ic] : T(0);
   return e;
 }
 
 template <typename T>
 __device__ __forceinline__ void epi_store(T *y, size_t iy, T v, const EpiIn<T> &e, const ChebFuse<T> &cf, size_t ic) {
+  if (cf.xform) {
+    y[iy] = e.x + T(cf.c1) * (e.x - e.d) + T(cf.c2) * v;
+    return;
+  }
   y[iy] = v;
   if (cf.dn) {
     const T dv = T(cf.c1) * e.d + T(cf.c2) * v;
@@ -432,8 +437,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
     TV wv{}, zv{}, dv{}, xv{}, dn;
     if (gamma != T(0)) wv = *reinterpret_cast<const TV *>(w + (size_t)row * ldw + c0);
     if (beta != T(0)) zv = *reinterpret_cast<const TV *>(z + (size_t)row * ldz + c0);
-    if (cf.dn) {
-      dv = *reinterpret_cast<const TV *>(cf.d + ic);
+    if (cf.on()) {
+      if (cf.d) dv = *reinterpret_cast<const TV *>(cf.d + ic);
       xv = *reinterpret_cast<const TV *>(cf.x + ic);
     }
     TV v;
@@ -448,6 +453,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
     if (beta != T(0)) {
 #pragma unroll
       for (int j = 0; j < VEC; ++j) v.v[j] += zscale * zv.v[j];
+    }
+    if (cf.xform) {
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) v.v[j] = xv.v[j] + T(cf.c1) * (xv.v[j] - dv.v[j]) + T(cf.c2) * v.v[j];
     }
     *reinterpret_cast<TV *>(y + (size_t)row * ldy + c0) = v;
     if (cf.dn) {
@@ -894,7 +903,8 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     const size_t va = std::min<size_t>(16, sizeof(T) * vec);
     auto al = [&](const void *p, int ld) { return (uintptr_t)p % va == 0 && ((size_t)ld * sizeof(T)) % va == 0; };
     const int vepi = al(y, ldy) && (gamma == T(0) || al(w, ldw)) && (beta == T(0) || al(z, ldz)) &&
-                     (cf.dn == nullptr || (al(cf.d, cf.ld) && al(cf.dn, cf.ld) && al(cf.x, cf.ld)));
+                     (!cf.on() || ((cf.d == nullptr || al(cf.d, cf.ld)) && (cf.dn == nullptr || al(cf.dn, cf.ld)) &&
+                                   al(cf.x, cf.ld)));
     const int rb = grid_for((int64_t)A.nrows * lw);
     const int nslab = (nc + W - 1) / W;
 #define EMF_WVEC(LW)                                                                                             \
@@ -941,9 +951,9 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     const T *wc = w ? w + c0 : nullptr;
     const T *zc = z ? z + c0 : nullptr;
     ChebFuse<T> cfc = cf;
-    if (cf.dn) {
-      cfc.d += c0;
-      cfc.dn += c0;
+    if (cf.on()) {
+      if (cfc.d) cfc.d += c0;
+      if (cfc.dn) cfc.dn += c0;
       cfc.x += c0;
     }
 #define EMF_CASE(K)                                                                                             \
@@ -1529,6 +1539,14 @@ __global__ void k_cheb_step(int64_t n, T *d, const T *__restrict__ rres, T *x, d
 }
 
 template <typename T>
+__global__ void k_cheb_first(int64_t n, int nc, const T *__restrict__ dinv, const T *__restrict__ b, T *x,
+                             double inv_theta) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  x[i] = (T)(inv_theta * (double)(dinv[i / nc] * b[i]));
+}
+
+template <typename T>
 __global__ void k_diag_scale(int64_t n, int nc, const T *__restrict__ dinv, const T *__restrict__ b, T *x) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -1635,6 +1653,12 @@ void cheb_step(hipStream_t s, int rows, int nc, T *d, const T *rres, T *x, doubl
   EMF_HIP(hipGetLastError());
 }
 template <typename T>
+void cheb_first(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x, double inv_theta) {
+  const int64_t n = (int64_t)rows * nc;
+  hipLaunchKernelGGL((k_cheb_first<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, dinv, b, x, inv_theta);
+  EMF_HIP(hipGetLastError());
+}
+template <typename T>
 void diag_scale(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x) {
   const int64_t n = (int64_t)rows * nc;
   hipLaunchKernelGGL((k_diag_scale<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, dinv, b, x);
@@ -1650,6 +1674,7 @@ void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T
 #define EMF_INST(T)                                                                                        \
   template void cheb_init<T>(hipStream_t, int, int, const T *, const T *, T *, T *, T *, double);           \
   template void cheb_step<T>(hipStream_t, int, int, T *, const T *, T *, double, double);                  \
+  template void cheb_first<T>(hipStream_t, int, int, const T *, const T *, T *, double);                   \
   template void diag_scale<T>(hipStream_t, int, int, const T *, const T *, T *);                           \
   template void dense_solve<T>(hipStream_t, int, int, const double *, const T *, T *);
 EMF_INST(float)
