@@ -72,6 +72,7 @@ const std::map<std::string, std::string> &option_defaults() {
       {"ams_theta", "0.08"},
       {"ams_smooth_levels", "0"},
       {"ams_cycle", "additive"},
+      {"ams_transfer", "structured"},
       {"surrogate_cycles", "false"},
       {"surrogate_min_corr", "0.9"},
   };
@@ -90,6 +91,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"use_graph", {"true", "false"}},
       {"spmm_rows", {"auto", "lanes", "rowblock", "pipelined", "xblock", "2", "4", "8", "16"}},
       {"ams_cycle", {"multiplicative", "additive"}},
+      {"ams_transfer", {"structured", "csr"}},
       {"surrogate_cycles", {"true", "false"}},
       {"save_mesh", {"true", "false"}},
   };

@@ -157,6 +157,36 @@ template <typename TX, typename TY = TX>
 void coldot(hipStream_t s, int rows, int nc, const TX *x, int ldx, const TY *y, int ldy, double *out, double *work,
             const Seg &sg = Seg());
 
+// The auxiliary-space transfers through the structure of the Nedelec interpolation
+// Pi = [G0 | Pi_x | Pi_y | Pi_z] (nblk = 4; [Pi_x | Pi_y | Pi_z] when nblk = 3): edge
+// e = (a, b) couples the interior vertices among its endpoints with -1 (a) / +1 (b) in
+// G0 and with t_d / 2 (t = x_b - x_a) in Pi_d, so the 8-entry rows of Pi need only
+// the endpoints and the half tangent.
+//   ev[e]    interior indices (a', b') of the endpoints, -1 on the boundary
+//   th[e][4] (t_x, t_y, t_z) / 2 and a zero pad
+//   vrp/vcode  the edges of each interior vertex in ascending order (vrp [nv0 + 1]),
+//              bit 31 set where the vertex is the edge's first endpoint
+template <typename T>
+struct AuxMap {
+  int ne = 0, nv0 = 0, nblk = 0;
+  const int2 *ev = nullptr;
+  const int *vrp = nullptr, *vcode = nullptr;
+  const T *th = nullptr;
+};
+// vb = Pi^T r: vb [nblk nv0][nc] (block k at rows k nv0), r fp64 [ne][nc]; each vertex
+// sums its edges in ascending order, in T.
+template <typename T>
+void aux_restrict(hipStream_t s, const AuxMap<T> &M, int nc, const double *r, T *vb);
+// y += Pi vx: vx [nblk nv0][nc], y [ne][nc].
+template <typename T>
+void aux_prolong(hipStream_t s, const AuxMap<T> &M, int nc, const T *vx, T *y);
+
+// out1 = x . y1 and out2 = x . y2 per column in one pass over x (each bitwise equal to
+// the corresponding coldot); work as for coldot, twice.
+template <typename TX>
+void coldot2(hipStream_t s, int rows, int nc, const TX *x, int ldx, const double *y1, const double *y2, int ldy,
+             double *out1, double *out2, double *work, const Seg &sg = Seg());
+
 // Multi-dot for Gram-Schmidt: out[i*nl + seg*nc + c] = sum_r V_i[r][c] w[r][c] for
 // i < m, where V_i = V + i * vstride.  work must hold dot_work_size(rows, nc, sg) * 8
 // doubles.
@@ -196,9 +226,10 @@ template <typename T>
 void cheb_init(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *rres, T *d, T *x, double inv_theta);
 template <typename T>
 void cheb_step(hipStream_t s, int rows, int nc, T *d, const T *rres, T *x, double c1, double c2);
-// x = inv_theta * dinv[row] .* b  (the first Chebyshev iterate from zero)
+// x = inv_theta * dinv[row] .* T(b)  (the first Chebyshev iterate from zero), and
+// bt = T(b) when bt is not null (the right-hand side in the cycle's precision)
 template <typename T>
-void cheb_first(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x, double inv_theta);
+void cheb_first(hipStream_t s, int rows, int nc, const T *dinv, const double *b, T *bt, T *x, double inv_theta);
 
 // x = Dinv .* b (one l1-Jacobi sweep from a zero guess)
 template <typename T>

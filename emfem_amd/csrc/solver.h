@@ -47,6 +47,9 @@ struct PcOptions {
   // when possible, 4 = persistent pipelined row blocks when possible, 10 + n = n
   // lanes per row
   int spmm_rows = 0;
+  // auxiliary-space transfers by the CSR matrices Pi / Pi^T instead of the structured
+  // kernels (the same operator)
+  bool aux_csr = false;
 };
 
 // Preconditioner for B on nc real columns: flexible CG in fp64 with the

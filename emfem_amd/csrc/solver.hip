@@ -201,11 +201,16 @@ class DevAms {
       G0T_.upload(X.G0T, s);
       amgG_.upload(H.amgG, s);
     }
-    Pi_.upload(X.Pi, s);
-    PiT_.upload(X.PiT, s);
     amgPi_.upload(H.amgPi, s);
     nv0_ = X.nv0;
     nblk_ = X.nblk;
+    aux_csr_ = o.aux_csr;
+    if (aux_csr_) {
+      Pi_.upload(X.Pi, s);
+      PiT_.upload(X.PiT, s);
+    } else {
+      upload_aux_map(X, s);
+    }
     spmm_rows_ = o.spmm_rows;
     deg_ = std::max(1, o.cheb_degree);
     const double lmax = 1.0;  // Gershgorin: the l1-scaled operator has spectrum in (0, 1]
@@ -461,26 +466,26 @@ class DevAms {
   void launch_cycle(hipStream_t s, int nc, const double *b, T *out) {
     T *rres = rres_.get(), *d = d_.get();
     const T *bt;
+    T *btw = nullptr;  // the fp32 copy of b, when the cycle runs in fp32
     if constexpr (std::is_same<T, double>::value) {
       bt = b;
     } else {
-      axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, bt_.get(), nc);
-      bt = bt_.get();
+      btw = bt_.get();
+      bt = btw;
     }
     if (fused(nc)) {
       // iterate-form Chebyshev in two blocks, d and out: pre-smoothing from zero starts
       // in d; the buffers are chosen so that post-smoothing (deg writes, alternating)
       // ends in out: the middle iterate lives in d for odd deg, in out for even deg
       T *mid = (deg_ & 1) ? d : out, *other = (deg_ & 1) ? out : d;
-      cheb_first<T>(s, ne_, nc, l1inv_.get(), bt, d, 1.0 / theta_);
+      cheb_first<T>(s, ne_, nc, l1inv_.get(), b, btw, d, 1.0 / theta_);
       T *xm = cheb_x(s, nc, bt, out, d, true);
       EMF_REQUIRE(xm == mid, kErrState, "Chebyshev buffer plan");
       if (!additive_) aux_gradient(s, nc, b, mid);
       residual64(s, nc, b, mid);
-      spmm_mixed<T, double, T>(s, PiT_.view(), nc, r64_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0,
-                               T(0), nullptr, 0);
+      restrict_aux(s, nc);
       amgPi_.vcycle(s, nc, vb_.get(), vx_.get());
-      spmm<T>(s, Pi_.view(), nc, vx_.get(), nc, mid, nc, T(1), nullptr, T(0), nullptr, 0, T(1), mid, nc);
+      prolong_aux(s, nc, mid);
       if (!additive_) aux_gradient(s, nc, b, mid);
       // post-smoothing from the current iterate: x_1 = x_0 + Dinv .* (bt - B x_0) / theta
       ChebFuse<T> cf;
@@ -494,6 +499,7 @@ class DevAms {
       EMF_REQUIRE(xm == out, kErrState, "Chebyshev buffer plan");
       return;
     }
+    if (btw) axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, btw, nc);
     // pre-smoothing (Chebyshev from zero)
     cheb_init<T>(s, ne_, nc, l1inv_.get(), bt, rres, d, out, 1.0 / theta_);
     cheb_iterate(s, nc, rres, d, out);
@@ -501,10 +507,9 @@ class DevAms {
     residual64(s, nc, b, out);
     // the nodal-vector components (and, additive, the gradient) in one
     // block-diagonal auxiliary solve
-    spmm_mixed<T, double, T>(s, PiT_.view(), nc, r64_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0, T(0),
-                             nullptr, 0);
+    restrict_aux(s, nc);
     amgPi_.vcycle(s, nc, vb_.get(), vx_.get());
-    spmm<T>(s, Pi_.view(), nc, vx_.get(), nc, out, nc, T(1), nullptr, T(0), nullptr, 0, T(1), out, nc);
+    prolong_aux(s, nc, out);
     if (!additive_) aux_gradient(s, nc, b, out);
     // post-smoothing (Chebyshev from the current iterate)
     bspmm(s, nc, out, rres, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr);
@@ -573,6 +578,56 @@ class DevAms {
     }
   }
 
+  // vb = Pi^T r64 and x += Pi vx: the structured kernels, or the CSR matrices on request
+  void restrict_aux(hipStream_t s, int nc) {
+    if (aux_csr_)
+      spmm_mixed<T, double, T>(s, PiT_.view(), nc, r64_.get(), nc, vb_.get(), nc, T(1), nullptr, T(0), nullptr, 0,
+                               T(0), nullptr, 0);
+    else
+      aux_restrict<T>(s, aux_map(), nc, r64_.get(), vb_.get());
+  }
+  void prolong_aux(hipStream_t s, int nc, T *x) {
+    if (aux_csr_)
+      spmm<T>(s, Pi_.view(), nc, vx_.get(), nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), x, nc);
+    else
+      aux_prolong<T>(s, aux_map(), nc, vx_.get(), x);
+  }
+  AuxMap<T> aux_map() const {
+    AuxMap<T> m;
+    m.ne = ne_;
+    m.nv0 = nv0_;
+    m.nblk = nblk_;
+    m.ev = ev_.get();
+    m.vrp = vrp_.get();
+    m.vcode = vcode_.get();
+    m.th = th_.get();
+    return m;
+  }
+  // endpoints, half tangents and vertex-edge lists of the interpolation from the
+  // mesh-only matrices G0 (-1 / +1 per edge row), Pi_d (t_d / 2 at both endpoints)
+  // and G0^T (each vertex's edges in ascending order)
+  void upload_aux_map(const AmsAux &X, hipStream_t s) {
+    const int ne = X.G0.n;
+    EMF_REQUIRE(ne == ne_ && X.G0T.n == X.nv0, kErrState, "auxiliary map: shapes");
+    std::vector<int2> ev(ne, make_int2(-1, -1));
+    std::vector<T> th((size_t)ne * 4, T(0));
+    for (int e = 0; e < ne; ++e) {
+      for (int p = X.G0.rp[e]; p < X.G0.rp[e + 1]; ++p) (X.G0.v[p] < 0 ? ev[e].x : ev[e].y) = X.G0.ci[p];
+      for (int d = 0; d < 3; ++d)
+        if (X.P[d].rp[e] < X.P[d].rp[e + 1]) th[(size_t)e * 4 + d] = (T)X.P[d].v[X.P[d].rp[e]];
+    }
+    std::vector<int> vcode(X.G0T.ci.size());
+    for (size_t p = 0; p < vcode.size(); ++p) {
+      EMF_REQUIRE(X.G0T.ci[p] >= 0, kErrState, "auxiliary map: edge index");
+      vcode[p] = (int)((uint32_t)X.G0T.ci[p] | (X.G0T.v[p] < 0 ? 0x80000000u : 0u));
+    }
+    ev_.upload(ev, s);
+    th_.upload(th, s);
+    vrp_.upload(X.G0T.rp, s);
+    vcode_.upload(vcode, s);
+    EMF_HIP(hipStreamSynchronize(s));
+  }
+
   // Chebyshev steps k = 1 .. deg-1 in the iterate form, one SpMM pass each:
   // x_{k+1} = x_k + c1 (x_k - x_{k-1}) + c2 Dinv .* (bt - B x_k), written over x_{k-1}
   // (the SpMM gathers x_k; x_{k-1} is read only at its own row).  x_{k-1} starts in
@@ -611,6 +666,10 @@ class DevAms {
   DevBuf<T> bvals_, l1inv_, rres_, d_, d2_, bt_, vb_, vx_;
   DevBuf<double> b64vals_, r64_;
   DevCsr<T> G0_, G0T_, Pi_, PiT_;
+  bool aux_csr_ = false;
+  DevBuf<int2> ev_;
+  DevBuf<int> vrp_, vcode_;
+  DevBuf<T> th_;
   DevAmg<T> amgG_, amgPi_;
   std::map<std::tuple<int, const void *, const void *>, hipGraphExec_t> graphs_;
   std::map<int, int> lpr_, lpr64_;
@@ -723,28 +782,33 @@ class InnerPcImpl : public InnerPc {
     h_active_.resize(nl);
   }
 
-  // z = M r: the cycle in its precision; z in fp64 (conversion from the fp32 cycle)
-  // or in fp32 (the cycle's own output block)
+  // z = M r: the cycle in its precision, z in that precision (the FCG kernels widen
+  // fp32 entries exactly as they read them)
   void precondition(hipStream_t s, int nc, const double *r, double *z, SolveStats &st) {
     ++st.vcycles;
-    if (!fp32_) {
-      a64_->apply(s, nc, r, z, o_.use_graph);
-    } else {
-      a32_->apply(s, nc, r, zt_.get(), o_.use_graph);
-      axpby<float, double>(s, ne_, nc, 1.0, zt_.get(), nc, 0.0, z, nc);
-    }
+    a64_->apply(s, nc, r, z, o_.use_graph);
+  }
+  void precondition(hipStream_t s, int nc, const double *r, float *z, SolveStats &st) {
+    ++st.vcycles;
+    a32_->apply(s, nc, r, z, o_.use_graph);
   }
   void solve(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) {
-    fcg<double>(s, nc, b, x, p_.get(), q_.get(), z_.get(), st);
+    if (fp32_)
+      fcg<float>(s, nc, b, x, zt_.get(), st);
+    else
+      fcg<double>(s, nc, b, x, z_.get(), st);
   }
 
   // Flexible CG per logical column (segment, column): each stops on its own
-  // unpreconditioned residual; frozen columns keep alpha = beta = 0.  All vectors are
-  // fp64: q = B p for a gradient-like p in the air is a small result of large
+  // unpreconditioned residual; frozen columns keep alpha = beta = 0.  x, r, p and q
+  // are fp64: q = B p for a gradient-like p in the air is a small result of large
   // cancelling curl terms, which an fp32 product turns into rounding noise (fp32
-  // search vectors with an fp32 B stalled the outer solve at 1e-10).
-  template <typename F>
-  void fcg(hipStream_t s, int nc, const double *b, double *x, F *p, F *q, F *z, SolveStats &st) {
+  // search vectors with an fp32 B stalled the outer solve at 1e-10).  The
+  // preconditioned residual z stays in the cycle's precision TZ: z . q and r . z
+  // share one pass over it, and p = z + beta p widens it on the fly.
+  template <typename TZ>
+  void fcg(hipStream_t s, int nc, const double *b, double *x, TZ *z, SolveStats &st) {
+    double *p = p_.get(), *q = q_.get();
     double *r = r_.get();
     double *w = work_.get();
     const int nl = sg_.nseg * nc;
@@ -765,13 +829,13 @@ class InnerPcImpl : public InnerPc {
     const double tol2 = o_.inner_rtol * o_.inner_rtol;
     // z = M r ; p = z ; q = B p
     precondition(s, nc, r, z, st);
-    EMF_HIP(hipMemcpyAsync(p, z, sizeof(F) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
+    axpby<TZ, double>(s, ne_, nc, 1.0, z, nc, 0.0, p, nc);
     bspmm(s, nc, p, q);
-    coldot<F, F>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
-    coldot<double, F>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
+    coldot<double, double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
+    coldot<double, TZ>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
     for (int it = 1; it <= o_.inner_max_it; ++it) {
       fcg_alpha(s, nl, rz_.get(), pq_.get(), active_.get(), alpha_.get());
-      fcg_xr<double, F>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w, sg_);
+      fcg_xr<double, double>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w, sg_);
       ++st.inner_its;
       EMF_HIP(hipMemcpyAsync(h_rr_.data(), rr_.get(), sizeof(double) * nl, hipMemcpyDeviceToHost, s));
       EMF_HIP(hipStreamSynchronize(s));
@@ -783,12 +847,11 @@ class InnerPcImpl : public InnerPc {
       if (!any || it == o_.inner_max_it) break;
       active_.upload(h_active_.data(), nl, s);
       precondition(s, nc, r, z, st);
-      coldot<F, F>(s, ne_, nc, z, nc, q, nc, zq_.get(), w, sg_);
-      coldot<double, F>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
+      coldot2<TZ>(s, ne_, nc, z, nc, q, r, nc, zq_.get(), rz_.get(), w, sg_);
       fcg_beta(s, nl, zq_.get(), pq_.get(), active_.get(), beta_.get());
-      axpby_cols<F, F>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc, sg_);
+      axpby_cols<TZ, double>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc, sg_);
       bspmm(s, nc, p, q);
-      coldot<F, F>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
+      coldot<double, double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
     }
   }
 

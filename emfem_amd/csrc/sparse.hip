@@ -1307,7 +1307,20 @@ __global__ __launch_bounds__(kBlock) void k_coldot(int rows, int nc, const TX *_
   double acc = 0.0;
   if (t < per_block) {
     const int c = t % nc;
-    for (int r = r0 + t / nc; r < r1; r += rpp) acc += (double)x[(size_t)r * ldx + c] * (double)y[(size_t)r * ldy + c];
+    // four rows per trip with every load issued first (clamped, masked), summed in
+    // row order
+    for (int r = r0 + t / nc; r < r1; r += 4 * rpp) {
+      double xv[4], yv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int ru = min(r + u * rpp, r1 - 1);
+        xv[u] = (double)x[(size_t)ru * ldx + c];
+        yv[u] = (double)y[(size_t)ru * ldy + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (r + u * rpp < r1) acc += xv[u] * yv[u];
+    }
   }
   sh[t] = acc;
   __syncthreads();
@@ -1315,6 +1328,53 @@ __global__ __launch_bounds__(kBlock) void k_coldot(int rows, int nc, const TX *_
     double sum = 0.0;
     for (int k = t; k < per_block; k += nc) sum += sh[k];
     work[(size_t)blockIdx.x * nc + t] = sum;
+  }
+}
+
+// Two dot products against one shared operand: x . y1 and x . y2 per column, in the
+// order of k_coldot (so each equals the corresponding coldot bitwise); the partials
+// go to work[0][block][c] and work[1][block][c].
+template <typename TX>
+__global__ __launch_bounds__(kBlock) void k_coldot2(int rows, int nc, const TX *__restrict__ x, int ldx,
+                                                    const double *__restrict__ y1, const double *__restrict__ y2,
+                                                    int ldy, double *work, Seg sg) {
+  __shared__ double sh[2][kBlock];
+  const int rpp = kBlock / nc;
+  const int per_block = rpp * nc;
+  const int t = threadIdx.x;
+  int r0, r1;
+  chunk_rows(sg, rows, blockIdx.x, gridDim.x, r0, r1);
+  double acc1 = 0.0, acc2 = 0.0;
+  if (t < per_block) {
+    const int c = t % nc;
+    for (int r = r0 + t / nc; r < r1; r += 4 * rpp) {
+      double xv[4], av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int ru = min(r + u * rpp, r1 - 1);
+        xv[u] = (double)x[(size_t)ru * ldx + c];
+        av[u] = y1[(size_t)ru * ldy + c];
+        bv[u] = y2[(size_t)ru * ldy + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (r + u * rpp < r1) {
+          acc1 += xv[u] * av[u];
+          acc2 += xv[u] * bv[u];
+        }
+    }
+  }
+  sh[0][t] = acc1;
+  sh[1][t] = acc2;
+  __syncthreads();
+  if (t < nc) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int k = t; k < per_block; k += nc) {
+      s1 += sh[0][k];
+      s2 += sh[1][k];
+    }
+    work[(size_t)blockIdx.x * nc + t] = s1;
+    work[((size_t)gridDim.x + blockIdx.x) * nc + t] = s2;
   }
 }
 
@@ -1374,6 +1434,131 @@ void coldot(hipStream_t s, int rows, int nc, const TX *x, int ldx, const TY *y, 
   hipLaunchKernelGGL(k_finish, dim3(sg.nseg * nc), dim3(kBlock), 0, s, nb, nc, work, out, sg);
   EMF_HIP(hipGetLastError());
 }
+namespace {
+
+// One lane per (interior vertex, column): the vertex's edges four at a time (codes,
+// then the residual entries and half tangents, all loads before the sums).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_aux_restrict(int nv0, int nblk, int nc, const int *__restrict__ vrp,
+                                                         const int *__restrict__ vcode, const T *__restrict__ th,
+                                                         const double *__restrict__ r, T *vb) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)nv0 * nc) return;
+  const int v = (int)(i / nc), c = (int)(i % nc);
+  T g = T(0), px = T(0), py = T(0), pz = T(0);
+  const int p0 = vrp[v], p1 = vrp[v + 1];
+  for (int p = p0; p < p1; p += 4) {
+    int code[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) code[u] = vcode[min(p + u, p1 - 1)];
+    double rv[4];
+    VecT<T, 4> t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = code[u] & 0x7fffffff;
+      rv[u] = r[(size_t)e * nc + c];
+      t[u] = *reinterpret_cast<const VecT<T, 4> *>(th + (size_t)e * 4);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (p + u < p1) {
+        const T x = (T)rv[u];
+        g += code[u] < 0 ? -x : x;
+        px += t[u].v[0] * x;
+        py += t[u].v[1] * x;
+        pz += t[u].v[2] * x;
+      }
+  }
+  const size_t o = (size_t)v * nc + c, blk = (size_t)nv0 * nc;
+  int k = 0;
+  if (nblk == 4) vb[o] = g, k = 1;
+  vb[k * blk + o] = px;
+  vb[(k + 1) * blk + o] = py;
+  vb[(k + 2) * blk + o] = pz;
+}
+
+// One lane per (edge, column): the blocks' entries of both interior endpoints, all
+// loads first.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_aux_prolong(int ne, int nv0, int nblk, int nc, const int2 *__restrict__ ev,
+                                                        const T *__restrict__ th, const T *__restrict__ vx, T *y) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)ne * nc) return;
+  const int e = (int)(i / nc), c = (int)(i % nc);
+  const int2 ab = ev[e];
+  const VecT<T, 4> t = *reinterpret_cast<const VecT<T, 4> *>(th + (size_t)e * 4);
+  const size_t blk = (size_t)nv0 * nc;
+  const int k0 = nblk == 4 ? 1 : 0;
+  T v[2][4];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int w = q == 0 ? ab.x : ab.y;
+    const size_t o = (size_t)max(w, 0) * nc + c;
+    v[q][0] = nblk == 4 ? vx[o] : T(0);
+    v[q][1] = vx[k0 * blk + o];
+    v[q][2] = vx[(k0 + 1) * blk + o];
+    v[q][3] = vx[(k0 + 2) * blk + o];
+  }
+  const size_t iy = (size_t)e * nc + c;
+  const T y0 = y[iy];
+  T acc = T(0);
+  if (ab.x >= 0) acc += -v[0][0] + t.v[0] * v[0][1] + t.v[1] * v[0][2] + t.v[2] * v[0][3];
+  if (ab.y >= 0) acc += v[1][0] + t.v[0] * v[1][1] + t.v[1] * v[1][2] + t.v[2] * v[1][3];
+  y[iy] = y0 + acc;
+}
+
+}  // namespace
+
+template <typename T>
+void aux_restrict(hipStream_t s, const AuxMap<T> &M, int nc, const double *r, T *vb) {
+  EMF_REQUIRE(M.nblk == 3 || M.nblk == 4, kErrArg, "auxiliary transfer: 3 or 4 blocks");
+  const int64_t n = (int64_t)M.nv0 * nc;
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_aux_restrict<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, M.nv0, M.nblk, nc, M.vrp, M.vcode,
+                     M.th, r, vb);
+  EMF_HIP(hipGetLastError());
+}
+template <typename T>
+void aux_prolong(hipStream_t s, const AuxMap<T> &M, int nc, const T *vx, T *y) {
+  EMF_REQUIRE(M.nblk == 3 || M.nblk == 4, kErrArg, "auxiliary transfer: 3 or 4 blocks");
+  const int64_t n = (int64_t)M.ne * nc;
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_aux_prolong<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, M.ne, M.nv0, M.nblk, nc, M.ev, M.th,
+                     vx, y);
+  EMF_HIP(hipGetLastError());
+}
+template void aux_restrict<float>(hipStream_t, const AuxMap<float> &, int, const double *, float *);
+template void aux_restrict<double>(hipStream_t, const AuxMap<double> &, int, const double *, double *);
+template void aux_prolong<float>(hipStream_t, const AuxMap<float> &, int, const float *, float *);
+template void aux_prolong<double>(hipStream_t, const AuxMap<double> &, int, const double *, double *);
+
+template <typename TX>
+void coldot2(hipStream_t s, int rows, int nc, const TX *x, int ldx, const double *y1, const double *y2, int ldy,
+             double *out1, double *out2, double *work, const Seg &sg) {
+  if (nc > kBlock) {
+    EMF_REQUIRE(sg.nseg == 1, kErrArg, "segmented reductions support at most 256 columns");
+    for (int c0 = 0; c0 < nc; c0 += kBlock)
+      coldot2<TX>(s, rows, std::min(kBlock, nc - c0), x + c0, ldx, y1 + c0, y2 + c0, ldy, out1 + c0, out2 + c0, work,
+                  sg);
+    return;
+  }
+  const int nb = reduce_blocks(rows, nc, sg);
+  const int nl = sg.nseg * nc;
+  hipLaunchKernelGGL((k_coldot2<TX>), dim3(nb), dim3(kBlock), 0, s, rows, nc, x, ldx, y1, y2, ldy, work, sg);
+  // k_finish writes [i][nl]: the two results land in out1 when out2 == out1 + nl,
+  // otherwise each half is finished on its own
+  if (out2 == out1 + nl) {
+    hipLaunchKernelGGL(k_finish, dim3(2 * nl), dim3(kBlock), 0, s, nb, nc, work, out1, sg);
+  } else {
+    hipLaunchKernelGGL(k_finish, dim3(nl), dim3(kBlock), 0, s, nb, nc, work, out1, sg);
+    hipLaunchKernelGGL(k_finish, dim3(nl), dim3(kBlock), 0, s, nb, nc, work + (size_t)nb * nc, out2, sg);
+  }
+  EMF_HIP(hipGetLastError());
+}
+template void coldot2<float>(hipStream_t, int, int, const float *, int, const double *, const double *, int, double *,
+                             double *, double *, const Seg &);
+template void coldot2<double>(hipStream_t, int, int, const double *, int, const double *, const double *, int,
+                              double *, double *, double *, const Seg &);
 template void coldot<float, float>(hipStream_t, int, int, const float *, int, const float *, int, double *, double *,
                                    const Seg &);
 template void coldot<double, double>(hipStream_t, int, int, const double *, int, const double *, int, double *,
@@ -1539,11 +1724,13 @@ __global__ void k_cheb_step(int64_t n, T *d, const T *__restrict__ rres, T *x, d
 }
 
 template <typename T>
-__global__ void k_cheb_first(int64_t n, int nc, const T *__restrict__ dinv, const T *__restrict__ b, T *x,
+__global__ void k_cheb_first(int64_t n, int nc, const T *__restrict__ dinv, const double *__restrict__ b, T *bt, T *x,
                              double inv_theta) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  x[i] = (T)(inv_theta * (double)(dinv[i / nc] * b[i]));
+  const T bv = (T)b[i];
+  if (bt) bt[i] = bv;
+  x[i] = (T)(inv_theta * (double)(dinv[i / nc] * bv));
 }
 
 template <typename T>
@@ -1611,6 +1798,8 @@ template void axpby_cols<float, float>(hipStream_t, int, int, const double *, co
                                        float *, int, const Seg &);
 template void axpby_cols<double, double>(hipStream_t, int, int, const double *, const double *, int, const double *,
                                          double *, int, const Seg &);
+template void axpby_cols<float, double>(hipStream_t, int, int, const double *, const float *, int, const double *,
+                                        double *, int, const Seg &);
 
 template <typename TX, typename TY>
 void axpby(hipStream_t s, int rows, int nc, double alpha, const TX *x, int ldx, double beta, TY *y, int ldy) {
@@ -1653,9 +1842,9 @@ void cheb_step(hipStream_t s, int rows, int nc, T *d, const T *rres, T *x, doubl
   EMF_HIP(hipGetLastError());
 }
 template <typename T>
-void cheb_first(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x, double inv_theta) {
+void cheb_first(hipStream_t s, int rows, int nc, const T *dinv, const double *b, T *bt, T *x, double inv_theta) {
   const int64_t n = (int64_t)rows * nc;
-  hipLaunchKernelGGL((k_cheb_first<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, dinv, b, x, inv_theta);
+  hipLaunchKernelGGL((k_cheb_first<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, dinv, b, bt, x, inv_theta);
   EMF_HIP(hipGetLastError());
 }
 template <typename T>
@@ -1674,7 +1863,7 @@ void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T
 #define EMF_INST(T)                                                                                        \
   template void cheb_init<T>(hipStream_t, int, int, const T *, const T *, T *, T *, T *, double);           \
   template void cheb_step<T>(hipStream_t, int, int, T *, const T *, T *, double, double);                  \
-  template void cheb_first<T>(hipStream_t, int, int, const T *, const T *, T *, double);                   \
+  template void cheb_first<T>(hipStream_t, int, int, const T *, const double *, T *, T *, double);         \
   template void diag_scale<T>(hipStream_t, int, int, const T *, const T *, T *);                           \
   template void dense_solve<T>(hipStream_t, int, int, const double *, const T *, T *);
 EMF_INST(float)

@@ -210,6 +210,40 @@ def test_pc_apply_matches_fcg_tolerance(tiny, n_rhs, oracle_mod):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("cycle", ["additive", "multiplicative"])
+def test_structured_transfers_match_csr(tiny, precision, cycle):
+    """The structured auxiliary-space transfers (edge endpoints and half tangents) apply
+    the same Pi and Pi^T as the CSR matrices of the interpolation: one FCG step
+    (x = alpha M b) agrees to the cycle's rounding, on 2 and 6 real columns.  The
+    gradient space is nearly singular in the air (only the tiny mass term), so the
+    auxiliary solves amplify rounding differences (fp64: ~1e-10 relative here); a wrong
+    entry of Pi would differ at O(1)."""
+    m, om = tiny
+    ctx = _ctx(m, "-inner_precision %s -B_ksp_max_it 1 -ams_cycle %s" % (precision, cycle))
+    ctx.create_linear_system(1)
+    ctx.assemble_matrix(0)
+    ne = ctx.mesh_info()["n_edges"]
+    rng = np.random.default_rng(21)
+    tol = 1e-4 if precision == "fp32" else 1e-8
+    for n_rhs in (1, 3):
+        b = rng.standard_normal((ne, 2 * n_rhs))
+        bd = ctx.device_array(b)
+        out = {}
+        for tr in ("csr", "structured"):
+            ctx.process_options("-ams_transfer %s" % tr)
+            ctx.create_pc()
+            xd = ctx.device_array(np.zeros_like(b))
+            ctx.pc_apply_b(n_rhs, bd.ptr, xd.ptr)
+            out[tr] = xd.numpy()
+        assert np.all(np.isfinite(out["structured"]))
+        for c in range(2 * n_rhs):
+            d = np.linalg.norm(out["structured"][:, c] - out["csr"][:, c])
+            rel = d / np.linalg.norm(out["csr"][:, c])
+            print("transfer", precision, cycle, n_rhs, c, rel)
+            assert rel <= tol, (n_rhs, c, rel)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_graph_replay_matches_plain_launches(tiny, precision):
     """The captured auxiliary-space cycle gives bitwise the launches' result on every
     replay, with eager work and synchronisation between replays."""
