@@ -55,10 +55,13 @@ constexpr int kXBlockRows = 64;
 constexpr int kXnCols = 256;  // distinct columns per narrow x-block
 constexpr int kXnRows = 64;
 
-// Nonzeros staged per workgroup by the row-block SpMM: 16 KB of products.
+// Nonzeros staged per workgroup by the row-block SpMM: 16 KB of products in fp32,
+// 24 KB in fp64 (a multiple of the 256-thread block, at most 2048).
 template <typename T, int NC>
 constexpr int stream_tile() {
-  return (16384 / (NC * (int)sizeof(T))) / 256 * 256 > 2048 ? 2048 : (16384 / (NC * (int)sizeof(T))) / 256 * 256;
+  constexpr int bytes = sizeof(T) == 8 ? 24576 : 16384;
+  constexpr int t = (bytes / (NC * (int)sizeof(T))) / 256 * 256;
+  return t > 2048 ? 2048 : t;
 }
 int stream_tile_for(int nc, bool fp64);
 
