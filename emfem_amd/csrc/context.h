@@ -227,6 +227,7 @@ inline PcOptions pc_options(const emfem_context *c) {
   PcOptions o;
   o.cheb_degree = c->opti("ams_cheb_degree");
   o.cheb_ratio = c->optd("ams_cheb_ratio");
+  o.cheb_fourth = c->opt("ams_cheb_kind") == "fourth";
   o.inner_rtol = c->optd("B_ksp_rtol");
   o.inner_max_it = c->opti("B_ksp_max_it");
   o.amg.theta = c->optd("ams_theta");

@@ -69,6 +69,7 @@ const std::map<std::string, std::string> &option_defaults() {
       {"spmm_rows", "auto"},
       {"ams_cheb_degree", "5"},
       {"ams_cheb_ratio", "60"},
+      {"ams_cheb_kind", "first"},
       {"ams_theta", "0.08"},
       {"ams_smooth_levels", "0"},
       {"ams_cycle", "additive"},
@@ -92,6 +93,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"spmm_rows", {"auto", "lanes", "rowblock", "pipelined", "xblock", "2", "4", "8", "16"}},
       {"ams_cycle", {"multiplicative", "additive"}},
       {"ams_transfer", {"structured", "csr"}},
+      {"ams_cheb_kind", {"first", "fourth"}},
       {"surrogate_cycles", {"true", "false"}},
       {"save_mesh", {"true", "false"}},
   };

@@ -38,6 +38,7 @@ double time_operator(hipStream_t s, int ne, int nnz, const int *rp, const int *c
 struct PcOptions {
   int cheb_degree = 5;
   double cheb_ratio = 60.0;
+  bool cheb_fourth = false;  // fourth-kind Chebyshev (no ratio) instead of first kind
   double inner_rtol = 1e-2;
   int inner_max_it = 100;
   AmgParams amg;

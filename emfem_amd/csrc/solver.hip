@@ -214,9 +214,31 @@ class DevAms {
     spmm_rows_ = o.spmm_rows;
     deg_ = std::max(1, o.cheb_degree);
     const double lmax = 1.0;  // Gershgorin: the l1-scaled operator has spectrum in (0, 1]
-    const double lmin = lmax / o.cheb_ratio;
-    theta_ = 0.5 * (lmax + lmin);
-    delta_ = 0.5 * (lmax - lmin);
+    // step coefficients: x_1 = x_0 + c0 Dinv r_0, then d_k = c1[k] d_{k-1} + c2[k] Dinv r_k
+    c1_.assign(deg_, 0.0);
+    c2_.assign(deg_, 0.0);
+    if (o.cheb_fourth) {
+      // fourth-kind Chebyshev smoother (Lottes 2022, "Optimal polynomial smoothers for
+      // multigrid V-cycles", unit weights): no lower bound of the spectrum
+      c0_ = 4.0 / (3.0 * lmax);
+      for (int k = 1; k < deg_; ++k) {
+        c1_[k] = (2.0 * k - 1.0) / (2.0 * k + 3.0);
+        c2_[k] = (8.0 * k + 4.0) / ((2.0 * k + 3.0) * lmax);
+      }
+    } else {
+      // first kind on [lmax / ratio, lmax] (Saad, Iterative Methods, Alg. 12.1)
+      const double lmin = lmax / o.cheb_ratio;
+      const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+      const double sigma = theta / delta;
+      double rho = 1.0 / sigma;
+      c0_ = 1.0 / theta;
+      for (int k = 1; k < deg_; ++k) {
+        const double rho_new = 1.0 / (2.0 * sigma - rho);
+        c1_[k] = rho_new * rho;
+        c2_[k] = 2.0 * rho_new / delta;
+        rho = rho_new;
+      }
+    }
     EMF_HIP(hipStreamSynchronize(s));
   }
 
@@ -478,7 +500,7 @@ class DevAms {
       // in d; the buffers are chosen so that post-smoothing (deg writes, alternating)
       // ends in out: the middle iterate lives in d for odd deg, in out for even deg
       T *mid = (deg_ & 1) ? d : out, *other = (deg_ & 1) ? out : d;
-      cheb_first<T>(s, ne_, nc, l1inv_.get(), b, btw, d, 1.0 / theta_);
+      cheb_first<T>(s, ne_, nc, l1inv_.get(), b, btw, d, c0_);
       T *xm = cheb_x(s, nc, bt, out, d, true);
       EMF_REQUIRE(xm == mid, kErrState, "Chebyshev buffer plan");
       if (!additive_) aux_gradient(s, nc, b, mid);
@@ -493,7 +515,7 @@ class DevAms {
       cf.x = mid;
       cf.ld = nc;
       cf.c1 = 0.0;
-      cf.c2 = 1.0 / theta_;
+      cf.c2 = c0_;
       bspmm(s, nc, mid, other, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr, cf);
       xm = cheb_x(s, nc, bt, mid, other, false);
       EMF_REQUIRE(xm == out, kErrState, "Chebyshev buffer plan");
@@ -501,7 +523,7 @@ class DevAms {
     }
     if (btw) axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, btw, nc);
     // pre-smoothing (Chebyshev from zero)
-    cheb_init<T>(s, ne_, nc, l1inv_.get(), bt, rres, d, out, 1.0 / theta_);
+    cheb_init<T>(s, ne_, nc, l1inv_.get(), bt, rres, d, out, c0_);
     cheb_iterate(s, nc, rres, d, out);
     if (!additive_) aux_gradient(s, nc, b, out);
     residual64(s, nc, b, out);
@@ -513,7 +535,7 @@ class DevAms {
     if (!additive_) aux_gradient(s, nc, b, out);
     // post-smoothing (Chebyshev from the current iterate)
     bspmm(s, nc, out, rres, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr);
-    axpby<T, T>(s, ne_, nc, 1.0 / theta_, rres, nc, 0.0, d, nc);
+    axpby<T, T>(s, ne_, nc, c0_, rres, nc, 0.0, d, nc);
     axpby<T, T>(s, ne_, nc, 1.0, d, nc, 1.0, out, nc);
     cheb_iterate(s, nc, rres, d, out);
   }
@@ -569,12 +591,9 @@ class DevAms {
   // the fused step): rres -= l1inv .* (B d), then the streaming update d = c1 d + c2 rres
   // and x += d.
   void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x) {
-    double sigma = theta_ / delta_, rho = 1.0 / sigma;
     for (int k = 1; k < deg_; ++k) {
       bspmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres);
-      const double rho_new = 1.0 / (2.0 * sigma - rho);
-      cheb_step<T>(s, ne_, nc, d, rres, x, rho_new * rho, 2.0 * rho_new / delta_);
-      rho = rho_new;
+      cheb_step<T>(s, ne_, nc, d, rres, x, c1_[k], c2_[k]);
     }
   }
 
@@ -633,19 +652,16 @@ class DevAms {
   // (the SpMM gathers x_k; x_{k-1} is read only at its own row).  x_{k-1} starts in
   // prev (zero when prev_zero), x_k in cur; returns the block holding x_deg.
   T *cheb_x(hipStream_t s, int nc, const T *bt, T *prev, T *cur, bool prev_zero) {
-    double sigma = theta_ / delta_, rho = 1.0 / sigma;
     for (int k = 1; k < deg_; ++k) {
-      const double rho_new = 1.0 / (2.0 * sigma - rho);
       ChebFuse<T> cf;
       cf.xform = true;
       cf.x = cur;
       cf.d = prev_zero && k == 1 ? nullptr : prev;
       cf.ld = nc;
-      cf.c1 = rho_new * rho;
-      cf.c2 = 2.0 * rho_new / delta_;
+      cf.c1 = c1_[k];
+      cf.c2 = c2_[k];
       bspmm(s, nc, cur, prev, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr, cf);
       std::swap(prev, cur);
-      rho = rho_new;
     }
     return cur;
   }
@@ -660,7 +676,8 @@ class DevAms {
 
   int ne_ = 0, nv0_ = 0, nc_ = 0, deg_ = 3, nblk_ = 3, spmm_rows_ = 0;
   bool additive_ = false;
-  double theta_ = 0.5, delta_ = 0.5;
+  double c0_ = 1.0;
+  std::vector<double> c1_, c2_;
   Csr<T> B_;
   Csr<double> B64_;
   DevBuf<T> bvals_, l1inv_, rres_, d_, d2_, bt_, vb_, vx_;

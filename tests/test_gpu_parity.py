@@ -185,10 +185,12 @@ def test_estimator_matches_oracle(oracle_mod):
 @pytest.mark.parametrize("n_rhs", [1, 2, 3])
 def test_pc_apply_matches_fcg_tolerance(tiny, n_rhs, oracle_mod):
     """pc_apply_b gives x with ||b - B x|| <= rtol ||b|| per column (src/em_la.cc:208-228),
-    on 2, 4 and 6 real columns (6: the padded reduce-scatter of the SpMMs)."""
+    on 2, 4 and 6 real columns (6: the padded reduce-scatter of the SpMMs), with the
+    first-kind Chebyshev smoother and with the fourth-kind one."""
     m, om = tiny
-    for prec, tol in (("fp64", 1e-2), ("fp32", 1e-2), ("fp64", 1e-6)):
-        ctx = _ctx(m, "-inner_precision %s -B_ksp_rtol %g" % (prec, tol))
+    for prec, tol, kind in (("fp64", 1e-2, "first"), ("fp32", 1e-2, "first"), ("fp64", 1e-6, "first"),
+                            ("fp32", 1e-2, "fourth"), ("fp64", 1e-6, "fourth")):
+        ctx = _ctx(m, "-inner_precision %s -B_ksp_rtol %g -ams_cheb_kind %s" % (prec, tol, kind))
         ctx.create_linear_system(1)
         ctx.assemble_matrix(0)
         ctx.create_pc()
