@@ -562,13 +562,9 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
     }
     c->stats["op_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * 20 + double(ne) * n * 32;
     const double vb = c->pc->fp32() ? 4.0 : 8.0;
-    // row pointers, columns and values, Dinv, and per row: fused (iterate-form
-    // Chebyshev step) x_k gathered once, the right-hand side, x_{k-1} in, x_{k+1} out;
-    // otherwise (residual update rres -= Dinv .* (B d)) d once, rres in and out
-    const bool fused = c->pc->smoother_fused(s, nc);
-    c->stats["b_spmm_fused"] = fused ? 1.0 : 0.0;
-    c->stats["b_spmm_bytes"] =
-        double(ne + 1) * 4 + double(c->nnz) * (4 + vb) + double(ne) * (nc * (fused ? 4 : 3) + 1) * vb;
+    // row pointers, columns and values, Dinv, and per row of the Chebyshev step: x_k
+    // gathered once, the right-hand side, x_{k-1} in, x_{k+1} out
+    c->stats["b_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * (4 + vb) + double(ne) * (nc * 4 + 1) * vb;
     c->stats["fcg_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * 12 + double(ne) * nc * 16;
     for (int i = 0; i < n; ++i) {
       if (iters) iters[i] = st.its[i];

@@ -70,24 +70,17 @@ int stream_tile_for(int nc, bool fp64);
 // than the tile).
 std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
 
-// A Chebyshev step fused into an SpMM epilogue, in one of two forms (off when dn is
-// null and xform is false):
-//  - residual form (dn set): with every output entry y(row, j) also dn(row, j) =
-//    c1 d(row, j) + c2 y(row, j) (in T with the coefficients rounded to T, as
-//    cheb_step) and x(row, j) += dn(row, j).  d may be the SpMM's input block; dn must
-//    not alias it.
-//  - iterate form (xform): the SpMM's value v is not stored; y(row, j) receives
-//    x + c1 (x - d) + c2 v at (row, j) instead (d null = 0).  x is normally the SpMM's
-//    input block and d the output block (each read only at its own row).
-// d, dn and x are [rows][ld].
+// A Chebyshev step fused into an SpMM epilogue (off when x is null): the SpMM's value
+// v is not stored; y(row, j) receives x + c1 (x - d) + c2 v at (row, j) instead (in T
+// with the coefficients rounded to T; d null = 0).  x is normally the SpMM's input
+// block and d the output block (each read only at its own row); d and x are
+// [rows][ld].
 template <typename T>
 struct ChebFuse {
-  const T *d = nullptr;
-  T *dn = nullptr, *x = nullptr;
+  const T *d = nullptr, *x = nullptr;
   int ld = 0;
   double c1 = 0.0, c2 = 0.0;
-  bool xform = false;
-  __host__ __device__ bool on() const { return dn != nullptr || xform; }
+  __host__ __device__ bool on() const { return x != nullptr; }
 };
 
 // Picks the fastest row mapping of the narrow SpMM for A on nc columns by timing
@@ -222,13 +215,6 @@ template <typename T, typename TP = T>
 void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const TP *p, const TP *q, T *x, T *r, double *rr,
             double *work, const Seg &sg = Seg());
 
-// Chebyshev smoother helpers (Dinv = 1 / l1 row norms):
-//   init: rres = Dinv .* b ; d = rres / theta ; x = d
-//   step: d = c1 * d + c2 * rres ; x += d
-template <typename T>
-void cheb_init(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *rres, T *d, T *x, double inv_theta);
-template <typename T>
-void cheb_step(hipStream_t s, int rows, int nc, T *d, const T *rres, T *x, double c1, double c2);
 // x = inv_theta * dinv[row] .* T(b)  (the first Chebyshev iterate from zero), and
 // bt = T(b) when bt is not null (the right-hand side in the cycle's precision)
 template <typename T>

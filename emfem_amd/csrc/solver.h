@@ -78,8 +78,6 @@ class InnerPc {
   // row mapping the smoother SpMM uses on nc columns: lanes per row, -1 = row blocks,
   // 0 = many-column kernel
   virtual int smoother_rows(hipStream_t s, int nc) = 0;
-  // whether the smoother SpMM on nc columns carries the fused Chebyshev update
-  virtual bool smoother_fused(hipStream_t s, int nc) = 0;
 };
 
 std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp,

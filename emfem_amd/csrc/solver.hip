@@ -261,7 +261,7 @@ class DevAms {
     if (nc <= nc_) return;
     nc_ = nc;
     r64_.resize((size_t)ne_ * nc);
-    rres_.resize((size_t)ne_ * nc);
+    wtmp_.resize((size_t)ne_ * nc);
     d_.resize((size_t)ne_ * nc);
     d2_.resize((size_t)ne_ * nc);
     if (!std::is_same<T, double>::value) bt_.resize((size_t)ne_ * nc);
@@ -329,17 +329,14 @@ class DevAms {
       // step (gather x_k, one right-hand-side block, x_{k-1} and x_k at the own row,
       // x_{k+1} over x_{k-1}), else the l1-scaled residual update
       d_.zero(s);
-      rres_.zero(s);
+      d2_.zero(s);
       ChebFuse<T> cf;
-      if (nc < 16) {
-        cf.xform = true;
-        cf.x = d_.get();
-        cf.d = rres_.get();
-        cf.ld = nc;
-        cf.c1 = 0.5;
-        cf.c2 = 0.5;
-      }
-      lpr_[nc] = tune_lpr<T, T, T>(s, B, nc, d_.get(), rres_.get(), l1inv_.get(), cf);
+      cf.x = d_.get();
+      cf.d = d2_.get();
+      cf.ld = nc;
+      cf.c1 = 0.5;
+      cf.c2 = 0.5;
+      lpr_[nc] = tune_lpr<T, T, T>(s, B, nc, d_.get(), d2_.get(), l1inv_.get(), cf);
       EMF_HIP(hipStreamSynchronize(s));
     }
   }
@@ -408,13 +405,6 @@ class DevAms {
   // 16 columns; at 16 or more only with the 16-byte-gather kernel (odd code or the
   // untuned default), whose epilogue moves whole vectors (the others store one element
   // at a time, slower than the separate streaming update).
-  bool fused(int nc) const {
-    if (nc < 16) return true;
-    auto it = lpr_.find(nc);
-    const int code = it == lpr_.end() ? 0 : it->second;
-    const bool vec_cols = nc % (16 / (int)sizeof(T)) == 0;
-    return vec_cols && (code == 0 || (code >= 17 && (code & 1)));
-  }
 
   Csr<T> smoother(int nc) {
     Csr<T> B = B_;
@@ -456,37 +446,35 @@ class DevAms {
   }
 
   // ms per launch of the smoother SpMM in the cycle's form on nc columns (lanes per
-  // row as tuned), on the cycle's own buffers: where fused(nc) the iterate-form
-  // Chebyshev step x_{k+1} = x_k + c1 (x_k - x_{k-1}) + c2 Dinv .* (w - B x_k) (x_k in
-  // d, x_{k-1} and x_{k+1} in d2, w = rres), else the residual update
-  // rres -= Dinv .* (B d)
+  // row as tuned), on the cycle's own buffers: the Chebyshev step x_{k+1} = x_k +
+  // c1 (x_k - x_{k-1}) + c2 Dinv .* (w - B x_k) with x_k in d, x_{k-1} and x_{k+1} in
+  // d2 and w a scratch block
   double time_b(hipStream_t s, int nc, int reps) {
     tune(s, nc);
     d_.zero(s);
     d2_.zero(s);
-    rres_.zero(s);
+    wtmp_.zero(s);
     ChebFuse<T> cf;
-    if (fused(nc)) {
-      cf.xform = true;
-      cf.x = d_.get();
-      cf.d = d2_.get();
-      cf.ld = nc;
-      cf.c1 = 0.5;
-      cf.c2 = 0.5;
-    }
+    cf.x = d_.get();
+    cf.d = d2_.get();
+    cf.ld = nc;
+    cf.c1 = 0.5;
+    cf.c2 = 0.5;
     const double ms = time_launches(s, reps, [&]() {
-      if (fused(nc))
-        bspmm(s, nc, d_.get(), d2_.get(), T(-1), l1inv_.get(), T(1), rres_.get(), T(0), nullptr, cf);
-      else
-        bspmm(s, nc, d_.get(), rres_.get(), T(-1), l1inv_.get(), T(0), nullptr, T(1), rres_.get(), cf);
+      bspmm(s, nc, d_.get(), d2_.get(), T(-1), l1inv_.get(), T(1), wtmp_.get(), T(0), nullptr, cf);
     });
     EMF_HIP(hipStreamSynchronize(s));
     return ms;
   }
 
  private:
+  // One cycle: Chebyshev from zero, the fp64 residual, the auxiliary corrections, and
+  // Chebyshev from the current iterate.  The Chebyshev iterates alternate between two
+  // blocks, d and out: pre-smoothing starts in d, and the blocks are chosen so that
+  // post-smoothing (deg writes) ends in out: the middle iterate lives in d for odd deg,
+  // in out for even deg.
   void launch_cycle(hipStream_t s, int nc, const double *b, T *out) {
-    T *rres = rres_.get(), *d = d_.get();
+    T *d = d_.get();
     const T *bt;
     T *btw = nullptr;  // the fp32 copy of b, when the cycle runs in fp32
     if constexpr (std::is_same<T, double>::value) {
@@ -495,56 +483,34 @@ class DevAms {
       btw = bt_.get();
       bt = btw;
     }
-    if (fused(nc)) {
-      // iterate-form Chebyshev in two blocks, d and out: pre-smoothing from zero starts
-      // in d; the buffers are chosen so that post-smoothing (deg writes, alternating)
-      // ends in out: the middle iterate lives in d for odd deg, in out for even deg
-      T *mid = (deg_ & 1) ? d : out, *other = (deg_ & 1) ? out : d;
-      cheb_first<T>(s, ne_, nc, l1inv_.get(), b, btw, d, c0_);
-      T *xm = cheb_x(s, nc, bt, out, d, true);
-      EMF_REQUIRE(xm == mid, kErrState, "Chebyshev buffer plan");
-      if (!additive_) aux_gradient(s, nc, b, mid);
-      residual64(s, nc, b, mid);
-      restrict_aux(s, nc);
-      amgPi_.vcycle(s, nc, vb_.get(), vx_.get());
-      prolong_aux(s, nc, mid);
-      if (!additive_) aux_gradient(s, nc, b, mid);
-      // post-smoothing from the current iterate: x_1 = x_0 + Dinv .* (bt - B x_0) / theta
-      ChebFuse<T> cf;
-      cf.xform = true;
-      cf.x = mid;
-      cf.ld = nc;
-      cf.c1 = 0.0;
-      cf.c2 = c0_;
-      bspmm(s, nc, mid, other, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr, cf);
-      xm = cheb_x(s, nc, bt, mid, other, false);
-      EMF_REQUIRE(xm == out, kErrState, "Chebyshev buffer plan");
-      return;
-    }
-    if (btw) axpby<double, T>(s, ne_, nc, 1.0, b, nc, 0.0, btw, nc);
-    // pre-smoothing (Chebyshev from zero)
-    cheb_init<T>(s, ne_, nc, l1inv_.get(), bt, rres, d, out, c0_);
-    cheb_iterate(s, nc, rres, d, out);
-    if (!additive_) aux_gradient(s, nc, b, out);
-    residual64(s, nc, b, out);
-    // the nodal-vector components (and, additive, the gradient) in one
-    // block-diagonal auxiliary solve
+    T *mid = (deg_ & 1) ? d : out, *other = (deg_ & 1) ? out : d;
+    cheb_first<T>(s, ne_, nc, l1inv_.get(), b, btw, d, c0_);
+    T *xm = cheb_x(s, nc, bt, out, d, true);
+    EMF_REQUIRE(xm == mid, kErrState, "Chebyshev buffer plan");
+    if (!additive_) aux_gradient(s, nc, b, mid);
+    residual64(s, nc, b, mid);
+    // the nodal-vector components (and, additive, the gradient) in one block-diagonal
+    // auxiliary solve
     restrict_aux(s, nc);
     amgPi_.vcycle(s, nc, vb_.get(), vx_.get());
-    prolong_aux(s, nc, out);
-    if (!additive_) aux_gradient(s, nc, b, out);
-    // post-smoothing (Chebyshev from the current iterate)
-    bspmm(s, nc, out, rres, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr);
-    axpby<T, T>(s, ne_, nc, c0_, rres, nc, 0.0, d, nc);
-    axpby<T, T>(s, ne_, nc, 1.0, d, nc, 1.0, out, nc);
-    cheb_iterate(s, nc, rres, d, out);
+    prolong_aux(s, nc, mid);
+    if (!additive_) aux_gradient(s, nc, b, mid);
+    // post-smoothing from the current iterate: x_1 = x_0 + c0 Dinv .* (bt - B x_0)
+    ChebFuse<T> cf;
+    cf.x = mid;
+    cf.ld = nc;
+    cf.c1 = 0.0;
+    cf.c2 = c0_;
+    bspmm(s, nc, mid, other, T(-1), l1inv_.get(), T(1), bt, T(0), nullptr, cf);
+    xm = cheb_x(s, nc, bt, mid, other, false);
+    EMF_REQUIRE(xm == out, kErrState, "Chebyshev buffer plan");
   }
 
   void bspmm(hipStream_t s, int nc, const T *x, T *y, T alpha, const T *rs, T gamma, const T *w, T beta,
              const T *z, const ChebFuse<T> &cf = ChebFuse<T>()) {
     // in-cycle timing (plain launches only): HIP events around the first launches of
     // the fused Chebyshev form
-    const bool timed = clock_on_ && cf.on() == fused(nc) && clock_.size() < 2 * kClockPairs;
+    const bool timed = clock_on_ && cf.on() && clock_.size() < 2 * kClockPairs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) {
       EMF_HIP(hipEventCreate(&e0));
@@ -585,16 +551,6 @@ class DevAms {
   // r64 = b - B x in fp64 (the residual the auxiliary spaces see)
   void residual64(hipStream_t s, int nc, const double *b, const T *x) {
     spmm_mixed<double, T, double>(s, B64(nc), nc, x, nc, r64_.get(), nc, -1.0, nullptr, 1.0, b, nc, 0.0, nullptr, 0);
-  }
-
-  // The Chebyshev steps in the residual form (cycles whose smoother SpMM does not take
-  // the fused step): rres -= l1inv .* (B d), then the streaming update d = c1 d + c2 rres
-  // and x += d.
-  void cheb_iterate(hipStream_t s, int nc, T *rres, T *d, T *x) {
-    for (int k = 1; k < deg_; ++k) {
-      bspmm(s, nc, d, rres, T(-1), l1inv_.get(), T(0), nullptr, T(1), rres);
-      cheb_step<T>(s, ne_, nc, d, rres, x, c1_[k], c2_[k]);
-    }
   }
 
   // vb = Pi^T r64 and x += Pi vx: the structured kernels, or the CSR matrices on request
@@ -654,7 +610,6 @@ class DevAms {
   T *cheb_x(hipStream_t s, int nc, const T *bt, T *prev, T *cur, bool prev_zero) {
     for (int k = 1; k < deg_; ++k) {
       ChebFuse<T> cf;
-      cf.xform = true;
       cf.x = cur;
       cf.d = prev_zero && k == 1 ? nullptr : prev;
       cf.ld = nc;
@@ -680,7 +635,7 @@ class DevAms {
   std::vector<double> c1_, c2_;
   Csr<T> B_;
   Csr<double> B64_;
-  DevBuf<T> bvals_, l1inv_, rres_, d_, d2_, bt_, vb_, vx_;
+  DevBuf<T> bvals_, l1inv_, d_, d2_, wtmp_, bt_, vb_, vx_;
   DevBuf<double> b64vals_, r64_;
   DevCsr<T> G0_, G0T_, Pi_, PiT_;
   bool aux_csr_ = false;
@@ -727,15 +682,6 @@ class InnerPcImpl : public InnerPc {
   int smoother_rows(hipStream_t s, int nc) override {
     ensure(nc);
     return fp32_ ? a32_->rows(s, nc) : a64_->rows(s, nc);
-  }
-  bool smoother_fused(hipStream_t s, int nc) override {
-    ensure(nc);
-    if (fp32_) {
-      a32_->tune(s, nc);
-      return a32_->fused(nc);
-    }
-    a64_->tune(s, nc);
-    return a64_->fused(nc);
   }
   double time_fcg(hipStream_t s, int nc, int reps) override {
     ensure(nc);

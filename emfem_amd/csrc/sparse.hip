@@ -33,9 +33,8 @@ __device__ __forceinline__ T shfl_xor(T v, int mask, int width) {
 
 // SpMM epilogue of one output entry in two steps: epi_load issues every operand load
 // (w, z, the row scale and the fused Chebyshev d and x) before any arithmetic, so they
-// are in flight together; epi_store writes y and, with a fused Chebyshev step
-// (ChebFuse in kernels.h), dn = c1 d + c2 y and x + dn in T, as cheb_step does, or in
-// the iterate form y = x + c1 (x - d) + c2 v in place of v.
+// are in flight together; epi_store writes y = v or, with a fused Chebyshev step
+// (ChebFuse in kernels.h), y = x + c1 (x - d) + c2 v.
 template <typename T>
 struct EpiIn {
   T w, z, s, d, x;
@@ -55,16 +54,8 @@ __device__ __forceinline__ EpiIn<T> epi_load(const T *w, size_t iw, bool gw, con
 
 template <typename T>
 __device__ __forceinline__ void epi_store(T *y, size_t iy, T v, const EpiIn<T> &e, const ChebFuse<T> &cf, size_t ic) {
-  if (cf.xform) {
-    y[iy] = e.x + T(cf.c1) * (e.x - e.d) + T(cf.c2) * v;
-    return;
-  }
-  y[iy] = v;
-  if (cf.dn) {
-    const T dv = T(cf.c1) * e.d + T(cf.c2) * v;
-    cf.dn[ic] = dv;
-    cf.x[ic] = e.x + dv;
-  }
+  (void)ic;
+  y[iy] = cf.on() ? e.x + T(cf.c1) * (e.x - e.d) + T(cf.c2) * v : v;
 }
 
 // Sums the NC per-lane partials over the LPR lanes of a row.  For power-of-two NC
@@ -434,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
     using TV = VecT<T, VEC>;
     // all operand vectors first, then the arithmetic
     const size_t ic = (size_t)row * cf.ld + c0;
-    TV wv{}, zv{}, dv{}, xv{}, dn;
+    TV wv{}, zv{}, dv{}, xv{};
     if (gamma != T(0)) wv = *reinterpret_cast<const TV *>(w + (size_t)row * ldw + c0);
     if (beta != T(0)) zv = *reinterpret_cast<const TV *>(z + (size_t)row * ldz + c0);
     if (cf.on()) {
@@ -454,20 +445,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
 #pragma unroll
       for (int j = 0; j < VEC; ++j) v.v[j] += zscale * zv.v[j];
     }
-    if (cf.xform) {
+    if (cf.on()) {
 #pragma unroll
       for (int j = 0; j < VEC; ++j) v.v[j] = xv.v[j] + T(cf.c1) * (xv.v[j] - dv.v[j]) + T(cf.c2) * v.v[j];
     }
     *reinterpret_cast<TV *>(y + (size_t)row * ldy + c0) = v;
-    if (cf.dn) {
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        dn.v[j] = T(cf.c1) * dv.v[j] + T(cf.c2) * v.v[j];
-        xv.v[j] += dn.v[j];
-      }
-      *reinterpret_cast<TV *>(cf.dn + ic) = dn;
-      *reinterpret_cast<TV *>(cf.x + ic) = xv;
-    }
     return;
   }
   EpiIn<T> e_[VEC];
@@ -903,8 +885,7 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     const size_t va = std::min<size_t>(16, sizeof(T) * vec);
     auto al = [&](const void *p, int ld) { return (uintptr_t)p % va == 0 && ((size_t)ld * sizeof(T)) % va == 0; };
     const int vepi = al(y, ldy) && (gamma == T(0) || al(w, ldw)) && (beta == T(0) || al(z, ldz)) &&
-                     (!cf.on() || ((cf.d == nullptr || al(cf.d, cf.ld)) && (cf.dn == nullptr || al(cf.dn, cf.ld)) &&
-                                   al(cf.x, cf.ld)));
+                     (!cf.on() || ((cf.d == nullptr || al(cf.d, cf.ld)) && al(cf.x, cf.ld)));
     const int rb = grid_for((int64_t)A.nrows * lw);
     const int nslab = (nc + W - 1) / W;
 #define EMF_WVEC(LW)                                                                                             \
@@ -953,7 +934,6 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     ChebFuse<T> cfc = cf;
     if (cf.on()) {
       if (cfc.d) cfc.d += c0;
-      if (cfc.dn) cfc.dn += c0;
       cfc.x += c0;
     }
 #define EMF_CASE(K)                                                                                             \
@@ -1703,27 +1683,6 @@ __global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, const doubl
 }
 
 template <typename T>
-__global__ void k_cheb_init(int64_t n, int nc, const T *__restrict__ dinv, const T *__restrict__ b, T *rres, T *d,
-                            T *x, double inv_theta) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const T rv = dinv[i / nc] * b[i];
-  rres[i] = rv;
-  const T dv = (T)(inv_theta * (double)rv);
-  d[i] = dv;
-  x[i] = dv;
-}
-
-template <typename T>
-__global__ void k_cheb_step(int64_t n, T *d, const T *__restrict__ rres, T *x, double c1, double c2) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const T dv = T(c1) * d[i] + T(c2) * rres[i];
-  d[i] = dv;
-  x[i] += dv;
-}
-
-template <typename T>
 __global__ void k_cheb_first(int64_t n, int nc, const T *__restrict__ dinv, const double *__restrict__ b, T *bt, T *x,
                              double inv_theta) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -1830,18 +1789,6 @@ template void fcg_xr<double, float>(hipStream_t, int, int, const double *, const
                                     double *, double *, double *, const Seg &);
 
 template <typename T>
-void cheb_init(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *rres, T *d, T *x, double inv_theta) {
-  const int64_t n = (int64_t)rows * nc;
-  hipLaunchKernelGGL((k_cheb_init<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, dinv, b, rres, d, x, inv_theta);
-  EMF_HIP(hipGetLastError());
-}
-template <typename T>
-void cheb_step(hipStream_t s, int rows, int nc, T *d, const T *rres, T *x, double c1, double c2) {
-  const int64_t n = (int64_t)rows * nc;
-  hipLaunchKernelGGL((k_cheb_step<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, rres, x, c1, c2);
-  EMF_HIP(hipGetLastError());
-}
-template <typename T>
 void cheb_first(hipStream_t s, int rows, int nc, const T *dinv, const double *b, T *bt, T *x, double inv_theta) {
   const int64_t n = (int64_t)rows * nc;
   hipLaunchKernelGGL((k_cheb_first<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, dinv, b, bt, x, inv_theta);
@@ -1861,8 +1808,6 @@ void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T
 }
 
 #define EMF_INST(T)                                                                                        \
-  template void cheb_init<T>(hipStream_t, int, int, const T *, const T *, T *, T *, T *, double);           \
-  template void cheb_step<T>(hipStream_t, int, int, T *, const T *, T *, double, double);                  \
   template void cheb_first<T>(hipStream_t, int, int, const T *, const double *, T *, T *, double);         \
   template void diag_scale<T>(hipStream_t, int, int, const T *, const T *, T *);                           \
   template void dense_solve<T>(hipStream_t, int, int, const double *, const T *, T *);
