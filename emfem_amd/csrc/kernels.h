@@ -209,11 +209,15 @@ void axpby(hipStream_t s, int rows, int nc, double alpha, const TX *x, int ldx, 
 void put_rows(hipStream_t s, int rows, int n, const int *idx, const double2 *v, double *y, int ldy);
 
 // Flexible CG updates on nc columns with device scalars:
-//   x += alpha[c] p ; r -= alpha[c] q ; rr[c] = r . r (fused reduction)
-// (x, r in T; the search directions p, q in TP; arithmetic in double)
+//   x += alpha[c] p (skipped when x is null) ; r -= alpha[c] q ; rr[c] = r . r
+// (fused reduction; x, r in T; the search directions p, q in TP; arithmetic in double)
 template <typename T, typename TP = T>
 void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const TP *p, const TP *q, T *x, T *r, double *rr,
             double *work, const Seg &sg = Seg());
+// x += alpha[k] p ; p = z + beta[k] p  (k = seg(row) * nc + column)
+template <typename TZ>
+void fcg_px(hipStream_t s, int rows, int nc, const double *alpha, const double *beta, const TZ *z, double *p, double *x,
+            const Seg &sg = Seg());
 
 // x = inv_theta * dinv[row] .* T(b)  (the first Chebyshev iterate from zero), and
 // bt = T(b) when bt is not null (the right-hand side in the cycle's precision)

@@ -796,9 +796,10 @@ class InnerPcImpl : public InnerPc {
     bspmm(s, nc, p, q);
     coldot<double, double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
     coldot<double, TZ>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
+    // x += alpha p is delayed into the pass that forms the next p (or done on exit)
     for (int it = 1; it <= o_.inner_max_it; ++it) {
       fcg_alpha(s, nl, rz_.get(), pq_.get(), active_.get(), alpha_.get());
-      fcg_xr<double, double>(s, ne_, nc, alpha_.get(), p, q, x, r, rr_.get(), w, sg_);
+      fcg_xr<double, double>(s, ne_, nc, alpha_.get(), nullptr, q, nullptr, r, rr_.get(), w, sg_);
       ++st.inner_its;
       EMF_HIP(hipMemcpyAsync(h_rr_.data(), rr_.get(), sizeof(double) * nl, hipMemcpyDeviceToHost, s));
       EMF_HIP(hipStreamSynchronize(s));
@@ -807,12 +808,15 @@ class InnerPcImpl : public InnerPc {
         if (h_active_[c] && h_rr_[c] <= tol2 * h_bb_[c]) h_active_[c] = 0;
         any |= h_active_[c] != 0;
       }
-      if (!any || it == o_.inner_max_it) break;
+      if (!any || it == o_.inner_max_it) {
+        axpby_cols<double, double>(s, ne_, nc, alpha_.get(), p, nc, ones_.get(), x, nc, sg_);
+        break;
+      }
       active_.upload(h_active_.data(), nl, s);
       precondition(s, nc, r, z, st);
       coldot2<TZ>(s, ne_, nc, z, nc, q, r, nc, zq_.get(), rz_.get(), w, sg_);
       fcg_beta(s, nl, zq_.get(), pq_.get(), active_.get(), beta_.get());
-      axpby_cols<TZ, double>(s, ne_, nc, ones_.get(), z, nc, beta_.get(), p, nc, sg_);
+      fcg_px<TZ>(s, ne_, nc, alpha_.get(), beta_.get(), z, p, x, sg_);
       bspmm(s, nc, p, q);
       coldot<double, double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
     }

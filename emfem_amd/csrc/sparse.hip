@@ -1665,12 +1665,28 @@ __global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, const doubl
   if (t < per_block && r0 < r1) {
     const int c = t % nc;
     const double a = alpha[scalar_index(sg.rseg, r0, nc, c)];  // a chunk lies in one segment
-    for (int row = r0 + t / nc; row < r1; row += rpp) {
-      const size_t o = (size_t)row * nc + c;
-      x[o] = (T)((double)x[o] + a * (double)p[o]);
-      const T rv = (T)((double)r[o] - a * (double)q[o]);
-      r[o] = rv;
-      acc += (double)rv * (double)rv;
+    // four rows per trip, every load issued first (clamped rows, masked updates)
+    for (int row = r0 + t / nc; row < r1; row += 4 * rpp) {
+      double rv[4], qv[4], xv[4], pv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const size_t o = (size_t)min(row + u * rpp, r1 - 1) * nc + c;
+        rv[u] = (double)r[o];
+        qv[u] = (double)q[o];
+        if (x) {
+          xv[u] = (double)x[o];
+          pv[u] = (double)p[o];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (row + u * rpp < r1) {
+          const size_t o = (size_t)(row + u * rpp) * nc + c;
+          if (x) x[o] = (T)(xv[u] + a * pv[u]);
+          const T nr = (T)(rv[u] - a * qv[u]);
+          r[o] = nr;
+          acc += (double)nr * (double)nr;
+        }
     }
   }
   sh[t] = acc;
@@ -1680,6 +1696,20 @@ __global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, const doubl
     for (int k = t; k < per_block; k += nc) sum += sh[k];
     work[(size_t)blockIdx.x * nc + t] = sum;
   }
+}
+
+// The delayed solution update of the flexible CG and the next search direction in
+// one pass: x += alpha p ; p = z + beta p (per (segment, column) scalars).
+template <typename TZ>
+__global__ void k_fcg_px(int rows, int nc, const double *__restrict__ alpha, const double *__restrict__ beta,
+                         const TZ *__restrict__ z, double *p, double *x, const int *rseg) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)rows * nc) return;
+  const int r = (int)(i / nc), c = (int)(i % nc);
+  const int k = scalar_index(rseg, r, nc, c);
+  const double pv = p[i];
+  x[i] = x[i] + alpha[k] * pv;
+  p[i] = (double)z[i] + beta[k] * pv;
 }
 
 template <typename T>
@@ -1787,6 +1817,19 @@ template void fcg_xr<double, double>(hipStream_t, int, int, const double *, cons
                                      double *, double *, double *, const Seg &);
 template void fcg_xr<double, float>(hipStream_t, int, int, const double *, const float *, const float *, double *,
                                     double *, double *, double *, const Seg &);
+
+template <typename TZ>
+void fcg_px(hipStream_t s, int rows, int nc, const double *alpha, const double *beta, const TZ *z, double *p, double *x,
+            const Seg &sg) {
+  const int64_t n = (int64_t)rows * nc;
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_fcg_px<TZ>), dim3(grid_for(n)), dim3(kBlock), 0, s, rows, nc, alpha, beta, z, p, x, sg.rseg);
+  EMF_HIP(hipGetLastError());
+}
+template void fcg_px<float>(hipStream_t, int, int, const double *, const double *, const float *, double *, double *,
+                            const Seg &);
+template void fcg_px<double>(hipStream_t, int, int, const double *, const double *, const double *, double *, double *,
+                             const Seg &);
 
 template <typename T>
 void cheb_first(hipStream_t s, int rows, int nc, const T *dinv, const double *b, T *bt, T *x, double inv_theta) {
