@@ -603,6 +603,13 @@ void galerkin_blocks(const AmsAux &X, const HostCsr &B, std::vector<double> (&va
 
 void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, AmsHost &H,
                std::shared_ptr<const AmsAux> aux) {
+  build_ams(
+      T, [&](const AmsAux &X, double *const out[4]) { galerkin_blocks(X, B, out); }, prm_in, H, std::move(aux));
+  H.l1inv = l1_inverse(B);
+}
+
+void build_ams(const Topology &T, const GalerkinFill &fill, const AmgParams &prm_in, AmsHost &H,
+               std::shared_ptr<const AmsAux> aux) {
   AmgParams prm = prm_in;
   prm.timing = &H.timing;
   double t0 = now_ms();
@@ -633,7 +640,7 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, Ams
     vg.resize(z);
     out[0] = vg.data();
   }
-  galerkin_blocks(*aux, B, out);
+  fill(*aux, out);
   H.timing["galerkin_ms"] += now_ms() - t0;
   if (!additive) {
     t0 = now_ms();
@@ -645,7 +652,6 @@ void build_ams(const Topology &T, const HostCsr &B, const AmgParams &prm_in, Ams
   t0 = now_ms();
   build_amg(std::move(AP), prm, H.amgPi);
   H.timing["amg_ms"] += now_ms() - t0;
-  H.l1inv = l1_inverse(B);
 }
 
 }  // namespace emfem

@@ -13,6 +13,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -98,6 +99,13 @@ struct AmsHost {
 // B given on the edge pattern (fp64 values).  `aux` from an earlier call on the same
 // mesh (same additive flag) is reused; otherwise it is built.
 void build_ams(const Topology &T, const HostCsr &B, const AmgParams &p, AmsHost &H,
+               std::shared_ptr<const AmsAux> aux = nullptr);
+
+// The same with the four Galerkin value arrays (as galerkin_blocks writes them) filled
+// by `fill` instead of from a host copy of B; H.l1inv is left empty (the device forms
+// the edge-space l1 scaling itself).
+using GalerkinFill = std::function<void(const AmsAux &, double *const[4])>;
+void build_ams(const Topology &T, const GalerkinFill &fill, const AmgParams &p, AmsHost &H,
                std::shared_ptr<const AmsAux> aux = nullptr);
 
 }  // namespace emfem

@@ -26,6 +26,30 @@
 
 using namespace emfem;
 
+// Device copies of the mesh-only matrices of the auxiliary-space Galerkin products
+// (for `aux`, held so that the cache cannot outlive it), and the four product value
+// blocks.
+struct DevGalerkin {
+  std::shared_ptr<const AmsAux> aux;
+  DevBuf<int> rrp, rci, prp, pci, arp, aci, missing;
+  DevBuf<double> rv[4], pv[4], out[4];
+  GalerkinDev view() const {
+    GalerkinDev g;
+    g.nv0 = aux ? aux->nv0 : 0;
+    g.rrp = rrp.get();
+    g.rci = rci.get();
+    g.prp = prp.get();
+    g.pci = pci.get();
+    g.arp = arp.get();
+    g.aci = aci.get();
+    for (int k = 0; k < 4; ++k) {
+      g.rv[k] = rv[k].get();
+      g.pv[k] = pv[k].get();
+    }
+    return g;
+  }
+};
+
 struct emfem_context {
   hipStream_t stream = nullptr;
   bool device = false;  // false: host-only context (topology, options, point location)
@@ -63,6 +87,7 @@ struct emfem_context {
   // set-up) and the mesh-only auxiliary matrices, both kept while the linear system
   // lives
   HostCsr h_B;
+  DevGalerkin gal;
   std::shared_ptr<const AmsAux> ams_aux;
   std::unique_ptr<InnerPc> pc;
 

@@ -74,6 +74,7 @@ const std::map<std::string, std::string> &option_defaults() {
       {"ams_smooth_levels", "0"},
       {"ams_cycle", "additive"},
       {"ams_transfer", "structured"},
+      {"ams_setup", "device"},
       {"surrogate_cycles", "false"},
       {"surrogate_min_corr", "0.9"},
   };
@@ -93,6 +94,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"spmm_rows", {"auto", "lanes", "rowblock", "pipelined", "xblock", "2", "4", "8", "16"}},
       {"ams_cycle", {"multiplicative", "additive"}},
       {"ams_transfer", {"structured", "csr"}},
+      {"ams_setup", {"device", "host"}},
       {"ams_cheb_kind", {"first", "fourth"}},
       {"surrogate_cycles", {"true", "false"}},
       {"save_mesh", {"true", "false"}},
@@ -467,22 +469,52 @@ int emfem_get_solution(const emfem_context *c, int slot, double *re, double *im)
   });
 }
 
+namespace {
+
+// Uploads the mesh-only matrices of X for the device Galerkin products; the four
+// restrictions and the four interpolations must each share one pattern.
+void upload_galerkin(DevGalerkin &G, const std::shared_ptr<const AmsAux> &aux, hipStream_t s) {
+  const AmsAux &X = *aux;
+  const HostCsr *R[4] = {&X.G0T, &X.PT[0], &X.PT[1], &X.PT[2]};
+  const HostCsr *P[4] = {&X.G0, &X.P[0], &X.P[1], &X.P[2]};
+  for (int k = 1; k < 4; ++k)
+    EMF_REQUIRE(R[k]->rp == R[0]->rp && R[k]->ci == R[0]->ci && P[k]->rp == P[0]->rp && P[k]->ci == P[0]->ci,
+                kErrState, "auxiliary spaces: patterns differ between blocks");
+  G.rrp.upload(R[0]->rp, s);
+  G.rci.upload(R[0]->ci, s);
+  G.prp.upload(P[0]->rp, s);
+  G.pci.upload(P[0]->ci, s);
+  G.arp.upload(X.Apat.rp, s);
+  G.aci.upload(X.Apat.ci, s);
+  for (int k = 0; k < 4; ++k) {
+    G.rv[k].upload(R[k]->v, s);
+    G.pv[k].upload(P[k]->v, s);
+  }
+  G.missing.resize(1);
+  EMF_HIP(hipStreamSynchronize(s));
+  G.aux = aux;
+}
+
+}  // namespace
+
 int emfem_create_pc(emfem_context *c) {
   return guarded(c, [&]() {
     require_system(c);
     EMF_REQUIRE(c->assembled_fidx >= 0, kErrState, "matrix not assembled");
     const double t0 = now_ms();
     hipStream_t s = c->stream;
-    // B = C + M (formed on the device) on the host pattern for the auxiliary-space
-    // set-up
+    // B = C + M formed on the device; by default the Galerkin products of the auxiliary
+    // blocks and the l1 scaling are formed there too and only the products travel to
+    // the host AMG set-up (-ams_setup host: B travels and the host forms both)
+    const bool on_device = c->opt("ams_setup") == "device";
+    DevBuf<double> bsum(c->nnz);
+    make_b<double>(s, c->nnz, c->d_K.get(), bsum.get());
     HostCsr &B = c->h_B;
-    {
-      DevBuf<double> bsum(c->nnz);
-      make_b<double>(s, c->nnz, c->d_K.get(), bsum.get());
+    if (!on_device) {
       B.v.resize(c->nnz);
       bsum.download(B.v.data(), c->nnz, s);
-      EMF_HIP(hipStreamSynchronize(s));
     }
+    EMF_HIP(hipStreamSynchronize(s));
     PcOptions o = pc_options(c);
     c->pc.reset();
     HostCsr recycle;
@@ -490,7 +522,28 @@ int emfem_create_pc(emfem_context *c) {
     c->ams_host.reset(new AmsHost());
     c->ams_host->recycle = std::move(recycle);
     const double tb = now_ms();
-    build_ams(c->T, B, o.amg, *c->ams_host, c->ams_aux);
+    if (on_device) {
+      auto fill = [&](const AmsAux &X, double *const out[4]) {
+        DevGalerkin &G = c->gal;
+        EMF_REQUIRE(c->ams_host->aux.get() == &X, kErrState, "auxiliary spaces: set-up order");
+        if (G.aux != c->ams_host->aux) upload_galerkin(G, c->ams_host->aux, s);
+        const size_t z = X.Apat.ci.size();
+        double *dv[4];
+        for (int k = 0; k < 4; ++k) {
+          G.out[k].resize(z);
+          dv[k] = G.out[k].get();
+        }
+        galerkin_dev(s, G.view(), c->d_rp.get(), c->d_ci.get(), bsum.get(), dv, G.missing.get());
+        int miss = 0;
+        EMF_HIP(hipMemcpyAsync(&miss, G.missing.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        for (int k = 0; k < 4; ++k) G.out[k].download(out[k], z, s);
+        EMF_HIP(hipStreamSynchronize(s));
+        EMF_REQUIRE(miss == 0, kErrArg, "Galerkin blocks: B has couplings outside the tet pattern");
+      };
+      build_ams(c->T, fill, o.amg, *c->ams_host, c->ams_aux);
+    } else {
+      build_ams(c->T, B, o.amg, *c->ams_host, c->ams_aux);
+    }
     c->ams_aux = c->ams_host->aux;
     const double t1 = now_ms();
     c->stats["pc_download_ms"] = tb - t0;

@@ -153,6 +153,26 @@ template <typename TX, typename TY = TX>
 void coldot(hipStream_t s, int rows, int nc, const TX *x, int ldx, const TY *y, int ldy, double *out, double *work,
             const Seg &sg = Seg());
 
+// Device copy of the mesh-only matrices of the auxiliary-space Galerkin products: the
+// restrictions R_k = (G0^T, Pi_x^T, Pi_y^T, Pi_z^T) share one pattern (rrp, rci), the
+// interpolations P_k = (G0, Pi_x, Pi_y, Pi_z) another (prp, pci), and the four
+// products the pattern Apat (arp, aci) on the nv0 interior vertices.
+struct GalerkinDev {
+  int nv0 = 0;
+  const int *rrp = nullptr, *rci = nullptr, *prp = nullptr, *pci = nullptr, *arp = nullptr, *aci = nullptr;
+  const double *rv[4] = {nullptr, nullptr, nullptr, nullptr}, *pv[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+// vals[k] (Apat layout) = P_k^T B P_k for B on the edge pattern (brp, bci, bv): one
+// thread per vertex row, each value summed in the order of the host's galerkin_blocks
+// with separately rounded products and sums (bitwise equal to it).  missing (device
+// int, zeroed by the call) counts couplings of B outside Apat.
+void galerkin_dev(hipStream_t s, const GalerkinDev &G, const int *brp, const int *bci, const double *bv,
+                  double *const vals[4], int *missing);
+// l1[row] = T(1 / sum_j |v_j|) over the row in order (0 for a zero row), as the host's
+// l1_inverse followed by a conversion to T.
+template <typename T>
+void l1_inverse_dev(hipStream_t s, int rows, const int *rp, const double *v, T *l1);
+
 // The auxiliary-space transfers through the structure of the Nedelec interpolation
 // Pi = [G0 | Pi_x | Pi_y | Pi_z] (nblk = 4; [Pi_x | Pi_y | Pi_z] when nblk = 3): edge
 // e = (a, b) couples the interior vertices among its endpoints with -1 (a) / +1 (b) in

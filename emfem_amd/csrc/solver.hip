@@ -193,7 +193,13 @@ class DevAms {
       make_b<double>(s, nnz, K, b64vals_.get());
       B64_.v = b64vals_.get();
     }
-    upload_vec(l1inv_, H.l1inv, s);
+    if (H.l1inv.empty()) {
+      // the edge-space l1 scaling from the fp64 B on the device
+      l1inv_.resize(ne);
+      l1_inverse_dev<T>(s, ne, rp, B64_.v, l1inv_.get());
+    } else {
+      upload_vec(l1inv_, H.l1inv, s);
+    }
     const AmsAux &X = *H.aux;
     additive_ = X.additive;
     if (!additive_) {

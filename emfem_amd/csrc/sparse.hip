@@ -1489,6 +1489,79 @@ __global__ __launch_bounds__(kBlock) void k_aux_prolong(int ne, int nv0, int nbl
 
 }  // namespace
 
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_galerkin(GalerkinDev G, const int *__restrict__ brp,
+                                                     const int *__restrict__ bci, const double *__restrict__ bv,
+                                                     double *v0, double *v1, double *v2, double *v3, int *missing) {
+#pragma clang fp contract(off)
+  const int u = blockIdx.x * kBlock + threadIdx.x;
+  if (u >= G.nv0) return;
+  double *vals[4] = {v0, v1, v2, v3};
+  const int a0 = G.arp[u], a1 = G.arp[u + 1];
+  for (int t = a0; t < a1; ++t)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) vals[k][t] = 0.0;
+  int miss = 0;
+  for (int p = G.rrp[u]; p < G.rrp[u + 1]; ++p) {
+    const int ed = G.rci[p];
+    double r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = G.rv[k][p];
+    for (int q = brp[ed]; q < brp[ed + 1]; ++q) {
+      const int f = bci[q];
+      const double b = bv[q];
+      double ra[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) ra[k] = r[k] * b;
+      for (int s2 = G.prp[f]; s2 < G.prp[f + 1]; ++s2) {
+        const int col = G.pci[s2];
+        int t = a0;
+        while (t < a1 && G.aci[t] != col) ++t;
+        if (t == a1) {
+          ++miss;
+          continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double prod = ra[k] * G.pv[k][s2];
+          vals[k][t] = vals[k][t] + prod;
+        }
+      }
+    }
+  }
+  if (miss) atomicAdd(missing, miss);
+}
+
+template <typename T>
+__global__ void k_l1_inverse(int rows, const int *__restrict__ rp, const double *__restrict__ v, T *l1) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  double s = 0.0;
+  for (int p = rp[r]; p < rp[r + 1]; ++p) s += fabs(v[p]);
+  l1[r] = (T)(s > 0.0 ? 1.0 / s : 0.0);
+}
+
+}  // namespace
+
+void galerkin_dev(hipStream_t s, const GalerkinDev &G, const int *brp, const int *bci, const double *bv,
+                  double *const vals[4], int *missing) {
+  EMF_HIP(hipMemsetAsync(missing, 0, sizeof(int), s));
+  if (G.nv0 == 0) return;
+  hipLaunchKernelGGL(k_galerkin, dim3((G.nv0 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, G, brp, bci, bv, vals[0],
+                     vals[1], vals[2], vals[3], missing);
+  EMF_HIP(hipGetLastError());
+}
+template <typename T>
+void l1_inverse_dev(hipStream_t s, int rows, const int *rp, const double *v, T *l1) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL((k_l1_inverse<T>), dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rows, rp, v, l1);
+  EMF_HIP(hipGetLastError());
+}
+template void l1_inverse_dev<float>(hipStream_t, int, const int *, const double *, float *);
+template void l1_inverse_dev<double>(hipStream_t, int, const int *, const double *, double *);
+
 template <typename T>
 void aux_restrict(hipStream_t s, const AuxMap<T> &M, int nc, const double *r, T *vb) {
   EMF_REQUIRE(M.nblk == 3 || M.nblk == 4, kErrArg, "auxiliary transfer: 3 or 4 blocks");
