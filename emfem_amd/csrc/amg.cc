@@ -625,10 +625,15 @@ void build_ams(const Topology &T, const GalerkinFill &fill, const AmgParams &prm
   const bool additive = aux->additive;
   const int64_t z = aux->Apat.rp[aux->Apat.n];
   HostCsr AP = std::move(H.recycle);
-  AP.n = aux->APpat.n;
-  AP.m = aux->APpat.m;
-  AP.rp.assign(aux->APpat.rp.begin(), aux->APpat.rp.end());
-  AP.ci.assign(aux->APpat.ci.begin(), aux->APpat.ci.end());
+  const bool same = H.recycle_aux == aux && AP.n == aux->APpat.n && AP.m == aux->APpat.m &&
+                    AP.rp.size() == aux->APpat.rp.size() && AP.ci.size() == aux->APpat.ci.size();
+  H.recycle_aux.reset();
+  if (!same) {
+    AP.n = aux->APpat.n;
+    AP.m = aux->APpat.m;
+    AP.rp.assign(aux->APpat.rp.begin(), aux->APpat.rp.end());
+    AP.ci.assign(aux->APpat.ci.begin(), aux->APpat.ci.end());
+  }
   AP.v.resize(AP.ci.size());
   std::vector<double> vg;
   double *out[4];

@@ -88,8 +88,10 @@ void galerkin_blocks(const AmsAux &X, const HostCsr &B, std::vector<double> (&va
 
 struct AmsHost {
   std::shared_ptr<const AmsAux> aux;
-  // storage of an earlier set-up's block-diagonal Galerkin stack, reused by build_ams
+  // storage of an earlier set-up's block-diagonal Galerkin stack, reused by build_ams,
+  // and the auxiliary spaces it was built for (same spaces: its pattern is kept)
   HostCsr recycle;
+  std::shared_ptr<const AmsAux> recycle_aux;
   AmgHost amgG;   // AMG of G0^T B G0 (multiplicative cycle only)
   AmgHost amgPi;  // AMG of diag(P_k^T B P_k)
   std::vector<double> l1inv;  // edge-space l1 row norms of B, inverted

@@ -518,9 +518,14 @@ int emfem_create_pc(emfem_context *c) {
     PcOptions o = pc_options(c);
     c->pc.reset();
     HostCsr recycle;
-    if (c->ams_host && !c->ams_host->amgPi.levels.empty()) recycle = std::move(c->ams_host->amgPi.levels[0].A);
+    std::shared_ptr<const AmsAux> recycle_aux;
+    if (c->ams_host && !c->ams_host->amgPi.levels.empty()) {
+      recycle = std::move(c->ams_host->amgPi.levels[0].A);
+      recycle_aux = c->ams_host->aux;
+    }
     c->ams_host.reset(new AmsHost());
     c->ams_host->recycle = std::move(recycle);
+    c->ams_host->recycle_aux = std::move(recycle_aux);
     const double tb = now_ms();
     if (on_device) {
       auto fill = [&](const AmsAux &X, double *const out[4]) {
