@@ -75,11 +75,14 @@ std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
 // with the coefficients rounded to T; d null = 0).  x is normally the SpMM's input
 // block and d the output block (each read only at its own row); d and x are
 // [rows][ld].
+// Independently of the step, zcol (device, indexed by the column j of [rows][ld])
+// scales the SpMM's z term per column: beta * zcol[j] * z(row, j).
 template <typename T>
 struct ChebFuse {
   const T *d = nullptr, *x = nullptr;
   int ld = 0;
   double c1 = 0.0, c2 = 0.0;
+  const double *zcol = nullptr;
   __host__ __device__ bool on() const { return x != nullptr; }
 };
 

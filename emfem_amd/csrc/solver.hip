@@ -796,10 +796,17 @@ class InnerPcImpl : public InnerPc {
     if (!any) return;
     active_.upload(h_active_.data(), nl, s);
     const double tol2 = o_.inner_rtol * o_.inner_rtol;
-    // z = M r ; p = z ; q = B p
+    // z = M r ; p = z ; q = B p.  With an fp32 z on 16 or more columns of one segment
+    // the product is formed from z instead, q = B z + beta q (beta per column in the
+    // SpMM's z term): the many-column SpMM is bound by its gathers, and z gathers half
+    // the bytes of p
+    const bool from_z = std::is_same<TZ, float>::value && nc >= 16 && sg_.nseg == 1;
     precondition(s, nc, r, z, st);
     axpby<TZ, double>(s, ne_, nc, 1.0, z, nc, 0.0, p, nc);
-    bspmm(s, nc, p, q);
+    if (from_z)
+      spmm_mixed<double, TZ, double>(s, B64(s, nc), nc, z, nc, q, nc, 1.0, nullptr, 0.0, nullptr, 0, 0.0, nullptr, 0);
+    else
+      bspmm(s, nc, p, q);
     coldot<double, double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
     coldot<double, TZ>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
     // x += alpha p is delayed into the pass that forms the next p (or done on exit)
@@ -823,7 +830,15 @@ class InnerPcImpl : public InnerPc {
       coldot2<TZ>(s, ne_, nc, z, nc, q, r, nc, zq_.get(), rz_.get(), w, sg_);
       fcg_beta(s, nl, zq_.get(), pq_.get(), active_.get(), beta_.get());
       fcg_px<TZ>(s, ne_, nc, alpha_.get(), beta_.get(), z, p, x, sg_);
-      bspmm(s, nc, p, q);
+      if (from_z) {
+        ChebFuse<double> zb;
+        zb.zcol = beta_.get();
+        zb.ld = nc;
+        spmm_ex<double, TZ, double>(s, B64(s, nc), nc, z, nc, q, nc, 1.0, nullptr, 0.0, nullptr, 0, 1.0, q, nc,
+                                    nullptr, nullptr, zb);
+      } else {
+        bspmm(s, nc, p, q);
+      }
       coldot<double, double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
     }
   }

@@ -46,6 +46,7 @@ __device__ __forceinline__ EpiIn<T> epi_load(const T *w, size_t iw, bool gw, con
   EpiIn<T> e;
   e.w = gw ? w[iw] : T(0);
   e.z = gz ? z[iz] : T(0);
+  if (gz && cf.zcol) e.z *= (T)cf.zcol[ic - (size_t)row * cf.ld];
   e.s = rs ? rs[row] : T(1);
   e.d = cf.on() && cf.d ? cf.d[ic] : T(0);
   e.x = cf.on() ? cf.x[ic] : T(0);
@@ -427,7 +428,13 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
     const size_t ic = (size_t)row * cf.ld + c0;
     TV wv{}, zv{}, dv{}, xv{};
     if (gamma != T(0)) wv = *reinterpret_cast<const TV *>(w + (size_t)row * ldw + c0);
-    if (beta != T(0)) zv = *reinterpret_cast<const TV *>(z + (size_t)row * ldz + c0);
+    if (beta != T(0)) {
+      zv = *reinterpret_cast<const TV *>(z + (size_t)row * ldz + c0);
+      if (cf.zcol) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) zv.v[j] *= (T)cf.zcol[c0 + j];
+      }
+    }
     if (cf.on()) {
       if (cf.d) dv = *reinterpret_cast<const TV *>(cf.d + ic);
       xv = *reinterpret_cast<const TV *>(cf.x + ic);
@@ -935,6 +942,9 @@ void spmm_ex(hipStream_t s, const Csr<TA> &A, int nc, const TX *x, int ldx, T *y
     if (cf.on()) {
       if (cfc.d) cfc.d += c0;
       cfc.x += c0;
+    }
+    if (cfc.zcol) {
+      cfc.zcol += c0;
     }
 #define EMF_CASE(K)                                                                                             \
   case K:                                                                                                       \
