@@ -57,8 +57,16 @@ struct DevCsr {
     nnz = (int)H.ci.size();
     rp.upload(H.rp, s);
     ci.upload(H.ci, s);
-    std::vector<T> tv(H.v.begin(), H.v.end());
-    v.upload(tv, s);
+    std::vector<T> tv;  // kept until the copy has completed
+    if constexpr (std::is_same<T, double>::value) {
+      v.upload(H.v, s);
+    } else {
+      tv.resize(H.v.size());
+      parallel_ranges((int64_t)tv.size(), [&](int64_t b, int64_t e, int) {
+        for (int64_t i = b; i < e; ++i) tv[i] = (T)H.v[i];
+      });
+      v.upload(tv, s);
+    }
     EMF_HIP(hipStreamSynchronize(s));
   }
   Csr<T> view() const {
@@ -592,16 +600,19 @@ class DevAms {
     EMF_REQUIRE(ne == ne_ && X.G0T.n == X.nv0, kErrState, "auxiliary map: shapes");
     std::vector<int2> ev(ne, make_int2(-1, -1));
     std::vector<T> th((size_t)ne * 4, T(0));
-    for (int e = 0; e < ne; ++e) {
-      for (int p = X.G0.rp[e]; p < X.G0.rp[e + 1]; ++p) (X.G0.v[p] < 0 ? ev[e].x : ev[e].y) = X.G0.ci[p];
-      for (int d = 0; d < 3; ++d)
-        if (X.P[d].rp[e] < X.P[d].rp[e + 1]) th[(size_t)e * 4 + d] = (T)X.P[d].v[X.P[d].rp[e]];
-    }
+    parallel_ranges(ne, [&](int64_t b, int64_t e1, int) {
+      for (int64_t e = b; e < e1; ++e) {
+        for (int p = X.G0.rp[e]; p < X.G0.rp[e + 1]; ++p) (X.G0.v[p] < 0 ? ev[e].x : ev[e].y) = X.G0.ci[p];
+        for (int d = 0; d < 3; ++d)
+          if (X.P[d].rp[e] < X.P[d].rp[e + 1]) th[(size_t)e * 4 + d] = (T)X.P[d].v[X.P[d].rp[e]];
+      }
+    });
     std::vector<int> vcode(X.G0T.ci.size());
-    for (size_t p = 0; p < vcode.size(); ++p) {
-      EMF_REQUIRE(X.G0T.ci[p] >= 0, kErrState, "auxiliary map: edge index");
-      vcode[p] = (int)((uint32_t)X.G0T.ci[p] | (X.G0T.v[p] < 0 ? 0x80000000u : 0u));
-    }
+    for (size_t p = 0; p < vcode.size(); ++p) EMF_REQUIRE(X.G0T.ci[p] >= 0, kErrState, "auxiliary map: edge index");
+    parallel_ranges((int64_t)vcode.size(), [&](int64_t b, int64_t e1, int) {
+      for (int64_t p = b; p < e1; ++p)
+        vcode[p] = (int)((uint32_t)X.G0T.ci[p] | (X.G0T.v[p] < 0 ? 0x80000000u : 0u));
+    });
     ev_.upload(ev, s);
     th_.upload(th, s);
     vrp_.upload(X.G0T.rp, s);
