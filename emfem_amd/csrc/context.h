@@ -33,6 +33,25 @@ struct DevGalerkin {
   std::shared_ptr<const AmsAux> aux;
   DevBuf<int> rrp, rci, prp, pci, arp, aci, missing;
   DevBuf<double> rv[4], pv[4], out[4];
+  // pinned host staging of the four blocks (page-locked copies run at full link speed)
+  double *pinned = nullptr;
+  size_t pinned_n = 0;
+  DevGalerkin() = default;
+  DevGalerkin(const DevGalerkin &) = delete;
+  DevGalerkin &operator=(const DevGalerkin &) = delete;
+  ~DevGalerkin() {
+    if (pinned) (void)hipHostFree(pinned);
+  }
+  double *staging(size_t n) {
+    if (n > pinned_n) {
+      if (pinned) EMF_HIP(hipHostFree(pinned));
+      pinned = nullptr;
+      pinned_n = 0;
+      EMF_HIP(hipHostMalloc((void **)&pinned, n * sizeof(double), hipHostMallocDefault));
+      pinned_n = n;
+    }
+    return pinned;
+  }
   GalerkinDev view() const {
     GalerkinDev g;
     g.nv0 = aux ? aux->nv0 : 0;

@@ -541,9 +541,13 @@ int emfem_create_pc(emfem_context *c) {
         galerkin_dev(s, G.view(), c->d_rp.get(), c->d_ci.get(), bsum.get(), dv, G.missing.get());
         int miss = 0;
         EMF_HIP(hipMemcpyAsync(&miss, G.missing.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-        for (int k = 0; k < 4; ++k) G.out[k].download(out[k], z, s);
+        double *st = G.staging(4 * z);
+        for (int k = 0; k < 4; ++k) G.out[k].download(st + k * z, z, s);
         EMF_HIP(hipStreamSynchronize(s));
         EMF_REQUIRE(miss == 0, kErrArg, "Galerkin blocks: B has couplings outside the tet pattern");
+        parallel_ranges((int64_t)z, [&](int64_t b, int64_t e, int) {
+          for (int k = 0; k < 4; ++k) std::copy(st + k * z + b, st + k * z + e, out[k] + b);
+        });
       };
       build_ams(c->T, fill, o.amg, *c->ams_host, c->ams_aux);
     } else {

@@ -1501,29 +1501,25 @@ __global__ __launch_bounds__(kBlock) void k_aux_prolong(int ne, int nv0, int nbl
 
 namespace {
 
+// One thread per (vertex row, block k): the four blocks of a row are independent sums.
 __global__ __launch_bounds__(kBlock) void k_galerkin(GalerkinDev G, const int *__restrict__ brp,
                                                      const int *__restrict__ bci, const double *__restrict__ bv,
                                                      double *v0, double *v1, double *v2, double *v3, int *missing) {
 #pragma clang fp contract(off)
-  const int u = blockIdx.x * kBlock + threadIdx.x;
-  if (u >= G.nv0) return;
-  double *vals[4] = {v0, v1, v2, v3};
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)G.nv0 * 4) return;
+  const int u = (int)(i >> 2), k = (int)(i & 3);
+  double *vals = k == 0 ? v0 : (k == 1 ? v1 : (k == 2 ? v2 : v3));
+  const double *rv = G.rv[k], *pv = G.pv[k];
   const int a0 = G.arp[u], a1 = G.arp[u + 1];
-  for (int t = a0; t < a1; ++t)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) vals[k][t] = 0.0;
+  for (int t = a0; t < a1; ++t) vals[t] = 0.0;
   int miss = 0;
   for (int p = G.rrp[u]; p < G.rrp[u + 1]; ++p) {
     const int ed = G.rci[p];
-    double r[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) r[k] = G.rv[k][p];
+    const double r = rv[p];
     for (int q = brp[ed]; q < brp[ed + 1]; ++q) {
       const int f = bci[q];
-      const double b = bv[q];
-      double ra[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) ra[k] = r[k] * b;
+      const double ra = r * bv[q];
       for (int s2 = G.prp[f]; s2 < G.prp[f + 1]; ++s2) {
         const int col = G.pci[s2];
         int t = a0;
@@ -1532,15 +1528,12 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(GalerkinDev G, const int *_
           ++miss;
           continue;
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const double prod = ra[k] * G.pv[k][s2];
-          vals[k][t] = vals[k][t] + prod;
-        }
+        const double prod = ra * pv[s2];
+        vals[t] = vals[t] + prod;
       }
     }
   }
-  if (miss) atomicAdd(missing, miss);
+  if (miss && k == 0) atomicAdd(missing, miss);
 }
 
 template <typename T>
@@ -1559,8 +1552,8 @@ void galerkin_dev(hipStream_t s, const GalerkinDev &G, const int *brp, const int
                   double *const vals[4], int *missing) {
   EMF_HIP(hipMemsetAsync(missing, 0, sizeof(int), s));
   if (G.nv0 == 0) return;
-  hipLaunchKernelGGL(k_galerkin, dim3((G.nv0 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, G, brp, bci, bv, vals[0],
-                     vals[1], vals[2], vals[3], missing);
+  hipLaunchKernelGGL(k_galerkin, dim3(((int64_t)G.nv0 * 4 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, G, brp, bci, bv,
+                     vals[0], vals[1], vals[2], vals[3], missing);
   EMF_HIP(hipGetLastError());
 }
 template <typename T>
