@@ -768,6 +768,33 @@ int or_max_layers(void) { return OR_MAXLAY; }
 /* MT right-hand side (src/em_fwd.cc:402-580)                                 */
 /* ------------------------------------------------------------------------- */
 
+/* Weights of the four corner columns at the edge midpoint ec (src/em_fwd.cc:438-471): a
+ * cosine taper between the two corners of a side face (x = x_min, y = y_min, x = x_max,
+ * y = y_max, tested in that order), bilinear in (x, y) elsewhere (top and bottom faces).
+ * corners[4][3] = (x_min, y_min), (x_max, y_min), (x_max, y_max), (x_min, y_max). */
+void or_blend_weights(const double *corners, const double *ec, double *coeffs) {
+  const double(*tc)[3] = (const double(*)[3])corners;
+  if (fabs(ec[0] - tc[0][0]) < OR_EPS) {
+    double s = (1.0 - cos(OR_PI * (ec[1] - tc[0][1]) / (tc[2][1] - tc[0][1]))) / 2.0;
+    coeffs[0] = 1.0 - s; coeffs[1] = 0.0; coeffs[2] = 0.0; coeffs[3] = s;
+  } else if (fabs(ec[1] - tc[0][1]) < OR_EPS) {
+    double s = (1.0 - cos(OR_PI * (ec[0] - tc[0][0]) / (tc[2][0] - tc[0][0]))) / 2.0;
+    coeffs[0] = 1.0 - s; coeffs[1] = s; coeffs[2] = 0.0; coeffs[3] = 0.0;
+  } else if (fabs(ec[0] - tc[2][0]) < OR_EPS) {
+    double s = (1.0 - cos(OR_PI * (ec[1] - tc[0][1]) / (tc[2][1] - tc[0][1]))) / 2.0;
+    coeffs[0] = 0.0; coeffs[1] = 1.0 - s; coeffs[2] = s; coeffs[3] = 0.0;
+  } else if (fabs(ec[1] - tc[2][1]) < OR_EPS) {
+    double s = (1.0 - cos(OR_PI * (ec[0] - tc[0][0]) / (tc[2][0] - tc[0][0]))) / 2.0;
+    coeffs[0] = 0.0; coeffs[1] = 0.0; coeffs[2] = s; coeffs[3] = 1.0 - s;
+  } else {
+    double area = (tc[2][0] - tc[0][0]) * (tc[2][1] - tc[0][1]);
+    coeffs[0] = ((tc[2][0] - ec[0]) * (tc[2][1] - ec[1])) / area;
+    coeffs[1] = ((ec[0] - tc[3][0]) * (tc[3][1] - ec[1])) / area;
+    coeffs[2] = ((ec[0] - tc[0][0]) * (ec[1] - tc[0][1])) / area;
+    coeffs[3] = ((tc[1][0] - ec[0]) * (ec[1] - tc[1][1])) / area;
+  }
+}
+
 /* Boundary values: 1-D fields at the 4 corners blended over the edge midpoint, projected on
  * the edge (src/em_fwd.cc:402-486).  bval is (re, im) per edge; only boundary edges are set.
  * The first bdr cell (ascending tet index) that holds an edge defines its value. */
@@ -776,7 +803,6 @@ int or_boundary_values(int nt, const double *pts, const int *tets, const int *te
                        const double *lsig, double freq, int mode, int ne, double *bval, unsigned char *have) {
   or_pw pw[4];
   int k, t, i;
-  const double(*tc)[3] = (const double(*)[3])corners;
   for (k = 0; k < 4; ++k) pw_init(&pw[k], nlay[k], &ztop[k * OR_MAXLAY], &lsig[k * OR_MAXLAY], freq);
   memset(have, 0, (size_t)ne);
   for (t = 0; t < nt; ++t) {
@@ -791,25 +817,7 @@ int or_boundary_values(int nt, const double *pts, const int *tets, const int *te
       double ec[3], ed[3], coeffs[4], N[3];
       const double *xa = fv.x[fv.ev[i][0]], *xb = fv.x[fv.ev[i][1]];
       for (k = 0; k < 3; ++k) { ec[k] = (xa[k] + xb[k]) / 2.0; ed[k] = xb[k] - xa[k]; }
-      if (fabs(ec[0] - tc[0][0]) < OR_EPS) {
-        double s = (1.0 - cos(OR_PI * (ec[1] - tc[0][1]) / (tc[2][1] - tc[0][1]))) / 2.0;
-        coeffs[0] = 1.0 - s; coeffs[1] = 0.0; coeffs[2] = 0.0; coeffs[3] = s;
-      } else if (fabs(ec[1] - tc[0][1]) < OR_EPS) {
-        double s = (1.0 - cos(OR_PI * (ec[0] - tc[0][0]) / (tc[2][0] - tc[0][0]))) / 2.0;
-        coeffs[0] = 1.0 - s; coeffs[1] = s; coeffs[2] = 0.0; coeffs[3] = 0.0;
-      } else if (fabs(ec[0] - tc[2][0]) < OR_EPS) {
-        double s = (1.0 - cos(OR_PI * (ec[1] - tc[0][1]) / (tc[2][1] - tc[0][1]))) / 2.0;
-        coeffs[0] = 0.0; coeffs[1] = 1.0 - s; coeffs[2] = s; coeffs[3] = 0.0;
-      } else if (fabs(ec[1] - tc[2][1]) < OR_EPS) {
-        double s = (1.0 - cos(OR_PI * (ec[0] - tc[0][0]) / (tc[2][0] - tc[0][0]))) / 2.0;
-        coeffs[0] = 0.0; coeffs[1] = 0.0; coeffs[2] = s; coeffs[3] = 1.0 - s;
-      } else {
-        double area = (tc[2][0] - tc[0][0]) * (tc[2][1] - tc[0][1]);
-        coeffs[0] = ((tc[2][0] - ec[0]) * (tc[2][1] - ec[1])) / area;
-        coeffs[1] = ((ec[0] - tc[3][0]) * (tc[3][1] - ec[1])) / area;
-        coeffs[2] = ((ec[0] - tc[0][0]) * (ec[1] - tc[0][1])) / area;
-        coeffs[3] = ((tc[1][0] - ec[0]) * (ec[1] - tc[1][1])) / area;
-      }
+      or_blend_weights(corners, ec, coeffs);
       zdouble ep[3] = {0, 0, 0};
       for (k = 0; k < 4; ++k) {
         zdouble eh[4], e[3] = {0, 0, 0};

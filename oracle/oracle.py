@@ -61,6 +61,7 @@ def _declare(L):
         "or_find_cell": (I, [I, P, I, P, P, P, P]),
         "or_pw_eval": (None, [I, P, P, D, D, I, P, P]),
         "or_background_model": (I, [I, P, I, P, P, P, I, P, P, P, P, P]),
+        "or_blend_weights": (None, [P, P, P]),
         "or_boundary_values": (I, [I, P, P, P, P, P, P, P, P, P, D, I, I, P, P]),
         "or_assemble_rhs_mt": (I, [I, P, P, P, P, P, P, I, P, D, P, P, P]),
         "or_divide_line_source": (I, [P, I, P]),
@@ -230,6 +231,13 @@ def pw_eval(ztop, lsig, freq, z, mode):
     return e[0::2] + 1j * e[1::2], h[0::2] + 1j * h[1::2]
 
 
+def blend_weights(corners, ec):
+    """Weights of the four corner columns at a boundary-edge midpoint (src/em_fwd.cc:438-471)."""
+    out = np.empty(4)
+    lib().or_blend_weights(_p(_c(corners, np.float64)), _p(_c(ec, np.float64)), _p(out))
+    return out
+
+
 def boundary_values(mesh, bg, freq, mode):
     """Dirichlet DOF values on boundary edges (src/em_fwd.cc:402-486) -> complex[E] + mask."""
     bval = np.zeros((mesh.n_edges, 2))
@@ -304,7 +312,7 @@ def solve_direct(rowptr, col, C, M, s_re, s_im):
     import scipy.sparse.linalg as spla
     n = rowptr.size - 1
     K = csr(rowptr, col, C.astype(np.complex128) + 1j * M, n).tocsc()
-    lu = spla.splu(K)
+    lu = spla.splu(K, permc_spec="MMD_AT_PLUS_A")  # symmetric pattern: far less fill than COLAMD
     rhs = np.asarray(s_re) - 1j * np.asarray(s_im)
     return lu.solve(rhs.astype(np.complex128))
 
