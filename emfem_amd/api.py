@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "libemfem_amd.so")
 
 OK, ERR_USER, ERR_HIP, ERR_NOT_CONVERGED, ERR_STATE, ERR_ARG = 0, -1, -2, -3, -4, -5
 XY_POLAR, YX_POLAR, MT_TIDX = -1, -2, -3
+LAYOUT_INTERLEAVED, LAYOUT_SPLIT = 0, 1
 
 
 class EmfemError(RuntimeError):
@@ -68,6 +69,9 @@ SIGNATURES = {
     "emfem_solve_linear_system": (I, [P, I, I, I, D, P, P, P]),
     "emfem_matshell_mult_a": (I, [P, I, P, P]),
     "emfem_pc_apply_b": (I, [P, I, P, P]),
+    "emfem_create_vecs": (I, [P, I, I, ctypes.POINTER(P), ctypes.POINTER(P)]),
+    "emfem_matshell_mult_a_split": (I, [P, I, P, P, P, P]),
+    "emfem_pc_apply_b_split": (I, [P, I, P, P, P, P]),
     "emfem_estimate_error": (I, [P, I, I, P, P]),
     "emfem_interpolate_fields": (I, [P, I, I, I, P, P, P]),
     "emfem_calculate_response_mt": (I, [P, I, I, I, I, P, P]),
@@ -327,6 +331,32 @@ class Context:
 
     def pc_apply_b(self, n_rhs, b_ptr, x_ptr):
         self._check(self._L.emfem_pc_apply_b(self._h, int(n_rhs), P(b_ptr), P(x_ptr)))
+
+    def create_vecs(self, n_rhs, layout=LAYOUT_SPLIT):
+        """Zeroed device vectors of the operator (emfem_create_vecs): (re, im) device
+        pointers for LAYOUT_SPLIT, (x, None) for LAYOUT_INTERLEAVED; free each with
+        device_free."""
+        a, b = P(), P()
+        self._check(self._L.emfem_create_vecs(self._h, int(n_rhs), int(layout), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def device_free(self, ptr):
+        self._check(self._L.emfem_device_free(self._h, P(ptr)))
+
+    def copy_to_device(self, ptr, host):
+        h = np.ascontiguousarray(host)
+        self._check(self._L.emfem_copy_to_device(self._h, P(ptr), _p(h), ctypes.c_ulonglong(h.nbytes)))
+
+    def copy_to_host(self, ptr, out):
+        self._check(self._L.emfem_copy_to_host(self._h, _p(out), P(ptr), ctypes.c_ulonglong(out.nbytes)))
+        return out
+
+    def matshell_mult_a_split(self, n_rhs, xr, xi, yr, yi):
+        """y = A x on the VecNest halves (split device arrays of n_rhs * n_edges)."""
+        self._check(self._L.emfem_matshell_mult_a_split(self._h, int(n_rhs), P(xr), P(xi), P(yr), P(yi)))
+
+    def pc_apply_b_split(self, n_rhs, br, bi, xr, xi):
+        self._check(self._L.emfem_pc_apply_b_split(self._h, int(n_rhs), P(br), P(bi), P(xr), P(xi)))
 
     # -- estimator and responses ----------------------------------------------------
     def estimate_error(self, fidx, slots):

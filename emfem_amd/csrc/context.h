@@ -99,6 +99,12 @@ struct emfem_context {
   DevBuf<int> d_rp, d_ci;
   DevBuf<double2> d_K;
   DevBuf<double> d_S, d_X;  // [E][2 nslots]
+  // solve workspace kept across solves: the compact right-hand side and solution
+  // blocks and the outer FGMRES's basis (released with the linear system)
+  DevBuf<double> d_sb, d_sx;
+  FgmresWork fgw;
+  // interleaved staging of the split-layout operator and preconditioner entry points
+  DevBuf<double> d_li, d_lo;
 
   // preconditioner
   std::unique_ptr<AmsHost> ams_host;
@@ -193,6 +199,7 @@ inline Locate locator(emfem_context *c) {
     locate_points(s, n, q.get(), c->T.nv, c->d_pts.get(), c->d_tets.get(), c->d_nb.get(), c->d_v2t.get(),
                   nearest.get(), queue.get(), res.get());
     res.download(out.data(), (size_t)n, s);
+    EMF_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < n; ++i)
       EMF_REQUIRE(out[i] != -2, kErrUser,
                   "point location queued " + std::to_string(kLocateQueue) + " tets without an answer for point " +

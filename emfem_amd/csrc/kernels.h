@@ -231,6 +231,11 @@ void axpby(hipStream_t s, int rows, int nc, double alpha, const TX *x, int ldx, 
 // for i < n (distinct idx).
 void put_rows(hipStream_t s, int rows, int n, const int *idx, const double2 *v, double *y, int ldy);
 
+// Split {re, im} blocks of n vectors (vector j at offset j * rows) to interleaved
+// [rows][2 n] pairs, and back.
+void split_to_interleaved(hipStream_t s, int rows, int n, const double *re, const double *im, double *y);
+void interleaved_to_split(hipStream_t s, int rows, int n, const double *x, double *re, double *im);
+
 // Flexible CG updates on nc columns with device scalars:
 //   x += alpha[c] p (skipped when x is null) ; r -= alpha[c] q ; rr[c] = r . r
 // (fused reduction; x, r in T; the search directions p, q in TP; arithmetic in double)

@@ -27,6 +27,7 @@ struct SolveStats {
   std::vector<int> its;      // per right-hand side
   std::vector<double> relres;
   std::vector<uint8_t> converged;  // the reference's stopping test was met
+  int floor_accepts = 0;           // right-hand sides accepted at the fp64 residual floor
   std::vector<double> done_ms;     // per right-hand side: time from the start until it was accepted
 };
 
@@ -91,14 +92,38 @@ struct OuterOptions {
   // iterations)
   bool fp64_fallback = true;
   int stall_window = 15;
+  // A right-hand side whose implicit residual met rtol while its true residual did not
+  // is restarted from the true residual until that stops halving; it is then accepted
+  // only if the true residual is within floor_factor * rtol * ||b||, and otherwise
+  // reported as not converged.
+  double floor_factor = 10.0;
+};
+
+// Device workspace of the outer FGMRES, kept by the caller across solves so that
+// repeated solves allocate nothing: the Krylov basis V (min(restart, max_it) + 1
+// blocks), the preconditioned directions Z (min(restart, max_it) blocks), one work
+// block and the reduction scratch.  Buffers only grow.
+struct FgmresWork {
+  DevBuf<double> V, Z, w, dots, coef, scal, work;
+  // total device bytes held
+  size_t bytes() const {
+    return V.bytes() + Z.bytes() + w.bytes() + dots.bytes() + coef.bytes() + scal.bytes() + work.bytes();
+  }
 };
 
 // FGMRES on nrhs interleaved right-hand sides b [E][2 nrhs] -> x [E][2 nrhs].  With
 // segments (a block-diagonal multi-model batch) every (segment, right-hand side) is
 // an independent system with its own Arnoldi process and stopping test; rtol and the
 // per-RHS statistics are then indexed seg * nrhs + k.
+// ws (may be null: a workspace local to the call) holds the device buffers.
 void fgmres(hipStream_t s, int ne, int nnz, const int *d_rp, const int *d_ci, const double2 *d_K, InnerPc &pc,
             int nrhs, const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o,
-            SolveStats &st, const Seg &sg = Seg());
+            SolveStats &st, const Seg &sg = Seg(), FgmresWork *ws = nullptr);
+
+// Grows d to at least n elements (contents are not kept when it grows).
+template <typename T>
+void grow(DevBuf<T> &d, size_t n) {
+  if (d.size() < n) d.resize(n);
+}
 
 }  // namespace emfem

@@ -142,7 +142,6 @@ class DevAmg {
       return;
     }
     Level &L = levels_[l];
-    const int n = L.A.n;
     T *x = L.x.get(), *r = L.r.get();
     const T *dinv = L.l1inv.get();
     // pre-smoothing from a zero guess, x0 = Dinv b, folded into the next two
@@ -273,6 +272,9 @@ class DevAms {
 
   void ensure(int nc) {
     if (nc <= nc_) return;
+    // every buffer below (and the AMG levels') is reallocated: captured cycles hold
+    // the old addresses, so they are dropped and captured again on next use
+    drop_graphs();
     nc_ = nc;
     r64_.resize((size_t)ne_ * nc);
     wtmp_.resize((size_t)ne_ * nc);
@@ -431,9 +433,17 @@ class DevAms {
   }
 
   ~DevAms() {
-    for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    drop_graphs();
     for (auto e : clock_) (void)hipEventDestroy(e);
   }
+
+  // Destroys the captured cycles (callers synchronise the stream before any buffer a
+  // graph uses is freed; every entry point returns synchronised).
+  void drop_graphs() {
+    for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
+  }
+  size_t n_graphs() const { return graphs_.size(); }
 
   // out = M_AMS b: b fp64 compact [E][nc], out in T; out must not alias b.  With
   // use_graph the ~100 launches of one cycle are captured once per (nc, b, out)
@@ -893,18 +903,26 @@ namespace {
 // that cycle and re-checked (and, if needed, resumed) at the next restart.
 void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, InnerPc &pc,
                  int nrhs_seg, const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o,
-                 SolveStats &st, const Seg &sg) {
+                 SolveStats &st, const Seg &sg, FgmresWork &ws) {
   // nc columns per row; nrhs logical right-hand sides = segments x column pairs, their
   // scalars at [2 r], [2 r + 1] of nl-long arrays
   const int nc = 2 * nrhs_seg;
   const int nrhs = sg.nseg * nrhs_seg;
   const int nl = 2 * nrhs;
-  const int m = std::max(1, o.restart);
+  // a cycle never runs past max_it iterations, so min(restart, max_it) basis blocks
+  // give the same iterates as `restart` ones
+  const int m = std::max(1, std::min(o.restart, o.max_it));
   const size_t blk = (size_t)ne * nc;
-  DevBuf<double> V((m + 1) * blk), w(blk), dots((size_t)(m + 1) * nl), coef((size_t)(m + 1) * nl), scal(nl);
-  DevBuf<double> Z(m * blk);
+  grow(ws.V, (m + 1) * blk);
+  grow(ws.Z, m * blk);
+  grow(ws.w, blk);
+  grow(ws.dots, (size_t)(m + 1) * nl);
+  grow(ws.coef, (size_t)(m + 1) * nl);
+  grow(ws.scal, (size_t)nl);
+  grow(ws.work, dot_work_size(ne, nc, sg) * 8);
+  DevBuf<double> &V = ws.V, &w = ws.w, &dots = ws.dots, &coef = ws.coef, &scal = ws.scal, &Z = ws.Z,
+                 &work = ws.work;
   std::vector<std::vector<double>> hist(nrhs);
-  DevBuf<double> work(dot_work_size(ne, nc, sg) * 8);
   std::vector<double> h_dots((size_t)(m + 1) * nl), h_coef((size_t)(m + 1) * nl), h_scal(nl);
   auto op = [&](const auto *in, double *out, double alpha, double beta, const double *z) {
     op_spmm(s, ne, nnz, rp, ci, K, nrhs_seg, in, nc, out, nc, alpha, beta, z, nc);
@@ -950,15 +968,19 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
       const bool true_ok = bnorm[r] == 0.0 || beta[r] <= rtol[r] * bnorm[r];
       // The reference stops on FGMRES's implicit residual (PETSc's test).  When that
       // test passed but the true residual did not, restart as iterative refinement
-      // until the true residual stops improving (its evaluation floor, e.g. fields
-      // of ~1/(omega sigma_air) in the air), then accept.
-      bool floor = false;
+      // until the true residual stops halving (its evaluation floor, e.g. fields of
+      // ~1/(omega sigma_air) in the air).  At that floor the right-hand side is
+      // accepted if the true residual is within floor_factor * rtol, and otherwise
+      // stopped and reported as not converged.
+      bool stalled = false;
       if (!true_ok && est_ok[r]) {
-        floor = last_true[r] > 0.0 && beta[r] > 0.5 * last_true[r];
+        stalled = last_true[r] > 0.0 && beta[r] > 0.5 * last_true[r];
         last_true[r] = beta[r];
       }
-      done[r] = true_ok || floor;
-      if (done[r]) st.converged[r] = 1;
+      const bool floor_ok = stalled && beta[r] <= o.floor_factor * rtol[r] * bnorm[r];
+      done[r] = true_ok || stalled;
+      st.converged[r] = true_ok || floor_ok;
+      if (floor_ok && !true_ok) ++st.floor_accepts;
       // accepted: keep the time its Arnoldi process stopped; rejected: running again
       if (done[r] && st.done_ms[r] < 0) st.done_ms[r] = now_ms() - t_start;
       if (!done[r]) st.done_ms[r] = -1.0;
@@ -1075,6 +1097,15 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
     if (st.done_ms[r] < 0) st.done_ms[r] = now_ms() - t_start;
     st.relres[r] = bnorm[r] > 0 ? rn[r] / bnorm[r] : 0.0;
     st.outer_its = std::max(st.outer_its, st.its[r]);
+    // max_it reached before the next restart's check: judge the final iterate by the
+    // same rule (a right-hand side stopped at its floor keeps its verdict: the same
+    // residual fails the same bound)
+    if (!st.converged[r]) {
+      const bool ok = bnorm[r] == 0.0 || rn[r] <= rtol[r] * bnorm[r];
+      const bool floor_ok = est_ok[r] && rn[r] <= o.floor_factor * rtol[r] * bnorm[r];
+      st.converged[r] = ok || floor_ok;
+      if (floor_ok && !ok) ++st.floor_accepts;
+    }
   }
 }
 
@@ -1082,9 +1113,10 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
 
 void fgmres(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, const double2 *K, InnerPc &pc, int nrhs,
             const double *b, double *x, const std::vector<double> &rtol, const OuterOptions &o, SolveStats &st,
-            const Seg &sg) {
+            const Seg &sg, FgmresWork *ws) {
   pc.set_segments(sg);
-  fgmres_impl(s, ne, nnz, rp, ci, K, pc, nrhs, b, x, rtol, o, st, sg);
+  FgmresWork local;
+  fgmres_impl(s, ne, nnz, rp, ci, K, pc, nrhs, b, x, rtol, o, st, sg, ws ? *ws : local);
 }
 
 }  // namespace emfem

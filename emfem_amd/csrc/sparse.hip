@@ -1258,6 +1258,40 @@ __global__ void k_put_rows(int rows, int n, const int *__restrict__ idx, const d
 }
 }  // namespace
 
+// Split {re, im} blocks (n vectors of `rows` entries each, vector j at j * rows) <->
+// interleaved [rows][2 n]; one thread per (row, vector), rows fastest so the split side
+// is read and written coalesced.
+__global__ void k_split_interleave(int rows, int n, const double *__restrict__ re, const double *__restrict__ im,
+                                   double *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)rows * n) return;
+  const int r = (int)(i % rows), j = (int)(i / rows);
+  const double a = re[i], b = im[i];
+  y[(size_t)r * 2 * n + 2 * j] = a;
+  y[(size_t)r * 2 * n + 2 * j + 1] = b;
+}
+
+__global__ void k_interleave_split(int rows, int n, const double *__restrict__ x, double *__restrict__ re,
+                                   double *__restrict__ im) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)rows * n) return;
+  const int r = (int)(i % rows), j = (int)(i / rows);
+  re[i] = x[(size_t)r * 2 * n + 2 * j];
+  im[i] = x[(size_t)r * 2 * n + 2 * j + 1];
+}
+
+void split_to_interleaved(hipStream_t s, int rows, int n, const double *re, const double *im, double *y) {
+  const int64_t m = (int64_t)rows * n;
+  hipLaunchKernelGGL(k_split_interleave, dim3(grid_for(m)), dim3(kBlock), 0, s, rows, n, re, im, y);
+  EMF_HIP(hipGetLastError());
+}
+
+void interleaved_to_split(hipStream_t s, int rows, int n, const double *x, double *re, double *im) {
+  const int64_t m = (int64_t)rows * n;
+  hipLaunchKernelGGL(k_interleave_split, dim3(grid_for(m)), dim3(kBlock), 0, s, rows, n, x, re, im);
+  EMF_HIP(hipGetLastError());
+}
+
 void put_rows(hipStream_t s, int rows, int n, const int *idx, const double2 *v, double *y, int ldy) {
   hipLaunchKernelGGL(k_put_rows, dim3(grid_for(rows)), dim3(kBlock), 0, s, rows, n, idx, v, y, ldy, 0);
   if (n > 0) hipLaunchKernelGGL(k_put_rows, dim3(grid_for(n)), dim3(kBlock), 0, s, rows, n, idx, v, y, ldy, 1);

@@ -62,7 +62,11 @@ int emfem_process_options(emfem_context *ctx, const char *options_text);
  * an unknown key. */
 int emfem_get_option(const emfem_context *ctx, const char *key, char *buf, int buflen);
 
-/* Space-separated list of option keys that were given but are not used. */
+/* Space-separated list of option keys that were given but are not used, or whose
+ * given value is not honoured: -inner_pc_type direct / mixed with -pc_threshold and
+ * -direct_solver_type select the reference's direct inner solver
+ * (src/em_la.cc:147-155), which does not exist here; the inner solve then stays
+ * FCG + AMS (the default, -inner_pc_type ams). */
 int emfem_unused_options(const emfem_context *ctx, char *buf, int buflen);
 
 /* ---- mesh, model and survey ------------------------------------------------- */
@@ -159,11 +163,14 @@ int emfem_destroy_pc(emfem_context *ctx);
  * rtol for all when rtols is NULL), zero initial guess, at most max_it outer
  * iterations.  The stopping test is the reference's (FGMRES's implicit residual
  * <= rtol ||s||); when it passes but the true residual does not, the solve restarts
- * from the true residual until that stops improving.  iters[i] and relres[i] (the
- * true FP64 residual ||s - A e|| / ||s||) are written when the pointers are
- * non-NULL.  Returns EMFEM_ERR_NOT_CONVERGED (after storing the partial solution)
- * if any right-hand side missed the stopping test within max_it, which the
- * reference never reports. */
+ * from the true residual until that stops halving, and then accepts the right-hand
+ * side only if its true residual is within -true_residual_slack (10) x rtol.
+ * iters[i] and relres[i] (the true FP64 residual ||s - A e|| / ||s||) are written
+ * when the pointers are non-NULL.  Returns EMFEM_ERR_NOT_CONVERGED (after storing
+ * the partial solution, with the worst true residual in the error message) if any
+ * right-hand side missed these tests within max_it, which the reference never
+ * reports.  Device workspace is kept by the context across calls (released by
+ * emfem_destroy_linear_system), so repeated solves do not allocate. */
 int emfem_solve_linear_system(emfem_context *ctx, int first, int n, int max_it, double rtol,
                               const double *rtols, int *iters, double *relres);
 
@@ -190,6 +197,29 @@ int emfem_matshell_mult_a(emfem_context *ctx, int n_rhs, const double *x_dev, do
 /* pc_apply_b (src/em_la.cc:208-228) on device arrays: x = diag(B,B)^-1 b by the
  * inner Krylov solve (FCG + AMS) for n_rhs interleaved right-hand sides. */
 int emfem_pc_apply_b(emfem_context *ctx, int n_rhs, const double *b_dev, double *x_dev);
+
+/* Vector layouts of emfem_create_vecs. */
+#define EMFEM_LAYOUT_INTERLEAVED 0
+#define EMFEM_LAYOUT_SPLIT 1
+
+/* matshell_createvecs_a (MATOP_CREATE_VECS, src/em_la.cc:89-111, registered at
+ * src/em_la.cc:170): zeroed device vectors for n_rhs right-hand sides of the
+ * operator.  EMFEM_LAYOUT_SPLIT gives the VecNest {re, im} halves the reference
+ * creates: *x = re and *x_im = im, n_rhs * n_edges doubles each (vector j at offset
+ * j * n_edges).  EMFEM_LAYOUT_INTERLEAVED gives one [n_edges][2 * n_rhs] array in
+ * *x (x_im may be NULL; it is set to NULL).  Free with emfem_device_free. */
+int emfem_create_vecs(emfem_context *ctx, int n_rhs, int layout, double **x, double **x_im);
+
+/* matshell_mult_a (src/em_la.cc:65-87) on the VecNest halves: y = A x with
+ * x = {x_re, x_im} and y = {y_re, y_im} split device arrays (EMFEM_LAYOUT_SPLIT),
+ * n_rhs vectors each.  In place (x == y) is allowed. */
+int emfem_matshell_mult_a_split(emfem_context *ctx, int n_rhs, const double *x_re, const double *x_im,
+                                double *y_re, double *y_im);
+
+/* pc_apply_b (src/em_la.cc:208-228) on the VecNest halves: {x_re, x_im} =
+ * B^-1 {b_re, b_im} (split device arrays, n_rhs vectors each). */
+int emfem_pc_apply_b_split(emfem_context *ctx, int n_rhs, const double *b_re, const double *b_im, double *x_re,
+                           double *x_im);
 
 /* ---- estimator, responses, marking ----------------------------------------- */
 

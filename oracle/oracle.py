@@ -306,15 +306,37 @@ def apply_operator(rowptr, col, C, M, xr, xi):
     return Cm @ xr - Mm @ xi, -(Mm @ xr) - Cm @ xi
 
 
-def solve_direct(rowptr, col, C, M, s_re, s_im):
-    """Exact solution e = e_re + i e_im of A [e_re; e_im] = [s_re; s_im], i.e.
-    (C + iM) e = s_re - i s_im (src/em_la.cc:35-87)."""
+def direct_solver(rowptr, col, C, M):
+    """Factorises K = C + iM once; returns solve(s_re, s_im) -> e for
+    A [e_re; e_im] = [s_re; s_im], i.e. (C + iM) e = s_re - i s_im
+    (src/em_la.cc:35-87).  K is complex symmetric with a symmetric pattern, so the
+    sparse LU runs in symmetric mode (diagonal pivots, minimum-degree ordering of
+    K + K^T), which has far less fill than the default; every solve is checked by
+    its residual, and a factorisation with threshold pivoting replaces the symmetric
+    one if that check fails."""
     import scipy.sparse.linalg as spla
     n = rowptr.size - 1
     K = csr(rowptr, col, C.astype(np.complex128) + 1j * M, n).tocsc()
-    lu = spla.splu(K, permc_spec="MMD_AT_PLUS_A")  # symmetric pattern: far less fill than COLAMD
-    rhs = np.asarray(s_re) - 1j * np.asarray(s_im)
-    return lu.solve(rhs.astype(np.complex128))
+    state = {"lu": spla.splu(K, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+                             options=dict(SymmetricMode=True)), "pivoting": False}
+
+    def solve(s_re, s_im):
+        rhs = (np.asarray(s_re) - 1j * np.asarray(s_im)).astype(np.complex128)
+        x = state["lu"].solve(rhs)
+        nb = np.linalg.norm(rhs)
+        if nb > 0 and np.linalg.norm(K @ x - rhs) > 1e-11 * nb and not state["pivoting"]:
+            state["lu"] = spla.splu(K, permc_spec="MMD_AT_PLUS_A")
+            state["pivoting"] = True
+            x = state["lu"].solve(rhs)
+        return x
+
+    return solve
+
+
+def solve_direct(rowptr, col, C, M, s_re, s_im):
+    """Exact solution e = e_re + i e_im of A [e_re; e_im] = [s_re; s_im], i.e.
+    (C + iM) e = s_re - i s_im (src/em_la.cc:35-87)."""
+    return direct_solver(rowptr, col, C, M)(s_re, s_im)
 
 
 def interpolate_fields(mesh, t, p, freq, e):

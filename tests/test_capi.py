@@ -56,6 +56,17 @@ def test_options_parsing():
     assert ctx.get_option("dual_rtol") == "1e-6"         # default (src/em_ctx.cc:148-149)
     assert ctx.get_option("pc_threshold") == "500000"    # default (src/em_ctx.cc:136-137)
     assert ctx.unused_options() == ["B_pc_hypre_ams_cycle_type"]
+    assert api.Context().get_option("inner_pc_type") == "ams"
+    # the direct inner solver (src/em_la.cc:147-155) is not implemented: its options
+    # are parsed and reported as not honoured
+    for text, unused in (("-inner_pc_type mixed", ["inner_pc_type"]),
+                         ("-inner_pc_type direct -direct_solver_type mumps",
+                          ["direct_solver_type", "inner_pc_type"]),
+                         ("-pc_threshold 1000", ["pc_threshold"]),
+                         ("-inner_pc_type ams", [])):
+        c2 = api.Context(text)
+        assert sorted(c2.unused_options()) == unused, text
+        c2.close()
     with pytest.raises(api.EmfemError) as e:
         ctx.process_options("-inner_pc_type bogus")
     assert e.value.code == api.ERR_USER
