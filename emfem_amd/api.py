@@ -72,6 +72,7 @@ SIGNATURES = {
     "emfem_create_vecs": (I, [P, I, I, ctypes.POINTER(P), ctypes.POINTER(P)]),
     "emfem_matshell_mult_a_split": (I, [P, I, P, P, P, P]),
     "emfem_pc_apply_b_split": (I, [P, I, P, P, P, P]),
+    "emfem_matshell_mult_a_split_noalloc": (I, [P, I, P, P, P, P]),
     "emfem_estimate_error": (I, [P, I, I, P, P]),
     "emfem_interpolate_fields": (I, [P, I, I, I, P, P, P]),
     "emfem_calculate_response_mt": (I, [P, I, I, I, I, P, P]),
@@ -354,6 +355,11 @@ class Context:
     def matshell_mult_a_split(self, n_rhs, xr, xi, yr, yi):
         """y = A x on the VecNest halves (split device arrays of n_rhs * n_edges)."""
         self._check(self._L.emfem_matshell_mult_a_split(self._h, int(n_rhs), P(xr), P(xi), P(yr), P(yi)))
+
+    def matshell_mult_a_split_noalloc(self, n_rhs, xr, xi, yr, yi):
+        """matshell_mult_a_split staged in the library's fixed device array (allocates
+        nothing; n_edges * n_rhs <= 2097152)."""
+        self._check(self._L.emfem_matshell_mult_a_split_noalloc(self._h, int(n_rhs), P(xr), P(xi), P(yr), P(yi)))
 
     def pc_apply_b_split(self, n_rhs, br, bi, xr, xi):
         self._check(self._L.emfem_pc_apply_b_split(self._h, int(n_rhs), P(br), P(bi), P(xr), P(xi)))

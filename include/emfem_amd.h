@@ -216,6 +216,17 @@ int emfem_create_vecs(emfem_context *ctx, int n_rhs, int layout, double **x, dou
 int emfem_matshell_mult_a_split(emfem_context *ctx, int n_rhs, const double *x_re, const double *x_im,
                                 double *y_re, double *y_im);
 
+/* No counterpart in the reference: emfem_matshell_mult_a_split for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and result; the interleaved
+ * staging lives in a fixed device array of the library (64 MiB per GPU, held until
+ * the process ends) instead of context buffers, so the call allocates and frees
+ * nothing.  n_edges * n_rhs must not exceed 2097152 (EMFEM_ERR_ARG otherwise).  All
+ * calls on one GPU share that array: calls that overlap in time on one GPU (from
+ * several host threads or streams) race. */
+int emfem_matshell_mult_a_split_noalloc(emfem_context *ctx, int n_rhs, const double *x_re, const double *x_im,
+                                        double *y_re, double *y_im);
+
 /* pc_apply_b (src/em_la.cc:208-228) on the VecNest halves: {x_re, x_im} =
  * B^-1 {b_re, b_im} (split device arrays, n_rhs vectors each). */
 int emfem_pc_apply_b_split(emfem_context *ctx, int n_rhs, const double *b_re, const double *b_im, double *x_re,
