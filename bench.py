@@ -83,54 +83,97 @@ def cpu_baseline(budget_s=20.0):
                        "estimator, %.1f s" % (om.n_edges, m.freqs[fidx], len(sols), dt))
 
 
-def csem_leg(api, mg, cells, n_tx=64, n_plain=4, fidx=1, graph=True):
-    """Marine-CSEM batched multi-RHS solve (one frequency, n_tx x-directed dipoles as
-    one block of right-hand sides, fp32 inner cycle in fp64 FGMRES) against the plain
-    fp64 single-RHS path on the first n_plain transmitters of the same operator.
-    Returns DOF*RHS/s of both, the speed-up, and the largest receiver-field difference
-    between the two paths relative to the field's norm over the receivers."""
+def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1):
+    """Marine CSEM (the north star's third configuration): n_tx x-directed dipoles at
+    each of the model's frequencies (0.25 / 0.75 / 1.25 Hz), each frequency one block of
+    n_tx right-hand sides (fp32 inner cycle in fp64 FGMRES, true residual 1e-8),
+    against the plain fp64 single-RHS path on the first n_plain transmitters of one
+    frequency.  DOF*RHS/s = n_edges * right-hand sides / solve wall time.  The block
+    solves run with plain launches (at 128 columns a kernel lasts milliseconds, so
+    graph replay saves nothing), which lets the library time every smoother SpMM in
+    the cycles with HIP events for the roofline.  Also reports the largest difference
+    of the receiver fields between the two paths relative to each field's norm over
+    the receivers."""
     m = mg.marine_csem(cells=cells, n_tx=n_tx)
-    out = {}
+    out = {"workload": None, "per_frequency": []}
     fields = {}
-    for name, opts, batched in (("mixed_batched", "-inner_precision fp32", True),
-                                ("plain_fp64_single_rhs", "-inner_precision fp64", False)):
-        ctx = api.Context(opts + " -e_rtol 1e-8 -use_graph %s" % ("true" if graph else "false"))
-        ctx.load_model(m)
-        nt = n_tx if batched else n_plain
-        ctx.create_linear_system(nt)
-        E = ctx.mesh_info()["n_edges"]
-        ctx.assemble_matrix(fidx)
+    t0 = time.perf_counter()
+    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -use_graph false")
+    ctx.load_model(m)
+    ctx.create_linear_system(n_tx)
+    info = ctx.mesh_info()
+    E, nnz = info["n_edges"], info["nnz"]
+    log("csem: %d tets, %d edges, nnz %d (%.1fs)" % (info["n_tets"], E, nnz, time.perf_counter() - t0))
+    tot_solve = tot_step = 0.0
+    roof = None
+    for fidx in range(len(m.freqs)):
         t0 = time.perf_counter()
+        ctx.assemble_matrix(fidx)
         ctx.create_pc()
-        t1 = time.perf_counter()
-        for t in range(nt):
+        for t in range(n_tx):
             ctx.assemble_rhs_csem(fidx, t, t)
+        t1 = time.perf_counter()
+        its, rr = ctx.solve_linear_system(0, n_tx, 200, 1e-8)
         t2 = time.perf_counter()
-        if batched:
-            its, rr = ctx.solve_linear_system(0, nt, 200, 1e-8)
-            its, rr = its.tolist(), rr.tolist()
-        else:
-            its, rr = [], []
-            for t in range(nt):
-                i, r = ctx.solve_linear_system(t, 1, 200, 1e-8)
-                its += i.tolist()
-                rr += r.tolist()
-        t3 = time.perf_counter()
-        fields[name] = [np.concatenate(ctx.interpolate_fields(fidx, t, m.rx), axis=1) for t in range(n_plain)]
-        out[name] = {"dof_rhs_per_s": round(E * nt / (t3 - t2), 1), "rhs": nt, "solve_s": round(t3 - t2, 3),
-                     "pc_s": round(t1 - t0, 3), "max_outer_its": int(max(its)), "max_true_relres": float(max(rr))}
-        log("csem %s: E=%d rhs=%d pc %.2fs solve %.2fs its max %d relres max %.1e" %
-            (name, E, nt, t1 - t0, t3 - t2, max(its), max(rr)))
-        ctx.close()
+        tot_solve += t2 - t1
+        tot_step += t2 - t0
+        row = {"freq_hz": float(m.freqs[fidx]), "setup_s": round(t1 - t0, 3), "solve_s": round(t2 - t1, 3),
+               "max_outer_its": int(its.max()), "max_true_relres": float(rr.max()),
+               "inner_its": int(ctx.stat("inner_its"))}
+        out["per_frequency"].append(row)
+        log("csem mixed f=%.2f Hz: set-up %.2fs solve %.2fs its max %d relres max %.1e inner %d"
+            % (m.freqs[fidx], t1 - t0, t2 - t1, its.max(), rr.max(), row["inner_its"]))
+        if fidx == plain_fidx:
+            fields["mixed"] = [np.concatenate(ctx.interpolate_fields(fidx, t, m.rx), axis=1) for t in range(n_plain)]
+            b_ms, b_bytes = ctx.kernel_time("b_spmm")
+            if b_ms > 0:
+                bw = b_bytes / (b_ms * 1e-3) / 1e9
+                roof = {"bound": "hbm", "achieved": round(bw, 1), "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(bw / 8000.0, 4), "launch_ms": round(b_ms, 4), "algorithmic_bytes": b_bytes,
+                        "kernel": "smoother SpMM on %d fp32 columns (Chebyshev step in the epilogue), %s"
+                                  % (2 * n_tx, smoother_kernel_name(int(ctx.stat("b_spmm_rows"))))}
+    ctx.close()
+    out["mixed_batched"] = {"dof_rhs_per_s": round(E * n_tx * len(m.freqs) / tot_solve, 1),
+                            "dof_rhs_per_s_with_setup": round(E * n_tx * len(m.freqs) / tot_step, 1),
+                            "rhs": n_tx * len(m.freqs), "solve_s": round(tot_solve, 3)}
+    ctxp = api.Context("-inner_precision fp64 -e_rtol 1e-8")
+    ctxp.load_model(m)
+    ctxp.create_linear_system(n_plain)
+    ctxp.assemble_matrix(plain_fidx)
+    ctxp.create_pc()
+    for t in range(n_plain):
+        ctxp.assemble_rhs_csem(plain_fidx, t, t)
+    t0 = time.perf_counter()
+    pits = []
+    for t in range(n_plain):
+        i, r = ctxp.solve_linear_system(t, 1, 200, 1e-8)
+        pits.append(int(i[0]))
+    dt = time.perf_counter() - t0
+    fields["plain"] = [np.concatenate(ctxp.interpolate_fields(plain_fidx, t, m.rx), axis=1) for t in range(n_plain)]
+    ctxp.close()
+    out["plain_fp64_single_rhs"] = {"dof_rhs_per_s": round(E * n_plain / dt, 1), "rhs": n_plain,
+                                    "solve_s": round(dt, 3), "freq_hz": float(m.freqs[plain_fidx]), "outer_its": pits}
+    log("csem plain fp64: %d single-RHS solves %.2fs" % (n_plain, dt))
     err = 0.0
-    for a, b in zip(fields["mixed_batched"], fields["plain_fp64_single_rhs"]):
+    for a, b in zip(fields["mixed"], fields["plain"]):
         for c in range(a.shape[1]):
             err = max(err, float(np.linalg.norm(a[:, c] - b[:, c]) / max(np.linalg.norm(b[:, c]), 1e-300)))
-    out["workload"] = ("marine CSEM %dx%dx%d cells, %d edges, f=%.2f Hz, %d x-dipoles batched"
-                       % (cells[0], cells[1], cells[2], E, m.freqs[fidx], n_tx))
-    out["speedup"] = round(out["mixed_batched"]["dof_rhs_per_s"] / out["plain_fp64_single_rhs"]["dof_rhs_per_s"], 3)
-    out["max_receiver_rel_diff"] = err
+    out["workload"] = ("marine CSEM %dx%dx%d cells (%d tets, %d edges, %d nonzeros), %d x-dipoles x %d frequencies "
+                       "(%s Hz) = %d right-hand sides, true residual 1e-8"
+                       % (cells[0], cells[1], cells[2], info["n_tets"], E, nnz, n_tx, len(m.freqs),
+                          "/".join("%g" % f for f in m.freqs), n_tx * len(m.freqs)))
+    out["speedup_vs_plain"] = round(out["mixed_batched"]["dof_rhs_per_s"] / out["plain_fp64_single_rhs"]["dof_rhs_per_s"], 3)
+    out["max_receiver_rel_diff_vs_plain"] = err
+    out["roofline"] = roof
     return out
+
+
+def smoother_kernel_name(rows):
+    return ("k_spmm_stream (LDS-staged row blocks)" if rows == -1 else
+            "k_spmm_pstream (persistent pipelined LDS-staged row blocks)" if rows == -3 else
+            "k_spmm_xnarrow (LDS x-blocks, %d lanes per row)" % -rows if rows < -1 else
+            "many-column kernel (column slabs)" if rows == 0 else
+            "k_spmm_narrow role 1 (%d lanes per row)" % rows)
 
 
 def batch_leg(api, mg, n_models, n_single=8):
@@ -275,8 +318,9 @@ def main():
     ap.add_argument("--cells", type=int, default=60, help="grid cells per axis")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain fp64 single-RHS comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-csem", action="store_true", help="skip the batched marine-CSEM comparison")
-    ap.add_argument("--csem-cells", type=str, default="48,48,40", help="marine-CSEM grid cells x,y,z")
+    ap.add_argument("--no-csem", action="store_true", help="skip the marine-CSEM 64-transmitter leg")
+    ap.add_argument("--csem-cells", type=str, default="80,80,66",
+                    help="marine-CSEM grid cells x,y,z (80,80,66: 3.0M edges)")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive accuracy-versus-time comparison")
     ap.add_argument("--batch-models", type=int, default=64,
                     help="models of the uneven multi-model batch (0 skips it)")
@@ -335,11 +379,7 @@ def main():
     bw = (b_bytes / (b_ms * 1e-3) / 1e9) if b_ms > 0 else 0.0
     rows = int(ctx.stat("b_spmm_rows"))
     kname = ("%s: Chebyshev step x' = x + c1 (x - x_prev) + c2 Dinv .* (b - B x) in one pass, "
-             "B = C + M fp32, XY+YX+dual = 6 real columns"
-             % ("k_spmm_stream (LDS-staged row blocks)" if rows == -1 else
-                "k_spmm_pstream (persistent pipelined LDS-staged row blocks)" if rows == -3 else
-                "k_spmm_xnarrow (LDS x-blocks, %d lanes per row)" % -rows if rows < -1 else
-                "k_spmm_narrow role 1 (%d lanes per row)" % rows))
+             "B = C + M fp32, XY+YX+dual = 6 real columns" % smoother_kernel_name(rows))
     # HBM traffic of the same kernel and workload from the committed PMC summary
     # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on k_make_b), when it exists
     traffic = None
@@ -389,8 +429,7 @@ def main():
         extra["speedup_vs_plain_solve"] = round(p_solve / mixed_solve, 3)
         ctxp.close()
     if not args.no_csem:
-        extra["csem_batched_64tx"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")),
-                                              graph=not args.no_graph)
+        extra["config3_marine_csem"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")))
     if not args.no_adaptive:
         extra["adaptive"] = adaptive_leg(api, mg)
     if args.batch_models > 0:

@@ -303,6 +303,7 @@ class DevAms {
       blk_[nc].upload(blk, s);
       B.blk_row = blk_[nc].get();
       B.nblk = (int)blk.size() - 1;
+      EMF_HIP(hipStreamSynchronize(s));  // blk is host memory of this scope
     }
     if (nc >= 16) {
       if (spmm_rows_ == 3) x_blocks_table(s);  // x-blocks only on request

@@ -715,6 +715,104 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ 
   }
 }
 
+// Vector chunk of W consecutive values, loaded and stored as one W-wide access
+// (dwordx2 / dwordx3 / dwordx4); W = 3 needs only the element alignment.
+template <typename T, int W>
+struct alignas(sizeof(T) * (W == 3 ? 1 : W)) ChunkT {
+  T v[W];
+};
+
+// Widest chunk that divides NC: 4, 3 or 2 values (fp32), 2 values (fp64).
+template <typename T, int NC>
+constexpr int row_chunk() {
+  return sizeof(T) == 8 ? (NC % 2 == 0 ? 2 : 1) : (NC % 4 == 0 ? 4 : (NC % 3 == 0 ? 3 : (NC % 2 == 0 ? 2 : 1)));
+}
+
+// k_spmm_stream on compact blocks (every leading dimension = NC): each gathered
+// x row is read as NC / W vector chunks, and in the epilogue a thread owns one W-wide
+// chunk of a row, loading its operands and storing its result as vectors.  The
+// products and the summation order are those of k_spmm_stream.
+template <typename TA, typename TX, typename T, int NC, int TILE>
+__global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                          const TA *__restrict__ av, const TX *__restrict__ x,
+                                                          T *y, T alpha, const T *__restrict__ rs, T gamma,
+                                                          const T *w, T beta, const T *z,
+                                                          const int *__restrict__ blk_row, ChebFuse<T> cf) {
+  constexpr int WX = row_chunk<TX, NC>(), W = row_chunk<T, NC>();
+  constexpr int CX = NC / WX, CY = NC / W;
+  using VX = ChunkT<TX, WX>;
+  using VY = ChunkT<T, W>;
+  __shared__ T prod[TILE * NC];
+  __shared__ int srp[TILE + 1];
+  const int b = xcd_block();
+  const int r0 = blk_row[b], r1 = blk_row[b + 1];
+  const int p0 = rp[r0];
+  const int nnz = rp[r1] - p0;
+  const int nr = r1 - r0;
+  for (int i = threadIdx.x; i <= nr; i += kBlock) srp[i] = rp[r0 + i] - p0;
+  constexpr int kTrips = TILE / kBlock;
+  int c[kTrips];
+  T a[kTrips];
+  const int last = max(nnz - 1, 0);
+#pragma unroll
+  for (int u = 0; u < kTrips; ++u) {
+    const int pc = p0 + min((int)threadIdx.x + u * kBlock, last);
+    c[u] = ci[pc];
+    a[u] = (T)av[pc];
+  }
+  VX xv[kTrips][CX];
+#pragma unroll
+  for (int u = 0; u < kTrips; ++u) {
+    const VX *xr = reinterpret_cast<const VX *>(x + (size_t)c[u] * NC);
+#pragma unroll
+    for (int k = 0; k < CX; ++k) xv[u][k] = xr[k];
+  }
+#pragma unroll
+  for (int u = 0; u < kTrips; ++u) {
+    const int idx = threadIdx.x + u * kBlock;
+    if (idx < nnz) {
+#pragma unroll
+      for (int k = 0; k < CX; ++k)
+#pragma unroll
+        for (int i = 0; i < WX; ++i) prod[idx * NC + k * WX + i] = a[u] * (T)xv[u][k].v[i];
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nr * CY; q += kBlock) {
+    const int lr = q / CY, j0 = (q % CY) * W, row = r0 + lr;
+    const size_t o = (size_t)row * NC + j0;
+    VY wv, zv, xk, xd;
+    if (gamma != T(0)) wv = *reinterpret_cast<const VY *>(w + o);
+    if (beta != T(0)) zv = *reinterpret_cast<const VY *>(z + o);
+    if (cf.on()) {
+      xk = *reinterpret_cast<const VY *>(cf.x + o);
+      if (cf.d) xd = *reinterpret_cast<const VY *>(cf.d + o);
+    }
+    const T sc = rs ? rs[row] : T(1);
+    T acc[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) acc[i] = T(0);
+    for (int k = srp[lr]; k < srp[lr + 1]; ++k) {
+#pragma unroll
+      for (int i = 0; i < W; ++i) acc[i] += prod[k * NC + j0 + i];
+    }
+    VY out;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      T v = alpha * acc[i];
+      if (gamma != T(0)) v += gamma * wv.v[i];
+      v *= sc;
+      if (beta != T(0)) v += beta * zv.v[i];
+      if (cf.on()) {
+        const T xo = xk.v[i], dv = cf.d ? xd.v[i] : T(0);
+        v = xo + T(cf.c1) * (xo - dv) + T(cf.c2) * v;
+      }
+      out.v[i] = v;
+    }
+    *reinterpret_cast<VY *>(y + o) = out;
+  }
+}
+
 template <typename TA, typename TX, typename T, int NC>
 void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, int ldy, T alpha, const T *rs,
                    T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs,
@@ -758,8 +856,21 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
     // LDS-staged row blocks (tables built by row_blocks for this NC and T)
     EMF_REQUIRE(A.blk_row != nullptr && A.nblk > 0, kErrArg, "row-block SpMM without its block table");
     EMF_REQUIRE(xs == nullptr && zs == nullptr, kErrArg, "row-block SpMM has no scaled operands");
-    hipLaunchKernelGGL((k_spmm_stream<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(A.nblk), dim3(kBlock), 0, s, A.rp,
-                       A.ci, A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, A.blk_row, cf);
+    // compact blocks with vector-aligned bases take the vector-chunk kernel
+    constexpr int WX = row_chunk<TX, NC>(), WY = row_chunk<T, NC>();
+    auto aligned = [](const void *p, size_t a) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) % a) == 0; };
+    const size_t ax = sizeof(TX) * (WX == 3 ? 1 : WX), ay = sizeof(T) * (WY == 3 ? 1 : WY);
+    const bool vec = WX > 1 && WY > 1 && ldx == NC && ldy == NC && (gamma == T(0) || ldw == NC) &&
+                     (beta == T(0) || ldz == NC) && (!cf.on() || cf.ld == NC) && cf.zcol == nullptr &&
+                     aligned(x, ax) && aligned(y, ay) && aligned(gamma == T(0) ? nullptr : w, ay) &&
+                     aligned(beta == T(0) ? nullptr : z, ay) && aligned(cf.on() ? cf.x : nullptr, ay) &&
+                     aligned(cf.on() ? cf.d : nullptr, ay);
+    if (vec)
+      hipLaunchKernelGGL((k_spmm_stream_v<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(A.nblk), dim3(kBlock), 0, s,
+                         A.rp, A.ci, A.v, x, y, alpha, rs, gamma, w, beta, z, A.blk_row, cf);
+    else
+      hipLaunchKernelGGL((k_spmm_stream<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(A.nblk), dim3(kBlock), 0, s,
+                         A.rp, A.ci, A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, A.blk_row, cf);
     return;
   }
   // one trip of 4 nonzeros per lane covers a row of average length
