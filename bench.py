@@ -169,7 +169,7 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1):
 
 
 def smoother_kernel_name(rows):
-    return ("k_spmm_stream (LDS-staged row blocks)" if rows == -1 else
+    return ("k_spmm_stream_v (LDS-staged row blocks, vector-chunk gathers and epilogue)" if rows == -1 else
             "k_spmm_pstream (persistent pipelined LDS-staged row blocks)" if rows == -3 else
             "k_spmm_xnarrow (LDS x-blocks, %d lanes per row)" % -rows if rows < -1 else
             "many-column kernel (column slabs)" if rows == 0 else
@@ -384,7 +384,7 @@ def main():
     # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on k_make_b), when it exists
     traffic = None
     prof = os.path.join(ROOT, "profiles", "%s_smoother_adaptive_mt_%d.md"
-                        % ("k_spmm_stream" if rows == -1 else "k_spmm_pstream" if rows == -3 else
+                        % ("k_spmm_stream_v" if rows == -1 else "k_spmm_pstream" if rows == -3 else
                            "k_spmm_xnarrow" if rows < -1 else "k_spmm_narrow",
                            args.cells))
     if os.path.exists(prof):
