@@ -83,6 +83,14 @@ def cpu_baseline(budget_s=20.0):
                        "estimator, %.1f s" % (om.n_edges, m.freqs[fidx], len(sols), dt))
 
 
+# Edge smoother of the CSEM solves (both paths): Chebyshev degree 8 on [lmax / 120,
+# lmax].  On the marine model it needs fewer inner iterations per unit of work than
+# the MT-tuned default (degree 5 on [lmax / 60, lmax]): the 64-transmitter solve at
+# 663,688 edges takes 8.55 s against 9.38 s, a plain single-RHS solve 0.41 s against
+# 0.45 s.
+CSEM_SMOOTHER = "-ams_cheb_degree 8 -ams_cheb_ratio 120"
+
+
 def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1):
     """Marine CSEM (the north star's third configuration): n_tx x-directed dipoles at
     each of the model's frequencies (0.25 / 0.75 / 1.25 Hz), each frequency one block of
@@ -98,7 +106,7 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1):
     out = {"workload": None, "per_frequency": []}
     fields = {}
     t0 = time.perf_counter()
-    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -use_graph false")
+    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -use_graph false " + CSEM_SMOOTHER)
     ctx.load_model(m)
     ctx.create_linear_system(n_tx)
     info = ctx.mesh_info()
@@ -136,7 +144,7 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1):
     out["mixed_batched"] = {"dof_rhs_per_s": round(E * n_tx * len(m.freqs) / tot_solve, 1),
                             "dof_rhs_per_s_with_setup": round(E * n_tx * len(m.freqs) / tot_step, 1),
                             "rhs": n_tx * len(m.freqs), "solve_s": round(tot_solve, 3)}
-    ctxp = api.Context("-inner_precision fp64 -e_rtol 1e-8")
+    ctxp = api.Context("-inner_precision fp64 -e_rtol 1e-8 " + CSEM_SMOOTHER)
     ctxp.load_model(m)
     ctxp.create_linear_system(n_plain)
     ctxp.assemble_matrix(plain_fidx)

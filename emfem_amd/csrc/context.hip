@@ -636,6 +636,7 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
     c->stats["solve_ms"] = ms;
     c->stats["outer_its"] = st.outer_its;
     c->stats["inner_its"] = (double)st.inner_its;
+    c->stats["inner_col_its"] = (double)st.inner_col_its;
     c->stats["pc_applies"] = (double)st.pc_applies;
     c->stats["vcycles"] = (double)st.vcycles;
     c->stats["fell_back"] = (double)st.fell_back;

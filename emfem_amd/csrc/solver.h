@@ -22,6 +22,7 @@ struct SolveStats {
   int fell_back = 0;         // 1 if the solve switched to the fp64 cycle
   int outer_its = 0;         // max over right-hand sides
   long inner_its = 0;        // FCG iterations summed over preconditioner applications
+  long inner_col_its = 0;    // FCG iterations summed over the columns still active in them
   long pc_applies = 0;       // preconditioner applications (each on 2k columns)
   long vcycles = 0;          // AMS cycles
   std::vector<int> its;      // per right-hand side

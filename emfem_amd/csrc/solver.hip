@@ -840,6 +840,7 @@ class InnerPcImpl : public InnerPc {
       EMF_HIP(hipStreamSynchronize(s));
       any = false;
       for (int c = 0; c < nl; ++c) {
+        st.inner_col_its += h_active_[c] != 0;
         if (h_active_[c] && h_rr_[c] <= tol2 * h_bb_[c]) h_active_[c] = 0;
         any |= h_active_[c] != 0;
       }

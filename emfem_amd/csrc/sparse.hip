@@ -391,33 +391,48 @@ __global__ __launch_bounds__(kBlock) void k_spmm_wvec(int nrows, int rblocks, co
   const int gid = (bid - slab * rblocks) * kBlock + threadIdx.x;
   const int row = gid / LW;
   const int c0 = slab * LW * VEC + (gid % LW) * VEC;
-  if (row >= nrows || c0 >= nc) return;
+  if (row >= nrows) return;
   T acc[VEC];
 #pragma unroll
   for (int j = 0; j < VEC; ++j) acc[j] = T(0);
   const int b = rp[row], e = rp[row + 1];
-  constexpr int kU = 4;
-  for (int p0 = b; p0 < e; p0 += kU) {
-    int cu[kU];
-    T au[kU];
+  {
+    // The LW lanes of the row load up to LW of its column indices and values at once
+    // (one per lane) and pass them round by shuffles; eight 16-byte gathers are then
+    // in flight per lane instead of a load -> gather dependency every four nonzeros.
+    // Lanes past the last column slab (c0 >= nc) take part in the loads and shuffles
+    // and gather a valid column block they discard.
+    const int lane = gid % LW;
+    const int cx = c0 < nc ? c0 : 0;
+    constexpr int kG = 8;
+    for (int q0 = b; q0 < e; q0 += LW) {
+      const int n = min(LW, e - q0);
+      const int pl = q0 + min(lane, n - 1);
+      const int cl = ci[pl];
+      T al = (T)av[pl];
+      if (xs) al *= xs[cl];
+      for (int u0 = 0; u0 < n; u0 += kG) {
+        int cu[kG];
+        T au[kG];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const bool ok = p0 + u < e;
-      const int pc = min(p0 + u, e - 1);
-      cu[u] = ci[pc];
-      const T av_ = (T)av[pc];
-      au[u] = ok ? av_ : T(0);
-      if (xs && ok) au[u] *= xs[cu[u]];
+        for (int u = 0; u < kG; ++u) {
+          const int src = min(u0 + u, n - 1);
+          cu[u] = __shfl(cl, src, LW);
+          const T a_ = __shfl(al, src, LW);
+          au[u] = u0 + u < n ? a_ : T(0);
+        }
+        V xv[kG];
+#pragma unroll
+        for (int u = 0; u < kG; ++u) xv[u] = *reinterpret_cast<const V *>(x + (size_t)cu[u] * ldx + cx);
+#pragma unroll
+        for (int u = 0; u < kG; ++u) {
+          const TX *xe = reinterpret_cast<const TX *>(&xv[u]);
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) acc[j] += au[u] * (T)xe[j];
+        }
+      }
     }
-    V xv[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) xv[u] = *reinterpret_cast<const V *>(x + (size_t)cu[u] * ldx + c0);
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const TX *xe = reinterpret_cast<const TX *>(&xv[u]);
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) acc[j] += au[u] * (T)xe[j];
-    }
+    if (c0 >= nc) return;
   }
   const T scale = rs ? rs[row] : T(1);
   const T zscale = zs ? beta * zs[row] : beta;
