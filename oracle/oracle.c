@@ -537,6 +537,44 @@ int or_assemble_matrix(int nt, const double *pts, const int *tets, const int *te
   return 0;
 }
 
+/* or_assemble_matrix over nthreads OpenMP threads (the CPU baseline's timing only):
+ * element matrices in parallel, scatter-adds as atomic updates, so the sums agree with
+ * the serial loop to rounding but not bit for bit.  C, M must be zeroed. */
+int or_assemble_matrix_mt(int nt, const double *pts, const int *tets, const int *tet2edge, const int *edgelist,
+                          const unsigned char *edge_bdr, const int *attr, int aniso_form, const double *rho,
+                          double freq, const int *rowptr, const int *col, double *Cv, double *Mv,
+                          unsigned char *bdr_cell, int nthreads) {
+  double omega = 2 * OR_PI * freq;
+  int nc = rho_cols(aniso_form), bad = 0, t;
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(| : bad)
+  for (t = 0; t < nt; ++t) {
+    fe_values fv;
+    double sigma[9], C[6][6], M[6][6];
+    int i, j;
+    if (or_sigma(aniso_form, &rho[attr[t] * nc], sigma) != 0) { bad = 1; continue; }
+    fe_reinit(&fv, pts, tets, tet2edge, edgelist, t);
+    element_matrices(&fv, sigma, omega, C, M);
+    bdr_cell[t] = 0;
+    for (i = 0; i < 6; ++i)
+      if (edge_bdr[fv.edge[i]]) {
+        for (j = 0; j < 6; ++j) {
+          if (j != i) C[i][j] = C[j][i] = 0.0;
+          M[i][j] = M[j][i] = 0.0;
+        }
+        bdr_cell[t] = 1;
+      }
+    for (i = 0; i < 6; ++i)
+      for (j = 0; j < 6; ++j) {
+        int p = csr_find(rowptr, col, fv.edge[i], fv.edge[j]);
+#pragma omp atomic
+        Cv[p] += C[i][j];
+#pragma omp atomic
+        Mv[p] += M[i][j];
+      }
+  }
+  return bad ? -1 : 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Point location (src/em_mesh.cc:504-566, src/em_mesh.h:208-230)             */
 /* ------------------------------------------------------------------------- */
@@ -1189,6 +1227,89 @@ int or_estimate_error(int nt, const double *pts, const int *tets, const int *tet
 /* ------------------------------------------------------------------------- */
 /* Marking (src/em_fwd.cc:1182-1240)                                         */
 /* ------------------------------------------------------------------------- */
+
+/* or_estimate_error over nthreads OpenMP threads (the CPU baseline's timing only), with
+ * the serial loop's arithmetic: each interior face is evaluated from its lower-index
+ * tet, which writes both sides' slots (no other thread writes them), then every tet
+ * sums its cell and face terms.  Results equal the serial ones bit for bit. */
+int or_estimate_error_mt(int nt, const double *pts, const int *tets, const int *tet2edge, const int *edgelist,
+                         const int *neighbor, const int *non, const int *attr, int aniso_form, const double *rho,
+                         double freq, int ns, const double *const *sols, double *errors, int nthreads) {
+  double omega = 2 * OR_PI * freq;
+  int nc = rho_cols(aniso_form), bad = 0, t;
+  double *face_err = (double *)calloc((size_t)ns * nt * 4, sizeof(double));
+  double *cell_err = (double *)calloc((size_t)ns * nt, sizeof(double));
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(| : bad)
+  for (t = 0; t < nt; ++t) {
+    fe_values fv;
+    double sigma[9], qp[4][3], jxw[4];
+    int f, s, q, k;
+    if (or_sigma(aniso_form, &rho[attr[t] * nc], sigma) != 0) { bad = 1; continue; }
+    fe_reinit(&fv, pts, tets, tet2edge, edgelist, t);
+    cell_quadrature(&fv, qp, jxw);
+    double h = or_tet_diameter(pts, tets, t);
+    for (s = 0; s < ns; ++s) {
+      double res = 0.0;
+      for (q = 0; q < 4; ++q) {
+        zdouble e[3], se[3];
+        fn_value(&fv, qp[q], sols[s], e);
+        sig_mul(sigma, e, se);
+        double nrm = 0;
+        for (k = 0; k < 3; ++k) { zdouble v = I * omega * se[k]; nrm += creal(v) * creal(v) + cimag(v) * cimag(v); }
+        res += nrm * jxw[q];
+      }
+      cell_err[s * nt + t] = res * h * h;
+    }
+    for (f = 0; f < 4; ++f) {
+      int n = neighbor[t * 4 + f];
+      if (n < t) continue;  /* boundary faces stay 0; shared faces belong to the lower index */
+      int nn = non[t * 4 + f];
+      fe_values fvn;
+      double sign[9], fqp[3][3], fjxw[3], fn[3];
+      or_sigma(aniso_form, &rho[attr[n] * nc], sign);
+      fe_reinit(&fvn, pts, tets, tet2edge, edgelist, n);
+      face_normal(&fv, f, fn);
+      face_quadrature(&fv, f, fqp, fjxw);
+      double hf = or_face_diameter(pts, tets, t, f);
+      for (s = 0; s < ns; ++s) {
+        double dotj = 0.0, crossh = 0.0;
+        for (q = 0; q < 3; ++q) {
+          zdouble e[3], en[3], se[3], sen[3], ce[3], cen[3], d, cr[3];
+          fn_value(&fv, fqp[q], sols[s], e);
+          fn_value(&fvn, fqp[q], sols[s], en);
+          sig_mul(sigma, e, se);
+          sig_mul(sign, en, sen);
+          d = (se[0] - sen[0]) * fn[0] + (se[1] - sen[1]) * fn[1] + (se[2] - sen[2]) * fn[2];
+          dotj += (creal(d) * creal(d) + cimag(d) * cimag(d)) * fjxw[q];
+          fn_curl(&fv, sols[s], ce);
+          fn_curl(&fvn, sols[s], cen);
+          zdouble dc[3] = {ce[0] - cen[0], ce[1] - cen[1], ce[2] - cen[2]};
+          cr[0] = dc[1] * fn[2] - dc[2] * fn[1];
+          cr[1] = dc[2] * fn[0] - dc[0] * fn[2];
+          cr[2] = dc[0] * fn[1] - dc[1] * fn[0];
+          double c2 = 0;
+          for (k = 0; k < 3; ++k) c2 += creal(cr[k]) * creal(cr[k]) + cimag(cr[k]) * cimag(cr[k]);
+          crossh += c2 * fjxw[q];
+        }
+        double v = (dotj + crossh) * hf / 2.0;
+        face_err[(s * nt + t) * 4 + f] = v;
+        face_err[(s * nt + n) * 4 + nn] = v;
+      }
+    }
+  }
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (t = 0; t < nt; ++t) {
+    int s, f;
+    for (s = 0; s < ns; ++s) {
+      double v = 0.0 + cell_err[s * nt + t];
+      for (f = 0; f < 4; ++f) v += face_err[(s * nt + t) * 4 + f];
+      errors[s * nt + t] = v;
+    }
+  }
+  free(face_err);
+  free(cell_err);
+  return bad ? -1 : 0;
+}
 
 static int cmp_desc(const void *a, const void *b) {
   double x = *(const double *)a, y = *(const double *)b;

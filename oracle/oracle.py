@@ -55,6 +55,8 @@ def _declare(L):
         "or_element_matrices": (I, [P, P, P, P, I, P, D, P, P]),
         "or_tet_geometry": (None, [P, P, I, P, P]),
         "or_assemble_matrix": (I, [I, P, P, P, P, P, P, I, P, D, P, P, P, P, P]),
+        "or_assemble_matrix_mt": (I, [I, P, P, P, P, P, P, I, P, D, P, P, P, P, P, I]),
+        "or_estimate_error_mt": (I, [I, P, P, P, P, P, P, P, I, P, D, I, P, P, I]),
         "or_contains_point": (I, [P, P, I, P]),
         "or_vertex_to_tet": (None, [I, P, P]),
         "or_nearest_vertex": (I, [I, P, P]),
@@ -184,17 +186,21 @@ def tet_geometry(mesh, t):
     return coef.reshape(4, 4), float(det[0])
 
 
-def assemble(mesh, model, freq, pattern=None):
-    """C, M values on the CSR pattern plus the bdr-cell flags (src/em_fwd.cc:318-400)."""
+def assemble(mesh, model, freq, pattern=None, threads=1):
+    """C, M values on the CSR pattern plus the bdr-cell flags (src/em_fwd.cc:318-400).
+    threads > 1 runs the OpenMP loop (atomic scatter: equal to rounding, not bitwise)."""
     rowptr, col = pattern if pattern is not None else mesh.pattern()
     C = np.zeros(col.size)
     M = np.zeros(col.size)
     bdr = np.zeros(mesh.n_tets, dtype=np.uint8)
     rho = _c(model.rho, np.float64)
     attr = _c(model.attr, np.int32)
-    rc = lib().or_assemble_matrix(mesh.n_tets, _p(mesh.points), _p(mesh.tets), _p(mesh.tet2edge),
-                                  _p(mesh.edgelist), _p(mesh.edge_bdr), _p(attr), int(model.aniso_form),
-                                  _p(rho), float(freq), _p(rowptr), _p(col), _p(C), _p(M), _p(bdr))
+    args = (mesh.n_tets, _p(mesh.points), _p(mesh.tets), _p(mesh.tet2edge), _p(mesh.edgelist), _p(mesh.edge_bdr),
+            _p(attr), int(model.aniso_form), _p(rho), float(freq), _p(rowptr), _p(col), _p(C), _p(M), _p(bdr))
+    if threads > 1:
+        rc = lib().or_assemble_matrix_mt(*args, int(threads))
+    else:
+        rc = lib().or_assemble_matrix(*args)
     if rc != 0:
         raise ValueError("assembly failed")
     return rowptr, col, C, M, bdr
@@ -389,17 +395,21 @@ def responses_mt(mesh, model, fidx, e_xy, e_yx):
     return out
 
 
-def estimate_error(mesh, model, freq, sols):
-    """Per-tet squared indicators for each solution (src/em_fwd.cc:1005-1170)."""
+def estimate_error(mesh, model, freq, sols, threads=1):
+    """Per-tet squared indicators for each solution (src/em_fwd.cc:1005-1170);
+    threads > 1 runs the OpenMP loop (bitwise the serial result)."""
     L = lib()
     nt = mesh.n_tets
     bufs = [np.ascontiguousarray(np.stack([s.real, s.imag], axis=1)) for s in sols]
     arr = (ctypes.c_void_p * len(bufs))(*[b.ctypes.data for b in bufs])
     errors = np.empty((len(sols), nt))
-    rc = L.or_estimate_error(nt, _p(mesh.points), _p(mesh.tets), _p(mesh.tet2edge), _p(mesh.edgelist),
-                             _p(mesh.neighbor), _p(mesh.non), _p(_c(model.attr, np.int32)), int(model.aniso_form),
-                             _p(_c(model.rho, np.float64)), float(freq), len(sols), ctypes.cast(arr, P),
-                             _p(errors))
+    args = (nt, _p(mesh.points), _p(mesh.tets), _p(mesh.tet2edge), _p(mesh.edgelist), _p(mesh.neighbor),
+            _p(mesh.non), _p(_c(model.attr, np.int32)), int(model.aniso_form), _p(_c(model.rho, np.float64)),
+            float(freq), len(sols), ctypes.cast(arr, P), _p(errors))
+    if threads > 1:
+        rc = L.or_estimate_error_mt(*args, int(threads))
+    else:
+        rc = L.or_estimate_error(*args)
     if rc != 0:
         raise ValueError("estimator failed")
     return errors

@@ -363,3 +363,19 @@ def test_direct_solution_solves_real_equivalent_system(oracle_mod):
     eb = mesh.edge_bdr.astype(bool)
     assert np.array_equal(have.astype(bool), eb)
     assert np.allclose(e[eb], bv[eb], rtol=1e-12, atol=1e-14 * np.abs(bv).max())
+
+
+def test_threaded_assembly_and_estimator_match_serial(oracle_mod):
+    """The OpenMP loops of the CPU baseline compute the serial oracle's values."""
+    m = mg.tiny_mt(n=3, aniso_form=3, seed=6, freq=0.4)
+    mesh = oracle_mod.Mesh(m.points, m.tets)
+    rp, ci, C, M, b = oracle_mod.assemble(mesh, m, m.freqs[0])
+    _, _, C4, M4, b4 = oracle_mod.assemble(mesh, m, m.freqs[0], (rp, ci), threads=4)
+    assert np.max(np.abs(C4 - C)) <= 1e-14 * np.max(np.abs(C))
+    assert np.max(np.abs(M4 - M)) <= 1e-14 * np.max(np.abs(M))
+    assert np.array_equal(b, b4)
+    rng = np.random.default_rng(2)
+    sols = [rng.standard_normal(mesh.n_edges) + 1j * rng.standard_normal(mesh.n_edges) for _ in range(2)]
+    e1 = oracle_mod.estimate_error(mesh, m, m.freqs[0], sols)
+    e4 = oracle_mod.estimate_error(mesh, m, m.freqs[0], sols, threads=4)
+    assert np.array_equal(e1, e4)
