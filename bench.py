@@ -226,10 +226,13 @@ def smoother_kernel_name(rows):
 
 
 def batch_leg(api, mg, n_models, n_single=8):
-    """Uneven multi-model batch: n_models independent MT models (edges log-uniform in
-    2e4..2e5, one random frequency each), XY and YX per model, solved as one
-    block-diagonal batch (one launch per operation for all models, per-model Krylov
-    processes) against the same models solved one at a time (the first n_single)."""
+    """Uneven multi-model batch (the north star's fourth configuration): n_models
+    independent MT models (edges log-uniform in 2e4..2e5, one random frequency each),
+    XY and YX per model, solved as one block-diagonal batch with per-model Krylov
+    processes, once in lock-step (the default) and once in stages that retire converged
+    models; beside them the first n_single models solved one at a time.  Per-model
+    time = from the start of the batch solve until both of its right-hand sides were
+    accepted."""
     models = mg.uneven_mt_batch(n_models=n_models)
     t0 = time.perf_counter()
     ctxs = []
@@ -243,13 +246,27 @@ def batch_leg(api, mg, n_models, n_single=8):
         ctxs.append(c)
     t1 = time.perf_counter()
     edges = np.array([c.mesh_info()["n_edges"] for c in ctxs])
-    its, rr, ms = api.solve_batch(ctxs, 0, 2, 300, 1e-8)
-    t2 = time.perf_counter()
-    setup = ctxs[0].stat("batch_setup_ms") / 1e3
-    solve = ctxs[0].stat("batch_solve_ms") / 1e3
-    per_model = ms.max(axis=1)
-    log("batch: %d models, %d edges in all, contexts %.1fs, set-up %.2fs, solve %.2fs, its max %d, relres max %.1e"
-        % (len(ctxs), edges.sum(), t1 - t0, setup, solve, its.max(), rr.max()))
+    out = {"workload": "%d uneven MT models, %d..%d edges (%d in all), f log-uniform 1e-2..10 Hz, XY+YX"
+                       % (len(ctxs), edges.min(), edges.max(), edges.sum())}
+    for name, retire in (("lock_step", "false"), ("staged", "true")):
+        ctxs[0].process_options("-batch_retire %s" % retire)
+        ta = time.perf_counter()
+        its, rr, ms = api.solve_batch(ctxs, 0, 2, 300, 1e-8)
+        wall = time.perf_counter() - ta
+        per_model = ms.max(axis=1)
+        setup = ctxs[0].stat("batch_setup_ms") / 1e3
+        solve = ctxs[0].stat("batch_solve_ms") / 1e3
+        out[name] = {"dof_rhs_per_s": round(2.0 * edges.sum() / wall, 1), "wall_s": round(wall, 3),
+                     "setup_s": round(setup, 3), "solve_s": round(solve, 3),
+                     "stages": int(ctxs[0].stat("batch_stages")),
+                     "per_model_ms_p50": round(float(np.percentile(per_model, 50)), 1),
+                     "per_model_ms_p99": round(float(np.percentile(per_model, 99)), 1),
+                     "max_outer_its": int(its.max()), "max_true_relres": float(rr.max())}
+        log("batch %s: %d models, %d edges, contexts %.1fs, wall %.2fs (set-up %.2fs, solve %.2fs, %d stages), "
+            "p50 %.0f ms p99 %.0f ms, its max %d, relres max %.1e"
+            % (name, len(ctxs), edges.sum(), t1 - t0, wall, setup, solve, out[name]["stages"],
+               out[name]["per_model_ms_p50"], out[name]["per_model_ms_p99"], its.max(), rr.max()))
+    ctxs[0].process_options("-batch_retire false")
     seq = []
     for c in ctxs[:n_single]:
         ta = time.perf_counter()
@@ -258,19 +275,9 @@ def batch_leg(api, mg, n_models, n_single=8):
         seq.append(time.perf_counter() - ta)
         c.destroy_pc()
     seq_rate = 2.0 * edges[:n_single].sum() / sum(seq)
-    out = {
-        "workload": "%d uneven MT models, %d..%d edges (%d in all), f log-uniform 1e-2..10 Hz, XY+YX"
-                    % (len(ctxs), edges.min(), edges.max(), edges.sum()),
-        "dof_rhs_per_s": round(2.0 * edges.sum() / (setup + solve), 1),
-        "solve_dof_rhs_per_s": round(2.0 * edges.sum() / solve, 1),
-        "setup_s": round(setup, 3), "solve_s": round(solve, 3),
-        "per_model_ms_p50": round(float(np.percentile(per_model, 50)), 1),
-        "per_model_ms_p99": round(float(np.percentile(per_model, 99)), 1),
-        "max_outer_its": int(its.max()), "max_true_relres": float(rr.max()),
-        "one_at_a_time_dof_rhs_per_s": round(seq_rate, 1),
-        "one_at_a_time_models": n_single,
-    }
-    out["speedup_vs_one_at_a_time"] = round(out["dof_rhs_per_s"] / seq_rate, 3)
+    out["one_at_a_time_dof_rhs_per_s"] = round(seq_rate, 1)
+    out["one_at_a_time_models"] = n_single
+    out["speedup_vs_one_at_a_time"] = round(out["lock_step"]["dof_rhs_per_s"] / seq_rate, 3)
     for c in ctxs:
         c.close()
     return out
@@ -471,7 +478,7 @@ def main():
     ap.add_argument("--csem-cells", type=str, default="80,80,66",
                     help="marine-CSEM grid cells x,y,z (80,80,66: 3.0M edges)")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive accuracy-versus-time comparison")
-    ap.add_argument("--batch-models", type=int, default=64,
+    ap.add_argument("--batch-models", type=int, default=256,
                     help="models of the uneven multi-model batch (0 skips it)")
     ap.add_argument("--spmm-rows", default="auto", choices=["auto", "lanes", "rowblock", "pipelined", "xblock", "2", "4", "8", "16"],
                     help="row mapping of the smoother SpMM (auto: autotuned)")

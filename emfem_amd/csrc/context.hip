@@ -70,6 +70,10 @@ const std::map<std::string, std::string> &option_defaults() {
       {"fp64_fallback", "true"},
       // acceptance bound at the fp64 residual floor, in units of the requested rtol
       {"true_residual_slack", "10"},
+      // multi-model batches: true = stages that retire converged models (each stage
+      // repeats the auxiliary-space set-up of the models left), false = one lock-step
+      // solve; on 256 uneven MT models lock-step is faster (18.1 s against 21.7 s)
+      {"batch_retire", "false"},
       {"use_graph", "true"},
       {"spmm_rows", "auto"},
       {"ams_cheb_degree", "5"},
@@ -95,6 +99,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"inner_precision", {"fp32", "fp64"}},
       {"dual_precision", {"same", "fp32", "fp64"}},
       {"fp64_fallback", {"true", "false"}},
+      {"batch_retire", {"true", "false"}},
       {"use_graph", {"true", "false"}},
       {"spmm_rows", {"auto", "lanes", "rowblock", "pipelined", "xblock", "2", "4", "8", "16"}},
       {"ams_cycle", {"multiplicative", "additive"}},

@@ -98,6 +98,12 @@ struct OuterOptions {
   // only if the true residual is within floor_factor * rtol * ||b||, and otherwise
   // reported as not converged.
   double floor_factor = 10.0;
+  // warm start: x holds the initial guess on entry (otherwise the solve starts from 0)
+  bool x_given = false;
+  // > 0: return once at most this many right-hand sides are still unconverged (the
+  // Arnoldi cycle in progress is closed when that many are still active, and the
+  // following restart's true-residual check returns); 0 runs to completion
+  int stop_remaining = 0;
 };
 
 // Device workspace of the outer FGMRES, kept by the caller across solves so that

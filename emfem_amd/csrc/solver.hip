@@ -9,6 +9,7 @@
 // the Chebyshev edge smoothing, the auxiliary transfers and the AMG V-cycles in fp32.
 #include "solver.h"
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <tuple>
@@ -953,7 +954,7 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
   std::vector<uint8_t> done(nrhs, 0), est_ok(nrhs, 0);
   std::vector<double> last_true(nrhs, 0.0);
   for (int r = 0; r < nrhs; ++r) done[r] = bnorm[r] == 0.0;
-  EMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * blk, s));
+  if (!o.x_given) EMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * blk, s));
   int total = 0;
   std::vector<double> H((size_t)nrhs * (m + 1) * m), cs((size_t)nrhs * m), sn((size_t)nrhs * m),
       g((size_t)nrhs * (m + 1));
@@ -990,6 +991,7 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
       any |= active[r] != 0;
     }
     if (!any) break;
+    if (o.stop_remaining > 0 && std::count(active.begin(), active.end(), 1) <= o.stop_remaining) break;
     bool switch_now = false;
     std::vector<double> inv(nrhs);
     for (int r = 0; r < nrhs; ++r) inv[r] = active[r] && beta[r] > 0 ? 1.0 / beta[r] : 0.0;
@@ -1064,6 +1066,7 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
       }
       scale_cols(w.get(), V.get() + (size_t)(j + 1) * blk, fac);
       if (!still || switch_now) break;
+      if (o.stop_remaining > 0 && std::count(active.begin(), active.end(), 1) <= o.stop_remaining) break;
     }
     if (switch_now) {
       pc.set_fp32(false, s);

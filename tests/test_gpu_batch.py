@@ -58,3 +58,31 @@ def test_batch_agrees_with_single_model_solves():
             e = c.get_solution(s)
             assert np.linalg.norm(batch[k][s] - e) <= 1e-6 * np.linalg.norm(e)
     assert len(set(its_b[:, 0].tolist())) > 1 or len(set(its_b[:, 1].tolist())) > 1
+
+
+def test_batch_stages_retire_converged_models(oracle_mod):
+    """Twelve models: the staged batch retires the models whose right-hand sides have
+    all converged and restacks the rest, warm-started; every model reaches the exact
+    discrete solution as in one lock-step stage, and models accepted earlier report
+    earlier acceptance times."""
+    models = mg.uneven_mt_batch(n_models=12, seed=7, e_min=3e3, e_max=2.5e4)
+    rtol = 1e-10
+    out = {}
+    for retire in ("true", "false"):
+        ctxs = [_ready(m, "-inner_precision fp32 -batch_retire %s" % retire) for m in models]
+        its, rr, ms = api.solve_batch(ctxs, 0, 2, 300, rtol)
+        out[retire] = (its, ms, ctxs[0].stat("batch_stages"))
+        assert np.all(rr <= rtol * 1.0001), rr
+        for c in ctxs[::3]:
+            rp, ci, C, M = c.get_matrix()
+            for slot in range(2):
+                e = oracle_mod.solve_direct(rp, ci, C, M, *c.get_rhs(slot))
+                assert np.linalg.norm(c.get_solution(slot) - e) <= 1e4 * rtol * np.linalg.norm(e)
+        for c in ctxs:
+            c.close()
+    its, ms, stages = out["true"]
+    assert stages >= 2 and out["false"][2] == 1
+    per_model = ms.max(axis=1)
+    assert np.all(per_model > 0)
+    print("stages", stages, "per-model ms", np.round(per_model, 1).tolist(),
+          "lock-step", np.round(out["false"][1].max(axis=1), 1).tolist())
