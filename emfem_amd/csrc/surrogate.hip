@@ -36,7 +36,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kCat = 2 * kSurHidden;  // [h, mean_nb h] row width
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // fp32 -> bf16 bits, round to nearest even (inputs are finite)
@@ -46,9 +47,10 @@ __device__ __forceinline__ short to_bf16(float f) {
   return (short)(u >> 16);
 }
 
-// One wave per 16 x 16 output tile, four waves per block along N.  Lane l holds
-// A(m0 + l % 16, k0 + 4 (l / 16) + j) and B(k0 + 4 (l / 16) + j, n0 + l % 16), j < 4,
-// and receives D(m0 + 4 (l / 16) + j, n0 + l % 16).
+// One wave per 16 x 16 output tile, four waves per block along N, 32 k per step
+// (v_mfma_f32_16x16x32_bf16).  Lane l holds A(m0 + l % 16, k0 + 8 (l / 16) + j) and
+// B(k0 + 8 (l / 16) + j, n0 + l % 16), j < 8, and receives
+// D(m0 + 4 (l / 16) + j, n0 + l % 16), j < 4.
 __global__ __launch_bounds__(kBlock) void k_gemm_bf16(int M, int N, int K, const float *__restrict__ A, int64_t sam,
                                                       int64_t sak, const float *__restrict__ B, int64_t sbk,
                                                       int64_t sbn, float *C, int64_t ldc, int kchunk,
@@ -61,16 +63,17 @@ __global__ __launch_bounds__(kBlock) void k_gemm_bf16(int M, int N, int K, const
   const int r = lane % 16, q = lane / 16;
   const int m = m0 + r, n = n0 + r;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = kb; k0 < ke; k0 += 16) {
-    bf16x4 a, b;
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+    s16x8 a, b;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = k0 + 4 * q + j;
+    for (int j = 0; j < 8; ++j) {
+      const int k = k0 + 8 * q + j;
       const bool kin = k < ke;
       a[j] = (kin && m < M) ? to_bf16(A[(int64_t)m * sam + (int64_t)k * sak]) : (short)0;
       b[j] = (kin && n < N) ? to_bf16(B[(int64_t)k * sbk + (int64_t)n * sbn]) : (short)0;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                  0, 0, 0);
   }
   const bool split = gridDim.z > 1;
 #pragma unroll
@@ -189,7 +192,7 @@ inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 
 void gemm_bf16(hipStream_t s, int M, int N, int K, const float *A, int64_t sam, int64_t sak, const float *B,
                int64_t sbk, int64_t sbn, float *C, int kchunk, const float *bias, bool relu) {
   if (M <= 0 || N <= 0) return;
-  kchunk = std::max(16, (kchunk + 15) / 16 * 16);
+  kchunk = std::max(32, (kchunk + 31) / 32 * 32);  // whole 32-deep MFMA steps
   const int nch = std::max(1, (K + kchunk - 1) / kchunk);
   dim3 grid((M + 15) / 16, (N + 63) / 64, nch);
   hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(kBlock), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, (int64_t)N, kchunk,
