@@ -1072,9 +1072,9 @@ int or_mt_response(int otype, double freq, const double *exy, const double *hxy,
 int or_csem_response(int otype, const double *es, const double *hs, double *out) {
   const zdouble II = I;
   int comp = otype / 10 % 10 - 1;   /* 1..6 -> Ex..Hz */
-  int ap = otype % 10 == 2;
+  int form = otype % 10, ap = form == 2;   /* 1 = RI, 2 = AP (src/em_defs.h:49-61) */
   zdouble v;
-  if (otype < 111 || otype > 162 || comp < 0 || comp > 5) return -1;
+  if (otype < 111 || otype > 162 || comp < 0 || comp > 5 || (form != 1 && form != 2)) return -1;
   if (comp < 3) v = es[comp * 2] + II * es[comp * 2 + 1];
   else v = hs[(comp - 3) * 2] + II * hs[(comp - 3) * 2 + 1];
   if (ap) v = clog(v);

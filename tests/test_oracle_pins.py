@@ -382,3 +382,20 @@ def test_halfspace_impedance_converges(oracle_mod):
     assert np.all(rates > 0.85), (errs, rates)
     assert errs[-1] < 0.03, errs
     assert phase[-1] < 1.5 and phase[-1] < phase[0], phase
+
+
+def test_csem_response_codes(oracle_mod):
+    """calculate_response_csem (src/em_fwd.cc:804-871): code 1C1 returns component C
+    of E (C = 1..3) or H (C = 4..6) as is, code 1C2 its complex logarithm
+    log|v| + i arg v; other codes are rejected."""
+    E = np.array([0.3 - 1.2j, -2.0 + 0.5j, 1e-3 + 4e-3j])
+    H = np.array([5.0 + 1.0j, -0.2 - 0.7j, 3e-2 - 1e-2j])
+    comps = list(E) + list(H)
+    for c in range(6):
+        v = comps[c]
+        assert oracle_mod.csem_response(100 + 10 * (c + 1) + 1, E, H) == v
+        lv = oracle_mod.csem_response(100 + 10 * (c + 1) + 2, E, H)
+        assert abs(lv.real - np.log(abs(v))) < 1e-15 and abs(lv.imag - np.angle(v)) < 1e-15
+    for bad in (110, 113, 171, 211, 321):
+        with pytest.raises(ValueError):
+            oracle_mod.csem_response(bad, E, H)
