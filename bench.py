@@ -130,7 +130,7 @@ def cpu_baseline(cells_full=60, cells_sample=16):
 CSEM_SMOOTHER = "-ams_cheb_degree 8 -ams_cheb_ratio 120"
 
 
-def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1):
+def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1, n_freqs=None):
     """Marine CSEM (the north star's third configuration): n_tx x-directed dipoles at
     each of the model's frequencies (0.25 / 0.75 / 1.25 Hz), each frequency one block of
     n_tx right-hand sides (fp32 inner cycle in fp64 FGMRES, true residual 1e-8),
@@ -142,6 +142,9 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1):
     of the receiver fields between the two paths relative to each field's norm over
     the receivers."""
     m = mg.marine_csem(cells=cells, n_tx=n_tx)
+    if n_freqs is not None:
+        m.freqs = m.freqs[:n_freqs]
+        plain_fidx = min(plain_fidx, len(m.freqs) - 1)
     out = {"workload": None, "per_frequency": []}
     fields = {}
     t0 = time.perf_counter()
@@ -477,6 +480,8 @@ def main():
     ap.add_argument("--no-csem", action="store_true", help="skip the marine-CSEM 64-transmitter leg")
     ap.add_argument("--csem-cells", type=str, default="80,80,66",
                     help="marine-CSEM grid cells x,y,z (80,80,66: 3.0M edges)")
+    ap.add_argument("--csem-freqs", type=int, default=None,
+                    help="solve only the first N marine-CSEM frequencies (profiling)")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive accuracy-versus-time comparison")
     ap.add_argument("--batch-models", type=int, default=256,
                     help="models of the uneven multi-model batch (0 skips it)")
@@ -585,7 +590,8 @@ def main():
         extra["speedup_vs_plain_solve"] = round(p_solve / mixed_solve, 3)
         ctxp.close()
     if not args.no_csem:
-        extra["config3_marine_csem"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")))
+        extra["config3_marine_csem"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")),
+                                                n_freqs=args.csem_freqs)
     if not args.no_adaptive:
         extra["adaptive"] = adaptive_leg(api, mg)
         params = extra["adaptive"].pop("_params")
