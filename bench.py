@@ -36,6 +36,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# Set by --no-graph: every context of every leg then runs with plain launches, so a
+# kernel tracer sees no hipGraph capture or replay anywhere in the run.
+NO_GRAPH = False
+
+
+def context(api, opts=""):
+    return api.Context(opts + (" -use_graph false" if NO_GRAPH else ""))
+
+
 def run_step(ctx, m, fidx, batched=True):
     t0 = time.perf_counter()
     ctx.assemble_matrix(fidx)
@@ -148,7 +157,7 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1, n_freqs=None):
     out = {"workload": None, "per_frequency": []}
     fields = {}
     t0 = time.perf_counter()
-    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -use_graph false " + CSEM_SMOOTHER)
+    ctx = context(api, "-inner_precision fp32 -e_rtol 1e-8 -use_graph false " + CSEM_SMOOTHER)
     ctx.load_model(m)
     ctx.create_linear_system(n_tx)
     info = ctx.mesh_info()
@@ -186,7 +195,14 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1, n_freqs=None):
     out["mixed_batched"] = {"dof_rhs_per_s": round(E * n_tx * len(m.freqs) / tot_solve, 1),
                             "dof_rhs_per_s_with_setup": round(E * n_tx * len(m.freqs) / tot_step, 1),
                             "rhs": n_tx * len(m.freqs), "solve_s": round(tot_solve, 3)}
-    ctxp = api.Context("-inner_precision fp64 -e_rtol 1e-8 " + CSEM_SMOOTHER)
+    out["workload"] = ("marine CSEM %dx%dx%d cells (%d tets, %d edges, %d nonzeros), %d x-dipoles x %d frequencies "
+                       "(%s Hz) = %d right-hand sides, true residual 1e-8"
+                       % (cells[0], cells[1], cells[2], info["n_tets"], E, nnz, n_tx, len(m.freqs),
+                          "/".join("%g" % f for f in m.freqs), n_tx * len(m.freqs)))
+    out["roofline"] = roof
+    if n_plain == 0:
+        return out
+    ctxp = context(api, "-inner_precision fp64 -e_rtol 1e-8 " + CSEM_SMOOTHER)
     ctxp.load_model(m)
     ctxp.create_linear_system(n_plain)
     ctxp.assemble_matrix(plain_fidx)
@@ -208,13 +224,8 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1, n_freqs=None):
     for a, b in zip(fields["mixed"], fields["plain"]):
         for c in range(a.shape[1]):
             err = max(err, float(np.linalg.norm(a[:, c] - b[:, c]) / max(np.linalg.norm(b[:, c]), 1e-300)))
-    out["workload"] = ("marine CSEM %dx%dx%d cells (%d tets, %d edges, %d nonzeros), %d x-dipoles x %d frequencies "
-                       "(%s Hz) = %d right-hand sides, true residual 1e-8"
-                       % (cells[0], cells[1], cells[2], info["n_tets"], E, nnz, n_tx, len(m.freqs),
-                          "/".join("%g" % f for f in m.freqs), n_tx * len(m.freqs)))
     out["speedup_vs_plain"] = round(out["mixed_batched"]["dof_rhs_per_s"] / out["plain_fp64_single_rhs"]["dof_rhs_per_s"], 3)
     out["max_receiver_rel_diff_vs_plain"] = err
-    out["roofline"] = roof
     return out
 
 
@@ -240,7 +251,7 @@ def batch_leg(api, mg, n_models, n_single=8):
     t0 = time.perf_counter()
     ctxs = []
     for m in models:
-        c = api.Context("-inner_precision fp32")
+        c = context(api, "-inner_precision fp32")
         c.load_model(m)
         c.create_linear_system(2)
         c.assemble_matrix(0)
@@ -303,7 +314,7 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     m.freqs = m.freqs[[0, 3]]
     m.rx = m.rx + np.array([0.0, 0.0, 1.0])
     m.obs = mg.mt_obs(len(m.freqs), len(m.rx), types=(mg.Z_XY_RI, mg.Z_YX_RI))
-    ref = api.Context("-n_uniform_refinements 2 -max_adaptive_refinements 0 -inner_precision fp64 "
+    ref = context(api, "-n_uniform_refinements 2 -max_adaptive_refinements 0 -inner_precision fp64 "
                       "-e_rtol 1e-8 -save_mesh false")
     ref.load_model(m)
     ref.set_observations(m.obs)
@@ -318,7 +329,7 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     F, NB, Y = sur.training_data(mg.uneven_mt_batch(n_models=16, seed=11, e_min=8e3, e_max=4e4, f_min=1e-3), cycles=5,
                                  fraction=fraction)
     t1 = time.perf_counter()
-    trainer = api.Context()
+    trainer = context(api)
     trainer.surrogate_init(1)
     loss = trainer.surrogate_train(F, NB, Y, epochs=600, lr=3e-3)
     t2 = time.perf_counter()
@@ -336,7 +347,7 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
             continue
         trace = []
         for k in range(max_cycles + 1):
-            c = api.Context("%s -max_adaptive_refinements %d -refine_strategy fixed_number -refine_fraction %g "
+            c = context(api, "%s -max_adaptive_refinements %d -refine_strategy fixed_number -refine_fraction %g "
                             "-e_rtol 1e-8 -max_dofs 100000000 -save_mesh false" % (opts, k, fraction))
             c.load_model(m)
             c.set_observations(m.obs)
@@ -435,7 +446,7 @@ def layered_leg(api, mg, params, max_cycles=6, fraction=0.2):
             continue
         trace = []
         for k in range(max_cycles + 1):
-            c = api.Context("%s -max_adaptive_refinements %d -refine_strategy fixed_number -refine_fraction %g "
+            c = context(api, "%s -max_adaptive_refinements %d -refine_strategy fixed_number -refine_fraction %g "
                             "-e_rtol 1e-8 -max_dofs 100000000 -save_mesh false" % (opts, k, fraction))
             c.load_model(m)
             c.set_observations(m.obs)
@@ -482,6 +493,8 @@ def main():
                     help="marine-CSEM grid cells x,y,z (80,80,66: 3.0M edges)")
     ap.add_argument("--csem-freqs", type=int, default=None,
                     help="solve only the first N marine-CSEM frequencies (profiling)")
+    ap.add_argument("--csem-plain", type=int, default=2,
+                    help="transmitters of the plain fp64 marine-CSEM comparison (0 skips it)")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive accuracy-versus-time comparison")
     ap.add_argument("--batch-models", type=int, default=256,
                     help="models of the uneven multi-model batch (0 skips it)")
@@ -490,6 +503,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of hipGraph-captured cycles (for kernel tracers)")
     args = ap.parse_args()
+    global NO_GRAPH
+    NO_GRAPH = args.no_graph
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
@@ -500,7 +515,7 @@ def main():
 
     t0 = time.perf_counter()
     m = mg.adaptive_mt(cells=(args.cells, args.cells, args.cells))
-    ctx = api.Context("-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6 -use_graph %s -spmm_rows %s"
+    ctx = context(api, "-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6 -use_graph %s -spmm_rows %s"
                       % ("false" if args.no_graph else "true", args.spmm_rows))
     ctx.load_model(m)
     ctx.create_linear_system(3)
@@ -575,7 +590,7 @@ def main():
             pass
     extra["pc_last_ms"] = pcb
     if not args.no_plain:
-        ctxp = api.Context("-inner_precision fp64")
+        ctxp = context(api, "-inner_precision fp64")
         ctxp.load_model(m)
         ctxp.create_linear_system(3)
         plain = [run_step(ctxp, m, k % nf, batched=False) for k in range(min(args.steps, 2))]
@@ -591,7 +606,7 @@ def main():
         ctxp.close()
     if not args.no_csem:
         extra["config3_marine_csem"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")),
-                                                n_freqs=args.csem_freqs)
+                                                n_plain=args.csem_plain, n_freqs=args.csem_freqs)
     if not args.no_adaptive:
         extra["adaptive"] = adaptive_leg(api, mg)
         params = extra["adaptive"].pop("_params")
