@@ -23,8 +23,8 @@ struct Csr {
   int lpr = 0;  // lanes per row of the narrow SpMM kernel; 0 = chosen from the row length
   int role = 0; // 1 = edge-space smoother operator (a separate kernel instance in traces)
   // lpr = -1: LDS-staged row blocks (-3: the same, persistent and software-pipelined);
-  // blk_row [nblk + 1] are the first rows of the
-  // blocks (row_blocks), for the column count and precision the table was built for
+  // blk_row [2 (nblk + 1)] are the first rows of the blocks and then their first
+  // nonzeros (row_blocks), for the column count and precision the table was built for
   const int *blk_row = nullptr;
   int nblk = 0;
   // lpr = -2 (16 or more columns): LDS x-blocks of 64-column slabs.
@@ -66,8 +66,9 @@ constexpr int stream_tile() {
 int stream_tile_for(int nc, bool fp64);
 
 // Row blocks for the row-block SpMM: consecutive rows while their nonzeros fit in
-// `tile`; returns the first row of every block and nrows (empty when a row is longer
-// than the tile).
+// `tile`; returns the first row of every block and nrows (nblk + 1 entries), followed
+// by the first nonzero of every block and nnz (nblk + 1 entries); empty when a row is
+// longer than the tile.
 std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
 
 // A Chebyshev step fused into an SpMM epilogue (off when x is null): the SpMM's value

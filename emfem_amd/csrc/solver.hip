@@ -266,7 +266,7 @@ class DevAms {
       // fp64 row-block tables for this column count
       const DevBuf<int> &t = blk64_.at(nc);
       B.blk_row = t.get();
-      B.nblk = (int)t.size() - 1;
+      B.nblk = (int)t.size() / 2 - 1;
     }
     return B;
   }
@@ -303,7 +303,7 @@ class DevAms {
     if (!blk.empty()) {
       blk_[nc].upload(blk, s);
       B.blk_row = blk_[nc].get();
-      B.nblk = (int)blk.size() - 1;
+      B.nblk = (int)blk.size() / 2 - 1;
       EMF_HIP(hipStreamSynchronize(s));  // blk is host memory of this scope
     }
     if (nc >= 16) {
@@ -332,7 +332,7 @@ class DevAms {
         if (!b64.empty()) {
           blk64_[nc].upload(b64, s);
           B64t.blk_row = blk64_[nc].get();
-          B64t.nblk = (int)b64.size() - 1;
+          B64t.nblk = (int)b64.size() / 2 - 1;
         }
       }
       lpr64_[nc] = tune_lpr<double, double, double>(s, B64t, nc, r64_.get(), tmp.get());
@@ -429,7 +429,7 @@ class DevAms {
     B.lpr = lpr_[nc];
     if (B.lpr < 0) {
       B.blk_row = blk_[nc].get();
-      B.nblk = (int)(blk_[nc].size() - 1);
+      B.nblk = (int)(blk_[nc].size() / 2 - 1);
     }
     return B;
   }

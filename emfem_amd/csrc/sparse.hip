@@ -730,6 +730,30 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ 
   }
 }
 
+// Sum of the W-wide chunks prod[k * NC + j0 .. + W) over k in [k0, k1), added in
+// increasing k: the LDS reads are issued eight rows at a time so that a row's sum
+// waits for about one LDS latency per eight nonzeros instead of one per nonzero.
+template <typename T, int NC, int W>
+__device__ __forceinline__ void lds_row_sum(const T *prod, int k0, int k1, int j0, T (&acc)[W]) {
+  constexpr int kU = 8;
+  int k = k0;
+  for (; k + kU <= k1; k += kU) {
+    T v[kU][W];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int i = 0; i < W; ++i) v[u][i] = prod[(k + u) * NC + j0 + i];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int i = 0; i < W; ++i) acc[i] += v[u][i];
+  }
+  for (; k < k1; ++k) {
+#pragma unroll
+    for (int i = 0; i < W; ++i) acc[i] += prod[k * NC + j0 + i];
+  }
+}
+
 // Vector chunk of W consecutive values, loaded and stored as one W-wide access
 // (dwordx2 / dwordx3 / dwordx4); W = 3 needs only the element alignment.
 template <typename T, int W>
@@ -747,7 +771,17 @@ constexpr int row_chunk() {
 // x row is read as NC / W vector chunks, and in the epilogue a thread owns one W-wide
 // chunk of a row, loading its operands and storing its result as vectors.  The
 // products and the summation order are those of k_spmm_stream.
-template <typename TA, typename TX, typename T, int NC, int TILE>
+// Every load of the gather phase is issued before its first wait and every LDS store
+// is unconditional (entries past the tile's nonzeros hold a zero product and are
+// never summed): the compiler otherwise sinks a load into the conditional block of
+// its only use, behind a wait for every load issued before it.
+// FORM selects the epilogue at compile time: kFormAny (runtime operands, loaded after
+// the barrier), kFormPlain (y = alpha A x), kFormCheb (the fused Chebyshev step
+// y = x + c1 (x - d) + c2 rs .* (alpha A x + gamma w) with cf.x, cf.d, rs and w
+// given): its operands of the first pass are loaded before the barrier, so their
+// latency overlaps the gathers'.
+constexpr int kFormAny = 0, kFormPlain = 1, kFormCheb = 2;
+template <typename TA, typename TX, typename T, int NC, int TILE, int FORM>
 __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict__ rp, const int *__restrict__ ci,
                                                           const TA *__restrict__ av, const TX *__restrict__ x,
                                                           T *y, T alpha, const T *__restrict__ rs, T gamma,
@@ -759,22 +793,26 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
   using VY = ChunkT<T, W>;
   __shared__ T prod[TILE * NC];
   __shared__ int srp[TILE + 1];
+  static_assert(kBlock <= TILE + 1, "row pointers are staged one per thread");
   const int b = xcd_block();
   const int r0 = blk_row[b], r1 = blk_row[b + 1];
-  const int p0 = rp[r0];
-  const int nnz = rp[r1] - p0;
+  const int *blk_ptr = blk_row + gridDim.x + 1;  // one workgroup per block
+  const int p0 = blk_ptr[b];
+  const int nnz = blk_ptr[b + 1] - p0;
   const int nr = r1 - r0;
-  for (int i = threadIdx.x; i <= nr; i += kBlock) srp[i] = rp[r0 + i] - p0;
   constexpr int kTrips = TILE / kBlock;
   int c[kTrips];
   T a[kTrips];
   const int last = max(nnz - 1, 0);
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
-    const int pc = p0 + min((int)threadIdx.x + u * kBlock, last);
+    const int idx = (int)threadIdx.x + u * kBlock;
+    const int pc = p0 + min(idx, last);
+    const T aval = (T)av[pc];
     c[u] = ci[pc];
-    a[u] = (T)av[pc];
+    a[u] = idx < nnz ? aval : T(0);
   }
+  const int rpv = rp[r0 + min((int)threadIdx.x, nr)];
   VX xv[kTrips][CX];
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
@@ -782,18 +820,68 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
 #pragma unroll
     for (int k = 0; k < CX; ++k) xv[u][k] = xr[k];
   }
+  // the Chebyshev operands of this thread's first (row, chunk), from a clamped row
+  const int nq = nr * CY;
+  const int q0 = min((int)threadIdx.x, nq - 1);
+  const int lr0 = q0 / CY, j00 = (q0 % CY) * W;
+  VY w0, xk0, xd0;
+  T sc0 = T(1);
+  if constexpr (FORM == kFormCheb) {
+    const size_t o = (size_t)(r0 + lr0) * NC + j00;
+    w0 = *reinterpret_cast<const VY *>(w + o);
+    xk0 = *reinterpret_cast<const VY *>(cf.x + o);
+    xd0 = *reinterpret_cast<const VY *>(cf.d + o);
+    sc0 = rs[r0 + lr0];
+  }
+  // entries past nr (clamped to the last row pointer) are never read
+  srp[threadIdx.x] = rpv - p0;
+  for (int i = threadIdx.x + kBlock; i <= nr; i += kBlock) srp[i] = rp[r0 + i] - p0;  // tiles of short rows
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
     const int idx = threadIdx.x + u * kBlock;
-    if (idx < nnz) {
 #pragma unroll
-      for (int k = 0; k < CX; ++k)
+    for (int k = 0; k < CX; ++k)
 #pragma unroll
-        for (int i = 0; i < WX; ++i) prod[idx * NC + k * WX + i] = a[u] * (T)xv[u][k].v[i];
-    }
+      for (int i = 0; i < WX; ++i) prod[idx * NC + k * WX + i] = a[u] * (T)xv[u][k].v[i];
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < nr * CY; q += kBlock) {
+  if constexpr (FORM != kFormAny) {
+    auto finish = [&](int lr, int j0, const VY &wv, const VY &xk, const VY &xd, T sc) {
+      T acc[W];
+#pragma unroll
+      for (int i = 0; i < W; ++i) acc[i] = T(0);
+      lds_row_sum<T, NC, W>(prod, srp[lr], srp[lr + 1], j0, acc);
+      VY out;
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        T v = alpha * acc[i];
+        if constexpr (FORM == kFormCheb) {
+          v += gamma * wv.v[i];
+          v *= sc;
+          const T xo = xk.v[i], dv = xd.v[i];
+          v = xo + T(cf.c1) * (xo - dv) + T(cf.c2) * v;
+        }
+        out.v[i] = v;
+      }
+      *reinterpret_cast<VY *>(y + (size_t)(r0 + lr) * NC + j0) = out;
+    };
+    if ((int)threadIdx.x < nq) finish(lr0, j00, w0, xk0, xd0, sc0);
+    for (int q = threadIdx.x + kBlock; q < nq; q += kBlock) {
+      const int lr = q / CY, j0 = (q % CY) * W;
+      VY wv, xk, xd;
+      T sc = T(1);
+      if constexpr (FORM == kFormCheb) {
+        const size_t o = (size_t)(r0 + lr) * NC + j0;
+        wv = *reinterpret_cast<const VY *>(w + o);
+        xk = *reinterpret_cast<const VY *>(cf.x + o);
+        xd = *reinterpret_cast<const VY *>(cf.d + o);
+        sc = rs[r0 + lr];
+      }
+      finish(lr, j0, wv, xk, xd, sc);
+    }
+    return;
+  }
+  for (int q = threadIdx.x; q < nq; q += kBlock) {
     const int lr = q / CY, j0 = (q % CY) * W, row = r0 + lr;
     const size_t o = (size_t)row * NC + j0;
     VY wv, zv, xk, xd;
@@ -807,10 +895,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
     T acc[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) acc[i] = T(0);
-    for (int k = srp[lr]; k < srp[lr + 1]; ++k) {
-#pragma unroll
-      for (int i = 0; i < W; ++i) acc[i] += prod[k * NC + j0 + i];
-    }
+    lds_row_sum<T, NC, W>(prod, srp[lr], srp[lr + 1], j0, acc);
     VY out;
 #pragma unroll
     for (int i = 0; i < W; ++i) {
@@ -833,6 +918,15 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
                    T gamma, const T *w, int ldw, T beta, const T *z, int ldz, const T *xs, const T *zs,
                    const ChebFuse<T> &cf) {
   const double avg = A.nrows ? double(A.nnz) / A.nrows : 0.0;
+  // compact blocks with vector-aligned bases take the vector-chunk row-block kernels
+  constexpr int WX = row_chunk<TX, NC>(), WY = row_chunk<T, NC>();
+  auto aligned = [](const void *p, size_t a) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) % a) == 0; };
+  const size_t ax = sizeof(TX) * (WX == 3 ? 1 : WX), ay = sizeof(T) * (WY == 3 ? 1 : WY);
+  const bool vec = WX > 1 && WY > 1 && ldx == NC && ldy == NC && (gamma == T(0) || ldw == NC) &&
+                   (beta == T(0) || ldz == NC) && (!cf.on() || cf.ld == NC) && cf.zcol == nullptr &&
+                   aligned(x, ax) && aligned(y, ay) && aligned(gamma == T(0) ? nullptr : w, ay) &&
+                   aligned(beta == T(0) ? nullptr : z, ay) && aligned(cf.on() ? cf.x : nullptr, ay) &&
+                   aligned(cf.on() ? cf.d : nullptr, ay);
   if (A.lpr == -3) {
     // persistent pipelined row blocks: about 8 workgroups per CU
     EMF_REQUIRE(A.blk_row != nullptr && A.nblk > 0, kErrArg, "row-block SpMM without its block table");
@@ -871,19 +965,22 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
     // LDS-staged row blocks (tables built by row_blocks for this NC and T)
     EMF_REQUIRE(A.blk_row != nullptr && A.nblk > 0, kErrArg, "row-block SpMM without its block table");
     EMF_REQUIRE(xs == nullptr && zs == nullptr, kErrArg, "row-block SpMM has no scaled operands");
-    // compact blocks with vector-aligned bases take the vector-chunk kernel
-    constexpr int WX = row_chunk<TX, NC>(), WY = row_chunk<T, NC>();
-    auto aligned = [](const void *p, size_t a) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) % a) == 0; };
-    const size_t ax = sizeof(TX) * (WX == 3 ? 1 : WX), ay = sizeof(T) * (WY == 3 ? 1 : WY);
-    const bool vec = WX > 1 && WY > 1 && ldx == NC && ldy == NC && (gamma == T(0) || ldw == NC) &&
-                     (beta == T(0) || ldz == NC) && (!cf.on() || cf.ld == NC) && cf.zcol == nullptr &&
-                     aligned(x, ax) && aligned(y, ay) && aligned(gamma == T(0) ? nullptr : w, ay) &&
-                     aligned(beta == T(0) ? nullptr : z, ay) && aligned(cf.on() ? cf.x : nullptr, ay) &&
-                     aligned(cf.on() ? cf.d : nullptr, ay);
-    if (vec)
-      hipLaunchKernelGGL((k_spmm_stream_v<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(A.nblk), dim3(kBlock), 0, s,
-                         A.rp, A.ci, A.v, x, y, alpha, rs, gamma, w, beta, z, A.blk_row, cf);
-    else
+    if (vec) {
+      // the epilogue forms compiled without runtime operands: y = alpha A x, and the
+      // fused Chebyshev step with its row scaling and right-hand side
+      const bool plain = gamma == T(0) && beta == T(0) && rs == nullptr && !cf.on();
+      const bool cheb = gamma != T(0) && beta == T(0) && rs != nullptr && cf.on() && cf.d != nullptr;
+#define EMF_SV(F)                                                                                                \
+  hipLaunchKernelGGL((k_spmm_stream_v<TA, TX, T, NC, stream_tile<T, NC>(), F>), dim3(A.nblk), dim3(kBlock), 0, s, \
+                     A.rp, A.ci, A.v, x, y, alpha, rs, gamma, w, beta, z, A.blk_row, cf)
+      if (cheb)
+        EMF_SV(kFormCheb);
+      else if (plain)
+        EMF_SV(kFormPlain);
+      else
+        EMF_SV(kFormAny);
+#undef EMF_SV
+    } else
       hipLaunchKernelGGL((k_spmm_stream<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(A.nblk), dim3(kBlock), 0, s,
                          A.rp, A.ci, A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, A.blk_row, cf);
     return;
@@ -932,6 +1029,9 @@ std::vector<int> row_blocks(const std::vector<int> &rp, int tile) {
     blk.push_back(e);
     r = e;
   }
+  // first nonzero of every block, so a workgroup's bounds need no row-pointer load
+  const size_t nb = blk.size();
+  for (size_t b = 0; b < nb; ++b) blk.push_back(rp[blk[b]]);
   return blk;
 }
 
