@@ -778,9 +778,10 @@ constexpr int row_chunk() {
 // FORM selects the epilogue at compile time: kFormAny (runtime operands, loaded after
 // the barrier), kFormPlain (y = alpha A x), kFormCheb (the fused Chebyshev step
 // y = x + c1 (x - d) + c2 rs .* (alpha A x + gamma w) with cf.x, cf.d, rs and w
-// given): its operands of the first pass are loaded before the barrier, so their
-// latency overlaps the gathers'.
-constexpr int kFormAny = 0, kFormPlain = 1, kFormCheb = 2;
+// given), kFormCheb0 (the same with d = 0, cf.d null), kFormAxpy (y = alpha A x +
+// gamma w, the residual form): the operands of the first pass are loaded before the
+// barrier, so their latency overlaps the gathers'.
+constexpr int kFormAny = 0, kFormPlain = 1, kFormCheb = 2, kFormCheb0 = 3, kFormAxpy = 4;
 template <typename TA, typename TX, typename T, int NC, int TILE, int FORM>
 __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict__ rp, const int *__restrict__ ci,
                                                           const TA *__restrict__ av, const TX *__restrict__ x,
@@ -824,13 +825,15 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
   const int nq = nr * CY;
   const int q0 = min((int)threadIdx.x, nq - 1);
   const int lr0 = q0 / CY, j00 = (q0 % CY) * W;
+  constexpr bool kCheb = FORM == kFormCheb || FORM == kFormCheb0;
   VY w0, xk0, xd0;
   T sc0 = T(1);
-  if constexpr (FORM == kFormCheb) {
+  if constexpr (FORM == kFormAxpy) w0 = *reinterpret_cast<const VY *>(w + (size_t)(r0 + lr0) * NC + j00);
+  if constexpr (kCheb) {
     const size_t o = (size_t)(r0 + lr0) * NC + j00;
     w0 = *reinterpret_cast<const VY *>(w + o);
     xk0 = *reinterpret_cast<const VY *>(cf.x + o);
-    xd0 = *reinterpret_cast<const VY *>(cf.d + o);
+    if constexpr (FORM == kFormCheb) xd0 = *reinterpret_cast<const VY *>(cf.d + o);
     sc0 = rs[r0 + lr0];
   }
   // entries past nr (clamped to the last row pointer) are never read
@@ -855,10 +858,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
 #pragma unroll
       for (int i = 0; i < W; ++i) {
         T v = alpha * acc[i];
-        if constexpr (FORM == kFormCheb) {
+        if constexpr (FORM == kFormAxpy) v += gamma * wv.v[i];
+        if constexpr (kCheb) {
           v += gamma * wv.v[i];
           v *= sc;
-          const T xo = xk.v[i], dv = xd.v[i];
+          const T xo = xk.v[i], dv = FORM == kFormCheb ? xd.v[i] : T(0);
           v = xo + T(cf.c1) * (xo - dv) + T(cf.c2) * v;
         }
         out.v[i] = v;
@@ -870,11 +874,12 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
       const int lr = q / CY, j0 = (q % CY) * W;
       VY wv, xk, xd;
       T sc = T(1);
-      if constexpr (FORM == kFormCheb) {
+      if constexpr (FORM == kFormAxpy) wv = *reinterpret_cast<const VY *>(w + (size_t)(r0 + lr) * NC + j0);
+      if constexpr (kCheb) {
         const size_t o = (size_t)(r0 + lr) * NC + j0;
         wv = *reinterpret_cast<const VY *>(w + o);
         xk = *reinterpret_cast<const VY *>(cf.x + o);
-        xd = *reinterpret_cast<const VY *>(cf.d + o);
+        if constexpr (FORM == kFormCheb) xd = *reinterpret_cast<const VY *>(cf.d + o);
         sc = rs[r0 + lr];
       }
       finish(lr, j0, wv, xk, xd, sc);
@@ -969,14 +974,19 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
       // the epilogue forms compiled without runtime operands: y = alpha A x, and the
       // fused Chebyshev step with its row scaling and right-hand side
       const bool plain = gamma == T(0) && beta == T(0) && rs == nullptr && !cf.on();
-      const bool cheb = gamma != T(0) && beta == T(0) && rs != nullptr && cf.on() && cf.d != nullptr;
+      const bool cheb = gamma != T(0) && beta == T(0) && rs != nullptr && cf.on();
+      const bool axpy = gamma != T(0) && beta == T(0) && rs == nullptr && !cf.on();
 #define EMF_SV(F)                                                                                                \
   hipLaunchKernelGGL((k_spmm_stream_v<TA, TX, T, NC, stream_tile<T, NC>(), F>), dim3(A.nblk), dim3(kBlock), 0, s, \
                      A.rp, A.ci, A.v, x, y, alpha, rs, gamma, w, beta, z, A.blk_row, cf)
-      if (cheb)
+      if (cheb && cf.d)
         EMF_SV(kFormCheb);
+      else if (cheb)
+        EMF_SV(kFormCheb0);
       else if (plain)
         EMF_SV(kFormPlain);
+      else if (axpy)
+        EMF_SV(kFormAxpy);
       else
         EMF_SV(kFormAny);
 #undef EMF_SV
