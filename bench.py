@@ -305,7 +305,8 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     with k = 0..max_cycles refinement cycles (each run timed from scratch) in three
     modes: plain (fp64 inner solve), mixed (fp32 inner solve), and mixed with the
     learned indicator in the intermediate cycles (trained beforehand on plain cycles of
-    16 uneven-batch models at 1e-3..10 Hz; data generation and training are timed
+    16 uneven-batch models at 1e-3..10 Hz and 16 models of the same family with
+    topography and anisotropic blocks; data generation and training are timed
     separately).
     Reports, per mode, the time to reach a median / max receiver error and the
     surrogate's final-cycle rank correlation with the plain estimator."""
@@ -326,8 +327,9 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     ref.close()
     # surrogate: plain cycles of uneven-batch models, then training
     t0 = time.perf_counter()
-    F, NB, Y = sur.training_data(mg.uneven_mt_batch(n_models=16, seed=11, e_min=8e3, e_max=4e4, f_min=1e-3), cycles=5,
-                                 fraction=fraction)
+    train_models = (mg.uneven_mt_batch(n_models=16, seed=11, e_min=8e3, e_max=4e4, f_min=1e-3) +
+                    mg.varied_mt_batch(n_models=16, seed=21))
+    F, NB, Y = sur.training_data(train_models, cycles=5, fraction=fraction)
     t1 = time.perf_counter()
     trainer = context(api)
     trainer.surrogate_init(1)

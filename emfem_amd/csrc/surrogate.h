@@ -21,7 +21,7 @@
 
 namespace emfem {
 
-constexpr int kSurFeat = 16;   // padded feature width (14 used)
+constexpr int kSurFeat = 16;   // feature width
 constexpr int kSurHidden = 64;
 constexpr int kSurLayers = 3;  // message-passing layers after the input layer
 
@@ -71,11 +71,13 @@ class SurrogateNet {
 // log10 volume, log10 aspect (circumradius / (3 inradius)), log10 |E| and
 // log10 |curl E| over the XY and YX solutions (interleaved columns at e_xy / e_yx with
 // row stride ld), log10(1 + d/h) and log10(1 + d) of the distance d from the centroid
-// to the nearest of the nrx receivers rx [nrx][3] (h = volume^(1/3)), and log10 of
-// sum_r h / (d_r + h) over all receivers.
+// to the nearest of the nrx receivers rx [nrx][3] (h = volume^(1/3)), log10 of
+// sum_r h / (d_r + h) over all receivers, and log10 of the XY and YX estimator
+// indicators eta_xy, eta_yx (eprim [2][nt] holds their squares): the primal factors of
+// the goal-oriented indicator, which the surrogate cycles compute exactly.
 void surrogate_features(hipStream_t s, int nt, const int *tets, const int *tet2edge, const double *pts,
                         const int *attr, const double *sig6, double freq, const double *e_xy, const double *e_yx,
-                        int ld, const double *rx, int nrx, float *F);
+                        int ld, const double *rx, int nrx, const double *eprim, float *F);
 
 // F[t][j] = (F[t][j] - mean[j]) / std[j] on the device.
 void standardize(hipStream_t s, int nt, const float *mean, const float *stdv, float *F);

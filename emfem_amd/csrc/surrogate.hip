@@ -343,7 +343,7 @@ namespace {
 __global__ void k_features(int nt, const int *__restrict__ tets, const int *__restrict__ t2e,
                            const double *__restrict__ pts, const int *__restrict__ attr, const double *__restrict__ s6,
                            double freq, const double *__restrict__ exy, const double *__restrict__ eyx, int ld,
-                           const double *__restrict__ rx, int nrx, float *F) {
+                           const double *__restrict__ rx, int nrx, const double *__restrict__ eprim, float *F) {
   const int t = xcd_block() * kBlock + threadIdx.x;
   if (t >= nt) return;
   int vid[4];
@@ -432,7 +432,8 @@ __global__ void k_features(int nt, const int *__restrict__ tets, const int *__re
   f[11] = nrx > 0 ? (float)log10(sqrt(dmin) / h + 1.0) : 0.f;
   f[12] = nrx > 0 ? (float)log10(sqrt(dmin) + 1.0) : 0.f;
   f[13] = nrx > 0 ? (float)log10(dens) : 0.f;
-  for (int k = 14; k < kSurFeat; ++k) f[k] = 0.f;
+  f[14] = (float)(0.5 * log10(eprim[t] + 1e-300));
+  f[15] = (float)(0.5 * log10(eprim[(size_t)nt + t] + 1e-300));
 }
 
 __global__ void k_standardize(int n, const float *__restrict__ mean, const float *__restrict__ stdv, float *F) {
@@ -446,10 +447,10 @@ __global__ void k_standardize(int n, const float *__restrict__ mean, const float
 
 void surrogate_features(hipStream_t s, int nt, const int *tets, const int *tet2edge, const double *pts,
                         const int *attr, const double *sig6, double freq, const double *e_xy, const double *e_yx,
-                        int ld, const double *rx, int nrx, float *F) {
+                        int ld, const double *rx, int nrx, const double *eprim, float *F) {
   if (nt == 0) return;
   hipLaunchKernelGGL(k_features, dim3(grid_for(nt)), dim3(kBlock), 0, s, nt, tets, tet2edge, pts, attr, sig6, freq,
-                     e_xy, e_yx, ld, rx, nrx, F);
+                     e_xy, e_yx, ld, rx, nrx, eprim, F);
   EMF_HIP(hipGetLastError());
 }
 

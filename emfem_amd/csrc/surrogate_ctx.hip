@@ -20,19 +20,28 @@ void require_sur(const emfem_context *c) {
 }
 
 // raw features of the current mesh and the XY / YX solution slots, on the device
+// (the XY and YX estimator indicators, inputs of the last two features, are formed
+// here too)
 void device_features(emfem_context *c, int fidx, int sxy, int syx, DevBuf<float> &F) {
   require_system(c);
   require_slot(c, sxy);
   require_slot(c, syx);
   require_freq(c, fidx);
+  hipStream_t s = c->stream;
   F.resize((size_t)c->T.nt * kSurFeat);
   const int ld = 2 * c->nslots;
   DevBuf<double> rx;
-  if (!c->rx.empty()) rx.upload(c->rx, c->stream);
-  surrogate_features(c->stream, c->T.nt, c->d_tets.get(), c->d_t2e.get(), c->d_pts.get(), c->d_attr.get(),
-                     c->d_s6.get(), c->freqs[fidx], c->d_X.get() + 2 * sxy, c->d_X.get() + 2 * syx, ld, rx.get(),
-                     (int)(c->rx.size() / 3), F.get());
-  EMF_HIP(hipStreamSynchronize(c->stream));
+  if (!c->rx.empty()) rx.upload(c->rx, s);
+  const int slots[2] = {sxy, syx};
+  DevBuf<int> d_slots;
+  d_slots.upload(slots, 2, s);
+  DevBuf<double> eprim((size_t)2 * c->T.nt);
+  estimate(s, c->T.nt, c->d_tets.get(), c->d_t2e.get(), c->d_nb.get(), c->d_non.get(), c->d_pts.get(),
+           c->d_attr.get(), c->d_s6.get(), 2 * kPi * c->freqs[fidx], 2, d_slots.get(), c->d_X.get(), ld, eprim.get());
+  surrogate_features(s, c->T.nt, c->d_tets.get(), c->d_t2e.get(), c->d_pts.get(), c->d_attr.get(), c->d_s6.get(),
+                     c->freqs[fidx], c->d_X.get() + 2 * sxy, c->d_X.get() + 2 * syx, ld, rx.get(),
+                     (int)(c->rx.size() / 3), eprim.get(), F.get());
+  EMF_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace

@@ -436,3 +436,63 @@ def uneven_mt_batch(n_models=256, seed=4, e_min=2.0e4, e_max=2.0e5, f_min=1e-2, 
         m.name = "uneven_mt_%03d" % k
         models.append(m)
     return models
+
+
+def varied_mt_batch(n_models=32, seed=21, n_min=6, n_max=12, f_min=1e-3, f_max=10.0):
+    """Training models for the learned indicator: the uneven batch's half-spaces of
+    10-1000 ohm-m with 1-4 random blocks and one random frequency, widened by what the
+    adaptive-MT models add: a Gaussian-hill topography of random amplitude (0-400 m) and
+    width (1.5-5 km), and anisotropic tensors (arbitrary form, principal resistivities
+    within a factor of 1-10 of the block's value, random Euler angles) for about half of
+    the blocks and the background.  Receivers: a square grid of 2-7 per side spanning
+    +-0.5..4 km, 1 m below the discrete surface."""
+    rng = np.random.default_rng(seed)
+    models = []
+    for k in range(n_models):
+        n = int(rng.integers(n_min, n_max + 1))
+        n_out = max(3, n // 6 + 2)
+        xn = graded(4000.0, 8000.0 / n, 20000.0, n_out)
+        n_air = max(4, n // 3)
+        n_core_z = max(3, (2 * n) // 3)
+        earth = np.concatenate([np.linspace(0.0, 4000.0, n_core_z + 1),
+                                _geometric(4000.0, 30000.0, 4000.0 / n_core_z, max(3, n // 3))])
+        airn = -_geometric(0.0, 50000.0, 4000.0 / n_core_z, n_air)[::-1]
+        zn = np.concatenate([airn, earth])
+        pts, tets, _, _ = kuhn_grid(np.round(xn), np.round(xn), np.round(zn))
+        amp = float(rng.uniform(0.0, 400.0))
+        length = float(rng.uniform(1500.0, 5000.0))
+        x0, y0 = rng.uniform(-2000.0, 2000.0, 2)
+
+        def surf(x, y, amp=amp, length=length, x0=x0, y0=y0):
+            return -amp * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2 * length * length))
+
+        pts = _round(_warp(pts, surf, zn[0], zn[-1]))
+        _check(pts, tets)
+        c = centroids(pts, tets)
+        below = c[:, 2] > surf(c[:, 0], c[:, 1])
+        attr = np.where(below, 1, 0).astype(np.int32)
+
+        def tensor(rho0):
+            if rng.uniform() < 0.5:
+                return [rho0, rho0, rho0, 0.0, 0.0, 0.0]
+            r = rho0 * np.exp(rng.uniform(-0.5, 0.5, 3) * math.log(10.0))
+            return list(r) + list(rng.uniform(0.0, 360.0, 3))
+
+        rho_hs = float(np.exp(rng.uniform(np.log(10.0), np.log(1000.0))))
+        rho = [[AIR_RHO, AIR_RHO, AIR_RHO, 0.0, 0.0, 0.0], tensor(rho_hs)]
+        for b in range(int(rng.integers(1, 5))):
+            lo = rng.uniform([-1500.0, -1500.0, 100.0], [1000.0, 1000.0, 2500.0])
+            size = rng.uniform([300.0, 300.0, 200.0], [1500.0, 1500.0, 1200.0])
+            inside = below & np.all((c > lo) & (c < lo + size), axis=1)
+            attr[inside] = len(rho)
+            rho.append(tensor(float(np.exp(rng.uniform(np.log(1.0), np.log(1000.0))))))
+        n_rx = int(rng.integers(2, 8))
+        span = float(rng.uniform(500.0, 4000.0))
+        g = np.linspace(-span, span, n_rx)
+        X, Y = np.meshgrid(g, g, indexing="ij")
+        rx = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], axis=1)
+        rx[:, 2] = _surface_height_on_mesh(pts, tets, rx, surf)
+        freq = float(np.exp(rng.uniform(np.log(f_min), np.log(f_max))))
+        models.append(Model(pts, tets, attr, 3, np.asarray(rho), np.array([freq]), np.zeros((0, 7)), rx,
+                            mt_obs(1, rx.shape[0]), name="varied_mt_%03d" % k))
+    return models
