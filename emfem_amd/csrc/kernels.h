@@ -55,13 +55,22 @@ constexpr int kXBlockRows = 64;
 constexpr int kXnCols = 256;  // distinct columns per narrow x-block
 constexpr int kXnRows = 64;
 
-// Nonzeros staged per workgroup by the row-block SpMM: 16 KB of products in fp32,
-// 24 KB in fp64 (a multiple of the 256-thread block, at most 2048).
+// Threads per workgroup of the row-block SpMM: one wavefront for fp32 blocks of 3 to
+// 6 columns (seven nonzeros per thread, all of whose loads are in flight together),
+// else 256.
+template <typename T, int NC>
+constexpr int stream_block() {
+  return sizeof(T) == 4 && NC >= 3 && NC <= 6 ? 64 : 256;
+}
+
+// Nonzeros staged per workgroup by the row-block SpMM: 448 for the one-wavefront
+// blocks, else 16 KB of products in fp32, 24 KB in fp64 (a multiple of the 256-thread
+// block, at most 2048).
 template <typename T, int NC>
 constexpr int stream_tile() {
   constexpr int bytes = sizeof(T) == 8 ? 24576 : 16384;
   constexpr int t = (bytes / (NC * (int)sizeof(T))) / 256 * 256;
-  return t > 2048 ? 2048 : t;
+  return stream_block<T, NC>() == 64 ? 7 * 64 : (t > 2048 ? 2048 : t);
 }
 int stream_tile_for(int nc, bool fp64);
 

@@ -574,8 +574,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_xblk(int nblk, const int *__res
 // tile t+1 and the column/value/row-pointer loads of tile t+2 are already issued, so
 // a tile costs about one memory latency instead of three.  Products and summation
 // order are those of k_spmm_stream.
-template <typename TA, typename TX, typename T, int NC, int TILE>
-__global__ __launch_bounds__(kBlock) void k_spmm_pstream(const int *__restrict__ rp, const int *__restrict__ ci,
+template <typename TA, typename TX, typename T, int NC, int TILE, int BS = kBlock>
+__global__ __launch_bounds__(BS) void k_spmm_pstream(const int *__restrict__ rp, const int *__restrict__ ci,
                                                          const TA *__restrict__ av, const TX *__restrict__ x,
                                                          int ldx, T *y, int ldy, T alpha, const T *__restrict__ rs,
                                                          T gamma, const T *w, int ldw, T beta, const T *z, int ldz,
@@ -583,8 +583,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_pstream(const int *__restrict__
                                                          ChebFuse<T> cf) {
   __shared__ T prod[TILE * NC];
   __shared__ int srp[TILE + 1];
-  constexpr int kTrips = TILE / kBlock;
-  constexpr int kRp = (TILE + 1 + kBlock - 1) / kBlock;
+  static_assert(TILE % BS == 0, "a tile is a whole number of passes of the block");
+  constexpr int kTrips = TILE / BS;
+  constexpr int kRp = (TILE + 1 + BS - 1) / BS;
   struct Tile {
     int r0, r1, p0, nnz;
     int c[kTrips];
@@ -603,13 +604,13 @@ __global__ __launch_bounds__(kBlock) void k_spmm_pstream(const int *__restrict__
     I.nnz = rp[I.r1] - I.p0;
 #pragma unroll
     for (int k = 0; k < kRp; ++k) {
-      const int i = threadIdx.x + k * kBlock;
+      const int i = threadIdx.x + k * BS;
       I.rpv[k] = rp[I.r0 + min(i, I.r1 - I.r0)] - I.p0;
     }
     const int last = max(I.nnz - 1, 0);
 #pragma unroll
     for (int u = 0; u < kTrips; ++u) {
-      const int idx = threadIdx.x + u * kBlock;
+      const int idx = threadIdx.x + u * BS;
       const int pc = I.p0 + min(idx, last);
       const int cv = ci[pc];
       const T aval = (T)av[pc];
@@ -628,12 +629,12 @@ __global__ __launch_bounds__(kBlock) void k_spmm_pstream(const int *__restrict__
   auto stage = [&](const Tile &I, const TX (&xv)[kTrips][NC]) {
 #pragma unroll
     for (int k = 0; k < kRp; ++k) {
-      const int i = threadIdx.x + k * kBlock;
+      const int i = threadIdx.x + k * BS;
       if (i <= I.r1 - I.r0) srp[i] = I.rpv[k];
     }
 #pragma unroll
     for (int u = 0; u < kTrips; ++u) {
-      const int idx = threadIdx.x + u * kBlock;
+      const int idx = threadIdx.x + u * BS;
       if (idx < I.nnz) {
 #pragma unroll
         for (int j = 0; j < NC; ++j) prod[idx * NC + j] = I.a[u] * (T)xv[u][j];
@@ -652,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_pstream(const int *__restrict__
     if (has1) gather(nxt, xv);
     if (has2) load_idx(t + 2, nn);
     const int nr = cur.r1 - cur.r0;
-    for (int q = threadIdx.x; q < nr * NC; q += kBlock) {
+    for (int q = threadIdx.x; q < nr * NC; q += BS) {
       const int lr = q / NC, j = q % NC, row = cur.r0 + lr;
       const EpiIn<T> e_ = epi_load<T>(w, (size_t)row * ldw + j, gamma != T(0), z, (size_t)row * ldz + j,
                                       beta != T(0), rs, row, cf, (size_t)row * cf.ld + j);
@@ -674,8 +675,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_pstream(const int *__restrict__
   }
 }
 
-template <typename TA, typename TX, typename T, int NC, int TILE>
-__global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ rp, const int *__restrict__ ci,
+template <typename TA, typename TX, typename T, int NC, int TILE, int BS = kBlock>
+__global__ __launch_bounds__(BS) void k_spmm_stream(const int *__restrict__ rp, const int *__restrict__ ci,
                                                         const TA *__restrict__ av, const TX *__restrict__ x, int ldx,
                                                         T *y, int ldy, T alpha, const T *__restrict__ rs, T gamma,
                                                         const T *w, int ldw, T beta, const T *z, int ldz,
@@ -686,8 +687,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ 
   const int r0 = blk_row[b], r1 = blk_row[b + 1];
   const int p0 = rp[r0];
   const int nnz = rp[r1] - p0;
-  for (int i = threadIdx.x; i <= r1 - r0; i += kBlock) srp[i] = rp[r0 + i] - p0;
-  constexpr int kTrips = TILE / kBlock;
+  for (int i = threadIdx.x; i <= r1 - r0; i += BS) srp[i] = rp[r0 + i] - p0;
+  static_assert(TILE % BS == 0, "a tile is a whole number of passes of the block");
+  constexpr int kTrips = TILE / BS;
   // unconditional loads from clamped indices (masked at the LDS store), so all of a
   // thread's loads are in flight together
   int c[kTrips];
@@ -695,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ 
   const int last = max(nnz - 1, 0);
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
-    const int pc = p0 + min((int)threadIdx.x + u * kBlock, last);
+    const int pc = p0 + min((int)threadIdx.x + u * BS, last);
     c[u] = ci[pc];
     a[u] = (T)av[pc];
   }
@@ -708,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ 
   }
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
-    const int idx = threadIdx.x + u * kBlock;
+    const int idx = threadIdx.x + u * BS;
     if (idx < nnz) {
 #pragma unroll
       for (int j = 0; j < NC; ++j) prod[idx * NC + j] = a[u] * (T)xv[u][j];
@@ -716,7 +718,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream(const int *__restrict__ 
   }
   __syncthreads();
   const int nr = r1 - r0;
-  for (int q = threadIdx.x; q < nr * NC; q += kBlock) {
+  for (int q = threadIdx.x; q < nr * NC; q += BS) {
     const int lr = q / NC, j = q % NC, row = r0 + lr;
     const EpiIn<T> e_ = epi_load<T>(w, (size_t)row * ldw + j, gamma != T(0), z, (size_t)row * ldz + j,
                                     beta != T(0), rs, row, cf, (size_t)row * cf.ld + j);
@@ -780,10 +782,11 @@ constexpr int row_chunk() {
 // y = x + c1 (x - d) + c2 rs .* (alpha A x + gamma w) with cf.x, cf.d, rs and w
 // given), kFormCheb0 (the same with d = 0, cf.d null), kFormAxpy (y = alpha A x +
 // gamma w, the residual form): the operands of the first pass are loaded before the
-// barrier, so their latency overlaps the gathers'.
+// barrier, so their latency overlaps the gathers'.  BS threads per workgroup, TILE a
+// multiple of BS.
 constexpr int kFormAny = 0, kFormPlain = 1, kFormCheb = 2, kFormCheb0 = 3, kFormAxpy = 4;
-template <typename TA, typename TX, typename T, int NC, int TILE, int FORM>
-__global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict__ rp, const int *__restrict__ ci,
+template <typename TA, typename TX, typename T, int NC, int TILE, int FORM, int BS = kBlock>
+__global__ __launch_bounds__(BS) void k_spmm_stream_v(const int *__restrict__ rp, const int *__restrict__ ci,
                                                           const TA *__restrict__ av, const TX *__restrict__ x,
                                                           T *y, T alpha, const T *__restrict__ rs, T gamma,
                                                           const T *w, T beta, const T *z,
@@ -794,20 +797,21 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
   using VY = ChunkT<T, W>;
   __shared__ T prod[TILE * NC];
   __shared__ int srp[TILE + 1];
-  static_assert(kBlock <= TILE + 1, "row pointers are staged one per thread");
+  static_assert(BS <= TILE + 1, "row pointers are staged one per thread");
+  static_assert(TILE % BS == 0, "a tile is a whole number of passes of the block");
   const int b = xcd_block();
   const int r0 = blk_row[b], r1 = blk_row[b + 1];
   const int *blk_ptr = blk_row + gridDim.x + 1;  // one workgroup per block
   const int p0 = blk_ptr[b];
   const int nnz = blk_ptr[b + 1] - p0;
   const int nr = r1 - r0;
-  constexpr int kTrips = TILE / kBlock;
+  constexpr int kTrips = TILE / BS;
   int c[kTrips];
   T a[kTrips];
   const int last = max(nnz - 1, 0);
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
-    const int idx = (int)threadIdx.x + u * kBlock;
+    const int idx = (int)threadIdx.x + u * BS;
     const int pc = p0 + min(idx, last);
     const T aval = (T)av[pc];
     c[u] = ci[pc];
@@ -838,10 +842,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
   }
   // entries past nr (clamped to the last row pointer) are never read
   srp[threadIdx.x] = rpv - p0;
-  for (int i = threadIdx.x + kBlock; i <= nr; i += kBlock) srp[i] = rp[r0 + i] - p0;  // tiles of short rows
+  for (int i = threadIdx.x + BS; i <= nr; i += BS) srp[i] = rp[r0 + i] - p0;  // tiles of short rows
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
-    const int idx = threadIdx.x + u * kBlock;
+    const int idx = threadIdx.x + u * BS;
 #pragma unroll
     for (int k = 0; k < CX; ++k)
 #pragma unroll
@@ -870,7 +874,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
       *reinterpret_cast<VY *>(y + (size_t)(r0 + lr) * NC + j0) = out;
     };
     if ((int)threadIdx.x < nq) finish(lr0, j00, w0, xk0, xd0, sc0);
-    for (int q = threadIdx.x + kBlock; q < nq; q += kBlock) {
+    for (int q = threadIdx.x + BS; q < nq; q += BS) {
       const int lr = q / CY, j0 = (q % CY) * W;
       VY wv, xk, xd;
       T sc = T(1);
@@ -886,7 +890,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_stream_v(const int *__restrict_
     }
     return;
   }
-  for (int q = threadIdx.x; q < nq; q += kBlock) {
+  for (int q = threadIdx.x; q < nq; q += BS) {
     const int lr = q / CY, j0 = (q % CY) * W, row = r0 + lr;
     const size_t o = (size_t)row * NC + j0;
     VY wv, zv, xk, xd;
@@ -946,7 +950,8 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
     const int groups = std::min(A.nblk, 8 * n_cu);
     const int per = (A.nblk + groups - 1) / groups;
     const int used = (A.nblk + per - 1) / per;
-    hipLaunchKernelGGL((k_spmm_pstream<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(used), dim3(kBlock), 0, s, A.rp,
+    hipLaunchKernelGGL((k_spmm_pstream<TA, TX, T, NC, stream_tile<T, NC>(), stream_block<T, NC>()>), dim3(used),
+                       dim3(stream_block<T, NC>()), 0, s, A.rp,
                        A.ci, A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, A.blk_row, A.nblk, per, cf);
     return;
   }
@@ -977,7 +982,8 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
       const bool cheb = gamma != T(0) && beta == T(0) && rs != nullptr && cf.on();
       const bool axpy = gamma != T(0) && beta == T(0) && rs == nullptr && !cf.on();
 #define EMF_SV(F)                                                                                                \
-  hipLaunchKernelGGL((k_spmm_stream_v<TA, TX, T, NC, stream_tile<T, NC>(), F>), dim3(A.nblk), dim3(kBlock), 0, s, \
+  hipLaunchKernelGGL((k_spmm_stream_v<TA, TX, T, NC, stream_tile<T, NC>(), F, stream_block<T, NC>()>), dim3(A.nblk), \
+                     dim3(stream_block<T, NC>()), 0, s,                                                            \
                      A.rp, A.ci, A.v, x, y, alpha, rs, gamma, w, beta, z, A.blk_row, cf)
       if (cheb && cf.d)
         EMF_SV(kFormCheb);
@@ -991,7 +997,8 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
         EMF_SV(kFormAny);
 #undef EMF_SV
     } else
-      hipLaunchKernelGGL((k_spmm_stream<TA, TX, T, NC, stream_tile<T, NC>()>), dim3(A.nblk), dim3(kBlock), 0, s,
+      hipLaunchKernelGGL((k_spmm_stream<TA, TX, T, NC, stream_tile<T, NC>(), stream_block<T, NC>()>), dim3(A.nblk),
+                         dim3(stream_block<T, NC>()), 0, s,
                          A.rp, A.ci, A.v, x, ldx, y, ldy, alpha, rs, gamma, w, ldw, beta, z, ldz, A.blk_row, cf);
     return;
   }
