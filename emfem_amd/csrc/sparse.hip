@@ -813,8 +813,10 @@ __global__ __launch_bounds__(BS) void k_spmm_stream_v(const int *__restrict__ rp
   for (int u = 0; u < kTrips; ++u) {
     const int idx = (int)threadIdx.x + u * BS;
     const int pc = p0 + min(idx, last);
-    const T aval = (T)av[pc];
-    c[u] = ci[pc];
+    // the matrix is streamed once per launch: non-temporal loads, so that it does not
+    // evict the gathered x rows from the caches
+    const T aval = (T)__builtin_nontemporal_load(av + pc);
+    c[u] = __builtin_nontemporal_load(ci + pc);
     a[u] = idx < nnz ? aval : T(0);
   }
   const int rpv = rp[r0 + min((int)threadIdx.x, nr)];
