@@ -612,8 +612,8 @@ __global__ __launch_bounds__(BS) void k_spmm_pstream(const int *__restrict__ rp,
     for (int u = 0; u < kTrips; ++u) {
       const int idx = threadIdx.x + u * BS;
       const int pc = I.p0 + min(idx, last);
-      const int cv = ci[pc];
-      const T aval = (T)av[pc];
+      const int cv = __builtin_nontemporal_load(ci + pc);
+      const T aval = (T)__builtin_nontemporal_load(av + pc);
       I.c[u] = cv;
       I.a[u] = idx < I.nnz ? aval : T(0);
     }
@@ -698,8 +698,8 @@ __global__ __launch_bounds__(BS) void k_spmm_stream(const int *__restrict__ rp, 
 #pragma unroll
   for (int u = 0; u < kTrips; ++u) {
     const int pc = p0 + min((int)threadIdx.x + u * BS, last);
-    c[u] = ci[pc];
-    a[u] = (T)av[pc];
+    c[u] = __builtin_nontemporal_load(ci + pc);
+    a[u] = (T)__builtin_nontemporal_load(av + pc);
   }
   TX xv[kTrips][NC];
 #pragma unroll
