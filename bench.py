@@ -39,6 +39,15 @@ def log(*a):
 # Set by --no-graph: every context of every leg then runs with plain launches, so a
 # kernel tracer sees no hipGraph capture or replay anywhere in the run.
 NO_GRAPH = False
+# Outer preconditioner of the mixed/batched path (--pc): PRESB or the reference's
+# block-diagonal diag(B, B).  The plain fp64 comparison paths always run the
+# reference's block-diagonal preconditioner.
+PC = "presb"
+PLAIN_PC = "-A_pc_type blockdiag"
+
+
+def mixed_pc():
+    return "-A_pc_type %s" % PC
 
 
 def context(api, opts=""):
@@ -157,7 +166,7 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1, n_freqs=None):
     out = {"workload": None, "per_frequency": []}
     fields = {}
     t0 = time.perf_counter()
-    ctx = context(api, "-inner_precision fp32 -e_rtol 1e-8 -use_graph false " + CSEM_SMOOTHER)
+    ctx = context(api, "-inner_precision fp32 -e_rtol 1e-8 -use_graph false %s %s" % (CSEM_SMOOTHER, mixed_pc()))
     ctx.load_model(m)
     ctx.create_linear_system(n_tx)
     info = ctx.mesh_info()
@@ -190,7 +199,7 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1, n_freqs=None):
                 roof = {"bound": "hbm", "achieved": round(bw, 1), "peak": 8000.0, "unit": "GB/s",
                         "frac": round(bw / 8000.0, 4), "launch_ms": round(b_ms, 4), "algorithmic_bytes": b_bytes,
                         "kernel": "smoother SpMM on %d fp32 columns (Chebyshev step in the epilogue), %s"
-                                  % (2 * n_tx, smoother_kernel_name(int(ctx.stat("b_spmm_rows"))))}
+                                  % (int(ctx.stat("b_spmm_nc")), smoother_kernel_name(int(ctx.stat("b_spmm_rows"))))}
     ctx.close()
     out["mixed_batched"] = {"dof_rhs_per_s": round(E * n_tx * len(m.freqs) / tot_solve, 1),
                             "dof_rhs_per_s_with_setup": round(E * n_tx * len(m.freqs) / tot_step, 1),
@@ -202,29 +211,39 @@ def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1, n_freqs=None):
     out["roofline"] = roof
     if n_plain == 0:
         return out
-    ctxp = context(api, "-inner_precision fp64 -e_rtol 1e-8 " + CSEM_SMOOTHER)
-    ctxp.load_model(m)
-    ctxp.create_linear_system(n_plain)
-    ctxp.assemble_matrix(plain_fidx)
-    ctxp.create_pc()
-    for t in range(n_plain):
-        ctxp.assemble_rhs_csem(plain_fidx, t, t)
-    t0 = time.perf_counter()
-    pits = []
-    for t in range(n_plain):
-        i, r = ctxp.solve_linear_system(t, 1, 200, 1e-8)
-        pits.append(int(i[0]))
-    dt = time.perf_counter() - t0
-    fields["plain"] = [np.concatenate(ctxp.interpolate_fields(plain_fidx, t, m.rx), axis=1) for t in range(n_plain)]
-    ctxp.close()
-    out["plain_fp64_single_rhs"] = {"dof_rhs_per_s": round(E * n_plain / dt, 1), "rhs": n_plain,
-                                    "solve_s": round(dt, 3), "freq_hz": float(m.freqs[plain_fidx]), "outer_its": pits}
-    log("csem plain fp64: %d single-RHS solves %.2fs" % (n_plain, dt))
+    # the plain fp64 single-RHS path with the reference's block-diagonal preconditioner,
+    # and (for the preconditioner's own share of the speed-up) with the mixed path's
+    for key, pc in (("plain_fp64_single_rhs", PLAIN_PC), ("plain_fp64_single_rhs_same_pc", mixed_pc())):
+        if key.endswith("same_pc") and pc == PLAIN_PC:
+            continue
+        ctxp = context(api, "-inner_precision fp64 -e_rtol 1e-8 %s %s" % (CSEM_SMOOTHER, pc))
+        ctxp.load_model(m)
+        ctxp.create_linear_system(n_plain)
+        ctxp.assemble_matrix(plain_fidx)
+        ctxp.create_pc()
+        for t in range(n_plain):
+            ctxp.assemble_rhs_csem(plain_fidx, t, t)
+        t0 = time.perf_counter()
+        pits = []
+        for t in range(n_plain):
+            i, r = ctxp.solve_linear_system(t, 1, 200, 1e-8)
+            pits.append(int(i[0]))
+        dt = time.perf_counter() - t0
+        if key == "plain_fp64_single_rhs":
+            fields["plain"] = [np.concatenate(ctxp.interpolate_fields(plain_fidx, t, m.rx), axis=1)
+                               for t in range(n_plain)]
+        ctxp.close()
+        out[key] = {"dof_rhs_per_s": round(E * n_plain / dt, 1), "rhs": n_plain, "solve_s": round(dt, 3),
+                    "freq_hz": float(m.freqs[plain_fidx]), "outer_its": pits, "outer_pc": pc.split()[-1]}
+        log("csem plain fp64 (%s): %d single-RHS solves %.2fs" % (pc, n_plain, dt))
     err = 0.0
     for a, b in zip(fields["mixed"], fields["plain"]):
         for c in range(a.shape[1]):
             err = max(err, float(np.linalg.norm(a[:, c] - b[:, c]) / max(np.linalg.norm(b[:, c]), 1e-300)))
     out["speedup_vs_plain"] = round(out["mixed_batched"]["dof_rhs_per_s"] / out["plain_fp64_single_rhs"]["dof_rhs_per_s"], 3)
+    if "plain_fp64_single_rhs_same_pc" in out:
+        out["speedup_vs_plain_same_pc"] = round(
+            out["mixed_batched"]["dof_rhs_per_s"] / out["plain_fp64_single_rhs_same_pc"]["dof_rhs_per_s"], 3)
     out["max_receiver_rel_diff_vs_plain"] = err
     return out
 
@@ -251,7 +270,7 @@ def batch_leg(api, mg, n_models, n_single=8):
     t0 = time.perf_counter()
     ctxs = []
     for m in models:
-        c = context(api, "-inner_precision fp32")
+        c = context(api, "-inner_precision fp32 " + mixed_pc())
         c.load_model(m)
         c.create_linear_system(2)
         c.assemble_matrix(0)
@@ -342,8 +361,9 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
            "surrogate_training": {"elements": int(F.shape[0]), "data_s": round(t1 - t0, 2),
                                   "train_s": round(t2 - t1, 2), "loss_first": round(float(loss[0]), 4),
                                   "loss_last": round(float(loss[-1]), 4)}}
-    modes = (("plain_fp64", "-inner_precision fp64", False), ("mixed_fp32", "-inner_precision fp32", False),
-             ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true", True))
+    modes = (("plain_fp64", "-inner_precision fp64 " + PLAIN_PC, False),
+             ("mixed_fp32", "-inner_precision fp32 " + mixed_pc(), False),
+             ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true " + mixed_pc(), True))
     for name, opts, use_sur in modes:
         if use_sur and params is None:
             continue
@@ -441,8 +461,9 @@ def layered_leg(api, mg, params, max_cycles=6, fraction=0.2):
     out = {"workload": "flat layered earth (100 / 10 / 300 ohm-m, interfaces 600 m and 1500 m) on a graded Kuhn "
                        "mesh of %d edges, 2 frequencies (0.1, 10 Hz), 36 receivers, Z_xy and Z_yx; target: the exact "
                        "1-D impedance" % (7 * m.n_vertices)}
-    modes = (("plain_fp64", "-inner_precision fp64", False), ("mixed_fp32", "-inner_precision fp32", False),
-             ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true", True))
+    modes = (("plain_fp64", "-inner_precision fp64 " + PLAIN_PC, False),
+             ("mixed_fp32", "-inner_precision fp32 " + mixed_pc(), False),
+             ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true " + mixed_pc(), True))
     for name, opts, use_sur in modes:
         if use_sur and params is None:
             continue
@@ -504,8 +525,11 @@ def main():
                     help="row mapping of the smoother SpMM (auto: autotuned)")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of hipGraph-captured cycles (for kernel tracers)")
+    ap.add_argument("--pc", default="presb", choices=["presb", "blockdiag"],
+                    help="outer preconditioner of the mixed/batched path (the plain paths run blockdiag)")
     args = ap.parse_args()
-    global NO_GRAPH
+    global NO_GRAPH, PC
+    PC = args.pc
     NO_GRAPH = args.no_graph
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -517,8 +541,8 @@ def main():
 
     t0 = time.perf_counter()
     m = mg.adaptive_mt(cells=(args.cells, args.cells, args.cells))
-    ctx = context(api, "-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6 -use_graph %s -spmm_rows %s"
-                      % ("false" if args.no_graph else "true", args.spmm_rows))
+    ctx = context(api, "-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6 -use_graph %s -spmm_rows %s %s"
+                      % ("false" if args.no_graph else "true", args.spmm_rows, mixed_pc()))
     ctx.load_model(m)
     ctx.create_linear_system(3)
     info = ctx.mesh_info()
@@ -556,15 +580,17 @@ def main():
     peak = 8000.0
     bw = (b_bytes / (b_ms * 1e-3) / 1e9) if b_ms > 0 else 0.0
     rows = int(ctx.stat("b_spmm_rows"))
+    bnc = int(ctx.stat("b_spmm_nc"))
     kname = ("%s: Chebyshev step x' = x + c1 (x - x_prev) + c2 Dinv .* (b - B x) in one pass, "
-             "B = C + M fp32, XY+YX+dual = 6 real columns" % smoother_kernel_name(rows))
-    # HBM traffic of the same kernel and workload from the committed PMC summary
-    # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on k_make_b), when it exists
+             "B = C + M fp32, %d real columns (%s of XY, YX, dual)"
+             % (smoother_kernel_name(rows), bnc, "one PRESB half" if bnc == 3 else "re and im"))
+    # HBM traffic of the same kernel, workload and column count from the committed PMC
+    # summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on k_make_b), when it exists
     traffic = None
-    prof = os.path.join(ROOT, "profiles", "%s_smoother_adaptive_mt_%d.md"
+    prof = os.path.join(ROOT, "profiles", "%s_smoother_adaptive_mt_%d_nc%d.md"
                         % ("k_spmm_stream_v" if rows == -1 else "k_spmm_pstream" if rows == -3 else
                            "k_spmm_xnarrow" if rows < -1 else "k_spmm_narrow",
-                           args.cells))
+                           args.cells, bnc))
     if os.path.exists(prof):
         for line in open(prof):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -592,7 +618,7 @@ def main():
             pass
     extra["pc_last_ms"] = pcb
     if not args.no_plain:
-        ctxp = context(api, "-inner_precision fp64")
+        ctxp = context(api, "-inner_precision fp64 " + PLAIN_PC)
         ctxp.load_model(m)
         ctxp.create_linear_system(3)
         plain = [run_step(ctxp, m, k % nf, batched=False) for k in range(min(args.steps, 2))]
@@ -637,7 +663,7 @@ def main():
         "config": {"workload": "adaptive-MT model, final-cycle mesh (%d^3 cells, %d edges, %d tets), "
                                "3 RHS (XY, YX, dual), cycling over 8 frequencies 1e-3..100 Hz"
                                % (args.cells, E, info["n_tets"]),
-                   "n_edges": E, "nnz": info["nnz"], "rhs": 3, "parallelism": "single GPU"},
+                   "n_edges": E, "nnz": info["nnz"], "rhs": 3, "outer_pc": PC, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": extra,

@@ -155,6 +155,7 @@ void solve_stage(emfem_context **ctxs, const std::vector<int> &S, int first, int
   oo.restart = c0->opti("A_ksp_gmres_restart");
   oo.max_it = max_it;
   oo.fp64_fallback = c0->opt("fp64_fallback") == "true";
+  oo.presb = c0->opt("A_pc_type") == "presb";
   oo.floor_factor = c0->optd("true_residual_slack");
   oo.x_given = warm;
   oo.stop_remaining = stop_remaining;

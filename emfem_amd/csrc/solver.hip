@@ -901,6 +901,47 @@ std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int 
 
 namespace {
 
+// PRESB steps on k = nr column pairs of an interleaved [E][2 nr] block v (see
+// OuterOptions::presb): u = v_re - v_im (= f + g after the sign flip S) ...
+__global__ void k_presb_in(int64_t n, int nr, const double *__restrict__ v, double *__restrict__ u) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nr) return;
+  const int64_t e = i / nr;
+  const int r = (int)(i - e * nr);
+  u[i] = v[e * 2 * nr + 2 * r] - v[e * 2 * nr + 2 * r + 1];
+}
+
+// ... hx = (h, 0), so that the operator's imaginary half is -M h ...
+__global__ void k_presb_hx(int64_t n, int nr, const double *__restrict__ h, double *__restrict__ hx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nr) return;
+  const int64_t e = i / nr;
+  const int r = (int)(i - e * nr);
+  hx[e * 2 * nr + 2 * r] = h[i];
+  hx[e * 2 * nr + 2 * r + 1] = 0.0;
+}
+
+// ... t = g - M h = -v_im + (A hx)_im, written over u ...
+__global__ void k_presb_mid(int64_t n, int nr, const double *__restrict__ v, const double *__restrict__ ahx,
+                            double *__restrict__ t) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nr) return;
+  const int64_t e = i / nr;
+  const int r = (int)(i - e * nr);
+  t[i] = -v[e * 2 * nr + 2 * r + 1] + ahx[e * 2 * nr + 2 * r + 1];
+}
+
+// ... and z = (h - y, y).
+__global__ void k_presb_out(int64_t n, int nr, const double *__restrict__ h, const double *__restrict__ y,
+                            double *__restrict__ z) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nr) return;
+  const int64_t e = i / nr;
+  const int r = (int)(i - e * nr);
+  z[e * 2 * nr + 2 * r] = h[i] - y[i];
+  z[e * 2 * nr + 2 * r + 1] = y[i];
+}
+
 // Convergence is decided on the true fp64 residual at every restart: a right-hand
 // side whose Arnoldi estimate converged inside a cycle is frozen for the rest of
 // that cycle and re-checked (and, if needed, resumed) at the next restart.
@@ -929,6 +970,31 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
   std::vector<double> h_dots((size_t)(m + 1) * nl), h_coef((size_t)(m + 1) * nl), h_scal(nl);
   auto op = [&](const auto *in, double *out, double alpha, double beta, const double *z) {
     op_spmm(s, ne, nnz, rp, ci, K, nrhs_seg, in, nc, out, nc, alpha, beta, z, nc);
+  };
+  if (o.presb) {
+    grow(ws.pu, blk / 2);
+    grow(ws.ph, blk / 2);
+    grow(ws.py, blk / 2);
+    grow(ws.px, blk);
+  }
+  // z = P^-1 v: the block-diagonal preconditioner or PRESB (OuterOptions::presb)
+  auto precondition = [&](const double *v, double *z) {
+    if (!o.presb) {
+      pc.apply(s, nc, v, z, st);
+      return;
+    }
+    const int64_t n = ne;
+    const int nr = nrhs_seg;
+    const int grid = (int)((n * nr + 255) / 256);
+    double *u = ws.pu.get(), *h = ws.ph.get(), *y = ws.py.get(), *hx = ws.px.get();
+    hipLaunchKernelGGL(k_presb_in, dim3(grid), dim3(256), 0, s, n, nr, v, u);
+    pc.apply(s, nr, u, h, st);
+    hipLaunchKernelGGL(k_presb_hx, dim3(grid), dim3(256), 0, s, n, nr, h, hx);
+    op_spmm(s, ne, nnz, rp, ci, K, nrhs_seg, hx, nc, z, nc, 1.0, 0.0, (const double *)nullptr, nc);
+    hipLaunchKernelGGL(k_presb_mid, dim3(grid), dim3(256), 0, s, n, nr, v, z, u);
+    pc.apply(s, nr, u, y, st);
+    hipLaunchKernelGGL(k_presb_out, dim3(grid), dim3(256), 0, s, n, nr, h, y, z);
+    EMF_HIP(hipGetLastError());
   };
   auto rhs_norms = [&](const double *v, std::vector<double> &out) {
     coldot<double>(s, ne, nc, v, nc, v, nc, dots.get(), work.get(), sg);
@@ -1006,7 +1072,7 @@ void fgmres_impl(hipStream_t s, int ne, int nnz, const int *rp, const int *ci, c
       ++total;
       double *Vj = V.get() + (size_t)j * blk;
       double *Zj = Z.get() + (size_t)j * blk;
-      pc.apply(s, nc, Vj, Zj, st);
+      precondition(Vj, Zj);
       op(Zj, w.get(), 1.0, 0.0, nullptr);
       // classical Gram-Schmidt against V_0..V_j
       mdot(s, ne, nc, V.get(), blk, j + 1, w.get(), dots.get(), work.get(), sg);

@@ -3,11 +3,12 @@
 oracle's exact discrete solutions (tests/golden/make_golden.py):
 
 * plain MT at full size (62,581 edges, 1 Hz, 121 receivers), fp64 and fp32 inner
-  cycles, outer residual 1e-10: Z and T within 1e-6 (check_mt_responses);
+  cycles, the block-diagonal and the PRESB outer preconditioner, outer residual 1e-10:
+  Z and T within 1e-6 (check_mt_responses);
 * marine CSEM on 16 x 16 x 14 cells (27,246 edges) with all 64 x-directed dipoles as
   one block of right-hand sides at each of 0.25 / 0.75 / 1.25 Hz (forward_csem,
-  src/em_fwd.cc:1382-1439): E and H at the 29 receivers within 1e-6 of the largest
-  E (or H) value of that frequency and transmitter.
+  src/em_fwd.cc:1382-1439), both outer preconditioners: E and H at the 29 receivers
+  within 1e-6 of the largest E (or H) value of that frequency and transmitter.
 """
 import os
 
@@ -44,21 +45,23 @@ def _forward(m, opts):
     return rsp, info, nc
 
 
+@pytest.mark.parametrize("pc", ["blockdiag", "presb"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_config1_plain_mt(precision):
+def test_config1_plain_mt(precision, pc):
     obs, ref, ne = _golden("plain_mt.npz")
     m = mg.plain_mt()
     assert np.array_equal(obs, m.obs)
-    rsp, info, nc = _forward(m, "-inner_precision %s -e_rtol 1e-10" % precision)
+    rsp, info, nc = _forward(m, "-inner_precision %s -e_rtol 1e-10 -A_pc_type %s" % (precision, pc))
     assert info["n_edges"] == ne and nc == 0
     check_mt_responses(obs, rsp, {o: ref[o] for o in range(len(ref))})
 
 
-def test_config3_marine_csem_64tx_3freq():
+@pytest.mark.parametrize("pc", ["blockdiag", "presb"])
+def test_config3_marine_csem_64tx_3freq(pc):
     obs, ref, ne = _golden("marine_csem_16x16x14.npz")
     m = G.marine_csem_model()
     assert np.array_equal(obs, m.obs) and len(m.tx) == 64 and len(m.freqs) == 3
-    rsp, info, nc = _forward(m, "-inner_precision fp32 -e_rtol 1e-10")
+    rsp, info, nc = _forward(m, "-inner_precision fp32 -e_rtol 1e-10 -A_pc_type %s" % pc)
     assert info["n_edges"] == ne and nc == 0
     worst = 0.0
     for f in range(3):
