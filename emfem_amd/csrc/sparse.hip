@@ -763,10 +763,11 @@ struct alignas(sizeof(T) * (W == 3 ? 1 : W)) ChunkT {
   T v[W];
 };
 
-// Widest chunk that divides NC: 4, 3 or 2 values (fp32), 2 values (fp64).
+// Widest chunk that divides NC: 4, 3 or 2 values (fp32), 2 or 3 values (fp64).
 template <typename T, int NC>
 constexpr int row_chunk() {
-  return sizeof(T) == 8 ? (NC % 2 == 0 ? 2 : 1) : (NC % 4 == 0 ? 4 : (NC % 3 == 0 ? 3 : (NC % 2 == 0 ? 2 : 1)));
+  return sizeof(T) == 8 ? (NC % 2 == 0 ? 2 : (NC % 3 == 0 ? 3 : 1))
+                        : (NC % 4 == 0 ? 4 : (NC % 3 == 0 ? 3 : (NC % 2 == 0 ? 2 : 1)));
 }
 
 // k_spmm_stream on compact blocks (every leading dimension = NC): each gathered
