@@ -262,6 +262,15 @@ void solve_slots(emfem_context *c, int first, int n, const std::vector<double> &
   c->stats["forward_outer_its"] += *std::max_element(its.begin(), its.end());
 }
 
+// Wall time of one phase of the forward driver (every call it makes returns with its
+// work done), accumulated into the stat fwd_<name>_ms.
+template <typename F>
+void phase(emfem_context *c, const char *name, F &&f) {
+  const double t0 = now_ms();
+  f();
+  c->stats[std::string("fwd_") + name + "_ms"] += now_ms() - t0;
+}
+
 void forward_mt(emfem_context *c, int fidx) {
   restore_coarse(c);
   refine_points(c, c->rx, c->optd("max_rx_edge_length"), "Receiver");
@@ -275,21 +284,25 @@ void forward_mt(emfem_context *c, int fidx) {
     const bool last = cycle == max_cycles || c->T.ne >= c->opti("max_dofs");
     // learned indicator in intermediate cycles: no dual solve, no estimator
     const bool sur = c->opt("surrogate_cycles") == "true" && c->sur && !c->sur_rejected;
-    check(c, emfem_create_linear_system(c, 3));
-    check(c, emfem_assemble_matrix(c, fidx));
-    check(c, emfem_create_pc(c));
-    check(c, emfem_assemble_rhs_mt(c, fidx, EMFEM_XY_POLAR, 0));
-    check(c, emfem_assemble_rhs_mt(c, fidx, EMFEM_YX_POLAR, 1));
+    phase(c, "system", [&]() { check(c, emfem_create_linear_system(c, 3)); });
+    phase(c, "assemble", [&]() {
+      check(c, emfem_assemble_matrix(c, fidx));
+      check(c, emfem_assemble_rhs_mt(c, fidx, EMFEM_XY_POLAR, 0));
+      check(c, emfem_assemble_rhs_mt(c, fidx, EMFEM_YX_POLAR, 1));
+    });
+    phase(c, "pc", [&]() { check(c, emfem_create_pc(c)); });
     const double e_rtol = c->optd("e_rtol");
     if (sur && !last) {
       // primal indicators exactly; the dual solve's indicator from the network
-      solve_slots(c, 0, 2, {e_rtol, e_rtol});
+      phase(c, "solve", [&]() { solve_slots(c, 0, 2, {e_rtol, e_rtol}); });
       check(c, emfem_destroy_pc(c));
       const int nt = c->T.nt;
       const int slots[2] = {0, 1};
-      std::vector<double> err((size_t)2 * nt);
-      check(c, emfem_estimate_error(c, fidx, 2, slots, err.data()));
-      const std::vector<double> dual = surrogate_indicator(c, fidx, 0, 1);
+      std::vector<double> err((size_t)2 * nt), dual;
+      phase(c, "estimate", [&]() {
+        check(c, emfem_estimate_error(c, fidx, 2, slots, err.data()));
+        dual = surrogate_indicator(c, fidx, 0, 1);
+      });
       std::vector<double> exy(nt), eyx(nt);
       for (int t = 0; t < nt; ++t) {
         exy[t] = std::sqrt(err[t] * dual[t]);
@@ -297,17 +310,17 @@ void forward_mt(emfem_context *c, int fidx) {
       }
       c->stats["surrogate_cycles_used"] += 1;
       if (saving_mesh(c)) write_vtk(c, vtk_name(c, fidx, -3, cycle), -3, exy.data(), eyx.data());
-      check(c, emfem_destroy_linear_system(c));
-      refine_by_error(c, {exy, eyx});
+      phase(c, "system", [&]() { check(c, emfem_destroy_linear_system(c)); });
+      phase(c, "refine", [&]() { refine_by_error(c, {exy, eyx}); });
       continue;
     }
-    check(c, emfem_assemble_rhs_dual(c, 2));
-    solve_primal_dual(c, 0, 2);
+    phase(c, "assemble", [&]() { check(c, emfem_assemble_rhs_dual(c, 2)); });
+    phase(c, "solve", [&]() { solve_primal_dual(c, 0, 2); });
     check(c, emfem_destroy_pc(c));
     const int nt = c->T.nt;
     const int slots[3] = {0, 1, 2};
     std::vector<double> err((size_t)3 * nt);
-    check(c, emfem_estimate_error(c, fidx, 3, slots, err.data()));
+    phase(c, "estimate", [&]() { check(c, emfem_estimate_error(c, fidx, 3, slots, err.data())); });
     std::vector<double> exy(nt), eyx(nt);
     for (int t = 0; t < nt; ++t) {
       exy[t] = std::sqrt(err[t] * err[(size_t)2 * nt + t]);
@@ -330,15 +343,17 @@ void forward_mt(emfem_context *c, int fidx) {
     }
     if (last) {
       std::vector<double> rsp(c->obs4.size() / 2, NAN);
-      check(c, emfem_calculate_response_mt(c, fidx, 0, 1, (int)c->obs4.size() / 4, c->obs4.data(), rsp.data()));
+      phase(c, "responses", [&]() {
+        check(c, emfem_calculate_response_mt(c, fidx, 0, 1, (int)c->obs4.size() / 4, c->obs4.data(), rsp.data()));
+      });
       for (size_t o = 0; o < c->obs4.size() / 4; ++o)
         if (c->obs4[o * 4 + 1] == fidx && c->obs4[o * 4 + 2] == EMFEM_MT_TIDX)
           c->rsp[o] = cplx(rsp[o * 2], rsp[o * 2 + 1]);
       check(c, emfem_destroy_linear_system(c));
       break;
     }
-    check(c, emfem_destroy_linear_system(c));
-    refine_by_error(c, {exy, eyx});
+    phase(c, "system", [&]() { check(c, emfem_destroy_linear_system(c)); });
+    phase(c, "refine", [&]() { refine_by_error(c, {exy, eyx}); });
   }
 }
 
@@ -573,6 +588,8 @@ int emfem_forward(emfem_context *c) {
     }
     c->stats["not_converged"] = 0;
     c->stats["forward_outer_its"] = 0;
+    for (const char *k : {"system", "assemble", "pc", "solve", "estimate", "refine", "responses"})
+      c->stats[std::string("fwd_") + k + "_ms"] = 0.0;
     const double t0 = now_ms();
     for (auto &kv : csem) {
       forward_csem(c, kv.first, kv.second);

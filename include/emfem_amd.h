@@ -367,7 +367,10 @@ int emfem_copy_to_host(emfem_context *ctx, void *dst, const void *src, unsigned 
 /* ---- measurement ---------------------------------------------------------- */
 
 /* Named counters and timers of the last calls (e.g. "assemble_ms", "solve_ms",
- * "outer_its", "inner_its", "vcycles"); returns EMFEM_ERR_ARG for an unknown name. */
+ * "outer_its", "inner_its", "vcycles"; after emfem_forward "forward_ms" and the
+ * driver's phases "fwd_system_ms", "fwd_assemble_ms", "fwd_pc_ms", "fwd_solve_ms",
+ * "fwd_estimate_ms", "fwd_refine_ms", "fwd_responses_ms", like the reference's
+ * log_view events, src/em_ctx.cc:70-81); returns EMFEM_ERR_ARG for an unknown name. */
 int emfem_get_stat(const emfem_context *ctx, const char *name, double *value);
 
 /* Device time (ms per launch) and algorithmic bytes of the hot kernels:
