@@ -405,59 +405,16 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     return out
 
 
-MU0 = 4e-7 * np.pi
-
-
-def layered_impedance(tops, rho, freq, z):
-    """Exact MT impedance E_x / H_y at depth z (>= 0, below the surface) of a layered
-    earth (layer i from tops[i] to tops[i+1], the last one unbounded; e^{+i w t},
-    z down), by the impedance recursion from the bottom half-space up."""
-    w = 2 * np.pi * freq
-    k = np.sqrt(1j * w * MU0 / np.asarray(rho, dtype=float))
-    zi = 1j * w * MU0 / k                      # intrinsic impedances
-    Z = zi[-1]
-    layer = max(i for i in range(len(tops)) if z >= tops[i])
-    for i in range(len(tops) - 2, layer - 1, -1):
-        h = (tops[i + 1] - (z if i == layer else tops[i]))
-        t = np.tanh(k[i] * h)
-        Z = zi[i] * (Z + zi[i] * t) / (zi[i] + Z * t)
-    return Z
-
-
-def layered_model(mg, freqs, h=1000.0, hz=300.0):
-    """Flat layered earth under air on a graded Kuhn mesh: 100 ohm-m to 600 m, 10 ohm-m
-    to 1500 m, 300 ohm-m below; 6 x 6 receivers over +-3 km, 1 m below the surface."""
-    xn = mg.graded(4000.0, h, 20000.0, 4)
-    earth = np.concatenate([np.arange(0.0, 4200.0 + 1, hz), mg._geometric(4200.0, 30000.0, hz, 5)])
-    air = -mg._geometric(0.0, 50000.0, hz, 6)[::-1]
-    zn = np.concatenate([air, earth])
-    pts, tets, _, _ = mg.kuhn_grid(np.round(xn), np.round(xn), np.round(zn))
-    c = mg.centroids(pts, tets)
-    attr = np.select([c[:, 2] < 0, c[:, 2] < 600, c[:, 2] < 1500], [0, 1, 2], 3).astype(np.int32)
-    rho = np.array([[mg.AIR_RHO], [100.0], [10.0], [300.0]])
-    g = np.linspace(-3000.0, 3000.0, 6)
-    X, Y = np.meshgrid(g, g, indexing="ij")
-    rx = np.stack([X.ravel(), Y.ravel(), np.ones(X.size)], axis=1)
-    obs = mg.mt_obs(len(freqs), len(rx), types=(mg.Z_XY_RI, mg.Z_YX_RI))
-    m = mg.Model(pts, tets, attr, 0, rho, np.asarray(freqs, dtype=float), np.zeros((0, 7)), rx, obs,
-                 name="layered_mt")
-    target = np.zeros(len(obs), complex)
-    for o, (code, f, _, r) in enumerate(obs):
-        Z = layered_impedance([0.0, 600.0, 1500.0], [100.0, 10.0, 300.0], m.freqs[f], rx[r, 2])
-        target[o] = Z if code == mg.Z_XY_RI else -Z
-    return m, target
-
-
 def layered_leg(api, mg, params, max_cycles=6, fraction=0.2):
     """Time to 1% receiver error of the goal-oriented adaptive MT loop (em_forward)
     against an exact solution: on a flat layered earth the MT boundary data
     (src/em_fwd.cc:402-486) are the exact 1-D field, so the boundary-value problem's
     solution is that field and the loop's FE solutions converge to it; the target is
-    the closed-form impedance at each receiver (layered_impedance).  Two frequencies
+    the closed-form impedance at each receiver (meshgen.layered_impedance).  Two frequencies
     (0.1 and 10 Hz), Z_xy and Z_yx at 36 receivers.  Each k = 0..max_cycles runs the
     forward from scratch, plain (fp64 inner solve), mixed (fp32) and mixed with the
     learned indicator in the intermediate cycles."""
-    m, target = layered_model(mg, [0.1, 10.0])
+    m, target = mg.layered_mt([0.1, 10.0])
     out = {"workload": "flat layered earth (100 / 10 / 300 ohm-m, interfaces 600 m and 1500 m) on a graded Kuhn "
                        "mesh of %d edges, 2 frequencies (0.1, 10 Hz), 36 receivers, Z_xy and Z_yx; target: the exact "
                        "1-D impedance" % (7 * m.n_vertices)}

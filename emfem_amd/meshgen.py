@@ -496,3 +496,48 @@ def varied_mt_batch(n_models=32, seed=21, n_min=6, n_max=12, f_min=1e-3, f_max=1
         models.append(Model(pts, tets, attr, 3, np.asarray(rho), np.array([freq]), np.zeros((0, 7)), rx,
                             mt_obs(1, rx.shape[0]), name="varied_mt_%03d" % k))
     return models
+
+
+MU0 = 4e-7 * np.pi
+
+
+def layered_impedance(tops, rho, freq, z):
+    """Exact MT impedance E_x / H_y at depth z (>= 0, below the surface) of a layered
+    earth (layer i from tops[i] to tops[i+1], the last one unbounded; e^{+i w t},
+    z down), by the impedance recursion from the bottom half-space up."""
+    w = 2 * np.pi * freq
+    k = np.sqrt(1j * w * MU0 / np.asarray(rho, dtype=float))
+    zi = 1j * w * MU0 / k                      # intrinsic impedances
+    Z = zi[-1]
+    layer = max(i for i in range(len(tops)) if z >= tops[i])
+    for i in range(len(tops) - 2, layer - 1, -1):
+        h = (tops[i + 1] - (z if i == layer else tops[i]))
+        t = np.tanh(k[i] * h)
+        Z = zi[i] * (Z + zi[i] * t) / (zi[i] + Z * t)
+    return Z
+
+
+def layered_mt(freqs, h=1000.0, hz=300.0, tops=(0.0, 600.0, 1500.0), rho=(100.0, 10.0, 300.0)):
+    """Flat layered earth under air on a graded Kuhn mesh (default: 100 ohm-m to 600 m,
+    10 ohm-m to 1500 m, 300 ohm-m below); 6 x 6 receivers over +-3 km, 1 m below the
+    surface.  Returns (model, exact impedances of its observations): the MT boundary
+    data are the exact 1-D field here, so the discrete solutions converge to it."""
+    xn = graded(4000.0, h, 20000.0, 4)
+    earth = np.concatenate([np.arange(0.0, 4200.0 + 1, hz), _geometric(4200.0, 30000.0, hz, 5)])
+    air = -_geometric(0.0, 50000.0, hz, 6)[::-1]
+    zn = np.concatenate([air, earth])
+    pts, tets, _, _ = kuhn_grid(np.round(xn), np.round(xn), np.round(zn))
+    c = centroids(pts, tets)
+    conds = [c[:, 2] < 0] + [c[:, 2] < tp for tp in tops[1:]]
+    attr = np.select(conds, list(range(len(conds))), len(conds)).astype(np.int32)
+    rho_t = np.array([[AIR_RHO]] + [[r] for r in rho])
+    g = np.linspace(-3000.0, 3000.0, 6)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    rx = np.stack([X.ravel(), Y.ravel(), np.ones(X.size)], axis=1)
+    obs = mt_obs(len(freqs), len(rx), types=(Z_XY_RI, Z_YX_RI))
+    m = Model(pts, tets, attr, 0, rho_t, np.asarray(freqs, dtype=float), np.zeros((0, 7)), rx, obs, name="layered_mt")
+    target = np.zeros(len(obs), complex)
+    for o, (code, f, _, r) in enumerate(obs):
+        Z = layered_impedance(list(tops), list(rho), m.freqs[f], rx[r, 2])
+        target[o] = Z if code == Z_XY_RI else -Z
+    return m, target

@@ -399,3 +399,36 @@ def test_csem_response_codes(oracle_mod):
     for bad in (110, 113, 171, 211, 321):
         with pytest.raises(ValueError):
             oracle_mod.csem_response(bad, E, H)
+
+
+@pytest.mark.parametrize("freq", [1e-3, 0.1, 10.0])
+def test_layered_impedance_against_transfer_matrix(freq):
+    """The closed-form impedance the benchmark and the physics tests take as the exact
+    answer (meshgen.layered_impedance, the impedance recursion) equals
+    Z = E_x / H_y = -i w mu E_x / E_x' of the transfer-matrix solution above at every
+    depth, and a half-space gives sqrt(i w mu rho) with its 45 degree phase."""
+    tops, rho = [0.0, 600.0, 1500.0], [100.0, 10.0, 300.0]
+    sig = [1.0 / r for r in rho]
+    w = 2 * np.pi * freq
+    k = np.sqrt(1j * w * MU * np.asarray(sig))
+    for z in (0.0, 1.0, 300.0, 600.0, 1000.0, 1499.0, 2500.0):
+        # E_x and E_x' at z from the transfer matrix (same recursion as layered_1d, XY)
+        u, du = 1.0 + 0j, -k[-1]
+        top = [None] * 3
+        top[2] = (u, du)
+        for j in (1, 0):
+            h = tops[j + 1] - tops[j]
+            c, s = np.cosh(k[j] * h), np.sinh(k[j] * h)
+            u, du = u * c - du / k[j] * s, -u * k[j] * s + du * c
+            top[j] = (u, du)
+        j = max(i for i in range(3) if z >= tops[i])
+        u0, du0 = top[j]
+        d = z - tops[j]
+        c, s = np.cosh(k[j] * d), np.sinh(k[j] * d)
+        ez, dez = u0 * c + du0 / k[j] * s, u0 * k[j] * s + du0 * c
+        z_tm = -1j * w * MU * ez / dez
+        z_rec = mg.layered_impedance(tops, rho, freq, z)
+        assert abs(z_rec - z_tm) <= 1e-9 * abs(z_tm), (freq, z, z_rec, z_tm)
+    zh = mg.layered_impedance([0.0], [100.0], freq, 0.0)
+    assert abs(zh - np.sqrt(1j * w * MU * 100.0)) <= 1e-12 * abs(zh)
+    assert abs(np.degrees(np.angle(zh)) - 45.0) < 1e-9
