@@ -1717,20 +1717,23 @@ __global__ __launch_bounds__(kBlock) void k_aux_restrict(int nv0, int nblk, int 
   const int v = (int)(i / nc), c = (int)(i % nc);
   T g = T(0), px = T(0), py = T(0), pz = T(0);
   const int p0 = vrp[v], p1 = vrp[v + 1];
-  for (int p = p0; p < p1; p += 4) {
-    int code[4];
+  // sixteen edges per trip (an interior vertex of a Kuhn mesh has 14): all codes,
+  // then all gathers in flight together
+  constexpr int kU = 16;
+  for (int p = p0; p < p1; p += kU) {
+    int code[kU];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) code[u] = vcode[min(p + u, p1 - 1)];
-    double rv[4];
-    VecT<T, 4> t[4];
+    for (int u = 0; u < kU; ++u) code[u] = vcode[min(p + u, p1 - 1)];
+    double rv[kU];
+    VecT<T, 4> t[kU];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kU; ++u) {
       const int e = code[u] & 0x7fffffff;
       rv[u] = r[(size_t)e * nc + c];
       t[u] = *reinterpret_cast<const VecT<T, 4> *>(th + (size_t)e * 4);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < kU; ++u)
       if (p + u < p1) {
         const T x = (T)rv[u];
         g += code[u] < 0 ? -x : x;
