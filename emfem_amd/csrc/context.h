@@ -286,6 +286,7 @@ inline PcOptions pc_options(const emfem_context *c) {
   o.amg.additive = c->opt("ams_cycle") == "additive";
   o.use_graph = c->opt("use_graph") == "true";
   o.aux_csr = c->opt("ams_transfer") == "csr";
+  o.col16 = c->opt("spmm_col16") == "true";
   // spmm_rows: 0 tuned, 1 lanes per row tuned, 2 row blocks, 3 x-blocks, 4 persistent
   // pipelined row blocks, 10 + n fixed n lanes per row
   const std::string sr = c->opt("spmm_rows");

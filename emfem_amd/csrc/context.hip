@@ -82,6 +82,9 @@ const std::map<std::string, std::string> &option_defaults() {
       {"batch_retire", "false"},
       {"use_graph", "true"},
       {"spmm_rows", "auto"},
+      // row-block SpMMs of B on 16-bit columns (offsets from each 64-nonzero group's
+      // smallest column) when every group's columns span at most 65536
+      {"spmm_col16", "true"},
       {"ams_cheb_degree", "5"},
       {"ams_cheb_ratio", "60"},
       {"ams_cheb_kind", "first"},
@@ -108,6 +111,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"fp64_fallback", {"true", "false"}},
       {"batch_retire", {"true", "false"}},
       {"use_graph", {"true", "false"}},
+      {"spmm_col16", {"true", "false"}},
       {"spmm_rows", {"auto", "lanes", "rowblock", "pipelined", "xblock", "2", "4", "8", "16"}},
       {"ams_cycle", {"multiplicative", "additive"}},
       {"ams_transfer", {"structured", "csr"}},
@@ -599,6 +603,7 @@ int emfem_create_pc(emfem_context *c) {
     c->pc = make_inner_pc(c->opt("inner_precision") == "fp32", *c->ams_host, c->T.ne, c->nnz, c->d_rp.get(),
                           c->d_ci.get(), c->d_K.get(), o, s);
     EMF_HIP(hipStreamSynchronize(s));
+    c->stats["spmm_col16"] = c->pc->col16() ? 1.0 : 0.0;
     c->stats["pc_setup_host_ms"] = t1 - t0;
     c->stats["pc_setup_ms"] = now_ms() - t0;
   });
@@ -674,10 +679,13 @@ int emfem_solve_linear_system(emfem_context *c, int first, int n, int max_it, do
     }
     c->stats["op_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * 20 + double(ne) * n * 32;
     const double vb = c->pc->fp32() ? 4.0 : 8.0;
-    // row pointers, columns and values, Dinv, and per row of the Chebyshev step: x_k
-    // gathered once, the right-hand side, x_{k-1} in, x_{k+1} out
-    c->stats["b_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * (4 + vb) + double(ne) * (ncp * 4 + 1) * vb;
-    c->stats["fcg_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * 12 + double(ne) * ncp * 16;
+    // row pointers, columns (4 bytes, or 2 + 4 / 64 in their 16-bit form) and values,
+    // Dinv, and per row of the Chebyshev step: x_k gathered once, the right-hand side,
+    // x_{k-1} in, x_{k+1} out
+    const double ib = c->pc->index_bytes(s, ncp, false), ib64 = c->pc->index_bytes(s, ncp, true);
+    c->stats["b_spmm_index_bytes"] = ib;
+    c->stats["b_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * (ib + vb) + double(ne) * (ncp * 4 + 1) * vb;
+    c->stats["fcg_spmm_bytes"] = double(ne + 1) * 4 + double(c->nnz) * (ib64 + 8) + double(ne) * ncp * 16;
     for (int i = 0; i < n; ++i) {
       if (iters) iters[i] = st.its[i];
       if (relres) relres[i] = st.relres[i];

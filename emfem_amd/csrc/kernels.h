@@ -27,6 +27,10 @@ struct Csr {
   // nonzeros (row_blocks), for the column count and precision the table was built for
   const int *blk_row = nullptr;
   int nblk = 0;
+  // 16-bit column form read by the row blocks when set (see compress_columns): the
+  // column of nonzero p is ci16_base[p / 64] + ci16[p]
+  const uint16_t *ci16 = nullptr;
+  const int *ci16_base = nullptr;
   // lpr = -2 (16 or more columns): LDS x-blocks of 64-column slabs.
   // Block b takes the rows [xb_row[b], xb_row[b+1]); its distinct columns
   // xb_ucol[xb_uptr[b] .. xb_uptr[b+1]) (at most xb_umax, ascending) are staged in LDS
@@ -80,6 +84,15 @@ int stream_tile_for(int nc, bool fp64);
 // by the first nonzero of every block and nnz (nblk + 1 entries); empty when a row is
 // longer than the tile.
 std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
+
+// 16-bit column form of a CSR pattern: the nonzeros in aligned groups of kCol16Group,
+// each group's smallest column in base[ceil(nnz / kCol16Group)] and every column as its
+// offset from that in off[nnz].  The row-block SpMM then streams 2 + 4 / 64 bytes of
+// index per nonzero instead of 4.  Returns false (base and off unusable) when some
+// group spans more than 65536 columns; the pattern then keeps its 32-bit columns.
+// flag is one device int of scratch.  Synchronises s.
+constexpr int kCol16Group = 64;
+bool compress_columns(hipStream_t s, int nnz, const int *ci, int *base, uint16_t *off, int *flag);
 
 // A Chebyshev step fused into an SpMM epilogue (off when x is null): the SpMM's value
 // v is not stored; y(row, j) receives x + c1 (x - d) + c2 v at (row, j) instead (in T

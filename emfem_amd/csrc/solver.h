@@ -46,13 +46,17 @@ struct PcOptions {
   AmgParams amg;
   bool use_graph = true;  // replay each auxiliary-space cycle as one captured hipGraph
   // smoother SpMM row mapping: 0 = tuned, 1 = lanes per row only (tuned; lanes on
-  // columns at 16 or more columns), 2 = row blocks when possible, 3 = LDS x-blocks
+  // columns at 16 or more columns), 2 = row blocks when possible (for the fp64 FCG
+  // product too), 3 = LDS x-blocks
   // when possible, 4 = persistent pipelined row blocks when possible, 10 + n = n
   // lanes per row
   int spmm_rows = 0;
   // auxiliary-space transfers by the CSR matrices Pi / Pi^T instead of the structured
   // kernels (the same operator)
   bool aux_csr = false;
+  // row-block SpMMs of B read 16-bit columns (compress_columns) when the pattern
+  // allows; the results are the same bit for bit
+  bool col16 = true;
 };
 
 // Preconditioner for B on nc real columns: flexible CG in fp64 with the
@@ -80,6 +84,11 @@ class InnerPc {
   // row mapping the smoother SpMM uses on nc columns: lanes per row, -1 = row blocks,
   // 0 = many-column kernel
   virtual int smoother_rows(hipStream_t s, int nc) = 0;
+  // bytes of column index per nonzero the smoother SpMM (fcg false) or the fp64 FCG
+  // product (fcg true) reads on nc columns (4, or 2 + 4 / 64 on 16-bit columns)
+  virtual double index_bytes(hipStream_t s, int nc, bool fcg) = 0;
+  // whether B's row-block SpMMs read the 16-bit column form
+  virtual bool col16() const = 0;
 };
 
 std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp,

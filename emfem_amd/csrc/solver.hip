@@ -201,6 +201,20 @@ class DevAms {
       make_b<double>(s, nnz, K, b64vals_.get());
       B64_.v = b64vals_.get();
     }
+    if (o.col16) {
+      // the row-block SpMMs of B (smoother and FCG) stream 16-bit columns when every
+      // group of the pattern fits
+      ci16_.resize(nnz);
+      ci16_base_.resize((nnz + kCol16Group - 1) / kCol16Group);
+      col16_flag_.resize(1);
+      if (compress_columns(s, nnz, ci, ci16_base_.get(), ci16_.get(), col16_flag_.get())) {
+        B_.ci16 = B64_.ci16 = ci16_.get();
+        B_.ci16_base = B64_.ci16_base = ci16_base_.get();
+      } else {
+        ci16_.release();
+        ci16_base_.release();
+      }
+    }
     if (H.l1inv.empty()) {
       // the edge-space l1 scaling from the fp64 B on the device
       l1inv_.resize(ne);
@@ -335,7 +349,10 @@ class DevAms {
           B64t.nblk = (int)b64.size() / 2 - 1;
         }
       }
-      lpr64_[nc] = tune_lpr<double, double, double>(s, B64t, nc, r64_.get(), tmp.get());
+      if (spmm_rows_ == 2 && B64t.blk_row)
+        lpr64_[nc] = -1;  // row blocks on request, as for the smoother
+      else
+        lpr64_[nc] = tune_lpr<double, double, double>(s, B64t, nc, r64_.get(), tmp.get());
       EMF_HIP(hipStreamSynchronize(s));
     }
     if (spmm_rows_ >= 10 && nc < 16) lpr_[nc] = spmm_rows_ - 10;  // fixed lanes per row
@@ -417,6 +434,17 @@ class DevAms {
   int rows(hipStream_t s, int nc) {
     tune(s, nc);
     return lpr_[nc];
+  }
+
+  bool col16() const { return ci16_.get() != nullptr; }
+
+  // Bytes of column index per nonzero that the smoother SpMM (fcg false) or the fp64
+  // FCG product (fcg true) streams on nc columns: 2 + 4 / 64 for the row blocks on
+  // 16-bit columns, else 4.
+  double index_bytes(hipStream_t s, int nc, bool fcg) {
+    tune(s, nc);
+    const int lpr = fcg ? B64(nc).lpr : lpr_[nc];
+    return ci16_.get() && lpr == -1 && (nc == 3 || nc == 6) ? 2.0 + 4.0 / kCol16Group : 4.0;
   }
 
   // Whether the Chebyshev update rides in the smoother SpMM's epilogue: always below
@@ -666,6 +694,8 @@ class DevAms {
   Csr<double> B64_;
   DevBuf<T> bvals_, l1inv_, d_, d2_, wtmp_, bt_, vb_, vx_;
   DevBuf<double> b64vals_, r64_;
+  DevBuf<uint16_t> ci16_;
+  DevBuf<int> ci16_base_, col16_flag_;
   DevCsr<T> G0_, G0T_, Pi_, PiT_;
   bool aux_csr_ = false;
   DevBuf<int2> ev_;
@@ -711,6 +741,11 @@ class InnerPcImpl : public InnerPc {
   int smoother_rows(hipStream_t s, int nc) override {
     ensure(nc);
     return fp32_ ? a32_->rows(s, nc) : a64_->rows(s, nc);
+  }
+  bool col16() const override { return fp32_ ? a32_->col16() : a64_->col16(); }
+  double index_bytes(hipStream_t s, int nc, bool fcg) override {
+    ensure(nc);
+    return fp32_ ? a32_->index_bytes(s, nc, fcg) : a64_->index_bytes(s, nc, fcg);
   }
   double time_fcg(hipStream_t s, int nc, int reps) override {
     ensure(nc);

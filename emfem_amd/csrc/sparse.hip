@@ -784,10 +784,13 @@ constexpr int row_chunk() {
 // given), kFormCheb0 (the same with d = 0, cf.d null), kFormAxpy (y = alpha A x +
 // gamma w, the residual form): the operands of the first pass are loaded before the
 // barrier, so their latency overlaps the gathers'.  BS threads per workgroup, TILE a
-// multiple of BS.
+// multiple of BS.  C16: the columns are read in their 16-bit form (ci16, cbase; see
+// compress_columns) instead of from ci.
 constexpr int kFormAny = 0, kFormPlain = 1, kFormCheb = 2, kFormCheb0 = 3, kFormAxpy = 4;
-template <typename TA, typename TX, typename T, int NC, int TILE, int FORM, int BS = kBlock>
+template <typename TA, typename TX, typename T, int NC, int TILE, int FORM, int BS = kBlock, bool C16 = false>
 __global__ __launch_bounds__(BS) void k_spmm_stream_v(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                          const uint16_t *__restrict__ ci16,
+                                                          const int *__restrict__ cbase,
                                                           const TA *__restrict__ av, const TX *__restrict__ x,
                                                           T *y, T alpha, const T *__restrict__ rs, T gamma,
                                                           const T *w, T beta, const T *z,
@@ -817,7 +820,10 @@ __global__ __launch_bounds__(BS) void k_spmm_stream_v(const int *__restrict__ rp
     // the matrix is streamed once per launch: non-temporal loads, so that it does not
     // evict the gathered x rows from the caches
     const T aval = (T)__builtin_nontemporal_load(av + pc);
-    c[u] = __builtin_nontemporal_load(ci + pc);
+    if constexpr (C16)
+      c[u] = cbase[pc / kCol16Group] + (int)__builtin_nontemporal_load(ci16 + pc);
+    else
+      c[u] = __builtin_nontemporal_load(ci + pc);
     a[u] = idx < nnz ? aval : T(0);
   }
   const int rpv = rp[r0 + min((int)threadIdx.x, nr)];
@@ -984,20 +990,30 @@ void launch_narrow(hipStream_t s, const Csr<TA> &A, const TX *x, int ldx, T *y, 
       const bool plain = gamma == T(0) && beta == T(0) && rs == nullptr && !cf.on();
       const bool cheb = gamma != T(0) && beta == T(0) && rs != nullptr && cf.on();
       const bool axpy = gamma != T(0) && beta == T(0) && rs == nullptr && !cf.on();
-#define EMF_SV(F)                                                                                                \
-  hipLaunchKernelGGL((k_spmm_stream_v<TA, TX, T, NC, stream_tile<T, NC>(), F, stream_block<T, NC>()>), dim3(A.nblk), \
-                     dim3(stream_block<T, NC>()), 0, s,                                                            \
-                     A.rp, A.ci, A.v, x, y, alpha, rs, gamma, w, beta, z, A.blk_row, cf)
-      if (cheb && cf.d)
-        EMF_SV(kFormCheb);
-      else if (cheb)
-        EMF_SV(kFormCheb0);
-      else if (plain)
-        EMF_SV(kFormPlain);
-      else if (axpy)
-        EMF_SV(kFormAxpy);
-      else
-        EMF_SV(kFormAny);
+      // 16-bit columns at the column counts the auxiliary-space cycle and its FCG run
+      // on (3 and 6); the kFormAny fallback keeps 32-bit columns
+      constexpr bool kC16 = NC == 3 || NC == 6;
+#define EMF_SV(F, C)                                                                                                \
+  hipLaunchKernelGGL((k_spmm_stream_v<TA, TX, T, NC, stream_tile<T, NC>(), F, stream_block<T, NC>(), C>),          \
+                     dim3(A.nblk), dim3(stream_block<T, NC>()), 0, s, A.rp, A.ci, A.ci16, A.ci16_base, A.v, x, y,   \
+                     alpha, rs, gamma, w, beta, z, A.blk_row, cf)
+#define EMF_SV2(F)          \
+  if (kC16 && A.ci16)       \
+    EMF_SV(F, kC16);        \
+  else                      \
+    EMF_SV(F, false)
+      if (cheb && cf.d) {
+        EMF_SV2(kFormCheb);
+      } else if (cheb) {
+        EMF_SV2(kFormCheb0);
+      } else if (plain) {
+        EMF_SV2(kFormPlain);
+      } else if (axpy) {
+        EMF_SV2(kFormAxpy);
+      } else {
+        EMF_SV(kFormAny, false);
+      }
+#undef EMF_SV2
 #undef EMF_SV
     } else
       hipLaunchKernelGGL((k_spmm_stream<TA, TX, T, NC, stream_tile<T, NC>(), stream_block<T, NC>()>), dim3(A.nblk),
@@ -1053,6 +1069,48 @@ std::vector<int> row_blocks(const std::vector<int> &rp, int tile) {
   const size_t nb = blk.size();
   for (size_t b = 0; b < nb; ++b) blk.push_back(rp[blk[b]]);
   return blk;
+}
+
+namespace {
+
+// One thread per group of kCol16Group nonzeros: the group's smallest column, and a
+// 1 in *flag when its columns span more than 16 bits.
+__global__ void k_col16_base(int nnz, const int *__restrict__ ci, int *base, int *flag) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  const int p0 = g * kCol16Group;
+  if (p0 >= nnz) return;
+  const int p1 = min(p0 + kCol16Group, nnz);
+  int lo = ci[p0], hi = lo;
+  for (int p = p0 + 1; p < p1; ++p) {
+    lo = min(lo, ci[p]);
+    hi = max(hi, ci[p]);
+  }
+  base[g] = lo;
+  if (hi - lo > 65535) flag[0] = 1;
+}
+
+__global__ void k_col16_off(int nnz, const int *__restrict__ ci, const int *__restrict__ base, uint16_t *off) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nnz) return;
+  off[p] = (uint16_t)(ci[p] - base[p / kCol16Group]);
+}
+
+}  // namespace
+
+bool compress_columns(hipStream_t s, int nnz, const int *ci, int *base, uint16_t *off, int *flag) {
+  if (nnz <= 0) return false;
+  const int ng = (nnz + kCol16Group - 1) / kCol16Group;
+  EMF_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_col16_base, dim3((ng + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nnz, ci, base, flag);
+  EMF_HIP(hipGetLastError());
+  int wide = 0;
+  EMF_HIP(hipMemcpyAsync(&wide, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  EMF_HIP(hipStreamSynchronize(s));
+  if (wide) return false;
+  hipLaunchKernelGGL(k_col16_off, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, ci, base, off);
+  EMF_HIP(hipGetLastError());
+  EMF_HIP(hipStreamSynchronize(s));
+  return true;
 }
 
 XBlocks x_blocks(const std::vector<int> &rp, const std::vector<int> &ci, int ncols, int umax, int rmax) {

@@ -370,7 +370,9 @@ int emfem_copy_to_host(emfem_context *ctx, void *dst, const void *src, unsigned 
  * "outer_its", "inner_its", "vcycles"; after emfem_forward "forward_ms" and the
  * driver's phases "fwd_system_ms", "fwd_assemble_ms", "fwd_pc_ms", "fwd_solve_ms",
  * "fwd_estimate_ms", "fwd_refine_ms", "fwd_responses_ms", like the reference's
- * log_view events, src/em_ctx.cc:70-81); returns EMFEM_ERR_ARG for an unknown name. */
+ * log_view events, src/em_ctx.cc:70-81; after emfem_create_pc "spmm_col16", 1 when
+ * B's row-block SpMMs read 16-bit columns); returns EMFEM_ERR_ARG for an unknown
+ * name. */
 int emfem_get_stat(const emfem_context *ctx, const char *name, double *value);
 
 /* Device time (ms per launch) and algorithmic bytes of the hot kernels:
@@ -379,7 +381,8 @@ int emfem_get_stat(const emfem_context *ctx, const char *name, double *value);
  * precision, averaged over HIP events around each of its first 512 launches in that
  * solve's cycles), "op_spmm" (complex operator) and "fcg_spmm" (fp64 B SpMM of the
  * FCG), the last two from HIP events around 20 back-to-back launches on that solve's
- * operands; all on the context's stream. */
+ * operands; all on the context's stream.  The bytes of b_spmm and fcg_spmm count the
+ * column index at 2 + 4 / 64 bytes per nonzero where the 16-bit form is read. */
 int emfem_kernel_time(const emfem_context *ctx, const char *which, double *ms, double *bytes);
 
 #ifdef __cplusplus

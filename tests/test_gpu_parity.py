@@ -362,6 +362,48 @@ def test_row_block_spmm_matches_lane_spmm(precision, oracle_mod):
             assert np.linalg.norm(sols[rows][s] - e) <= 1e4 * 1e-10 * np.linalg.norm(e)
 
 
+@pytest.mark.parametrize("pc", ["blockdiag", "presb"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_row_blocks_on_16bit_columns_match_32bit_columns(precision, pc):
+    """The row-block SpMMs of B (smoother and fp64 FCG product) reading each column as
+    a 16-bit offset from its 64-nonzero group's smallest column give the solve of the
+    32-bit columns bit for bit, on 6 real columns (block-diagonal, XY + YX + dual) and
+    on 3 (PRESB halves)."""
+    m = mg.halfspace_mt(n=4, n_rx=3)
+    out = {}
+    for c16 in ("true", "false"):
+        ctx = _ctx(m, "-inner_precision %s -spmm_rows rowblock -A_pc_type %s -spmm_col16 %s" % (precision, pc, c16))
+        ctx.create_linear_system(3)
+        ctx.assemble_matrix(0)
+        ctx.assemble_rhs_mt(0, api.XY_POLAR, 0)
+        ctx.assemble_rhs_mt(0, api.YX_POLAR, 1)
+        ctx.assemble_rhs_dual(2)
+        ctx.create_pc()
+        assert ctx.stat("spmm_col16") == (1.0 if c16 == "true" else 0.0)
+        its, rr = ctx.solve_linear_system(0, 3, 300, 1e-8)
+        assert np.all(rr <= 1e-8 * 1.0001), rr
+        out[c16] = (its, [ctx.get_solution(s) for s in range(3)])
+        ctx.close()
+    assert np.array_equal(out["true"][0], out["false"][0])
+    for a, b in zip(out["true"][1], out["false"][1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("cells,fits", [((64, 64, 8), True), ((72, 72, 8), False)])
+def test_16bit_columns_only_where_every_group_fits(cells, fits):
+    """The 16-bit column form is taken when the columns of every aligned group of 64
+    nonzeros span at most 65536 (largest span 59,015 on 64 x 64 x 8 cells, 244,872
+    edges) and the 32-bit columns are kept otherwise (74,455 on 72 x 72 x 8 cells,
+    309,464 edges; spans computed from the oracle's pattern)."""
+    m = mg.adaptive_mt(cells=cells, n_freqs=1, n_rx_side=2)
+    ctx = _ctx(m, "-inner_precision fp32")
+    ctx.create_linear_system(1)
+    ctx.assemble_matrix(0)
+    ctx.create_pc()
+    assert ctx.stat("spmm_col16") == (1.0 if fits else 0.0)
+    ctx.close()
+
+
 @pytest.mark.parametrize("rows", ["auto", "lanes", "xblock"])
 @pytest.mark.parametrize("n_tx", [10, 40])
 def test_many_rhs_csem_block_solve(n_tx, rows, oracle_mod):
