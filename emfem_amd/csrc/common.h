@@ -95,6 +95,35 @@ class DevBuf {
   size_t n_ = 0;
 };
 
+// Pinned host array that kernels write directly, read on the host after the writing
+// stream is synchronised: mapped and coherent (uncached on the device), so the
+// kernel's stores reach host memory and not only the device caches.
+template <typename T>
+class HostPinned {
+ public:
+  HostPinned() = default;
+  ~HostPinned() { release(); }
+  HostPinned(const HostPinned &) = delete;
+  HostPinned &operator=(const HostPinned &) = delete;
+  void resize(size_t n) {
+    if (n == n_) return;
+    release();
+    if (n) EMF_HIP(hipHostMalloc((void **)&p_, n * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent));
+    n_ = n;
+  }
+  void release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T *get() { return p_; }
+  T &operator[](size_t i) { return p_[i]; }
+
+ private:
+  T *p_ = nullptr;
+  size_t n_ = 0;
+};
+
 inline double now_ms() {
   using namespace std::chrono;
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();

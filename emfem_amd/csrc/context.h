@@ -97,6 +97,11 @@ struct emfem_context {
   bool have_system = false;
   int nnz = 0, nslots = 0, assembled_fidx = -1;
   DevBuf<int> d_rp, d_ci;
+  // the pattern's 16-bit column form (compress_columns), built at the first
+  // preconditioner set-up of the linear system: -1 not yet, 0 does not fit, 1 built
+  DevBuf<uint16_t> d_ci16;
+  DevBuf<int> d_ci16_base;
+  int col16_state = -1;
   DevBuf<double2> d_K;
   DevBuf<double> d_S, d_X;  // [E][2 nslots]
   // solve workspace kept across solves: the compact right-hand side and solution

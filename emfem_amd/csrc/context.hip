@@ -367,6 +367,9 @@ int emfem_create_linear_system(emfem_context *c, int n_slots) {
     c->nnz = (int)ci.size();
     c->d_rp.upload(rp, c->stream);
     c->d_ci.upload(ci, c->stream);
+    c->d_ci16.release();
+    c->d_ci16_base.release();
+    c->col16_state = -1;
     c->d_K.resize(ci.size());
     c->nslots = n_slots;
     c->d_S.resize((size_t)c->T.ne * 2 * n_slots);
@@ -387,6 +390,9 @@ int emfem_destroy_linear_system(emfem_context *c) {
     c->h_B = HostCsr();
     c->d_rp.release();
     c->d_ci.release();
+    c->d_ci16.release();
+    c->d_ci16_base.release();
+    c->col16_state = -1;
     c->d_K.release();
     c->d_S.release();
     c->d_X.release();
@@ -600,8 +606,25 @@ int emfem_create_pc(emfem_context *c) {
     const double t1 = now_ms();
     c->stats["pc_download_ms"] = tb - t0;
     for (auto &kv : c->ams_host->timing) c->stats["pc_" + kv.first] = kv.second;
+    Col16 c16;
+    if (o.col16) {
+      // the pattern's 16-bit columns, once per linear system
+      if (c->col16_state < 0) {
+        c->d_ci16.resize(c->nnz);
+        c->d_ci16_base.resize((c->nnz + kCol16Group - 1) / kCol16Group);
+        DevBuf<int> flag(1);
+        c->col16_state = compress_columns(s, c->nnz, c->d_ci.get(), c->d_ci16_base.get(), c->d_ci16.get(), flag.get());
+        if (!c->col16_state) {
+          c->d_ci16.release();
+          c->d_ci16_base.release();
+        }
+      }
+      c16.built = true;
+      c16.off = c->d_ci16.get();
+      c16.base = c->d_ci16_base.get();
+    }
     c->pc = make_inner_pc(c->opt("inner_precision") == "fp32", *c->ams_host, c->T.ne, c->nnz, c->d_rp.get(),
-                          c->d_ci.get(), c->d_K.get(), o, s);
+                          c->d_ci.get(), c->d_K.get(), o, s, c16);
     EMF_HIP(hipStreamSynchronize(s));
     c->stats["spmm_col16"] = c->pc->col16() ? 1.0 : 0.0;
     c->stats["pc_setup_host_ms"] = t1 - t0;

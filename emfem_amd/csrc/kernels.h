@@ -260,16 +260,21 @@ void put_rows(hipStream_t s, int rows, int n, const int *idx, const double2 *v, 
 void split_to_interleaved(hipStream_t s, int rows, int n, const double *re, const double *im, double *y);
 void interleaved_to_split(hipStream_t s, int rows, int n, const double *x, double *re, double *im);
 
-// Flexible CG updates on nc columns with device scalars:
-//   x += alpha[c] p (skipped when x is null) ; r -= alpha[c] q ; rr[c] = r . r
-// (fused reduction; x, r in T; the search directions p, q in TP; arithmetic in double)
-template <typename T, typename TP = T>
-void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const TP *p, const TP *q, T *x, T *r, double *rr,
-            double *work, const Seg &sg = Seg());
-// x += alpha[k] p ; p = z + beta[k] p  (k = seg(row) * nc + column)
+// The residual step of the flexible CG on nc columns with device scalars, per
+// logical column k = seg(row) * nc + column:
+//   alpha[k] = active[k] && pq[k] != 0 ? rz[k] / pq[k] : 0 (stored for the delayed
+//   solution update), r -= alpha[k] q, rr[k] = r . r (fused, fixed-order reduction),
+// then the convergence test on the device: active[k] = 0 once rr[k] <= tol2 * bb[k].
+// rr and the updated flags are also written to h_rr and h_active, pinned host arrays
+// the caller reads after synchronising s.
+void fcg_residual(hipStream_t s, int rows, int nc, const double *rz, const double *pq, uint8_t *active,
+                  double *alpha, const double *q, double *r, double *rr, const double *bb, double tol2, double *h_rr,
+                  uint8_t *h_active, double *work, const Seg &sg = Seg());
+// x += alpha[k] p ; p = z + beta p with beta = active[k] && pq[k] != 0 ? -zq[k] / pq[k]
+// : 0 formed per element (k = seg(row) * nc + column)
 template <typename TZ>
-void fcg_px(hipStream_t s, int rows, int nc, const double *alpha, const double *beta, const TZ *z, double *p, double *x,
-            const Seg &sg = Seg());
+void fcg_px(hipStream_t s, int rows, int nc, const double *alpha, const double *zq, const double *pq,
+            const uint8_t *active, const TZ *z, double *p, double *x, const Seg &sg = Seg());
 
 // x = inv_theta * dinv[row] .* T(b)  (the first Chebyshev iterate from zero), and
 // bt = T(b) when bt is not null (the right-hand side in the cycle's precision)
@@ -286,8 +291,7 @@ constexpr int kDenseMax = 4096;
 template <typename T>
 void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T *x);
 
-// Tiny device helpers for the Krylov scalars.
-void fcg_alpha(hipStream_t s, int nc, const double *rz, const double *pq, const uint8_t *active, double *alpha);
+// beta[k] of fcg_px stored (for products that take it per column).
 void fcg_beta(hipStream_t s, int nc, const double *zq, const double *pq, const uint8_t *active, double *beta);
 
 // Estimator kernel: squared cell + face indicators per tet for ns solutions whose

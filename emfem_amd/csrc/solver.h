@@ -91,8 +91,18 @@ class InnerPc {
   virtual bool col16() const = 0;
 };
 
+// The 16-bit column form of B's pattern (compress_columns) when its owner built it
+// once for the pattern: built = true, off / base null when the pattern does not fit.
+// built = false: each preconditioner builds its own (PcOptions::col16).
+struct Col16 {
+  bool built = false;
+  const uint16_t *off = nullptr;
+  const int *base = nullptr;
+};
+
 std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp,
-                                       const int *d_ci, const double2 *d_K, const PcOptions &o, hipStream_t s);
+                                       const int *d_ci, const double2 *d_K, const PcOptions &o, hipStream_t s,
+                                       const Col16 &c16 = Col16());
 
 struct OuterOptions {
   int restart = 30;

@@ -173,7 +173,7 @@ template <typename T>
 class DevAms {
  public:
   void setup(const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K, const PcOptions &o,
-             hipStream_t s) {
+             hipStream_t s, const Col16 &c16) {
     ne_ = ne;
     auto view = [&](const void *v) {
       Csr<T> c;
@@ -201,7 +201,11 @@ class DevAms {
       make_b<double>(s, nnz, K, b64vals_.get());
       B64_.v = b64vals_.get();
     }
-    if (o.col16) {
+    if (o.col16 && c16.built) {
+      // the owner's 16-bit columns of this pattern
+      B_.ci16 = B64_.ci16 = c16.off;
+      B_.ci16_base = B64_.ci16_base = c16.base;
+    } else if (o.col16) {
       // the row-block SpMMs of B (smoother and FCG) stream 16-bit columns when every
       // group of the pattern fits
       ci16_.resize(nnz);
@@ -436,7 +440,7 @@ class DevAms {
     return lpr_[nc];
   }
 
-  bool col16() const { return ci16_.get() != nullptr; }
+  bool col16() const { return B_.ci16 != nullptr; }
 
   // Bytes of column index per nonzero that the smoother SpMM (fcg false) or the fp64
   // FCG product (fcg true) streams on nc columns: 2 + 4 / 64 for the row blocks on
@@ -444,7 +448,7 @@ class DevAms {
   double index_bytes(hipStream_t s, int nc, bool fcg) {
     tune(s, nc);
     const int lpr = fcg ? B64(nc).lpr : lpr_[nc];
-    return ci16_.get() && lpr == -1 && (nc == 3 || nc == 6) ? 2.0 + 4.0 / kCol16Group : 4.0;
+    return B_.ci16 && lpr == -1 && (nc == 3 || nc == 6) ? 2.0 + 4.0 / kCol16Group : 4.0;
   }
 
   // Whether the Chebyshev update rides in the smoother SpMM's epilogue: always below
@@ -724,8 +728,8 @@ class DevAms {
 class InnerPcImpl : public InnerPc {
  public:
   InnerPcImpl(bool fp32, const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K,
-              const PcOptions &o, hipStream_t s)
-      : H_(H), ne_(ne), nnz_(nnz), rp_(rp), ci_(ci), K_(K), o_(o) {
+              const PcOptions &o, hipStream_t s, const Col16 &c16)
+      : H_(H), ne_(ne), nnz_(nnz), rp_(rp), ci_(ci), K_(K), o_(o), c16_(c16) {
     set_fp32(fp32, s);
   }
 
@@ -756,12 +760,12 @@ class InnerPcImpl : public InnerPc {
     fp32_ = fp32;
     if (fp32 && !a32_) {
       a32_.reset(new DevAms<float>());
-      a32_->setup(H_, ne_, nnz_, rp_, ci_, K_, o_, s);
+      a32_->setup(H_, ne_, nnz_, rp_, ci_, K_, o_, s, c16_);
       if (nc_) a32_->ensure(nc_);
     }
     if (!fp32 && !a64_) {
       a64_.reset(new DevAms<double>());
-      a64_->setup(H_, ne_, nnz_, rp_, ci_, K_, o_, s);
+      a64_->setup(H_, ne_, nnz_, rp_, ci_, K_, o_, s, c16_);
       if (nc_) a64_->ensure(nc_);
     }
   }
@@ -796,7 +800,9 @@ class InnerPcImpl : public InnerPc {
     p_.resize(n);
     q_.resize(n);
     zt_.resize(n);
-    for (auto *v : {&rz_, &pq_, &zq_, &alpha_, &beta_, &rr_, &ones_}) v->resize(nl);
+    for (auto *v : {&rz_, &pq_, &zq_, &alpha_, &beta_, &rr_, &bb_, &ones_}) v->resize(nl);
+    h_rr_.resize(nl);
+    h_act_.resize(nl);
     active_.resize(nl);
     work_.resize(dot_work_size(ne_, nc, sg_) * 8);
     std::vector<double> one(nl, 1.0);
@@ -805,7 +811,6 @@ class InnerPcImpl : public InnerPc {
     if (a32_) a32_->ensure(nc);
     if (a64_) a64_->ensure(nc);
     h_bb_.resize(nl);
-    h_rr_.resize(nl);
     h_active_.resize(nl);
   }
 
@@ -843,8 +848,8 @@ class InnerPcImpl : public InnerPc {
     // x = 0, r = b
     EMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * (size_t)ne_ * nc, s));
     EMF_HIP(hipMemcpyAsync(r, b, sizeof(double) * (size_t)ne_ * nc, hipMemcpyDeviceToDevice, s));
-    coldot<double>(s, ne_, nc, b, nc, b, nc, rr_.get(), w, sg_);
-    EMF_HIP(hipMemcpyAsync(h_bb_.data(), rr_.get(), sizeof(double) * nl, hipMemcpyDeviceToHost, s));
+    coldot<double>(s, ne_, nc, b, nc, b, nc, bb_.get(), w, sg_);
+    EMF_HIP(hipMemcpyAsync(h_bb_.data(), bb_.get(), sizeof(double) * nl, hipMemcpyDeviceToHost, s));
     EMF_HIP(hipStreamSynchronize(s));
     bool any = false;
     for (int c = 0; c < nl; ++c) {
@@ -853,6 +858,9 @@ class InnerPcImpl : public InnerPc {
     }
     if (!any) return;
     active_.upload(h_active_.data(), nl, s);
+    // the convergence test runs on the device (fcg_residual), which writes r . r and
+    // the flags straight into pinned host memory: one synchronisation per iteration
+    // and no copies
     const double tol2 = o_.inner_rtol * o_.inner_rtol;
     // z = M r ; p = z ; q = B p.  With an fp32 z on 16 or more columns of one segment
     // the product is formed from z instead, q = B z + beta q (beta per column in the
@@ -869,27 +877,25 @@ class InnerPcImpl : public InnerPc {
     coldot<double, TZ>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
     // x += alpha p is delayed into the pass that forms the next p (or done on exit)
     for (int it = 1; it <= o_.inner_max_it; ++it) {
-      fcg_alpha(s, nl, rz_.get(), pq_.get(), active_.get(), alpha_.get());
-      fcg_xr<double, double>(s, ne_, nc, alpha_.get(), nullptr, q, nullptr, r, rr_.get(), w, sg_);
+      fcg_residual(s, ne_, nc, rz_.get(), pq_.get(), active_.get(), alpha_.get(), q, r, rr_.get(), bb_.get(), tol2,
+                   h_rr_.get(), h_act_.get(), w, sg_);
       ++st.inner_its;
-      EMF_HIP(hipMemcpyAsync(h_rr_.data(), rr_.get(), sizeof(double) * nl, hipMemcpyDeviceToHost, s));
       EMF_HIP(hipStreamSynchronize(s));
       any = false;
       for (int c = 0; c < nl; ++c) {
         st.inner_col_its += h_active_[c] != 0;
-        if (h_active_[c] && h_rr_[c] <= tol2 * h_bb_[c]) h_active_[c] = 0;
+        h_active_[c] = h_act_[c];
         any |= h_active_[c] != 0;
       }
       if (!any || it == o_.inner_max_it) {
         axpby_cols<double, double>(s, ne_, nc, alpha_.get(), p, nc, ones_.get(), x, nc, sg_);
         break;
       }
-      active_.upload(h_active_.data(), nl, s);
       precondition(s, nc, r, z, st);
       coldot2<TZ>(s, ne_, nc, z, nc, q, r, nc, zq_.get(), rz_.get(), w, sg_);
-      fcg_beta(s, nl, zq_.get(), pq_.get(), active_.get(), beta_.get());
-      fcg_px<TZ>(s, ne_, nc, alpha_.get(), beta_.get(), z, p, x, sg_);
+      fcg_px<TZ>(s, ne_, nc, alpha_.get(), zq_.get(), pq_.get(), active_.get(), z, p, x, sg_);
       if (from_z) {
+        fcg_beta(s, nl, zq_.get(), pq_.get(), active_.get(), beta_.get());
         ChebFuse<double> zb;
         zb.zcol = beta_.get();
         zb.ld = nc;
@@ -907,6 +913,7 @@ class InnerPcImpl : public InnerPc {
   }
 
   const AmsHost &H_;
+  Col16 c16_;
   Seg sg_;
   int ne_ = 0, nnz_ = 0, nc_ = 0;
   const int *rp_, *ci_;
@@ -917,17 +924,19 @@ class InnerPcImpl : public InnerPc {
   std::unique_ptr<DevAms<double>> a64_;
   DevBuf<double> r_, z_, p_, q_;
   DevBuf<float> zt_;
-  DevBuf<double> rz_, pq_, zq_, alpha_, beta_, rr_, ones_, work_;
+  DevBuf<double> rz_, pq_, zq_, alpha_, beta_, rr_, bb_, ones_, work_;
   DevBuf<uint8_t> active_;
-  std::vector<double> h_bb_, h_rr_;
+  HostPinned<double> h_rr_;
+  HostPinned<uint8_t> h_act_;
+  std::vector<double> h_bb_;
   std::vector<uint8_t> h_active_;
 };
 
 }  // namespace
 
 std::unique_ptr<InnerPc> make_inner_pc(bool fp32, const AmsHost &H, int ne, int nnz, const int *d_rp, const int *d_ci,
-                                       const double2 *d_K, const PcOptions &o, hipStream_t s) {
-  return std::unique_ptr<InnerPc>(new InnerPcImpl(fp32, H, ne, nnz, d_rp, d_ci, d_K, o, s));
+                                       const double2 *d_K, const PcOptions &o, hipStream_t s, const Col16 &c16) {
+  return std::unique_ptr<InnerPc>(new InnerPcImpl(fp32, H, ne, nnz, d_rp, d_ci, d_K, o, s, c16));
 }
 
 // ---------------------------------------------------------------------------------

@@ -2068,10 +2068,13 @@ __global__ void k_axpby(int rows, int nc, double alpha, const TX *__restrict__ x
   y[(size_t)r * ldy + c] = (TY)v;
 }
 
-template <typename T, typename TP>
-__global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, const double *__restrict__ alpha,
-                                                   const TP *__restrict__ p, const TP *__restrict__ q, T *x, T *r,
-                                                   double *work, Seg sg) {
+// The residual step of the flexible CG: alpha from rz, pq and the active flag (every
+// block of a segment forms the same value and stores it), r -= alpha q, and the block
+// partials of r . r in work[block][c].  A chunk of rows lies in one segment.
+__global__ __launch_bounds__(kBlock) void k_fcg_res(int rows, int nc, const double *__restrict__ rz,
+                                                    const double *__restrict__ pq,
+                                                    const uint8_t *__restrict__ active, double *alpha,
+                                                    const double *__restrict__ q, double *r, double *work, Seg sg) {
   __shared__ double sh[kBlock];
   const int rpp = kBlock / nc;
   const int per_block = rpp * nc;
@@ -2081,28 +2084,25 @@ __global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, const doubl
   double acc = 0.0;
   if (t < per_block && r0 < r1) {
     const int c = t % nc;
-    const double a = alpha[scalar_index(sg.rseg, r0, nc, c)];  // a chunk lies in one segment
+    const int k = scalar_index(sg.rseg, r0, nc, c);
+    const double a = (active[k] && pq[k] != 0.0) ? rz[k] / pq[k] : 0.0;
+    if (t < nc) alpha[k] = a;
     // four rows per trip, every load issued first (clamped rows, masked updates)
     for (int row = r0 + t / nc; row < r1; row += 4 * rpp) {
-      double rv[4], qv[4], xv[4], pv[4];
+      double rv[4], qv[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const size_t o = (size_t)min(row + u * rpp, r1 - 1) * nc + c;
-        rv[u] = (double)r[o];
-        qv[u] = (double)q[o];
-        if (x) {
-          xv[u] = (double)x[o];
-          pv[u] = (double)p[o];
-        }
+        rv[u] = r[o];
+        qv[u] = q[o];
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
         if (row + u * rpp < r1) {
           const size_t o = (size_t)(row + u * rpp) * nc + c;
-          if (x) x[o] = (T)(xv[u] + a * pv[u]);
-          const T nr = (T)(rv[u] - a * qv[u]);
+          const double nr = rv[u] - a * qv[u];
           r[o] = nr;
-          acc += (double)nr * (double)nr;
+          acc += nr * nr;
         }
     }
   }
@@ -2115,18 +2115,46 @@ __global__ __launch_bounds__(kBlock) void k_fcg_xr(int rows, int nc, const doubl
   }
 }
 
+// k_finish for r . r followed by the convergence test of its logical column: the
+// flag drops once rr <= tol2 * bb; rr and the flag also go to the pinned host arrays.
+__global__ __launch_bounds__(kBlock) void k_finish_conv(int nblocks, int nc, const double *__restrict__ work,
+                                                        double *out, Seg sg, const double *__restrict__ bb,
+                                                        double tol2, uint8_t *active, double *h_out,
+                                                        uint8_t *h_active) {
+  __shared__ double sh[kBlock];
+  const int idx = blockIdx.x;
+  const int sgi = idx / nc, c = idx % nc;
+  const int b0 = sg.seg_chunk ? sg.seg_chunk[sgi] : 0, b1 = sg.seg_chunk ? sg.seg_chunk[sgi + 1] : nblocks;
+  double s = 0.0;
+  for (int b = b0 + threadIdx.x; b < b1; b += kBlock) s += work[(size_t)b * nc + c];
+  sh[threadIdx.x] = s;
+  block_tree_sum(sh);
+  if (threadIdx.x == 0) {
+    const double v = sh[0];
+    uint8_t a = active[idx];
+    if (a && v <= tol2 * bb[idx]) a = 0;
+    out[idx] = v;
+    active[idx] = a;
+    h_out[idx] = v;
+    h_active[idx] = a;
+  }
+}
+
 // The delayed solution update of the flexible CG and the next search direction in
-// one pass: x += alpha p ; p = z + beta p (per (segment, column) scalars).
+// one pass: x += alpha p ; p = z + beta p (per (segment, column) scalars; beta formed
+// here from zq, pq and the active flag).
 template <typename TZ>
-__global__ void k_fcg_px(int rows, int nc, const double *__restrict__ alpha, const double *__restrict__ beta,
+__global__ void k_fcg_px(int rows, int nc, const double *__restrict__ alpha, const double *__restrict__ zq,
+                         const double *__restrict__ pq, const uint8_t *__restrict__ active,
                          const TZ *__restrict__ z, double *p, double *x, const int *rseg) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= (int64_t)rows * nc) return;
   const int r = (int)(i / nc), c = (int)(i % nc);
   const int k = scalar_index(rseg, r, nc, c);
-  const double pv = p[i];
-  x[i] = x[i] + alpha[k] * pv;
-  p[i] = (double)z[i] + beta[k] * pv;
+  const double pv = p[i], xv = x[i], zv = (double)z[i];
+  const double beta = (active[k] && pq[k] != 0.0) ? -zq[k] / pq[k] : 0.0;
+  x[i] = xv + alpha[k] * pv;
+  p[i] = zv + beta * pv;
 }
 
 template <typename T>
@@ -2147,27 +2175,29 @@ __global__ void k_diag_scale(int64_t n, int nc, const T *__restrict__ dinv, cons
 }
 
 template <typename T>
-// One block per column c: b[:, c] is staged in LDS, thread i accumulates row i
-// reading the transposed inverse AinvT[j][i] (consecutive threads, consecutive i:
-// coalesced).  n <= kDenseMax.
+// One block per (64 rows, column c): b[:, c] is staged in LDS; lane l of wavefront w
+// sums row i0 + l over the w-th quarter of j, reading the transposed inverse
+// AinvT[j][i] (consecutive lanes, consecutive i: coalesced), and the four quarters
+// are added in a fixed order.  n <= kDenseMax.
 __global__ __launch_bounds__(kBlock) void k_dense_solve(int n, int nc, const double *__restrict__ AinvT,
                                                         const T *__restrict__ b, T *x) {
+  static_assert(kBlock == 256, "four wavefronts per block");
   __shared__ double bs[kDenseMax];
-  const int c = blockIdx.x;
+  __shared__ double part[4][64];
+  const int c = blockIdx.y;
   for (int j = threadIdx.x; j < n; j += kBlock) bs[j] = (double)b[(size_t)j * nc + c];
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += kBlock) {
-    double s = 0.0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int q = (n + 3) / 4, j0 = w * q, j1 = min(n, j0 + q);
+  double s = 0.0;
+  if (i < n) {
 #pragma unroll 8
-    for (int j = 0; j < n; ++j) s += AinvT[(size_t)j * n + i] * bs[j];
-    x[(size_t)i * nc + c] = (T)s;
+    for (int j = j0; j < j1; ++j) s += AinvT[(size_t)j * n + i] * bs[j];
   }
-}
-
-__global__ void k_fcg_alpha(int nc, const double *rz, const double *pq, const uint8_t *active, double *alpha) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= nc) return;
-  alpha[c] = (active[c] && pq[c] != 0.0) ? rz[c] / pq[c] : 0.0;
+  part[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && i < n) x[(size_t)i * nc + c] = (T)(((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
 }
 
 __global__ void k_fcg_beta(int nc, const double *zq, const double *pq, const uint8_t *active, double *beta) {
@@ -2219,34 +2249,30 @@ template void axpby<double, double>(hipStream_t, int, int, double, const double 
 template void axpby<double, float>(hipStream_t, int, int, double, const double *, int, double, float *, int);
 template void axpby<float, double>(hipStream_t, int, int, double, const float *, int, double, double *, int);
 
-template <typename T, typename TP>
-void fcg_xr(hipStream_t s, int rows, int nc, const double *alpha, const TP *p, const TP *q, T *x, T *r, double *rr,
-            double *work, const Seg &sg) {
+void fcg_residual(hipStream_t s, int rows, int nc, const double *rz, const double *pq, uint8_t *active,
+                  double *alpha, const double *q, double *r, double *rr, const double *bb, double tol2, double *h_rr,
+                  uint8_t *h_active, double *work, const Seg &sg) {
   EMF_REQUIRE(nc <= kBlock, kErrArg, "inner solve supports at most 256 columns");
   const int nb = reduce_blocks(rows, nc, sg);
-  hipLaunchKernelGGL((k_fcg_xr<T, TP>), dim3(nb), dim3(kBlock), 0, s, rows, nc, alpha, p, q, x, r, work, sg);
-  hipLaunchKernelGGL(k_finish, dim3(sg.nseg * nc), dim3(kBlock), 0, s, nb, nc, work, rr, sg);
+  hipLaunchKernelGGL(k_fcg_res, dim3(nb), dim3(kBlock), 0, s, rows, nc, rz, pq, active, alpha, q, r, work, sg);
+  hipLaunchKernelGGL(k_finish_conv, dim3(sg.nseg * nc), dim3(kBlock), 0, s, nb, nc, work, rr, sg, bb, tol2, active,
+                     h_rr, h_active);
   EMF_HIP(hipGetLastError());
 }
-template void fcg_xr<float, float>(hipStream_t, int, int, const double *, const float *, const float *, float *,
-                                   float *, double *, double *, const Seg &);
-template void fcg_xr<double, double>(hipStream_t, int, int, const double *, const double *, const double *, double *,
-                                     double *, double *, double *, const Seg &);
-template void fcg_xr<double, float>(hipStream_t, int, int, const double *, const float *, const float *, double *,
-                                    double *, double *, double *, const Seg &);
 
 template <typename TZ>
-void fcg_px(hipStream_t s, int rows, int nc, const double *alpha, const double *beta, const TZ *z, double *p, double *x,
-            const Seg &sg) {
+void fcg_px(hipStream_t s, int rows, int nc, const double *alpha, const double *zq, const double *pq,
+            const uint8_t *active, const TZ *z, double *p, double *x, const Seg &sg) {
   const int64_t n = (int64_t)rows * nc;
   if (n == 0) return;
-  hipLaunchKernelGGL((k_fcg_px<TZ>), dim3(grid_for(n)), dim3(kBlock), 0, s, rows, nc, alpha, beta, z, p, x, sg.rseg);
+  hipLaunchKernelGGL((k_fcg_px<TZ>), dim3(grid_for(n)), dim3(kBlock), 0, s, rows, nc, alpha, zq, pq, active, z, p, x,
+                     sg.rseg);
   EMF_HIP(hipGetLastError());
 }
-template void fcg_px<float>(hipStream_t, int, int, const double *, const double *, const float *, double *, double *,
-                            const Seg &);
-template void fcg_px<double>(hipStream_t, int, int, const double *, const double *, const double *, double *, double *,
-                             const Seg &);
+template void fcg_px<float>(hipStream_t, int, int, const double *, const double *, const double *, const uint8_t *,
+                            const float *, double *, double *, const Seg &);
+template void fcg_px<double>(hipStream_t, int, int, const double *, const double *, const double *, const uint8_t *,
+                             const double *, double *, double *, const Seg &);
 
 template <typename T>
 void cheb_first(hipStream_t s, int rows, int nc, const T *dinv, const double *b, T *bt, T *x, double inv_theta) {
@@ -2263,7 +2289,9 @@ void diag_scale(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x
 template <typename T>
 void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T *x) {
   EMF_REQUIRE(n <= kDenseMax, kErrArg, "dense coarse solve larger than its LDS stage");
-  hipLaunchKernelGGL((k_dense_solve<T>), dim3(nc), dim3(kBlock), 0, s, n, nc, Ainv, b, x);
+  EMF_REQUIRE(nc <= 65535, kErrArg, "dense coarse solve: at most 65535 columns");
+  if (n <= 0 || nc <= 0) return;
+  hipLaunchKernelGGL((k_dense_solve<T>), dim3((n + 63) / 64, nc), dim3(kBlock), 0, s, n, nc, Ainv, b, x);
   EMF_HIP(hipGetLastError());
 }
 
@@ -2275,10 +2303,6 @@ EMF_INST(float)
 EMF_INST(double)
 #undef EMF_INST
 
-void fcg_alpha(hipStream_t s, int nc, const double *rz, const double *pq, const uint8_t *active, double *alpha) {
-  hipLaunchKernelGGL(k_fcg_alpha, dim3(grid_for(nc)), dim3(kBlock), 0, s, nc, rz, pq, active, alpha);
-  EMF_HIP(hipGetLastError());
-}
 void fcg_beta(hipStream_t s, int nc, const double *zq, const double *pq, const uint8_t *active, double *beta) {
   hipLaunchKernelGGL(k_fcg_beta, dim3(grid_for(nc)), dim3(kBlock), 0, s, nc, zq, pq, active, beta);
   EMF_HIP(hipGetLastError());
