@@ -482,6 +482,8 @@ def main():
                     help="row mapping of the smoother SpMM (auto: autotuned)")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of hipGraph-captured cycles (for kernel tracers)")
+    ap.add_argument("--options", default="",
+                    help="further library options for the headline's mixed context (e.g. '-ams_cheb_degree 6')")
     ap.add_argument("--pc", default="presb", choices=["presb", "blockdiag"],
                     help="outer preconditioner of the mixed/batched path (the plain paths run blockdiag)")
     args = ap.parse_args()
@@ -498,8 +500,8 @@ def main():
 
     t0 = time.perf_counter()
     m = mg.adaptive_mt(cells=(args.cells, args.cells, args.cells))
-    ctx = context(api, "-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6 -use_graph %s -spmm_rows %s %s"
-                      % ("false" if args.no_graph else "true", args.spmm_rows, mixed_pc()))
+    ctx = context(api, "-inner_precision fp32 -e_rtol 1e-8 -dual_rtol 1e-6 -use_graph %s -spmm_rows %s %s %s"
+                      % ("false" if args.no_graph else "true", args.spmm_rows, mixed_pc(), args.options))
     ctx.load_model(m)
     ctx.create_linear_system(3)
     info = ctx.mesh_info()

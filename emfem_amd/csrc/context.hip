@@ -85,7 +85,7 @@ const std::map<std::string, std::string> &option_defaults() {
       // row-block SpMMs of B on 16-bit columns (offsets from each 64-nonzero group's
       // smallest column) when every group's columns span at most 65536
       {"spmm_col16", "true"},
-      {"ams_cheb_degree", "5"},
+      {"ams_cheb_degree", "4"},
       {"ams_cheb_ratio", "60"},
       {"ams_cheb_kind", "first"},
       {"ams_theta", "0.08"},

@@ -38,7 +38,7 @@ double time_operator(hipStream_t s, int ne, int nnz, const int *rp, const int *c
                      const double *x, double *y, int reps);
 
 struct PcOptions {
-  int cheb_degree = 5;
+  int cheb_degree = 4;
   double cheb_ratio = 60.0;
   bool cheb_fourth = false;  // fourth-kind Chebyshev (no ratio) instead of first kind
   double inner_rtol = 1e-2;

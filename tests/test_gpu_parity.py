@@ -246,6 +246,30 @@ def test_structured_transfers_match_csr(tiny, precision, cycle):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_structured_transfers_match_csr_on_3_columns(tiny, precision, oracle_mod):
+    """On 3 real columns (the PRESB halves of three right-hand sides) the structured
+    prolongation runs a lane per edge for all columns: the solves with the structured
+    transfers and with the CSR matrices of Pi both reach the exact discrete solution."""
+    m, om = tiny
+    for tr in ("csr", "structured"):
+        ctx = _ctx(m, "-inner_precision %s -A_pc_type presb -ams_transfer %s" % (precision, tr))
+        ctx.create_linear_system(3)
+        ctx.assemble_matrix(0)
+        ctx.assemble_rhs_mt(0, api.XY_POLAR, 0)
+        ctx.assemble_rhs_mt(0, api.YX_POLAR, 1)
+        ctx.assemble_rhs_mt(0, api.XY_POLAR, 2)
+        ctx.create_pc()
+        its, rr = ctx.solve_linear_system(0, 3, 300, 1e-10)
+        assert np.all(rr <= 1e-10 * 1.0001), (tr, rr)
+        rp, ci, C, M = ctx.get_matrix()
+        for s in range(3):
+            e = oracle_mod.solve_direct(rp, ci, C, M, *ctx.get_rhs(s))
+            g = ctx.get_solution(s)
+            assert np.linalg.norm(g - e) <= 1e4 * 1e-10 * np.linalg.norm(e), (tr, s)
+        ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
 @pytest.mark.parametrize("cycle", ["additive", "multiplicative"])
 def test_device_setup_matches_host(tiny, precision, cycle):
     """The auxiliary-space Galerkin products and the l1 scaling formed on the device
