@@ -120,6 +120,9 @@ struct emfem_context {
   DevGalerkin gal;
   std::shared_ptr<const AmsAux> ams_aux;
   std::unique_ptr<InnerPc> pc;
+  // the preconditioner of an earlier matrix of this linear system (parked by
+  // emfem_assemble_matrix): not applied, only refreshed by the next emfem_create_pc
+  std::unique_ptr<InnerPc> pc_prev;
 
   std::map<std::string, double> stats;
   double assemble_kernel_ms = 0.0;

@@ -85,6 +85,10 @@ const std::map<std::string, std::string> &option_defaults() {
       // row-block SpMMs of B on 16-bit columns (offsets from each 64-nonzero group's
       // smallest column) when every group's columns span at most 65536
       {"spmm_col16", "true"},
+      // a new preconditioner set-up on the same linear system keeps the device state
+      // that depends only on the mesh (buffers, structured transfers, tuned row
+      // mappings) and refreshes the values
+      {"pc_reuse", "true"},
       {"ams_cheb_degree", "4"},
       {"ams_cheb_ratio", "60"},
       {"ams_cheb_kind", "first"},
@@ -112,6 +116,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"batch_retire", {"true", "false"}},
       {"use_graph", {"true", "false"}},
       {"spmm_col16", {"true", "false"}},
+      {"pc_reuse", {"true", "false"}},
       {"spmm_rows", {"auto", "lanes", "rowblock", "pipelined", "xblock", "2", "4", "8", "16"}},
       {"ams_cycle", {"multiplicative", "additive"}},
       {"ams_transfer", {"structured", "csr"}},
@@ -181,6 +186,7 @@ int emfem_destroy_context(emfem_context *c) {
   if (!c) return EMFEM_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->pc.reset();
+  c->pc_prev.reset();
   hipStream_t s = c->stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);
@@ -239,6 +245,7 @@ int emfem_set_mesh(emfem_context *c, int n_points, const double *points, int n_t
   return guarded(c, [&]() {
     EMF_REQUIRE(n_points > 0 && n_tets > 0 && points && tets && attributes, kErrArg, "empty or null mesh arrays");
     c->pc.reset();
+    c->pc_prev.reset();
     c->have_system = false;
     c->have_mesh = false;
     c->bg = Background();
@@ -261,6 +268,7 @@ int emfem_set_mesh_edges(emfem_context *c, int n_edges, const int *edgelist, con
     require_mesh(c);
     EMF_REQUIRE(edgelist && tet2edge, kErrArg, "null edge arrays");
     c->pc.reset();
+    c->pc_prev.reset();
     c->have_system = false;
     set_user_edges(c->T, n_edges, edgelist, tet2edge);
     upload_mesh(c);
@@ -354,6 +362,7 @@ int emfem_create_linear_system(emfem_context *c, int n_slots) {
     EMF_REQUIRE(n_slots > 0 && n_slots <= 512, kErrArg, "n_slots must be in 1..512");
     const double t0 = now_ms();
     c->pc.reset();
+    c->pc_prev.reset();
     c->ams_aux.reset();
     c->h_B = HostCsr();
     c->d_sb.release();
@@ -386,6 +395,7 @@ int emfem_create_linear_system(emfem_context *c, int n_slots) {
 int emfem_destroy_linear_system(emfem_context *c) {
   return guarded(c, [&]() {
     c->pc.reset();
+    c->pc_prev.reset();
     c->ams_aux.reset();
     c->h_B = HostCsr();
     c->d_rp.release();
@@ -435,7 +445,9 @@ int emfem_assemble_matrix(emfem_context *c, int fidx) {
     c->stats["assemble_ms"] = now_ms() - t0;
     c->stats["assemble_kernel_ms"] = ms;
     c->assembled_fidx = fidx;
-    c->pc.reset();
+    // the preconditioner no longer matches the matrix; its device state is kept for a
+    // refresh by the next emfem_create_pc
+    if (c->pc) c->pc_prev = std::move(c->pc);
   });
 }
 
@@ -565,7 +577,11 @@ int emfem_create_pc(emfem_context *c) {
     }
     EMF_HIP(hipStreamSynchronize(s));
     PcOptions o = pc_options(c);
-    c->pc.reset();
+    // the device side of the previous set-up is refreshed with the new values when it
+    // was built for this linear system's pattern and auxiliary spaces (-pc_reuse)
+    std::unique_ptr<InnerPc> prev = c->pc ? std::move(c->pc) : std::move(c->pc_prev);
+    c->pc_prev.reset();
+    if (c->opt("pc_reuse") != "true") prev.reset();
     HostCsr recycle;
     std::shared_ptr<const AmsAux> recycle_aux;
     if (c->ams_host && !c->ams_host->amgPi.levels.empty()) {
@@ -623,8 +639,15 @@ int emfem_create_pc(emfem_context *c) {
       c16.off = c->d_ci16.get();
       c16.base = c->d_ci16_base.get();
     }
-    c->pc = make_inner_pc(c->opt("inner_precision") == "fp32", *c->ams_host, c->T.ne, c->nnz, c->d_rp.get(),
-                          c->d_ci.get(), c->d_K.get(), o, s, c16);
+    if (prev && prev->refresh(*c->ams_host, c->d_K.get(), o, s)) {
+      c->pc = std::move(prev);
+      c->stats["pc_reused"] = 1.0;
+    } else {
+      prev.reset();
+      c->pc = make_inner_pc(c->opt("inner_precision") == "fp32", *c->ams_host, c->T.ne, c->nnz, c->d_rp.get(),
+                            c->d_ci.get(), c->d_K.get(), o, s, c16);
+      c->stats["pc_reused"] = 0.0;
+    }
     EMF_HIP(hipStreamSynchronize(s));
     c->stats["spmm_col16"] = c->pc->col16() ? 1.0 : 0.0;
     c->stats["pc_setup_host_ms"] = t1 - t0;
@@ -635,6 +658,7 @@ int emfem_create_pc(emfem_context *c) {
 int emfem_destroy_pc(emfem_context *c) {
   return guarded(c, [&]() {
     c->pc.reset();
+    c->pc_prev.reset();
     c->ams_host.reset();
   });
 }

@@ -89,6 +89,11 @@ class InnerPc {
   virtual double index_bytes(hipStream_t s, int nc, bool fcg) = 0;
   // whether B's row-block SpMMs read the 16-bit column form
   virtual bool col16() const = 0;
+  // Takes the set-up H of the next frequency of the same linear system (same pattern,
+  // same auxiliary spaces) and the new (C, M) values K, keeping the device state that
+  // depends only on the mesh; false (the preconditioner is then unusable) when H or
+  // the options o differ in what that state was built for.
+  virtual bool refresh(const AmsHost &H, const double2 *K, const PcOptions &o, hipStream_t s) = 0;
 };
 
 // The 16-bit column form of B's pattern (compress_columns) when its owner built it

@@ -97,6 +97,7 @@ template <typename T>
 class DevAmg {
  public:
   void upload(const AmgHost &H, hipStream_t s) {
+    nc_ = 0;  // the level blocks follow the (possibly new) shapes on the next ensure
     levels_.resize(H.levels.size());
     for (size_t l = 0; l < H.levels.size(); ++l) {
       levels_[l].A.upload(H.levels[l].A, s);
@@ -175,6 +176,9 @@ class DevAms {
   void setup(const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K, const PcOptions &o,
              hipStream_t s, const Col16 &c16) {
     ne_ = ne;
+    nnz_ = nnz;
+    rp_ = rp;
+    aux_ = H.aux.get();
     auto view = [&](const void *v) {
       Csr<T> c;
       c.nrows = c.ncols = ne;
@@ -442,6 +446,31 @@ class DevAms {
 
   bool col16() const { return B_.ci16 != nullptr; }
 
+  // New values on the pattern and auxiliary spaces this cycle was set up for (the
+  // next frequency of the same linear system): B in both precisions, the l1 scaling
+  // and the AMG hierarchies.  The mesh-only data (row pointers, 16-bit columns,
+  // structured transfers), the tuned row mappings and the scratch blocks are kept;
+  // captured cycles are dropped (the hierarchy's shapes may change).  False when H
+  // was built on other auxiliary spaces.
+  bool refresh(const AmsHost &H, const double2 *K, hipStream_t s) {
+    if (H.aux.get() != aux_ || H.aux->additive != additive_) return false;
+    drop_graphs();
+    make_b<T>(s, nnz_, K, bvals_.get());
+    if constexpr (!std::is_same<T, double>::value) make_b<double>(s, nnz_, K, b64vals_.get());
+    if (H.l1inv.empty())
+      l1_inverse_dev<T>(s, ne_, rp_, B64_.v, l1inv_.get());
+    else
+      upload_vec(l1inv_, H.l1inv, s);
+    if (!additive_) amgG_.upload(H.amgG, s);
+    amgPi_.upload(H.amgPi, s);
+    if (nc_) {
+      if (!additive_) amgG_.ensure(nc_);
+      amgPi_.ensure(nc_);
+    }
+    EMF_HIP(hipStreamSynchronize(s));
+    return true;
+  }
+
   // Bytes of column index per nonzero that the smoother SpMM (fcg false) or the fp64
   // FCG product (fcg true) streams on nc columns: 2 + 4 / 64 for the row blocks on
   // 16-bit columns, else 4.
@@ -690,7 +719,9 @@ class DevAms {
     spmm<T>(s, G0_.view(), nc, vx_.get(), nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), x, nc);
   }
 
-  int ne_ = 0, nv0_ = 0, nc_ = 0, deg_ = 3, nblk_ = 3, spmm_rows_ = 0;
+  int ne_ = 0, nnz_ = 0, nv0_ = 0, nc_ = 0, deg_ = 3, nblk_ = 3, spmm_rows_ = 0;
+  const int *rp_ = nullptr;
+  const AmsAux *aux_ = nullptr;
   bool additive_ = false;
   double c0_ = 1.0;
   std::vector<double> c1_, c2_;
@@ -729,7 +760,7 @@ class InnerPcImpl : public InnerPc {
  public:
   InnerPcImpl(bool fp32, const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K,
               const PcOptions &o, hipStream_t s, const Col16 &c16)
-      : H_(H), ne_(ne), nnz_(nnz), rp_(rp), ci_(ci), K_(K), o_(o), c16_(c16) {
+      : H_(&H), ne_(ne), nnz_(nnz), rp_(rp), ci_(ci), K_(K), o_(o), c16_(c16) {
     set_fp32(fp32, s);
   }
 
@@ -760,12 +791,12 @@ class InnerPcImpl : public InnerPc {
     fp32_ = fp32;
     if (fp32 && !a32_) {
       a32_.reset(new DevAms<float>());
-      a32_->setup(H_, ne_, nnz_, rp_, ci_, K_, o_, s, c16_);
+      a32_->setup(*H_, ne_, nnz_, rp_, ci_, K_, o_, s, c16_);
       if (nc_) a32_->ensure(nc_);
     }
     if (!fp32 && !a64_) {
       a64_.reset(new DevAms<double>());
-      a64_->setup(H_, ne_, nnz_, rp_, ci_, K_, o_, s, c16_);
+      a64_->setup(*H_, ne_, nnz_, rp_, ci_, K_, o_, s, c16_);
       if (nc_) a64_->ensure(nc_);
     }
   }
@@ -775,12 +806,30 @@ class InnerPcImpl : public InnerPc {
     solve(s, nc, b, x, st);
   }
 
+  bool refresh(const AmsHost &H, const double2 *K, const PcOptions &o, hipStream_t s) override {
+    if (!same_options(o)) return false;
+    if (a32_ && !a32_->refresh(H, K, s)) return false;
+    if (a64_ && !a64_->refresh(H, K, s)) return false;
+    H_ = &H;
+    K_ = K;
+    o_.use_graph = o.use_graph;
+    return true;
+  }
+
   void set_segments(const Seg &sg) override {
     sg_ = sg;
     nc_ = 0;  // scalar buffers are sized per logical column
   }
 
  private:
+  // the options the device set-up depends on
+  bool same_options(const PcOptions &o) const {
+    return o.cheb_degree == o_.cheb_degree && o.cheb_ratio == o_.cheb_ratio && o.cheb_fourth == o_.cheb_fourth &&
+           o.inner_rtol == o_.inner_rtol && o.inner_max_it == o_.inner_max_it && o.amg.theta == o_.amg.theta &&
+           o.amg.smooth_levels == o_.amg.smooth_levels && o.amg.additive == o_.amg.additive &&
+           o.spmm_rows == o_.spmm_rows && o.aux_csr == o_.aux_csr && o.col16 == o_.col16;
+  }
+
   Csr<double> B64(hipStream_t s, int nc) {
     if (a64_) {
       a64_->tune(s, nc);
@@ -912,7 +961,7 @@ class InnerPcImpl : public InnerPc {
     spmm<double>(s, B64(s, nc), nc, x, nc, y, nc, 1.0, nullptr, 0.0, nullptr, 0, 0.0, nullptr, 0);
   }
 
-  const AmsHost &H_;
+  const AmsHost *H_;
   Col16 c16_;
   Seg sg_;
   int ne_ = 0, nnz_ = 0, nc_ = 0;
