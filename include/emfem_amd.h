@@ -129,7 +129,10 @@ int emfem_create_linear_system(emfem_context *ctx, int n_slots);
 /* destroy_linear_system (src/em_fwd.cc:297-316). */
 int emfem_destroy_linear_system(emfem_context *ctx);
 
-/* assemble_matrix (src/em_fwd.cc:318-400): C and M on the device for frequency fidx. */
+/* assemble_matrix (src/em_fwd.cc:318-400): C and M on the device for frequency fidx.
+ * The preconditioner of the previous matrix is no longer applied (a solve needs
+ * emfem_create_pc again); with -pc_reuse true its device state is kept for that call
+ * to refresh. */
 int emfem_assemble_matrix(emfem_context *ctx, int fidx);
 
 /* Copies the assembled CSR to the host: rowptr [n_edges+1], col [nnz], C [nnz],
@@ -152,7 +155,10 @@ int emfem_get_solution(const emfem_context *ctx, int slot, double *re, double *i
 
 /* create_pc (src/em_la.cc:113-191): B = C + M, the auxiliary-space (AMS)
  * preconditioner for B and the outer FGMRES set-up.  Precision of the inner solve
- * follows the option -inner_precision (fp32 | fp64). */
+ * follows the option -inner_precision (fp32 | fp64).  On a linear system whose
+ * previous preconditioner was built with the same set-up options, its device state
+ * is refreshed with the new values rather than built again (-pc_reuse, stat
+ * "pc_reused"). */
 int emfem_create_pc(emfem_context *ctx);
 
 /* destroy_pc (src/em_la.cc:193-206). */
