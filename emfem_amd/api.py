@@ -65,6 +65,7 @@ SIGNATURES = {
     "emfem_get_rhs": (I, [P, I, P, P]),
     "emfem_get_solution": (I, [P, I, P, P]),
     "emfem_create_pc": (I, [P]),
+    "emfem_prepare_pc": (I, [P, I]),
     "emfem_destroy_pc": (I, [P]),
     "emfem_solve_linear_system": (I, [P, I, I, I, D, P, P, P]),
     "emfem_matshell_mult_a": (I, [P, I, P, P]),
@@ -300,6 +301,12 @@ class Context:
 
     def create_pc(self):
         self._check(self._L.emfem_create_pc(self._h))
+
+    def prepare_pc(self, fidx):
+        """Start the set-up for frequency fidx ahead of time (emfem_prepare_pc): the
+        next assemble_matrix(fidx) and create_pc take the prepared matrix and host
+        AMG set-up."""
+        self._check(self._L.emfem_prepare_pc(self._h, int(fidx)))
 
     def destroy_pc(self):
         self._check(self._L.emfem_destroy_pc(self._h))

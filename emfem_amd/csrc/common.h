@@ -129,9 +129,18 @@ inline double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// Upper bound on the threads of the host loops started from the calling thread (0 =
+// none): a set-up running beside a solve leaves cores to the thread that drives it.
+inline int &host_thread_cap() {
+  static thread_local int cap = 0;
+  return cap;
+}
+
 inline int host_threads() {
   unsigned n = std::thread::hardware_concurrency();
-  return (int)std::max(1u, std::min(n, 16u));
+  int t = (int)std::max(1u, std::min(n, 16u));
+  if (host_thread_cap() > 0) t = std::min(t, host_thread_cap());
+  return t;
 }
 
 // Static-partition parallel loop over [0, n): f(begin, end, thread_index).

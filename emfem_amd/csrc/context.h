@@ -8,9 +8,11 @@
 #include <cmath>
 
 #include <complex>
+#include <exception>
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "amg.h"
@@ -124,6 +126,28 @@ struct emfem_context {
   // emfem_assemble_matrix): not applied, only refreshed by the next emfem_create_pc
   std::unique_ptr<InnerPc> pc_prev;
 
+  // A set-up started ahead of its frequency (emfem_prepare_pc): the matrix of
+  // frequency fidx assembled on stream s2 into K, the auxiliary Galerkin products formed
+  // there and downloaded into staging, and the host AMG set-up built from them on the
+  // worker thread, while the context's stream runs other work.  The matrix is taken
+  // by emfem_assemble_matrix(fidx), the host set-up by the emfem_create_pc after it.
+  struct Prepared {
+    int fidx = -1;
+    bool k_taken = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr, done = nullptr;
+    DevBuf<double2> K;
+    DevBuf<double> bsum, out[4];
+    DevBuf<int> missing;
+    HostPinned<double> staging;
+    HostPinned<int> miss;
+    PcOptions o;
+    std::unique_ptr<AmsHost> host;
+    std::thread worker;
+    std::exception_ptr err;
+    double started_ms = 0.0, worker_ms = 0.0;
+  } prep;
+  hipStream_t s2 = nullptr;
+
   std::map<std::string, double> stats;
   double assemble_kernel_ms = 0.0;
 
@@ -144,6 +168,17 @@ struct emfem_context {
 using namespace emfem;
 
 namespace emfem_ctx {
+
+// Waits for a set-up started by emfem_prepare_pc (its stream and worker) and forgets it.
+inline void discard_prepared(emfem_context *c) {
+  auto &P = c->prep;
+  if (P.worker.joinable()) P.worker.join();
+  if (c->s2) (void)hipStreamSynchronize(c->s2);
+  P.fidx = -1;
+  P.k_taken = false;
+  P.host.reset();
+  P.err = nullptr;
+}
 
 template <typename F>
 inline int guarded(emfem_context *ctx, F &&f) {

@@ -185,11 +185,15 @@ int emfem_create_context(emfem_context **out) {
 int emfem_destroy_context(emfem_context *c) {
   if (!c) return EMFEM_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  discard_prepared(c);
   c->pc.reset();
   c->pc_prev.reset();
-  hipStream_t s = c->stream;
+  hipStream_t s = c->stream, s2 = c->s2;
+  for (hipEvent_t e : {c->prep.e0, c->prep.e1, c->prep.done})
+    if (e) (void)hipEventDestroy(e);
   delete c;
   if (s) (void)hipStreamDestroy(s);
+  if (s2) (void)hipStreamDestroy(s2);
   return EMFEM_OK;
 }
 
@@ -244,6 +248,7 @@ int emfem_set_mesh(emfem_context *c, int n_points, const double *points, int n_t
                    const int *attributes) {
   return guarded(c, [&]() {
     EMF_REQUIRE(n_points > 0 && n_tets > 0 && points && tets && attributes, kErrArg, "empty or null mesh arrays");
+    discard_prepared(c);  // it read the mesh and the conductivity table
     c->pc.reset();
     c->pc_prev.reset();
     c->have_system = false;
@@ -267,6 +272,7 @@ int emfem_set_mesh_edges(emfem_context *c, int n_edges, const int *edgelist, con
   return guarded(c, [&]() {
     require_mesh(c);
     EMF_REQUIRE(edgelist && tet2edge, kErrArg, "null edge arrays");
+    discard_prepared(c);  // it read the mesh and the conductivity table
     c->pc.reset();
     c->pc_prev.reset();
     c->have_system = false;
@@ -278,6 +284,7 @@ int emfem_set_mesh_edges(emfem_context *c, int n_edges, const int *edgelist, con
 int emfem_set_rho(emfem_context *c, int n_rho, int aniso_form, const double *rho) {
   return guarded(c, [&]() {
     EMF_REQUIRE(n_rho > 0 && rho, kErrArg, "empty resistivity table");
+    discard_prepared(c);  // it read the mesh and the conductivity table
     const int nc = rho_columns(aniso_form);
     EMF_REQUIRE(nc > 0, kErrUser, "Unsupported anisotropy form " + std::to_string(aniso_form) + ".");
     c->aniso_form = aniso_form;
@@ -294,6 +301,7 @@ int emfem_set_rho(emfem_context *c, int n_rho, int aniso_form, const double *rho
 
 int emfem_set_frequencies(emfem_context *c, int n, const double *f) {
   return guarded(c, [&]() {
+    discard_prepared(c);
     EMF_REQUIRE(n >= 0 && (n == 0 || f), kErrArg, "bad frequency array");
     c->freqs.assign(f, f + n);
   });
@@ -357,6 +365,7 @@ int emfem_update_background_model(emfem_context *c) {
 
 int emfem_create_linear_system(emfem_context *c, int n_slots) {
   return guarded(c, [&]() {
+    discard_prepared(c);
     require_mesh(c);
     require_device(c);
     EMF_REQUIRE(n_slots > 0 && n_slots <= 512, kErrArg, "n_slots must be in 1..512");
@@ -394,6 +403,7 @@ int emfem_create_linear_system(emfem_context *c, int n_slots) {
 
 int emfem_destroy_linear_system(emfem_context *c) {
   return guarded(c, [&]() {
+    discard_prepared(c);
     c->pc.reset();
     c->pc_prev.reset();
     c->ams_aux.reset();
@@ -417,30 +427,123 @@ int emfem_destroy_linear_system(emfem_context *c) {
   });
 }
 
+namespace {
+
+// Enqueues the assembly of C and M for frequency fidx into K on stream s, the colour
+// launches between events e0 and e1.
+void assemble_on(emfem_context *c, int fidx, hipStream_t s, double2 *K, hipEvent_t e0, hipEvent_t e1) {
+  const double omega = 2 * kPi * c->freqs[fidx];
+  EMF_HIP(hipMemsetAsync(K, 0, sizeof(double2) * c->nnz, s));
+  EMF_HIP(hipEventRecord(e0, s));
+  const auto &cp = c->T.color_ptr;
+  for (size_t k = 0; k + 1 < cp.size(); ++k)
+    assemble_colour(s, cp[k + 1] - cp[k], c->d_colour.get() + cp[k], c->d_tets.get(), c->d_t2e.get(),
+                    c->d_pts.get(), c->d_attr.get(), c->d_s6.get(), c->d_ebdr.get(), omega, c->d_rp.get(),
+                    c->d_ci.get(), K);
+  EMF_HIP(hipEventRecord(e1, s));
+}
+
+}  // namespace
+
+int emfem_prepare_pc(emfem_context *c, int fidx) {
+  return guarded(c, [&]() {
+    require_system(c);
+    require_freq(c, fidx);
+    EMF_REQUIRE(c->n_rho > 0, kErrState, "no resistivity table");
+    discard_prepared(c);
+    // the device path of the set-up on auxiliary spaces this linear system already has
+    // (built by its first emfem_create_pc); otherwise there is nothing to start
+    if (c->opt("ams_setup") != "device" || !c->ams_aux || c->gal.aux != c->ams_aux) {
+      c->stats["pc_prepare_started"] = 0.0;
+      return;
+    }
+    auto &P = c->prep;
+    if (!c->s2) EMF_HIP(hipStreamCreate(&c->s2));  // default priority, like the context's stream
+    for (hipEvent_t *e : {&P.e0, &P.e1, &P.done})
+      if (!*e) EMF_HIP(hipEventCreate(e));
+    hipStream_t s = c->s2;
+    const size_t z = c->ams_aux->Apat.ci.size();
+    P.K.resize(c->nnz);
+    P.bsum.resize(c->nnz);
+    double *dv[4];
+    for (int k = 0; k < 4; ++k) {
+      P.out[k].resize(z);
+      dv[k] = P.out[k].get();
+    }
+    P.missing.resize(1);
+    P.staging.resize(4 * z);
+    P.miss.resize(1);
+    P.started_ms = now_ms();
+    // the matrix and the Galerkin products on s2: the context's stream may still be
+    // using d_K meanwhile, s2 writes only the prepared buffers
+    assemble_on(c, fidx, s, P.K.get(), P.e0, P.e1);
+    make_b<double>(s, c->nnz, P.K.get(), P.bsum.get());
+    galerkin_dev(s, c->gal.view(), c->d_rp.get(), c->d_ci.get(), P.bsum.get(), dv, P.missing.get());
+    EMF_HIP(hipMemcpyAsync(P.miss.get(), P.missing.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    for (int k = 0; k < 4; ++k)
+      EMF_HIP(hipMemcpyAsync(P.staging.get() + k * z, dv[k], sizeof(double) * z, hipMemcpyDeviceToHost, s));
+    EMF_HIP(hipEventRecord(P.done, s));
+    P.fidx = fidx;
+    P.k_taken = false;
+    P.o = pc_options(c);
+    P.host.reset(new AmsHost());
+    // the host AMG set-up on the worker, at most 8 host threads (the context's thread
+    // keeps driving the GPU); it waits for the products' download
+    emfem_context::Prepared *pp = &P;
+    const Topology *T = &c->T;
+    std::shared_ptr<const AmsAux> aux = c->ams_aux;
+    const AmgParams amg = P.o.amg;
+    P.worker = std::thread([pp, T, aux, amg, z]() {
+      try {
+        host_thread_cap() = 8;
+        const double t0 = now_ms();
+        auto fill = [&](const AmsAux &, double *const out[4]) {
+          EMF_HIP(hipEventSynchronize(pp->done));
+          EMF_REQUIRE(pp->miss[0] == 0, kErrArg, "Galerkin blocks: B has couplings outside the tet pattern");
+          const double *st = pp->staging.get();
+          parallel_ranges((int64_t)z, [&](int64_t b, int64_t e, int) {
+            for (int k = 0; k < 4; ++k) std::copy(st + k * z + b, st + k * z + e, out[k] + b);
+          });
+        };
+        build_ams(*T, fill, amg, *pp->host, aux);
+        pp->worker_ms = now_ms() - t0;
+      } catch (...) {
+        pp->err = std::current_exception();
+      }
+    });
+    c->stats["pc_prepare_started"] = 1.0;
+  });
+}
+
 int emfem_assemble_matrix(emfem_context *c, int fidx) {
   return guarded(c, [&]() {
     require_system(c);
     require_freq(c, fidx);
     EMF_REQUIRE(c->n_rho > 0, kErrState, "no resistivity table");
-    const double omega = 2 * kPi * c->freqs[fidx];
     hipStream_t s = c->stream;
-    hipEvent_t e0, e1;
-    EMF_HIP(hipEventCreate(&e0));
-    EMF_HIP(hipEventCreate(&e1));
     const double t0 = now_ms();
-    EMF_HIP(hipMemsetAsync(c->d_K.get(), 0, sizeof(double2) * c->nnz, s));
-    EMF_HIP(hipEventRecord(e0, s));
-    const auto &cp = c->T.color_ptr;
-    for (size_t k = 0; k + 1 < cp.size(); ++k)
-      assemble_colour(s, cp[k + 1] - cp[k], c->d_colour.get() + cp[k], c->d_tets.get(), c->d_t2e.get(),
-                      c->d_pts.get(), c->d_attr.get(), c->d_s6.get(), c->d_ebdr.get(), omega, c->d_rp.get(),
-                      c->d_ci.get(), c->d_K.get());
-    EMF_HIP(hipEventRecord(e1, s));
-    EMF_HIP(hipStreamSynchronize(s));
+    auto &P = c->prep;
     float ms = 0.f;
-    EMF_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (P.fidx >= 0 && P.fidx != fidx) discard_prepared(c);  // another frequency was prepared
+    if (P.fidx == fidx && !P.k_taken) {
+      // the matrix was assembled on s2 by emfem_prepare_pc: taken once s2 has also
+      // formed B from it (the Galerkin products read only their own copy)
+      EMF_HIP(hipEventSynchronize(P.done));
+      EMF_HIP(hipEventElapsedTime(&ms, P.e0, P.e1));
+      std::swap(c->d_K, P.K);
+      P.k_taken = true;
+      c->stats["assemble_prepared"] = 1.0;
+    } else {
+      hipEvent_t e0, e1;
+      EMF_HIP(hipEventCreate(&e0));
+      EMF_HIP(hipEventCreate(&e1));
+      assemble_on(c, fidx, s, c->d_K.get(), e0, e1);
+      EMF_HIP(hipStreamSynchronize(s));
+      EMF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      c->stats["assemble_prepared"] = 0.0;
+    }
     c->assemble_kernel_ms = ms;
     c->stats["assemble_ms"] = now_ms() - t0;
     c->stats["assemble_kernel_ms"] = ms;
@@ -568,56 +671,81 @@ int emfem_create_pc(emfem_context *c) {
     // blocks and the l1 scaling are formed there too and only the products travel to
     // the host AMG set-up (-ams_setup host: B travels and the host forms both)
     const bool on_device = c->opt("ams_setup") == "device";
-    DevBuf<double> bsum(c->nnz);
-    make_b<double>(s, c->nnz, c->d_K.get(), bsum.get());
-    HostCsr &B = c->h_B;
-    if (!on_device) {
-      B.v.resize(c->nnz);
-      bsum.download(B.v.data(), c->nnz, s);
-    }
-    EMF_HIP(hipStreamSynchronize(s));
     PcOptions o = pc_options(c);
+    // a host set-up prepared for this matrix (emfem_prepare_pc) with the same options
+    auto &P = c->prep;
+    const bool prepared = on_device && P.fidx >= 0 && P.fidx == c->assembled_fidx && P.k_taken &&
+                          P.o.amg.theta == o.amg.theta && P.o.amg.smooth_levels == o.amg.smooth_levels &&
+                          P.o.amg.additive == o.amg.additive && P.o.amg.max_coarse == o.amg.max_coarse &&
+                          P.o.amg.max_levels == o.amg.max_levels;
+    if (P.fidx == c->assembled_fidx && !prepared) discard_prepared(c);  // unusable for this matrix
+    DevBuf<double> bsum;
+    HostCsr &B = c->h_B;
+    if (!prepared) {
+      bsum.resize(c->nnz);
+      make_b<double>(s, c->nnz, c->d_K.get(), bsum.get());
+      if (!on_device) {
+        B.v.resize(c->nnz);
+        bsum.download(B.v.data(), c->nnz, s);
+      }
+      EMF_HIP(hipStreamSynchronize(s));
+    }
     // the device side of the previous set-up is refreshed with the new values when it
     // was built for this linear system's pattern and auxiliary spaces (-pc_reuse)
     std::unique_ptr<InnerPc> prev = c->pc ? std::move(c->pc) : std::move(c->pc_prev);
     c->pc_prev.reset();
     if (c->opt("pc_reuse") != "true") prev.reset();
-    HostCsr recycle;
-    std::shared_ptr<const AmsAux> recycle_aux;
-    if (c->ams_host && !c->ams_host->amgPi.levels.empty()) {
-      recycle = std::move(c->ams_host->amgPi.levels[0].A);
-      recycle_aux = c->ams_host->aux;
-    }
-    c->ams_host.reset(new AmsHost());
-    c->ams_host->recycle = std::move(recycle);
-    c->ams_host->recycle_aux = std::move(recycle_aux);
     const double tb = now_ms();
-    if (on_device) {
-      auto fill = [&](const AmsAux &X, double *const out[4]) {
-        DevGalerkin &G = c->gal;
-        EMF_REQUIRE(c->ams_host->aux.get() == &X, kErrState, "auxiliary spaces: set-up order");
-        if (G.aux != c->ams_host->aux) upload_galerkin(G, c->ams_host->aux, s);
-        const size_t z = X.Apat.ci.size();
-        double *dv[4];
-        for (int k = 0; k < 4; ++k) {
-          G.out[k].resize(z);
-          dv[k] = G.out[k].get();
-        }
-        galerkin_dev(s, G.view(), c->d_rp.get(), c->d_ci.get(), bsum.get(), dv, G.missing.get());
-        int miss = 0;
-        EMF_HIP(hipMemcpyAsync(&miss, G.missing.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-        double *st = G.staging(4 * z);
-        for (int k = 0; k < 4; ++k) G.out[k].download(st + k * z, z, s);
-        EMF_HIP(hipStreamSynchronize(s));
-        EMF_REQUIRE(miss == 0, kErrArg, "Galerkin blocks: B has couplings outside the tet pattern");
-        parallel_ranges((int64_t)z, [&](int64_t b, int64_t e, int) {
-          for (int k = 0; k < 4; ++k) std::copy(st + k * z + b, st + k * z + e, out[k] + b);
-        });
-      };
-      build_ams(c->T, fill, o.amg, *c->ams_host, c->ams_aux);
+    if (prepared) {
+      const double tw = now_ms();
+      P.worker.join();
+      c->stats["pc_prepare_wait_ms"] = now_ms() - tw;
+      c->stats["pc_prepare_worker_ms"] = P.worker_ms;
+      if (P.err) {
+        std::exception_ptr e = P.err;
+        discard_prepared(c);
+        std::rethrow_exception(e);
+      }
+      c->ams_host = std::move(P.host);
+      discard_prepared(c);
     } else {
-      build_ams(c->T, B, o.amg, *c->ams_host, c->ams_aux);
+      HostCsr recycle;
+      std::shared_ptr<const AmsAux> recycle_aux;
+      if (c->ams_host && !c->ams_host->amgPi.levels.empty()) {
+        recycle = std::move(c->ams_host->amgPi.levels[0].A);
+        recycle_aux = c->ams_host->aux;
+      }
+      c->ams_host.reset(new AmsHost());
+      c->ams_host->recycle = std::move(recycle);
+      c->ams_host->recycle_aux = std::move(recycle_aux);
+      if (on_device) {
+        auto fill = [&](const AmsAux &X, double *const out[4]) {
+          DevGalerkin &G = c->gal;
+          EMF_REQUIRE(c->ams_host->aux.get() == &X, kErrState, "auxiliary spaces: set-up order");
+          if (G.aux != c->ams_host->aux) upload_galerkin(G, c->ams_host->aux, s);
+          const size_t z = X.Apat.ci.size();
+          double *dv[4];
+          for (int k = 0; k < 4; ++k) {
+            G.out[k].resize(z);
+            dv[k] = G.out[k].get();
+          }
+          galerkin_dev(s, G.view(), c->d_rp.get(), c->d_ci.get(), bsum.get(), dv, G.missing.get());
+          int miss = 0;
+          EMF_HIP(hipMemcpyAsync(&miss, G.missing.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+          double *st = G.staging(4 * z);
+          for (int k = 0; k < 4; ++k) G.out[k].download(st + k * z, z, s);
+          EMF_HIP(hipStreamSynchronize(s));
+          EMF_REQUIRE(miss == 0, kErrArg, "Galerkin blocks: B has couplings outside the tet pattern");
+          parallel_ranges((int64_t)z, [&](int64_t b, int64_t e, int) {
+            for (int k = 0; k < 4; ++k) std::copy(st + k * z + b, st + k * z + e, out[k] + b);
+          });
+        };
+        build_ams(c->T, fill, o.amg, *c->ams_host, c->ams_aux);
+      } else {
+        build_ams(c->T, B, o.amg, *c->ams_host, c->ams_aux);
+      }
     }
+    c->stats["pc_prepared"] = prepared ? 1.0 : 0.0;
     c->ams_aux = c->ams_host->aux;
     const double t1 = now_ms();
     c->stats["pc_download_ms"] = tb - t0;

@@ -760,7 +760,7 @@ class InnerPcImpl : public InnerPc {
  public:
   InnerPcImpl(bool fp32, const AmsHost &H, int ne, int nnz, const int *rp, const int *ci, const double2 *K,
               const PcOptions &o, hipStream_t s, const Col16 &c16)
-      : H_(&H), ne_(ne), nnz_(nnz), rp_(rp), ci_(ci), K_(K), o_(o), c16_(c16) {
+      : H_(&H), c16_(c16), ne_(ne), nnz_(nnz), rp_(rp), ci_(ci), K_(K), o_(o) {
     set_fp32(fp32, s);
   }
 

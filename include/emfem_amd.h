@@ -161,6 +161,19 @@ int emfem_get_solution(const emfem_context *ctx, int slot, double *re, double *i
  * "pc_reused"). */
 int emfem_create_pc(emfem_context *ctx);
 
+/* Starts the set-up for frequency fidx ahead of time, for drivers that loop over
+ * frequencies: the matrix of fidx is assembled on a second stream of the context,
+ * the auxiliary-space Galerkin products are formed there and the host AMG set-up is
+ * built from them on a worker thread, while the caller goes on (typically with the
+ * solve of the current frequency).  A later emfem_assemble_matrix(fidx) takes the
+ * prepared matrix and the emfem_create_pc after it the prepared host set-up (stats
+ * "assemble_prepared", "pc_prepared"); the results are those of the same calls
+ * without it.  A no-op before the first emfem_create_pc of the linear system and
+ * with -ams_setup host; discarded by the assembly of another frequency and by any
+ * change of the mesh, conductivities, frequencies or linear system.  No counterpart
+ * in the reference, which sets each frequency up in turn (src/em_fwd.cc:1341-1360). */
+int emfem_prepare_pc(emfem_context *ctx, int fidx);
+
 /* destroy_pc (src/em_la.cc:193-206). */
 int emfem_destroy_pc(emfem_context *ctx);
 

@@ -454,6 +454,57 @@ def test_refreshed_preconditioner_matches_a_fresh_one(precision):
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_prepared_setup_matches_setup_in_turn(precision):
+    """emfem_prepare_pc(next) started before the current solve: the next frequency's
+    matrix (assembled on the context's second stream), its auxiliary-space set-up
+    (built on the worker thread) and so its solve equal, bit for bit, those of the
+    same calls without it; a prepared frequency that is not the one assembled next is
+    discarded and changes nothing."""
+    m = mg.halfspace_mt(n=4, n_rx=3)
+    m.freqs = np.array([0.3, 3.0, 30.0])
+    opts = "-inner_precision %s -spmm_rows rowblock" % precision
+
+    def step(ctx, f, nxt=None):
+        ctx.assemble_matrix(f)
+        ctx.assemble_rhs_mt(f, api.XY_POLAR, 0)
+        ctx.assemble_rhs_mt(f, api.YX_POLAR, 1)
+        ctx.assemble_rhs_dual(2)
+        ctx.create_pc()
+        if nxt is not None:
+            ctx.prepare_pc(nxt)
+        its, rr = ctx.solve_linear_system(0, 3, 300, 1e-10)
+        assert np.all(rr <= 1e-10 * 1.0001), rr
+        _, _, C, M = ctx.get_matrix()
+        return its, [ctx.get_solution(s) for s in range(3)], C, M
+
+    a = _ctx(m, opts)
+    a.create_linear_system(3)
+    b = _ctx(m, opts)
+    b.create_linear_system(3)
+    step(a, 0, nxt=1)
+    assert a.stat("pc_prepare_started") == 1.0
+    step(b, 0)
+    ra = step(a, 1, nxt=0)
+    assert a.stat("assemble_prepared") == 1.0 and a.stat("pc_prepared") == 1.0
+    rb = step(b, 1)
+    assert b.stat("assemble_prepared") == 0.0 and b.stat("pc_prepared") == 0.0
+    for x, y in zip(ra[2:], rb[2:]):
+        assert np.array_equal(x, y)
+    assert np.array_equal(ra[0], rb[0])
+    for x, y in zip(ra[1], rb[1]):
+        assert np.array_equal(x, y)
+    # frequency 0 was prepared; assembling 2 instead discards it
+    ra = step(a, 2)
+    assert a.stat("assemble_prepared") == 0.0 and a.stat("pc_prepared") == 0.0
+    rb = step(b, 2)
+    assert np.array_equal(ra[0], rb[0])
+    for x, y in zip(ra[1], rb[1]):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("cells,fits", [((64, 64, 8), True), ((72, 72, 8), False)])
 def test_16bit_columns_only_where_every_group_fits(cells, fits):
     """The 16-bit column form is taken when the columns of every aligned group of 64
