@@ -86,9 +86,11 @@ def run_step(ctx, m, fidx, batched=True, next_fidx=None):
             its += i.tolist()
             rr += r.tolist()
     t3 = time.perf_counter()
+    inner = int(ctx.stat("inner_its")) if batched else None
     ctx.estimate_error(fidx, [0, 1, 2])
     t4 = time.perf_counter()
-    return dict(total=t4 - t0, assemble_rhs=t1 - t0, pc=t2 - t1, solve=t3 - t2, estimate=t4 - t3, its=its, relres=rr)
+    return dict(total=t4 - t0, assemble_rhs=t1 - t0, pc=t2 - t1, solve=t3 - t2, estimate=t4 - t3, its=its, relres=rr,
+                inner_its=inner)
 
 
 def cpu_threads():
@@ -578,6 +580,7 @@ def main():
         "phase_s": {k: round(sum(s[k] for s in steps) / len(steps), 4)
                     for k in ("assemble_rhs", "pc", "solve", "estimate")},
         "outer_its": [s["its"] for s in steps],
+        "inner_its": [s["inner_its"] for s in steps],
     }
     pcb = {}
     for k in ("download", "aux_build", "galerkin", "amg", "amg_aggregate", "amg_prolongator", "amg_transpose",

@@ -1904,8 +1904,48 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(GalerkinDev G, const int *_
   double *vals = k == 0 ? v0 : (k == 1 ? v1 : (k == 2 ? v2 : v3));
   const double *rv = G.rv[k], *pv = G.pv[k];
   const int a0 = G.arp[u], a1 = G.arp[u + 1];
-  for (int t = a0; t < a1; ++t) vals[t] = 0.0;
   int miss = 0;
+  // rows of up to kSlots entries (an interior vertex of a Kuhn mesh has 15) keep their
+  // columns and sums in registers: each product is added to its slot by a select over
+  // the unrolled slots, so every sum takes the same additions in the same order as
+  // the loop below (and the host pass)
+  constexpr int kSlots = 24;
+  if (a1 - a0 <= kSlots) {
+    const int len = a1 - a0;
+    int cols[kSlots];
+    double acc[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      cols[j] = j < len ? G.aci[a0 + j] : -1;
+      acc[j] = 0.0;
+    }
+    for (int p = G.rrp[u]; p < G.rrp[u + 1]; ++p) {
+      const int ed = G.rci[p];
+      const double r = rv[p];
+      for (int q = brp[ed]; q < brp[ed + 1]; ++q) {
+        const int f = bci[q];
+        const double ra = r * bv[q];
+        for (int s2 = G.prp[f]; s2 < G.prp[f + 1]; ++s2) {
+          const int col = G.pci[s2];
+          const double prod = ra * pv[s2];
+          bool hit = false;
+#pragma unroll
+          for (int j = 0; j < kSlots; ++j) {
+            const bool m = cols[j] == col;
+            acc[j] = m ? acc[j] + prod : acc[j];
+            hit |= m;
+          }
+          miss += hit ? 0 : 1;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j)
+      if (j < len) vals[a0 + j] = acc[j];
+    if (miss && k == 0) atomicAdd(missing, miss);
+    return;
+  }
+  for (int t = a0; t < a1; ++t) vals[t] = 0.0;
   for (int p = G.rrp[u]; p < G.rrp[u + 1]; ++p) {
     const int ed = G.rci[p];
     const double r = rv[p];
