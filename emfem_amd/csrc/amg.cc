@@ -11,8 +11,11 @@
 
 namespace emfem {
 
-// Two-pass transpose over row chunks: per-chunk column counts, then each chunk
-// fills its rows at its own offsets, so every column lists rows in ascending order.
+// Transpose: column counts (relaxed atomic increments, rows in parallel), their
+// prefix sum, a fill through atomic per-column cursors, and a sort of each output row
+// by source row, so every column lists its rows in ascending order (the order of a
+// sequential transpose) whatever the threads' interleaving.  Small matrices run on
+// one thread.
 HostCsr csr_transpose(const HostCsr &A) {
   HostCsr T;
   T.n = A.m;
@@ -20,47 +23,62 @@ HostCsr csr_transpose(const HostCsr &A) {
   T.rp.assign(A.m + 1, 0);
   T.ci.resize(A.ci.size());
   T.v.resize(A.v.size());
-  const int nch = A.ci.size() < 200000 ? 1 : host_threads();
-  std::vector<std::vector<int>> off(nch, std::vector<int>(A.m, 0));
-  auto run = [&](auto &&f) {
-    if (nch == 1) {
-      f(0);
-      return;
-    }
-    std::vector<std::thread> th;
-    for (int t = 0; t < nch; ++t) th.emplace_back([&, t]() { f(t); });
-    for (auto &x : th) x.join();
-  };
-  auto rows = [&](int t, int &b, int &e) {
-    b = (int)((int64_t)A.n * t / nch);
-    e = (int)((int64_t)A.n * (t + 1) / nch);
-  };
-  run([&](int t) {
-    int b, e;
-    rows(t, b, e);
-    int *cnt = off[t].data();
-    for (int p = A.rp[b]; p < A.rp[e]; ++p) cnt[A.ci[p]]++;
-  });
-  int run_sum = 0;
-  for (int c = 0; c < A.m; ++c) {
-    for (int t = 0; t < nch; ++t) {
-      const int k = off[t][c];
-      off[t][c] = run_sum;
-      run_sum += k;
-    }
-    T.rp[c + 1] = run_sum;
+  const int nth = A.ci.size() < 200000 ? 1 : host_threads();
+  std::vector<int> cnt(A.m, 0);
+  parallel_ranges(
+      A.n,
+      [&](int64_t b, int64_t e, int) {
+        for (int p = A.rp[b]; p < A.rp[e]; ++p) __atomic_fetch_add(&cnt[A.ci[p]], 1, __ATOMIC_RELAXED);
+      },
+      nth);
+  for (int c = 0; c < A.m; ++c) T.rp[c + 1] = T.rp[c] + cnt[c];
+  parallel_ranges(
+      A.m, [&](int64_t b, int64_t e, int) { std::copy(T.rp.begin() + b, T.rp.begin() + e, cnt.begin() + b); }, nth);
+  parallel_ranges(
+      A.n,
+      [&](int64_t b, int64_t e, int) {
+        for (int64_t r = b; r < e; ++r)
+          for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
+            const int q = __atomic_fetch_add(&cnt[A.ci[p]], 1, __ATOMIC_RELAXED);
+            T.ci[q] = (int)r;
+            T.v[q] = A.v[p];
+          }
+      },
+      nth);
+  if (nth > 1) {
+    // each output row sorted by source row: insertion sort on short rows, a pair sort
+    // on long ones
+    parallel_ranges(
+        A.m,
+        [&](int64_t b, int64_t e, int) {
+          std::vector<std::pair<int, double>> tmp;
+          for (int64_t c = b; c < e; ++c) {
+            if (T.rp[c + 1] - T.rp[c] > 32) {
+              tmp.clear();
+              for (int i = T.rp[c]; i < T.rp[c + 1]; ++i) tmp.emplace_back(T.ci[i], T.v[i]);
+              std::sort(tmp.begin(), tmp.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+              for (int i = T.rp[c]; i < T.rp[c + 1]; ++i) {
+                T.ci[i] = tmp[i - T.rp[c]].first;
+                T.v[i] = tmp[i - T.rp[c]].second;
+              }
+              continue;
+            }
+            for (int i = T.rp[c] + 1; i < T.rp[c + 1]; ++i) {
+              const int r = T.ci[i];
+              const double v = T.v[i];
+              int j = i - 1;
+              while (j >= T.rp[c] && T.ci[j] > r) {
+                T.ci[j + 1] = T.ci[j];
+                T.v[j + 1] = T.v[j];
+                --j;
+              }
+              T.ci[j + 1] = r;
+              T.v[j + 1] = v;
+            }
+          }
+        },
+        nth);
   }
-  run([&](int t) {
-    int b, e;
-    rows(t, b, e);
-    int *pos = off[t].data();
-    for (int r = b; r < e; ++r)
-      for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
-        const int q = pos[A.ci[p]]++;
-        T.ci[q] = r;
-        T.v[q] = A.v[p];
-      }
-  });
   return T;
 }
 
@@ -330,48 +348,91 @@ HostCsr smoothed_prolongator(const HostCsr &A, const std::vector<double> &d, con
 }
 
 // Dense SPD inverse by Cholesky; a vanishing pivot (singular coarse problem) is
-// regularised, which only affects the null-space component.
+// regularised, which only affects the null-space component.  The matrix is inverted
+// per connected component of its graph (the blocks of a block-diagonal matrix, e.g. the
+// auxiliary blocks and the models of a stacked batch), components in parallel; the
+// Cholesky of the whole matrix would do the same operations on each block plus exact
+// zeros, and the regularisation threshold stays relative to the largest diagonal entry
+// of the whole matrix.
 std::vector<double> dense_inverse(const HostCsr &A) {
   const int n = A.n;
-  std::vector<double> a((size_t)n * n, 0.0);
   double dmax = 0.0;
+  std::vector<int> root(n);
+  for (int i = 0; i < n; ++i) root[i] = i;
+  auto find = [&](int i) {
+    while (root[i] != i) i = root[i] = root[root[i]];
+    return i;
+  };
   for (int r = 0; r < n; ++r)
     for (int p = A.rp[r]; p < A.rp[r + 1]; ++p) {
-      a[(size_t)r * n + A.ci[p]] = A.v[p];
       if (A.ci[p] == r) dmax = std::max(dmax, std::abs(A.v[p]));
+      const int x = find(r), y = find(A.ci[p]);
+      if (x != y) root[std::max(x, y)] = std::min(x, y);
     }
-  // symmetrise (Galerkin products are symmetric up to rounding)
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < i; ++j) {
-      double v = 0.5 * (a[(size_t)i * n + j] + a[(size_t)j * n + i]);
-      a[(size_t)i * n + j] = a[(size_t)j * n + i] = v;
+  // components, each listing its rows in ascending order
+  std::vector<int> comp_of(n, -1);
+  std::vector<std::vector<int>> comps;
+  for (int i = 0; i < n; ++i) {
+    const int r = find(i);
+    if (comp_of[r] < 0) {
+      comp_of[r] = (int)comps.size();
+      comps.emplace_back();
     }
-  std::vector<double> L((size_t)n * n, 0.0);
-  for (int j = 0; j < n; ++j) {
-    double s = a[(size_t)j * n + j];
-    for (int k = 0; k < j; ++k) s -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
-    if (s <= 1e-14 * dmax) s = std::max(1e-14 * dmax, 1e-300);
-    const double ljj = std::sqrt(s);
-    L[(size_t)j * n + j] = ljj;
-    for (int i = j + 1; i < n; ++i) {
-      double t = a[(size_t)i * n + j];
-      for (int k = 0; k < j; ++k) t -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
-      L[(size_t)i * n + j] = t / ljj;
-    }
+    comps[comp_of[r]].push_back(i);
   }
-  std::vector<double> inv((size_t)n * n, 0.0), col(n);
-  for (int c = 0; c < n; ++c) {
-    for (int i = 0; i < n; ++i) {
-      double t = (i == c) ? 1.0 : 0.0;
-      for (int k = 0; k < i; ++k) t -= L[(size_t)i * n + k] * col[k];
-      col[i] = t / L[(size_t)i * n + i];
+  std::vector<double> inv((size_t)n * n, 0.0);
+  auto invert = [&](const std::vector<int> &idx) {
+    const int m = (int)idx.size();
+    std::vector<int> loc(n, -1);
+    for (int i = 0; i < m; ++i) loc[idx[i]] = i;
+    std::vector<double> a((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i)
+      for (int p = A.rp[idx[i]]; p < A.rp[idx[i] + 1]; ++p) a[(size_t)i * m + loc[A.ci[p]]] = A.v[p];
+    // symmetrise (Galerkin products are symmetric up to rounding)
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < i; ++j) {
+        double v = 0.5 * (a[(size_t)i * m + j] + a[(size_t)j * m + i]);
+        a[(size_t)i * m + j] = a[(size_t)j * m + i] = v;
+      }
+    std::vector<double> L((size_t)m * m, 0.0);
+    for (int j = 0; j < m; ++j) {
+      double s = a[(size_t)j * m + j];
+      for (int k = 0; k < j; ++k) s -= L[(size_t)j * m + k] * L[(size_t)j * m + k];
+      if (s <= 1e-14 * dmax) s = std::max(1e-14 * dmax, 1e-300);
+      const double ljj = std::sqrt(s);
+      L[(size_t)j * m + j] = ljj;
+      for (int i = j + 1; i < m; ++i) {
+        double t = a[(size_t)i * m + j];
+        for (int k = 0; k < j; ++k) t -= L[(size_t)i * m + k] * L[(size_t)j * m + k];
+        L[(size_t)i * m + j] = t / ljj;
+      }
     }
-    for (int i = n - 1; i >= 0; --i) {
-      double t = col[i];
-      for (int k = i + 1; k < n; ++k) t -= L[(size_t)k * n + i] * col[k];
-      col[i] = t / L[(size_t)i * n + i];
+    std::vector<double> col(m);
+    for (int c = 0; c < m; ++c) {
+      for (int i = 0; i < m; ++i) {
+        double t = (i == c) ? 1.0 : 0.0;
+        for (int k = 0; k < i; ++k) t -= L[(size_t)i * m + k] * col[k];
+        col[i] = t / L[(size_t)i * m + i];
+      }
+      for (int i = m - 1; i >= 0; --i) {
+        double t = col[i];
+        for (int k = i + 1; k < m; ++k) t -= L[(size_t)k * m + i] * col[k];
+        col[i] = t / L[(size_t)i * m + i];
+      }
+      for (int i = 0; i < m; ++i) inv[(size_t)idx[i] * n + idx[c]] = col[i];
     }
-    for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = col[i];
+  };
+  const int nc = (int)comps.size();
+  const int nth = std::min(nc, host_threads());
+  if (nth <= 1) {
+    for (const auto &idx : comps) invert(idx);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t)
+      th.emplace_back([&, t]() {
+        for (int k = t; k < nc; k += nth) invert(comps[k]);
+      });
+    for (auto &x : th) x.join();
   }
   return inv;
 }
@@ -439,13 +500,17 @@ std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive) {
     if (!T.vertex_bdr[v]) vmap[v] = nv0++;
   X->nv0 = nv0;
   EMF_REQUIRE(nv0 > 0, kErrUser, "mesh has no interior vertex");
+  // the row pointers G0 and the three Pi_d share: one entry per interior endpoint
+  std::vector<int> erp(T.ne + 1, 0);
+  parallel_ranges(T.ne, [&](int64_t b, int64_t e, int) {
+    for (int64_t i = b; i < e; ++i) erp[i + 1] = (vmap[T.edgelist[i * 2]] >= 0) + (vmap[T.edgelist[i * 2 + 1]] >= 0);
+  });
+  for (int e = 0; e < T.ne; ++e) erp[e + 1] += erp[e];
   auto make = [&](int comp) {
     HostCsr G;
     G.n = T.ne;
     G.m = nv0;
-    G.rp.assign(T.ne + 1, 0);
-    for (int e = 0; e < T.ne; ++e)
-      G.rp[e + 1] = G.rp[e] + (vmap[T.edgelist[e * 2]] >= 0) + (vmap[T.edgelist[e * 2 + 1]] >= 0);
+    G.rp = erp;
     G.ci.resize(G.rp[T.ne]);
     G.v.resize(G.rp[T.ne]);
     parallel_ranges(T.ne, [&](int64_t eb, int64_t ee, int) {
@@ -483,23 +548,79 @@ std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive) {
     X->PT[d] = csr_transpose(X->P[d]);
   }
   X->additive = additive;
-  std::vector<const HostCsr *> pb;
-  if (additive) pb.push_back(&X->G0);
-  for (int d = 0; d < 3; ++d) pb.push_back(&X->P[d]);
-  const int nb = (int)pb.size();
+  const int nb = additive ? 4 : 3;
   X->nblk = nb;
-  HostCsr &Pi = X->Pi;
-  Pi.n = T.ne;
-  Pi.m = nb * nv0;
-  Pi.rp.assign(T.ne + 1, 0);
-  for (int e = 0; e < T.ne; ++e) {
-    int k = 0;
-    for (const HostCsr *q : pb) k += q->rp[e + 1] - q->rp[e];
-    Pi.rp[e + 1] = Pi.rp[e] + k;
+  // Galerkin pattern: interior vertices u, v of a common tet (B couples the edges of a
+  // tet, and P maps an edge to its interior endpoints).  Every two vertices of a tet
+  // are the ends of one of its edges, so row u holds u and the other interior ends of
+  // the edges at u (the rows of G0^T), built per row in parallel.
+  {
+    HostCsr &A = X->Apat;
+    A.n = A.m = nv0;
+    std::vector<int> ivmap(nv0);
+    for (int v = 0; v < T.nv; ++v)
+      if (vmap[v] >= 0) ivmap[vmap[v]] = v;
+    const HostCsr &GT = X->G0T;
+    auto other = [&](int u, int e) {
+      const int a = T.edgelist[(size_t)e * 2], b = T.edgelist[(size_t)e * 2 + 1];
+      return vmap[a == ivmap[u] ? b : a];
+    };
+    A.rp.assign(nv0 + 1, 0);
+    parallel_ranges(nv0, [&](int64_t b, int64_t e, int) {
+      for (int64_t u = b; u < e; ++u) {
+        int k = 1;
+        for (int p = GT.rp[u]; p < GT.rp[u + 1]; ++p) k += other((int)u, GT.ci[p]) >= 0;
+        A.rp[u + 1] = k;
+      }
+    });
+    for (int u = 0; u < nv0; ++u) A.rp[u + 1] += A.rp[u];
+    A.ci.resize(A.rp[nv0]);
+    parallel_ranges(nv0, [&](int64_t b, int64_t e, int) {
+      for (int64_t u = b; u < e; ++u) {
+        int q = A.rp[u];
+        A.ci[q++] = (int)u;
+        for (int p = GT.rp[u]; p < GT.rp[u + 1]; ++p) {
+          const int v = other((int)u, GT.ci[p]);
+          if (v >= 0) A.ci[q++] = v;
+        }
+        std::sort(A.ci.begin() + A.rp[u], A.ci.begin() + A.rp[u + 1]);
+      }
+    });
+    // block-diagonal stack of the nblk Galerkin blocks
+    HostCsr &S = X->APpat;
+    const int64_t z = A.rp[nv0];
+    S.n = S.m = nb * nv0;
+    S.rp.resize((size_t)nb * nv0 + 1);
+    S.ci.resize((size_t)nb * z);
+    S.rp[0] = 0;
+    for (int d = 0; d < nb; ++d) {
+      parallel_ranges(nv0, [&](int64_t b, int64_t e, int) {
+        for (int64_t u = b; u < e; ++u) S.rp[(size_t)d * nv0 + u + 1] = (int)(d * z + A.rp[u + 1]);
+      });
+      parallel_ranges(z, [&](int64_t b, int64_t e, int) {
+        for (int64_t k = b; k < e; ++k) S.ci[d * z + k] = A.ci[k] + d * nv0;
+      });
+    }
   }
-  Pi.ci.resize(Pi.rp[T.ne]);
-  Pi.v.resize(Pi.rp[T.ne]);
-  parallel_ranges(T.ne, [&](int64_t eb, int64_t ee, int) {
+  return X;
+}
+
+void build_pi_csr(const AmsAux &X, HostCsr &Pi, HostCsr &PiT) {
+  std::vector<const HostCsr *> pb;
+  if (X.additive) pb.push_back(&X.G0);
+  for (int d = 0; d < 3; ++d) pb.push_back(&X.P[d]);
+  const int nb = (int)pb.size(), ne = X.G0.n, nv0 = X.nv0;
+  Pi = HostCsr();
+  Pi.n = ne;
+  Pi.m = nb * nv0;
+  Pi.rp.resize(ne + 1);
+  // every block has G0's row structure
+  parallel_ranges(ne + 1, [&](int64_t b, int64_t e, int) {
+    for (int64_t i = b; i < e; ++i) Pi.rp[i] = nb * X.G0.rp[i];
+  });
+  Pi.ci.resize(Pi.rp[ne]);
+  Pi.v.resize(Pi.rp[ne]);
+  parallel_ranges(ne, [&](int64_t eb, int64_t ee, int) {
     for (int64_t e = eb; e < ee; ++e) {
       int q = Pi.rp[e];
       for (int d = 0; d < nb; ++d)
@@ -509,45 +630,7 @@ std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive) {
         }
     }
   });
-  X->PiT = csr_transpose(Pi);
-  // Galerkin pattern: interior vertices u, v of a common tet (B couples the edges of a
-  // tet, and P maps an edge to its interior endpoints)
-  {
-    HostCsr &A = X->Apat;
-    A.n = A.m = nv0;
-    std::vector<std::vector<int>> adj(nv0);
-    for (int t = 0; t < T.nt; ++t)
-      for (int i = 0; i < 4; ++i) {
-        const int u = vmap[T.tets[(size_t)t * 4 + i]];
-        if (u < 0) continue;
-        for (int j = 0; j < 4; ++j) {
-          const int v = vmap[T.tets[(size_t)t * 4 + j]];
-          if (v >= 0) adj[u].push_back(v);
-        }
-      }
-    A.rp.assign(nv0 + 1, 0);
-    parallel_ranges(nv0, [&](int64_t b, int64_t e, int) {
-      for (int64_t u = b; u < e; ++u) {
-        std::sort(adj[u].begin(), adj[u].end());
-        adj[u].erase(std::unique(adj[u].begin(), adj[u].end()), adj[u].end());
-      }
-    });
-    for (int u = 0; u < nv0; ++u) A.rp[u + 1] = A.rp[u] + (int)adj[u].size();
-    A.ci.resize(A.rp[nv0]);
-    for (int u = 0; u < nv0; ++u) std::copy(adj[u].begin(), adj[u].end(), A.ci.begin() + A.rp[u]);
-    // block-diagonal stack of the nblk Galerkin blocks
-    HostCsr &S = X->APpat;
-    const int64_t z = A.rp[nv0];
-    S.n = S.m = nb * nv0;
-    S.rp.resize((size_t)nb * nv0 + 1);
-    S.ci.resize((size_t)nb * z);
-    S.rp[0] = 0;
-    for (int d = 0; d < nb; ++d) {
-      for (int u = 0; u < nv0; ++u) S.rp[(size_t)d * nv0 + u + 1] = (int)(d * z + A.rp[u + 1]);
-      for (int64_t k = 0; k < z; ++k) S.ci[d * z + k] = A.ci[k] + d * nv0;
-    }
-  }
-  return X;
+  PiT = csr_transpose(Pi);
 }
 
 // The four Galerkin products P_k^T B P_k (k = G0, Pi_x, Pi_y, Pi_z) in one pass over

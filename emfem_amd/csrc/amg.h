@@ -64,13 +64,12 @@ struct AmsAux {
   HostCsr G0, G0T;
   // nodal-vector components Pi_d (E x V0) and their transposes
   HostCsr P[3], PT[3];
-  // Auxiliary blocks side by side: E x (nblk V0), column k * V0 + v for block k of
-  // interior vertex v.  Blocks are the three nodal-vector components Pi_d
+  // Auxiliary blocks side by side (build_pi_csr): E x (nblk V0), column k * V0 + v for
+  // block k of interior vertex v.  Blocks are the three nodal-vector components Pi_d
   // (multiplicative cycle) or G0 followed by Pi_x, Pi_y, Pi_z (additive cycle).
   // Corrections of these blocks see the same residual, so their auxiliary problems
   // form one block-diagonal matrix diag(P_k^T B P_k) with one AMG hierarchy whose
   // aggregates never cross blocks.
-  HostCsr Pi, PiT;
   int nblk = 3;
   bool additive = false;
   int nv0 = 0;
@@ -80,6 +79,11 @@ struct AmsAux {
 };
 
 std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive);
+
+// The blocks of X side by side as one CSR matrix Pi and its transpose (the
+// interpolation of the CSR transfers, -ams_transfer csr; the structured transfers read
+// G0 and Pi_d directly).
+void build_pi_csr(const AmsAux &X, HostCsr &Pi, HostCsr &PiT);
 
 // vals[k] = values of P_k^T B P_k on X.Apat, k = G0, Pi_x, Pi_y, Pi_z (the first form
 // writes into caller arrays of Apat's nonzero count).

@@ -46,6 +46,7 @@ constexpr int kMinStageModels = 8;
 struct StageOut {
   SolveStats st;
   double setup_ms = 0.0, solve_ms = 0.0, solve_start_ms = 0.0;
+  std::map<std::string, double> timing;  // set-up phases (ms)
 };
 
 // One stage: the models `S` stacked block-diagonally, warm-started from their solution
@@ -80,10 +81,16 @@ void solve_stage(emfem_context **ctxs, const std::vector<int> &S, int first, int
   Topology U;
   U.nv = (int)nv;
   U.ne = (int)ne;
-  U.pts.reserve((size_t)nv * 3);
-  U.edgelist.reserve((size_t)ne * 2);
-  U.vertex_bdr.reserve((size_t)nv);
-  for (int k = 0; k < ns; ++k) {
+  std::vector<int64_t> toff(ns + 1, 0);
+  for (int k = 0; k < ns; ++k) toff[k + 1] = toff[k] + ctxs[S[k]]->T.nt;
+  U.nt = (int)toff[ns];
+  U.pts.resize((size_t)nv * 3);
+  U.edgelist.resize((size_t)ne * 2);
+  U.vertex_bdr.resize((size_t)nv);
+  U.tets.resize((size_t)toff[ns] * 4);
+  // every model fills its own ranges of the stacked arrays; the models are spread over
+  // host threads, each downloading on its model's stream
+  auto stack_model = [&](int k) {
     emfem_context *c = ctxs[S[k]];
     const int e0 = erow0[k], z0 = nzoff[k], v0 = voff[k];
     std::vector<int> hrp(c->T.ne + 1);
@@ -97,16 +104,35 @@ void solve_stage(emfem_context **ctxs, const std::vector<int> &S, int first, int
       B.v[z0 + i] = hK[i].x + hK[i].y;
       B.ci[z0 + i] += e0;
     }
-    EMF_HIP(hipMemcpyAsync(K.get() + z0, c->d_K.get(), sizeof(double2) * c->nnz, hipMemcpyDeviceToDevice, s));
-    U.pts.insert(U.pts.end(), c->T.pts.begin(), c->T.pts.end());
-    for (int v : c->T.edgelist) U.edgelist.push_back(v + v0);
-    U.vertex_bdr.insert(U.vertex_bdr.end(), c->T.vertex_bdr.begin(), c->T.vertex_bdr.end());
+    std::copy(c->T.pts.begin(), c->T.pts.end(), U.pts.begin() + (size_t)v0 * 3);
+    std::copy(c->T.vertex_bdr.begin(), c->T.vertex_bdr.end(), U.vertex_bdr.begin() + v0);
+    for (size_t i = 0; i < c->T.edgelist.size(); ++i) U.edgelist[(size_t)e0 * 2 + i] = c->T.edgelist[i] + v0;
     // tets (the pattern of the auxiliary Galerkin products)
-    for (int v : c->T.tets) U.tets.push_back(v + v0);
-    U.nt += c->T.nt;
+    for (size_t i = 0; i < c->T.tets.size(); ++i) U.tets[(size_t)toff[k] * 4 + i] = c->T.tets[i] + v0;
+  };
+  {
+    const int nth = std::max(1, std::min(ns, host_threads()));
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> err(nth);
+    for (int t = 0; t < nth; ++t)
+      th.emplace_back([&, t]() {
+        try {
+          for (int k = t; k < ns; k += nth) stack_model(k);
+        } catch (...) {
+          err[t] = std::current_exception();
+        }
+      });
+    for (auto &x : th) x.join();
+    for (auto &e : err)
+      if (e) std::rethrow_exception(e);
+  }
+  for (int k = 0; k < ns; ++k) {
+    emfem_context *c = ctxs[S[k]];
+    EMF_HIP(hipMemcpyAsync(K.get() + nzoff[k], c->d_K.get(), sizeof(double2) * c->nnz, hipMemcpyDeviceToDevice, s));
   }
   rp.upload(B.rp, s);
   ci.upload(B.ci, s);
+  out.timing["stack"] += now_ms() - t0;
   // segments: model k owns rows [erow0[k], erow0[k+1]); reduction chunks of at most
   // kChunkRows rows never straddle two models
   std::vector<int> chunk_row, seg_chunk(ns + 1, 0);
@@ -136,8 +162,11 @@ void solve_stage(emfem_context **ctxs, const std::vector<int> &S, int first, int
   PcOptions o = pc_options(c0);
   AmsHost H;
   build_ams(U, B, o.amg, H);
+  for (auto &kv : H.timing) out.timing[kv.first.substr(0, kv.first.size() - 3)] += kv.second;
+  const double tp = now_ms();
   auto pc = make_inner_pc(c0->opt("inner_precision") == "fp32", H, (int)ne, (int)nnz, rp.get(), ci.get(), K.get(), o,
                           s);
+  out.timing["device_pc"] += now_ms() - tp;
   pc->set_use_graph(c0->opt("use_graph") == "true");
   // gather the right-hand sides (and the warm-start iterates), solve, scatter
   const int nc = 2 * n;
@@ -203,6 +232,7 @@ extern "C" int emfem_solve_batch(emfem_context **ctxs, int n_ctx, int first, int
     for (int k = 0; k < n_ctx; ++k) live[k] = k;
     const double t_begin = now_ms();
     double setup = 0.0, solve = 0.0;
+    std::map<std::string, double> phase_ms;
     long inner = 0;
     int stages = 0, fell_back = 0;
     bool warm = false, complete = !retire;
@@ -222,6 +252,7 @@ extern "C" int emfem_solve_batch(emfem_context **ctxs, int n_ctx, int first, int
       ++stages;
       setup += so.setup_ms;
       solve += so.solve_ms;
+      for (auto &kv : so.timing) phase_ms[kv.first] += kv.second;
       inner += so.st.inner_its;
       fell_back |= so.st.fell_back;
       std::vector<int> next;
@@ -246,6 +277,7 @@ extern "C" int emfem_solve_batch(emfem_context **ctxs, int n_ctx, int first, int
       if (stop == 0) break;
     }
     c0->stats["batch_setup_ms"] = setup;
+    for (auto &kv : phase_ms) c0->stats["batch_setup_" + kv.first + "_ms"] = kv.second;
     c0->stats["batch_solve_ms"] = solve;
     c0->stats["batch_wall_ms"] = now_ms() - t_begin;
     c0->stats["batch_stages"] = stages;

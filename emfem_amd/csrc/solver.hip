@@ -242,8 +242,10 @@ class DevAms {
     nblk_ = X.nblk;
     aux_csr_ = o.aux_csr;
     if (aux_csr_) {
-      Pi_.upload(X.Pi, s);
-      PiT_.upload(X.PiT, s);
+      HostCsr Pi, PiT;
+      build_pi_csr(X, Pi, PiT);
+      Pi_.upload(Pi, s);
+      PiT_.upload(PiT, s);
     } else {
       upload_aux_map(X, s);
     }
