@@ -144,9 +144,18 @@ struct emfem_context {
     std::unique_ptr<AmsHost> host;
     std::thread worker;
     std::exception_ptr err;
+    // the MT right-hand sides of fidx (XY, YX; re and im) from a second worker thread,
+    // which needs no GPU; mt_ok once they are complete
+    std::thread rhs_worker;
+    std::vector<double> mt[2][2];
+    bool mt_ok = false;
     double started_ms = 0.0, worker_ms = 0.0;
   } prep;
   hipStream_t s2 = nullptr;
+  // the dual right-hand side depends only on the mesh and the receivers: kept until
+  // either changes
+  std::vector<double> dual_re, dual_im;
+  bool dual_ok = false;
 
   std::map<std::string, double> stats;
   double assemble_kernel_ms = 0.0;
@@ -173,6 +182,8 @@ namespace emfem_ctx {
 inline void discard_prepared(emfem_context *c) {
   auto &P = c->prep;
   if (P.worker.joinable()) P.worker.join();
+  if (P.rhs_worker.joinable()) P.rhs_worker.join();
+  P.mt_ok = false;
   if (c->s2) (void)hipStreamSynchronize(c->s2);
   P.fidx = -1;
   P.k_taken = false;

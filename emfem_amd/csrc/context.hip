@@ -249,6 +249,7 @@ int emfem_set_mesh(emfem_context *c, int n_points, const double *points, int n_t
   return guarded(c, [&]() {
     EMF_REQUIRE(n_points > 0 && n_tets > 0 && points && tets && attributes, kErrArg, "empty or null mesh arrays");
     discard_prepared(c);  // it read the mesh and the conductivity table
+    c->dual_ok = false;
     c->pc.reset();
     c->pc_prev.reset();
     c->have_system = false;
@@ -273,6 +274,7 @@ int emfem_set_mesh_edges(emfem_context *c, int n_edges, const int *edgelist, con
     require_mesh(c);
     EMF_REQUIRE(edgelist && tet2edge, kErrArg, "null edge arrays");
     discard_prepared(c);  // it read the mesh and the conductivity table
+    c->dual_ok = false;
     c->pc.reset();
     c->pc_prev.reset();
     c->have_system = false;
@@ -318,6 +320,7 @@ int emfem_set_receivers(emfem_context *c, int n, const double *rx) {
   return guarded(c, [&]() {
     EMF_REQUIRE(n >= 0 && (n == 0 || rx), kErrArg, "bad receiver array");
     c->rx.assign(rx, rx + (size_t)n * 3);
+    c->dual_ok = false;
   });
 }
 
@@ -359,6 +362,7 @@ int emfem_update_background_model(emfem_context *c) {
   return guarded(c, [&]() {
     require_mesh(c);
     EMF_REQUIRE(c->n_rho > 0, kErrState, "no resistivity table");
+    discard_prepared(c);  // its right-hand sides used the background model
     update_background(c->T, c->s6, c->bg);
   });
 }
@@ -493,6 +497,23 @@ int emfem_prepare_pc(emfem_context *c, int fidx) {
     const Topology *T = &c->T;
     std::shared_ptr<const AmsAux> aux = c->ams_aux;
     const AmgParams amg = P.o.amg;
+    if (c->bg.ready) {
+      // the MT right-hand sides of fidx on their own worker (host only)
+      const std::vector<double> *s6 = &c->s6;
+      const Background *bg = &c->bg;
+      const double freq = c->freqs[fidx];
+      P.mt_ok = false;
+      P.rhs_worker = std::thread([pp, T, s6, bg, freq]() {
+        try {
+          host_thread_cap() = 4;
+          rhs_mt(*T, *s6, *bg, freq, EMFEM_XY_POLAR, pp->mt[0][0], pp->mt[0][1]);
+          rhs_mt(*T, *s6, *bg, freq, EMFEM_YX_POLAR, pp->mt[1][0], pp->mt[1][1]);
+          pp->mt_ok = true;
+        } catch (...) {
+          pp->mt_ok = false;  // assemble_rhs_mt then computes them itself
+        }
+      });
+    }
     P.worker = std::thread([pp, T, aux, amg, z]() {
       try {
         host_thread_cap() = 8;
@@ -583,10 +604,22 @@ int emfem_assemble_rhs_mt(emfem_context *c, int fidx, int mode, int slot) {
     require_slot(c, slot);
     require_freq(c, fidx);
     EMF_REQUIRE(mode == EMFEM_XY_POLAR || mode == EMFEM_YX_POLAR, kErrArg, "mode must be -1 (XY) or -2 (YX)");
+    auto &P = c->prep;
+    if (P.fidx == fidx && (P.rhs_worker.joinable() || P.mt_ok)) {
+      // computed by emfem_prepare_pc's right-hand-side worker
+      if (P.rhs_worker.joinable()) P.rhs_worker.join();
+      if (P.mt_ok) {
+        const int m = mode == EMFEM_XY_POLAR ? 0 : 1;
+        put_slot(c, c->d_S, slot, P.mt[m][0].data(), P.mt[m][1].data());
+        c->stats["rhs_prepared"] = 1.0;
+        return;
+      }
+    }
     if (!c->bg.ready) update_background(c->T, c->s6, c->bg);
     std::vector<double> re, im;
     rhs_mt(c->T, c->s6, c->bg, c->freqs[fidx], mode, re, im);
     put_slot(c, c->d_S, slot, re.data(), im.data());
+    c->stats["rhs_prepared"] = 0.0;
   });
 }
 
@@ -604,9 +637,11 @@ int emfem_assemble_rhs_csem(emfem_context *c, int fidx, int tidx, int slot) {
 int emfem_assemble_rhs_dual(emfem_context *c, int slot) {
   return guarded(c, [&]() {
     require_slot(c, slot);
-    std::vector<double> re, im;
-    rhs_dual(c->T, c->rx, locator(c), re, im);
-    put_slot(c, c->d_S, slot, re.data(), im.data());
+    if (!c->dual_ok) {
+      rhs_dual(c->T, c->rx, locator(c), c->dual_re, c->dual_im);
+      c->dual_ok = true;
+    }
+    put_slot(c, c->d_S, slot, c->dual_re.data(), c->dual_im.data());
   });
 }
 

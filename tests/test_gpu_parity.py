@@ -457,10 +457,10 @@ def test_refreshed_preconditioner_matches_a_fresh_one(precision):
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_prepared_setup_matches_setup_in_turn(precision):
     """emfem_prepare_pc(next) started before the current solve: the next frequency's
-    matrix (assembled on the context's second stream), its auxiliary-space set-up
-    (built on the worker thread) and so its solve equal, bit for bit, those of the
-    same calls without it; a prepared frequency that is not the one assembled next is
-    discarded and changes nothing."""
+    matrix (assembled on the context's second stream), its MT right-hand sides and its
+    auxiliary-space set-up (built on worker threads) and so its solve equal, bit for
+    bit, those of the same calls without it; a prepared frequency that is not the one
+    assembled next is discarded and changes nothing."""
     m = mg.halfspace_mt(n=4, n_rx=3)
     m.freqs = np.array([0.3, 3.0, 30.0])
     opts = "-inner_precision %s -spmm_rows rowblock" % precision
@@ -476,7 +476,8 @@ def test_prepared_setup_matches_setup_in_turn(precision):
         its, rr = ctx.solve_linear_system(0, 3, 300, 1e-10)
         assert np.all(rr <= 1e-10 * 1.0001), rr
         _, _, C, M = ctx.get_matrix()
-        return its, [ctx.get_solution(s) for s in range(3)], C, M
+        rhs = [np.concatenate(ctx.get_rhs(s)) for s in range(3)]
+        return its, [ctx.get_solution(s) for s in range(3)], C, M, rhs
 
     a = _ctx(m, opts)
     a.create_linear_system(3)
@@ -487,9 +488,13 @@ def test_prepared_setup_matches_setup_in_turn(precision):
     step(b, 0)
     ra = step(a, 1, nxt=0)
     assert a.stat("assemble_prepared") == 1.0 and a.stat("pc_prepared") == 1.0
+    assert a.stat("rhs_prepared") == 1.0
     rb = step(b, 1)
     assert b.stat("assemble_prepared") == 0.0 and b.stat("pc_prepared") == 0.0
-    for x, y in zip(ra[2:], rb[2:]):
+    assert b.stat("rhs_prepared") == 0.0
+    for x, y in zip(ra[2:4], rb[2:4]):
+        assert np.array_equal(x, y)
+    for x, y in zip(ra[4], rb[4]):
         assert np.array_equal(x, y)
     assert np.array_equal(ra[0], rb[0])
     for x, y in zip(ra[1], rb[1]):
