@@ -501,7 +501,7 @@ std::shared_ptr<const AmsAux> build_ams_aux(const Topology &T, bool additive) {
   X->nv0 = nv0;
   EMF_REQUIRE(nv0 > 0, kErrUser, "mesh has no interior vertex");
   // the row pointers G0 and the three Pi_d share: one entry per interior endpoint
-  std::vector<int> erp(T.ne + 1, 0);
+  HostVec<int> erp(T.ne + 1, 0);
   parallel_ranges(T.ne, [&](int64_t b, int64_t e, int) {
     for (int64_t i = b; i < e; ++i) erp[i + 1] = (vmap[T.edgelist[i * 2]] >= 0) + (vmap[T.edgelist[i * 2 + 1]] >= 0);
   });
@@ -718,7 +718,7 @@ void build_ams(const Topology &T, const GalerkinFill &fill, const AmgParams &prm
     AP.ci.assign(aux->APpat.ci.begin(), aux->APpat.ci.end());
   }
   AP.v.resize(AP.ci.size());
-  std::vector<double> vg;
+  HostVec<double> vg;
   double *out[4];
   for (int k = 0; k < 4; ++k) {
     const int d = additive ? k : k - 1;

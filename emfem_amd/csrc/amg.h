@@ -23,10 +23,35 @@
 
 namespace emfem {
 
+// Allocator whose resize leaves new elements uninitialised (default-initialised): the
+// large host arrays of the set-up are then first written, and their pages first
+// touched, by the parallel loops that fill them instead of by a single-threaded fill
+// with zeros.  Explicit values (assign, resize with a value) behave as usual.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U> &) {}
+  template <typename U>
+  void construct(U *p) {
+    ::new (static_cast<void *>(p)) U;
+  }
+  template <typename U, typename... A>
+  void construct(U *p, A &&...a) {
+    ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using HostVec = std::vector<T, NoInitAlloc<T>>;
+
 struct HostCsr {
   int n = 0, m = 0;  // rows, columns
-  std::vector<int> rp, ci;
-  std::vector<double> v;
+  HostVec<int> rp, ci;
+  HostVec<double> v;
   int64_t nnz() const { return (int64_t)ci.size(); }
 };
 

@@ -94,16 +94,15 @@ void solve_stage(emfem_context **ctxs, const std::vector<int> &S, int first, int
     emfem_context *c = ctxs[S[k]];
     const int e0 = erow0[k], z0 = nzoff[k], v0 = voff[k];
     std::vector<int> hrp(c->T.ne + 1);
-    std::vector<double2> hK(c->nnz);
+    // B = C + M formed on the device: half the bytes of (C, M) travel
+    DevBuf<double> bm(c->nnz);
+    make_b<double>(c->stream, c->nnz, c->d_K.get(), bm.get());
     c->d_rp.download(hrp.data(), hrp.size(), c->stream);
     c->d_ci.download(&B.ci[z0], c->nnz, c->stream);
-    c->d_K.download(hK.data(), c->nnz, c->stream);
+    bm.download(&B.v[z0], c->nnz, c->stream);
     EMF_HIP(hipStreamSynchronize(c->stream));
     for (int r = 0; r < c->T.ne; ++r) B.rp[e0 + r + 1] = z0 + hrp[r + 1];
-    for (int i = 0; i < c->nnz; ++i) {
-      B.v[z0 + i] = hK[i].x + hK[i].y;
-      B.ci[z0 + i] += e0;
-    }
+    for (int i = 0; i < c->nnz; ++i) B.ci[z0 + i] += e0;
     std::copy(c->T.pts.begin(), c->T.pts.end(), U.pts.begin() + (size_t)v0 * 3);
     std::copy(c->T.vertex_bdr.begin(), c->T.vertex_bdr.end(), U.vertex_bdr.begin() + v0);
     for (size_t i = 0; i < c->T.edgelist.size(); ++i) U.edgelist[(size_t)e0 * 2 + i] = c->T.edgelist[i] + v0;

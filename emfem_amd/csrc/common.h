@@ -78,7 +78,10 @@ class DevBuf {
     resize(n);
     if (n) EMF_HIP(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
   }
-  void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
+  template <typename A>
+  void upload(const std::vector<T, A> &h, hipStream_t s) {
+    upload(h.data(), h.size(), s);
+  }
   void download(T *h, size_t n, hipStream_t s) const {
     if (n) EMF_HIP(hipMemcpyAsync(h, p_, n * sizeof(T), hipMemcpyDeviceToHost, s));
   }

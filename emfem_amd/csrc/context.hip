@@ -383,12 +383,15 @@ int emfem_create_linear_system(emfem_context *c, int n_slots) {
     c->fgw = FgmresWork();
     c->d_li.release();
     c->d_lo.release();
-    std::vector<int> &rp = c->h_B.rp, &ci = c->h_B.ci;
+    std::vector<int> rp, ci;
     build_pattern(c->T, rp, ci);
+    c->h_B.rp.assign(rp.begin(), rp.end());
+    c->h_B.ci.assign(ci.begin(), ci.end());
     c->h_B.n = c->h_B.m = c->T.ne;
     c->nnz = (int)ci.size();
     c->d_rp.upload(rp, c->stream);
     c->d_ci.upload(ci, c->stream);
+    EMF_HIP(hipStreamSynchronize(c->stream));  // rp and ci are this scope's
     c->d_ci16.release();
     c->d_ci16_base.release();
     c->col16_state = -1;
