@@ -712,11 +712,14 @@ int emfem_create_pc(emfem_context *c) {
     PcOptions o = pc_options(c);
     // a host set-up prepared for this matrix (emfem_prepare_pc) with the same options
     auto &P = c->prep;
-    const bool prepared = on_device && P.fidx >= 0 && P.fidx == c->assembled_fidx && P.k_taken &&
-                          P.o.amg.theta == o.amg.theta && P.o.amg.smooth_levels == o.amg.smooth_levels &&
+    const bool same_amg = P.o.amg.theta == o.amg.theta && P.o.amg.smooth_levels == o.amg.smooth_levels &&
                           P.o.amg.additive == o.amg.additive && P.o.amg.max_coarse == o.amg.max_coarse &&
                           P.o.amg.max_levels == o.amg.max_levels;
-    if (P.fidx == c->assembled_fidx && !prepared) discard_prepared(c);  // unusable for this matrix
+    const bool prepared = on_device && P.fidx >= 0 && P.fidx == c->assembled_fidx && P.k_taken && same_amg;
+    // a preparation unusable for this matrix is dropped, and so is one for another
+    // frequency made with other set-up options (this set-up may rebuild the device
+    // Galerkin pattern its stream reads)
+    if (P.fidx >= 0 && !prepared && (P.fidx == c->assembled_fidx || !same_amg || !on_device)) discard_prepared(c);
     DevBuf<double> bsum;
     HostCsr &B = c->h_B;
     if (!prepared) {
