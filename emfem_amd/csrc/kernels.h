@@ -94,6 +94,10 @@ std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
 constexpr int kCol16Group = 64;
 bool compress_columns(hipStream_t s, int nnz, const int *ci, int *base, uint16_t *off, int *flag);
 
+// out[p] = v[p] * d[ci[p]]: the values of A diag(d) on A's pattern.
+template <typename T>
+void scale_columns(hipStream_t s, int nnz, const int *ci, const T *v, const T *d, T *out);
+
 // A Chebyshev step fused into an SpMM epilogue (off when x is null): the SpMM's value
 // v is not stored; y(row, j) receives x + c1 (x - d) + c2 v at (row, j) instead (in T
 // with the coefficients rounded to T; d null = 0).  x is normally the SpMM's input

@@ -100,10 +100,29 @@ class DevAmg {
     nc_ = 0;  // the level blocks follow the (possibly new) shapes on the next ensure
     levels_.resize(H.levels.size());
     for (size_t l = 0; l < H.levels.size(); ++l) {
-      levels_[l].A.upload(H.levels[l].A, s);
-      levels_[l].P.upload(H.levels[l].P, s);
-      levels_[l].R.upload(H.levels[l].R, s);
-      upload_vec(levels_[l].l1inv, H.levels[l].l1inv, s);
+      Level &L = levels_[l];
+      L.A.upload(H.levels[l].A, s);
+      L.P.upload(H.levels[l].P, s);
+      L.R.upload(H.levels[l].R, s);
+      upload_vec(L.l1inv, H.levels[l].l1inv, s);
+      // large levels: the LDS-staged row-block SpMM on 16-bit columns, with the
+      // residual's diagonal scaling folded into a second value array A diag(l1inv)
+      L.blk.clear();
+      L.rowblock = false;
+      if (L.A.n >= kRowBlockRows && L.A.n == L.A.m) {
+        L.hrp.assign(H.levels[l].A.rp.begin(), H.levels[l].A.rp.end());
+        L.ci16.resize(L.A.nnz);
+        L.ci16_base.resize((L.A.nnz + kCol16Group - 1) / kCol16Group);
+        DevBuf<int> flag(1);
+        if (compress_columns(s, L.A.nnz, L.A.ci.get(), L.ci16_base.get(), L.ci16.get(), flag.get())) {
+          L.avs.resize(L.A.nnz);
+          scale_columns<T>(s, L.A.nnz, L.A.ci.get(), L.A.v.get(), L.l1inv.get(), L.avs.get());
+          L.rowblock = true;
+        } else {
+          L.ci16.release();
+          L.ci16_base.release();
+        }
+      }
     }
     ncoarse_ = H.ncoarse;
     std::vector<double> t((size_t)ncoarse_ * ncoarse_);
@@ -111,6 +130,21 @@ class DevAmg {
       for (int j = 0; j < ncoarse_; ++j) t[(size_t)j * ncoarse_ + i] = H.coarse_inv[(size_t)i * ncoarse_ + j];
     cinv_.upload(t, s);  // transposed for coalesced reads in the dense solve
     EMF_HIP(hipStreamSynchronize(s));
+  }
+
+  // row-block tables of the large levels for nc columns (outside graph capture: it
+  // synchronises s); a level whose rows do not fit a tile keeps the lane kernels
+  void prepare_rowblocks(hipStream_t s, int nc) {
+    if (nc != 3 && nc != 6) return;
+    for (Level &L : levels_) {
+      if (!L.rowblock || L.blk.count(nc)) continue;
+      const std::vector<int> t = row_blocks(L.hrp, stream_tile_for(nc, sizeof(T) == 8));
+      DevBuf<int> &d = L.blk[nc];
+      if (!t.empty()) {
+        d.upload(t, s);
+        EMF_HIP(hipStreamSynchronize(s));  // t is this scope's
+      }
+    }
   }
 
   void ensure(int nc) {
@@ -133,10 +167,34 @@ class DevAmg {
   void vcycle(hipStream_t s, int nc, const T *b, T *out) { cycle(s, 0, nc, b, out); }
 
  private:
+  // levels of at least this many rows use the row-block SpMM on 3 and 6 columns
+  static constexpr int kRowBlockRows = 100000;
   struct Level {
     DevCsr<T> A, P, R;
     DevBuf<T> l1inv, b, out, x, r;
+    bool rowblock = false;
+    std::vector<int> hrp;                 // host row pointers (row-block tables)
+    DevBuf<uint16_t> ci16;                // 16-bit columns of A
+    DevBuf<int> ci16_base;
+    DevBuf<T> avs;                        // A diag(l1inv)
+    std::map<int, DevBuf<int>> blk;       // row-block table per column count
   };
+
+  // A (or A diag(l1inv) when scaled) as a row-block view for nc columns; false when
+  // the level or nc does not take that kernel (tables from prepare_rowblocks)
+  bool rowblock_view(Level &L, int nc, bool scaled, Csr<T> &out) {
+    if (!L.rowblock || (nc != 3 && nc != 6)) return false;
+    auto it = L.blk.find(nc);
+    if (it == L.blk.end() || it->second.size() == 0) return false;
+    out = L.A.view();
+    if (scaled) out.v = L.avs.get();
+    out.lpr = -1;
+    out.blk_row = it->second.get();
+    out.nblk = (int)it->second.size() / 2 - 1;
+    out.ci16 = L.ci16.get();
+    out.ci16_base = L.ci16_base.get();
+    return true;
+  }
 
   void cycle(hipStream_t s, size_t l, int nc, const T *b, T *out) {
     if (l == levels_.size()) {
@@ -148,7 +206,11 @@ class DevAmg {
     const T *dinv = L.l1inv.get();
     // pre-smoothing from a zero guess, x0 = Dinv b, folded into the next two
     // launches: r = b - A Dinv b ; restrict
-    spmm_ex<T, T, T>(s, L.A.view(), nc, b, nc, r, nc, T(-1), nullptr, T(1), b, nc, T(0), nullptr, 0, dinv, nullptr);
+    Csr<T> Ab;
+    if (rowblock_view(L, nc, true, Ab))
+      spmm<T>(s, Ab, nc, b, nc, r, nc, T(-1), nullptr, T(1), b, nc, T(0), nullptr, 0);
+    else
+      spmm_ex<T, T, T>(s, L.A.view(), nc, b, nc, r, nc, T(-1), nullptr, T(1), b, nc, T(0), nullptr, 0, dinv, nullptr);
     const bool last = (l + 1 == levels_.size());
     T *cb = last ? coarse_b_.get() : levels_[l + 1].b.get();
     T *cx = last ? coarse_x_.get() : levels_[l + 1].out.get();
@@ -156,8 +218,19 @@ class DevAmg {
     cycle(s, l + 1, nc, cb, cx);
     // x = Dinv b + P xc
     spmm_ex<T, T, T>(s, L.P.view(), nc, cx, nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), b, nc, nullptr, dinv);
-    // post-smoothing: out = x + l1inv .* (b - A x)
-    spmm<T>(s, L.A.view(), nc, x, nc, out, nc, T(-1), L.l1inv.get(), T(1), b, nc, T(1), x, nc);
+    // post-smoothing: out = x + l1inv .* (b - A x) (the row blocks take it as the
+    // Chebyshev step x + 0 (x - 0) + 1 l1inv .* (b - A x))
+    if (rowblock_view(L, nc, false, Ab)) {
+      ChebFuse<T> cf;
+      cf.x = x;
+      cf.ld = nc;
+      cf.c1 = 0.0;
+      cf.c2 = 1.0;
+      spmm_ex<T, T, T>(s, Ab, nc, x, nc, out, nc, T(-1), L.l1inv.get(), T(1), b, nc, T(0), nullptr, 0, nullptr, nullptr,
+                       cf);
+    } else {
+      spmm<T>(s, L.A.view(), nc, x, nc, out, nc, T(-1), L.l1inv.get(), T(1), b, nc, T(1), x, nc);
+    }
   }
 
   std::vector<Level> levels_;
@@ -316,6 +389,8 @@ class DevAms {
   // row blocks), timed once per nc (outside any graph capture) on scratch blocks.
   void tune(hipStream_t s, int nc) {
     ensure(nc);
+    if (!additive_) amgG_.prepare_rowblocks(s, nc);
+    amgPi_.prepare_rowblocks(s, nc);
     if (lpr_.count(nc)) return;
     d_.zero(s);
     // narrow x-blocks only on request: tuned in isolation they can win for the fp64 B

@@ -1089,6 +1089,13 @@ __global__ void k_col16_base(int nnz, const int *__restrict__ ci, int *base, int
   if (hi - lo > 65535) flag[0] = 1;
 }
 
+template <typename T>
+__global__ void k_scale_columns(int nnz, const int *__restrict__ ci, const T *__restrict__ v, const T *__restrict__ d,
+                                T *out) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p < nnz) out[p] = v[p] * d[ci[p]];
+}
+
 __global__ void k_col16_off(int nnz, const int *__restrict__ ci, const int *__restrict__ base, uint16_t *off) {
   const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= nnz) return;
@@ -1096,6 +1103,15 @@ __global__ void k_col16_off(int nnz, const int *__restrict__ ci, const int *__re
 }
 
 }  // namespace
+
+template <typename T>
+void scale_columns(hipStream_t s, int nnz, const int *ci, const T *v, const T *d, T *out) {
+  if (nnz <= 0) return;
+  hipLaunchKernelGGL((k_scale_columns<T>), dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, ci, v, d, out);
+  EMF_HIP(hipGetLastError());
+}
+template void scale_columns<float>(hipStream_t, int, const int *, const float *, const float *, float *);
+template void scale_columns<double>(hipStream_t, int, const int *, const double *, const double *, double *);
 
 bool compress_columns(hipStream_t s, int nnz, const int *ci, int *base, uint16_t *off, int *flag) {
   if (nnz <= 0) return false;
