@@ -94,6 +94,13 @@ std::vector<int> row_blocks(const std::vector<int> &rp, int tile);
 constexpr int kCol16Group = 64;
 bool compress_columns(hipStream_t s, int nnz, const int *ci, int *base, uint16_t *off, int *flag);
 
+// y = P xc + zs .* z for a P with exactly one entry per row (column pci[row], value
+// pv[row]): the plain-aggregation prolongation of the AMG cycle as one elementwise
+// pass; vectors [rows][nc], xc [*][nc].
+template <typename T>
+void prolong_single(hipStream_t s, int rows, int nc, const int *pci, const T *pv, const T *xc, const T *zs,
+                    const T *z, T *y);
+
 // out[p] = v[p] * d[ci[p]]: the values of A diag(d) on A's pattern.
 template <typename T>
 void scale_columns(hipStream_t s, int nnz, const int *ci, const T *v, const T *d, T *out);

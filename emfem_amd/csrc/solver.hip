@@ -103,6 +103,13 @@ class DevAmg {
       Level &L = levels_[l];
       L.A.upload(H.levels[l].A, s);
       L.P.upload(H.levels[l].P, s);
+      {
+        // one entry per row (plain aggregation): the elementwise prolongation
+        const HostCsr &P = H.levels[l].P;
+        bool single = (int64_t)P.ci.size() == (int64_t)P.n;
+        for (int i = 0; single && i <= P.n; ++i) single = P.rp[i] == i;
+        L.p_single = single;
+      }
       L.R.upload(H.levels[l].R, s);
       upload_vec(L.l1inv, H.levels[l].l1inv, s);
       // large levels: the LDS-staged row-block SpMM on 16-bit columns, with the
@@ -173,6 +180,7 @@ class DevAmg {
     DevCsr<T> A, P, R;
     DevBuf<T> l1inv, b, out, x, r;
     bool rowblock = false;
+    bool p_single = false;                // P has one entry per row
     std::vector<int> hrp;                 // host row pointers (row-block tables)
     DevBuf<uint16_t> ci16;                // 16-bit columns of A
     DevBuf<int> ci16_base;
@@ -217,7 +225,10 @@ class DevAmg {
     spmm<T>(s, L.R.view(), nc, r, nc, cb, nc, T(1), nullptr, T(0), nullptr, 0, T(0), nullptr, 0);
     cycle(s, l + 1, nc, cb, cx);
     // x = Dinv b + P xc
-    spmm_ex<T, T, T>(s, L.P.view(), nc, cx, nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), b, nc, nullptr, dinv);
+    if (L.p_single)
+      prolong_single<T>(s, L.P.n, nc, L.P.ci.get(), L.P.v.get(), cx, dinv, b, x);
+    else
+      spmm_ex<T, T, T>(s, L.P.view(), nc, cx, nc, x, nc, T(1), nullptr, T(0), nullptr, 0, T(1), b, nc, nullptr, dinv);
     // post-smoothing: out = x + l1inv .* (b - A x) (the row blocks take it as the
     // Chebyshev step x + 0 (x - 0) + 1 l1inv .* (b - A x))
     if (rowblock_view(L, nc, false, Ab)) {

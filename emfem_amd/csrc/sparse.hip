@@ -1090,6 +1090,15 @@ __global__ void k_col16_base(int nnz, const int *__restrict__ ci, int *base, int
 }
 
 template <typename T>
+__global__ void k_prolong_single(int64_t n, int nc, const int *__restrict__ pci, const T *__restrict__ pv,
+                                 const T *__restrict__ xc, const T *__restrict__ zs, const T *__restrict__ z, T *y) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int row = (int)(i / nc), c = (int)(i % nc);
+  y[i] = pv[row] * xc[(size_t)pci[row] * nc + c] + zs[row] * z[i];
+}
+
+template <typename T>
 __global__ void k_scale_columns(int nnz, const int *__restrict__ ci, const T *__restrict__ v, const T *__restrict__ d,
                                 T *out) {
   const int p = blockIdx.x * kBlock + threadIdx.x;
@@ -1103,6 +1112,19 @@ __global__ void k_col16_off(int nnz, const int *__restrict__ ci, const int *__re
 }
 
 }  // namespace
+
+template <typename T>
+void prolong_single(hipStream_t s, int rows, int nc, const int *pci, const T *pv, const T *xc, const T *zs,
+                    const T *z, T *y) {
+  const int64_t n = (int64_t)rows * nc;
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_prolong_single<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, nc, pci, pv, xc, zs, z, y);
+  EMF_HIP(hipGetLastError());
+}
+template void prolong_single<float>(hipStream_t, int, int, const int *, const float *, const float *, const float *,
+                                    const float *, float *);
+template void prolong_single<double>(hipStream_t, int, int, const int *, const double *, const double *,
+                                     const double *, const double *, double *);
 
 template <typename T>
 void scale_columns(hipStream_t s, int nnz, const int *ci, const T *v, const T *d, T *out) {
