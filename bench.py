@@ -291,7 +291,7 @@ def batch_leg(api, mg, n_models, n_single=8):
     edges = np.array([c.mesh_info()["n_edges"] for c in ctxs])
     out = {"workload": "%d uneven MT models, %d..%d edges (%d in all), f log-uniform 1e-2..10 Hz, XY+YX"
                        % (len(ctxs), edges.min(), edges.max(), edges.sum())}
-    for name, retire in (("lock_step", "false"), ("staged", "true")):
+    for name, retire in (("staged", "true"), ("lock_step", "false")):
         ctxs[0].process_options("-batch_retire %s" % retire)
         ta = time.perf_counter()
         its, rr, ms = api.solve_batch(ctxs, 0, 2, 300, 1e-8)
@@ -309,7 +309,7 @@ def batch_leg(api, mg, n_models, n_single=8):
             "p50 %.0f ms p99 %.0f ms, its max %d, relres max %.1e"
             % (name, len(ctxs), edges.sum(), t1 - t0, wall, setup, solve, out[name]["stages"],
                out[name]["per_model_ms_p50"], out[name]["per_model_ms_p99"], its.max(), rr.max()))
-    ctxs[0].process_options("-batch_retire false")
+    ctxs[0].process_options("-batch_retire true")
     seq = []
     for c in ctxs[:n_single]:
         ta = time.perf_counter()

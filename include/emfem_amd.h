@@ -203,9 +203,11 @@ int emfem_solve_linear_system(emfem_context *ctx, int first, int n, int max_it, 
  * (any may be NULL) are indexed k * n + j for model k, slot first + j: outer
  * iterations, true FP64 relative residual, and the wall time (ms) from the start of
  * the call (stack set-up included) until that right-hand side was accepted.  With
- * -batch_retire true the solve runs in stages: each stops once half of the
- * right-hand sides still open have passed, models whose right-hand sides have all
- * passed retire, and the rest are restacked and continue from their iterates.
+ * -batch_retire true (the default) the solve runs in stages: each stops once half of
+ * the right-hand sides still open have passed, models whose right-hand sides have all
+ * passed retire, and the rest are restacked and continue from their iterates; at 8
+ * models or fewer one stage runs to completion.  -batch_retire false runs one
+ * lock-step solve.
  * Options (precision, restart, cycle, batch_retire) are taken from ctxs[0].  Returns
  * EMFEM_ERR_NOT_CONVERGED like emfem_solve_linear_system. */
 int emfem_solve_batch(emfem_context **ctxs, int n_ctx, int first, int n, int max_it, double rtol, int *iters,
