@@ -6,6 +6,9 @@
 #include <cmath>
 #include <numeric>
 #include <string>
+#include <thread>
+#include <unordered_map>
+#include <utility>
 
 #include "common.h"
 
@@ -347,14 +350,35 @@ HostCsr smoothed_prolongator(const HostCsr &A, const std::vector<double> &d, con
   return P;
 }
 
+// Runs f(0) .. f(count - 1) on the host threads, each task taken by the next free
+// thread; the tasks write disjoint outputs, so results do not depend on the order.
+template <typename F>
+void run_tasks(int count, F &&f) {
+  const int nth = std::min(count, host_threads());
+  if (nth <= 1) {
+    for (int k = 0; k < count; ++k) f(k);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; ++t)
+    th.emplace_back([&]() {
+      for (int k = next++; k < count; k = next++) f(k);
+    });
+  for (auto &x : th) x.join();
+}
+
 // Dense SPD inverse by Cholesky; a vanishing pivot (singular coarse problem) is
 // regularised, which only affects the null-space component.  The matrix is inverted
 // per connected component of its graph (the blocks of a block-diagonal matrix, e.g. the
-// auxiliary blocks and the models of a stacked batch), components in parallel; the
-// Cholesky of the whole matrix would do the same operations on each block plus exact
-// zeros, and the regularisation threshold stays relative to the largest diagonal entry
-// of the whole matrix.
-std::vector<double> dense_inverse(const HostCsr &A) {
+// auxiliary blocks and the models of a stacked batch): the Cholesky of the whole
+// matrix would do the same operations on each block plus exact zeros, and the
+// regularisation threshold stays relative to the largest diagonal entry of the whole
+// matrix.  The components are factored in parallel, then their columns are solved in
+// parallel chunks.  inv_t receives the inverse transposed (row j holds column j of the
+// inverse, zero outside its component), span[2 i] and span[2 i + 1] the first and one
+// past the last row of row i's component.
+void dense_inverse(const HostCsr &A, std::vector<double> &inv_t, std::vector<int> &span) {
   const int n = A.n;
   double dmax = 0.0;
   std::vector<int> root(n);
@@ -380,61 +404,112 @@ std::vector<double> dense_inverse(const HostCsr &A) {
     }
     comps[comp_of[r]].push_back(i);
   }
-  std::vector<double> inv((size_t)n * n, 0.0);
-  auto invert = [&](const std::vector<int> &idx) {
+  span.assign(2 * (size_t)n, 0);
+  for (const auto &idx : comps)
+    for (int i : idx) {
+      span[2 * (size_t)i] = idx.front();
+      span[2 * (size_t)i + 1] = idx.back() + 1;
+    }
+  inv_t.assign((size_t)n * n, 0.0);
+  // Cholesky factor L of each component and its transpose (the back substitution
+  // reads L by columns)
+  const int nc = (int)comps.size();
+  std::vector<std::vector<double>> Ls(nc), LTs(nc);
+  run_tasks(nc, [&](int kc) {
+    const std::vector<int> &idx = comps[kc];
     const int m = (int)idx.size();
-    std::vector<int> loc(n, -1);
+    std::unordered_map<int, int> loc;
+    loc.reserve(2 * (size_t)m);
     for (int i = 0; i < m; ++i) loc[idx[i]] = i;
     std::vector<double> a((size_t)m * m, 0.0);
     for (int i = 0; i < m; ++i)
-      for (int p = A.rp[idx[i]]; p < A.rp[idx[i] + 1]; ++p) a[(size_t)i * m + loc[A.ci[p]]] = A.v[p];
+      for (int p = A.rp[idx[i]]; p < A.rp[idx[i] + 1]; ++p) a[(size_t)i * m + loc.at(A.ci[p])] = A.v[p];
     // symmetrise (Galerkin products are symmetric up to rounding)
     for (int i = 0; i < m; ++i)
       for (int j = 0; j < i; ++j) {
         double v = 0.5 * (a[(size_t)i * m + j] + a[(size_t)j * m + i]);
         a[(size_t)i * m + j] = a[(size_t)j * m + i] = v;
       }
-    std::vector<double> L((size_t)m * m, 0.0);
+    std::vector<double> &L = Ls[kc];
+    L.assign((size_t)m * m, 0.0);
     for (int j = 0; j < m; ++j) {
+      const double *Lj = L.data() + (size_t)j * m;
       double s = a[(size_t)j * m + j];
-      for (int k = 0; k < j; ++k) s -= L[(size_t)j * m + k] * L[(size_t)j * m + k];
+      for (int k = 0; k < j; ++k) s -= Lj[k] * Lj[k];
       if (s <= 1e-14 * dmax) s = std::max(1e-14 * dmax, 1e-300);
       const double ljj = std::sqrt(s);
       L[(size_t)j * m + j] = ljj;
-      for (int i = j + 1; i < m; ++i) {
+      // four rows at a time: independent chains, each summed in the order of k
+      int i = j + 1;
+      for (; i + 4 <= m; i += 4) {
+        const double *L0 = L.data() + (size_t)i * m, *L1 = L0 + m, *L2 = L1 + m, *L3 = L2 + m;
+        double t0 = a[(size_t)i * m + j], t1 = a[(size_t)(i + 1) * m + j], t2 = a[(size_t)(i + 2) * m + j],
+               t3 = a[(size_t)(i + 3) * m + j];
+        for (int k = 0; k < j; ++k) {
+          const double l = Lj[k];
+          t0 -= L0[k] * l;
+          t1 -= L1[k] * l;
+          t2 -= L2[k] * l;
+          t3 -= L3[k] * l;
+        }
+        L[(size_t)i * m + j] = t0 / ljj;
+        L[(size_t)(i + 1) * m + j] = t1 / ljj;
+        L[(size_t)(i + 2) * m + j] = t2 / ljj;
+        L[(size_t)(i + 3) * m + j] = t3 / ljj;
+      }
+      for (; i < m; ++i) {
+        const double *Li = L.data() + (size_t)i * m;
         double t = a[(size_t)i * m + j];
-        for (int k = 0; k < j; ++k) t -= L[(size_t)i * m + k] * L[(size_t)j * m + k];
+        for (int k = 0; k < j; ++k) t -= Li[k] * Lj[k];
         L[(size_t)i * m + j] = t / ljj;
       }
     }
-    std::vector<double> col(m);
-    for (int c = 0; c < m; ++c) {
-      for (int i = 0; i < m; ++i) {
-        double t = (i == c) ? 1.0 : 0.0;
-        for (int k = 0; k < i; ++k) t -= L[(size_t)i * m + k] * col[k];
-        col[i] = t / L[(size_t)i * m + i];
+    std::vector<double> &LT = LTs[kc];
+    LT.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i)
+      for (int k = 0; k <= i; ++k) LT[(size_t)k * m + i] = L[(size_t)i * m + k];
+  });
+  // kCols columns c0 + [0, kCols) of a component's inverse at a time: L y = e_c (y is
+  // zero above c), then L^T x = y, the columns as independent chains over one pass of
+  // L (each summed in the order of k; the rows above a column's c add exact zeros)
+  constexpr int kCols = 8;
+  std::vector<std::pair<int, int>> tasks;
+  for (int kc = 0; kc < nc; ++kc)
+    for (int c0 = 0; c0 < (int)comps[kc].size(); c0 += kCols) tasks.emplace_back(kc, c0);
+  run_tasks((int)tasks.size(), [&](int task) {
+    const int kc = tasks[task].first, c0 = tasks[task].second;
+    const std::vector<int> &idx = comps[kc];
+    const std::vector<double> &L = Ls[kc], &LT = LTs[kc];
+    const int m = (int)idx.size();
+    const int nk = std::min(kCols, m - c0);
+    std::vector<double> y((size_t)m * kCols, 0.0);  // [row][column of the chunk]
+    for (int i = c0; i < m; ++i) {
+      const double *Li = L.data() + (size_t)i * m;
+      double t[kCols];
+      for (int q = 0; q < kCols; ++q) t[q] = (i == c0 + q) ? 1.0 : 0.0;
+      for (int k = c0; k < i; ++k) {
+        const double l = Li[k];
+        const double *yk = y.data() + (size_t)k * kCols;
+        for (int q = 0; q < kCols; ++q) t[q] -= l * yk[q];
       }
-      for (int i = m - 1; i >= 0; --i) {
-        double t = col[i];
-        for (int k = i + 1; k < m; ++k) t -= L[(size_t)k * m + i] * col[k];
-        col[i] = t / L[(size_t)i * m + i];
-      }
-      for (int i = 0; i < m; ++i) inv[(size_t)idx[i] * n + idx[c]] = col[i];
+      for (int q = 0; q < kCols; ++q) y[(size_t)i * kCols + q] = t[q] / Li[i];
     }
-  };
-  const int nc = (int)comps.size();
-  const int nth = std::min(nc, host_threads());
-  if (nth <= 1) {
-    for (const auto &idx : comps) invert(idx);
-  } else {
-    std::vector<std::thread> th;
-    for (int t = 0; t < nth; ++t)
-      th.emplace_back([&, t]() {
-        for (int k = t; k < nc; k += nth) invert(comps[k]);
-      });
-    for (auto &x : th) x.join();
-  }
-  return inv;
+    for (int i = m - 1; i >= 0; --i) {
+      const double *LTi = LT.data() + (size_t)i * m;
+      double t[kCols];
+      for (int q = 0; q < kCols; ++q) t[q] = y[(size_t)i * kCols + q];
+      for (int k = i + 1; k < m; ++k) {
+        const double l = LTi[k];
+        const double *yk = y.data() + (size_t)k * kCols;
+        for (int q = 0; q < kCols; ++q) t[q] -= l * yk[q];
+      }
+      for (int q = 0; q < kCols; ++q) y[(size_t)i * kCols + q] = t[q] / LTi[i];
+    }
+    for (int q = 0; q < nk; ++q) {
+      double *dst = inv_t.data() + (size_t)idx[c0 + q] * n;
+      for (int i = 0; i < m; ++i) dst[idx[i]] = y[(size_t)i * kCols + q];
+    }
+  });
 }
 
 }  // namespace
@@ -484,7 +559,7 @@ void build_amg(HostCsr A, const AmgParams &prm, AmgHost &H) {
               "auxiliary AMG stopped coarsening at " + std::to_string(A.n) + " unknowns");
   H.ncoarse = A.n;
   const double t0 = now_ms();
-  H.coarse_inv = dense_inverse(A);
+  dense_inverse(A, H.coarse_inv_t, H.coarse_span);
   tick("amg_coarse_ms", t0);
 }
 

@@ -67,12 +67,16 @@ struct AmgLevelHost {
 struct AmgHost {
   std::vector<AmgLevelHost> levels;  // levels[0].A is the fine matrix
   int ncoarse = 0;
-  std::vector<double> coarse_inv;    // dense ncoarse x ncoarse
+  // dense inverse of the coarsest matrix, transposed: row j holds column j (ncoarse x
+  // ncoarse); coarse_span[2 i], coarse_span[2 i + 1]: first and one past the last row of
+  // row i's connected component (the inverse is zero outside it)
+  std::vector<double> coarse_inv_t;
+  std::vector<int> coarse_span;
 };
 
 struct AmgParams {
   double theta = 0.08;     // strength threshold |a_ij| >= theta sqrt(a_ii a_jj)
-  int max_coarse = 256;    // dense solve below this size
+  int max_coarse = 256;    // dense solve at or below this size
   int max_levels = 20;
   int smooth_levels = 0;   // levels with a Jacobi-smoothed prolongator (then plain aggregation)
   bool additive = true;    // auxiliary cycle 0-(G+Pi)-0 instead of 0-G-Pi-G-0

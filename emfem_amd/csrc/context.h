@@ -337,6 +337,9 @@ inline PcOptions pc_options(const emfem_context *c) {
   o.inner_max_it = c->opti("B_ksp_max_it");
   o.amg.theta = c->optd("ams_theta");
   o.amg.smooth_levels = c->opti("ams_smooth_levels");
+  o.amg.max_coarse = c->opti("ams_max_coarse");
+  EMF_REQUIRE(o.amg.max_coarse >= 1 && o.amg.max_coarse <= kDenseMax, kErrArg,
+              "ams_max_coarse must lie in [1, " + std::to_string(kDenseMax) + "]");
   o.amg.additive = c->opt("ams_cycle") == "additive";
   o.use_graph = c->opt("use_graph") == "true";
   o.aux_csr = c->opt("ams_transfer") == "csr";

@@ -297,10 +297,12 @@ template <typename T>
 void diag_scale(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x);
 
 // Dense coarse solve in fp64: x[i][c] = sum_j AinvT[j][i] b[j][c], AinvT being the
-// transposed inverse (row j holds column j of the inverse); n <= kDenseMax.
+// transposed inverse (row j holds column j of the inverse); n <= kDenseMax.  range
+// (null: all of j) gives for every 64-row block i0 = 64 k the j-interval
+// [range[2k], range[2k + 1]) outside which AinvT[j][i] is zero for all of its rows.
 constexpr int kDenseMax = 4096;
 template <typename T>
-void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T *x);
+void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const int *range, const T *b, T *x);
 
 // beta[k] of fcg_px stored (for products that take it per column).
 void fcg_beta(hipStream_t s, int nc, const double *zq, const double *pq, const uint8_t *active, double *beta);

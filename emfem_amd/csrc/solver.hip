@@ -132,10 +132,21 @@ class DevAmg {
       }
     }
     ncoarse_ = H.ncoarse;
-    std::vector<double> t((size_t)ncoarse_ * ncoarse_);
-    for (int i = 0; i < ncoarse_; ++i)
-      for (int j = 0; j < ncoarse_; ++j) t[(size_t)j * ncoarse_ + i] = H.coarse_inv[(size_t)i * ncoarse_ + j];
-    cinv_.upload(t, s);  // transposed for coalesced reads in the dense solve
+    cinv_.upload(H.coarse_inv_t, s);  // transposed: coalesced reads in the dense solve
+    // the inverse is block-diagonal over the coarse matrix's connected components
+    // (the auxiliary blocks, the models of a batch): each 64-row block of the dense
+    // solve sums over the columns [lo, hi) of its rows' components only
+    std::vector<int> range(2 * (size_t)((ncoarse_ + 63) / 64));
+    for (int blk = 0; blk < (ncoarse_ + 63) / 64; ++blk) {
+      int lo = ncoarse_, hi = 0;
+      for (int i = blk * 64; i < std::min(ncoarse_, blk * 64 + 64); ++i) {
+        lo = std::min(lo, H.coarse_span[2 * (size_t)i]);
+        hi = std::max(hi, H.coarse_span[2 * (size_t)i + 1]);
+      }
+      range[2 * blk] = lo;
+      range[2 * blk + 1] = hi;
+    }
+    crange_.upload(range, s);
     EMF_HIP(hipStreamSynchronize(s));
   }
 
@@ -206,7 +217,7 @@ class DevAmg {
 
   void cycle(hipStream_t s, size_t l, int nc, const T *b, T *out) {
     if (l == levels_.size()) {
-      dense_solve<T>(s, ncoarse_, nc, cinv_.get(), b, out);
+      dense_solve<T>(s, ncoarse_, nc, cinv_.get(), crange_.get(), b, out);
       return;
     }
     Level &L = levels_[l];
@@ -247,6 +258,7 @@ class DevAmg {
   std::vector<Level> levels_;
   int ncoarse_ = 0, nc_ = 0;
   DevBuf<double> cinv_;
+  DevBuf<int> crange_;  // [(n + 63) / 64][2] column range of each 64-row block
   DevBuf<T> coarse_b_, coarse_x_;
 };
 
@@ -915,7 +927,7 @@ class InnerPcImpl : public InnerPc {
     return o.cheb_degree == o_.cheb_degree && o.cheb_ratio == o_.cheb_ratio && o.cheb_fourth == o_.cheb_fourth &&
            o.inner_rtol == o_.inner_rtol && o.inner_max_it == o_.inner_max_it && o.amg.theta == o_.amg.theta &&
            o.amg.smooth_levels == o_.amg.smooth_levels && o.amg.additive == o_.amg.additive &&
-           o.spmm_rows == o_.spmm_rows && o.aux_csr == o_.aux_csr && o.col16 == o_.col16;
+           o.amg.max_coarse == o_.amg.max_coarse && o.spmm_rows == o_.spmm_rows && o.aux_csr == o_.aux_csr && o.col16 == o_.col16;
   }
 
   Csr<double> B64(hipStream_t s, int nc) {

@@ -2314,30 +2314,47 @@ __global__ void k_diag_scale(int64_t n, int nc, const T *__restrict__ dinv, cons
   x[i] = dinv[i / nc] * b[i];
 }
 
-template <typename T>
-// One block per (64 rows, column c): b[:, c] is staged in LDS; lane l of wavefront w
-// sums row i0 + l over the w-th quarter of j, reading the transposed inverse
-// AinvT[j][i] (consecutive lanes, consecutive i: coalesced), and the four quarters
-// are added in a fixed order.  n <= kDenseMax.
+// One block per (64 rows, group of CG columns c0 + [0, CG)): lane l of wavefront w
+// sums row i0 + l over the w-th quarter of [jl, jh) for all CG columns, each load of
+// the transposed inverse AinvT[j][i] (consecutive lanes, consecutive i: coalesced)
+// serving the CG columns, whose b[j][c] are the same for all lanes; the four quarters
+// are added in a fixed order.  [jl, jh) is the block's column range (all of j
+// without a range table).  Columns past nc read the last column and are not stored.
+template <typename T, int CG>
 __global__ __launch_bounds__(kBlock) void k_dense_solve(int n, int nc, const double *__restrict__ AinvT,
-                                                        const T *__restrict__ b, T *x) {
+                                                        const int *__restrict__ range, const T *__restrict__ b,
+                                                        T *x) {
   static_assert(kBlock == 256, "four wavefronts per block");
-  __shared__ double bs[kDenseMax];
-  __shared__ double part[4][64];
-  const int c = blockIdx.y;
-  for (int j = threadIdx.x; j < n; j += kBlock) bs[j] = (double)b[(size_t)j * nc + c];
-  __syncthreads();
+  __shared__ double part[4][CG][64];
+  const int c0 = blockIdx.y * CG;
+  const int jl = range ? range[2 * blockIdx.x] : 0, jh = range ? range[2 * blockIdx.x + 1] : n;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + lane;
-  const int q = (n + 3) / 4, j0 = w * q, j1 = min(n, j0 + q);
-  double s = 0.0;
+  const int q = (jh - jl + 3) / 4, j0 = jl + w * q, j1 = min(jh, j0 + q);
+  double s[CG];
+#pragma unroll
+  for (int k = 0; k < CG; ++k) s[k] = 0.0;
   if (i < n) {
-#pragma unroll 8
-    for (int j = j0; j < j1; ++j) s += AinvT[(size_t)j * n + i] * bs[j];
+    const double *a = AinvT + i;
+    const T *bc = b + c0;
+    const int last = nc - 1 - c0;
+#pragma unroll 4
+    for (int j = j0; j < j1; ++j) {
+      const double av = a[(size_t)j * n];
+#pragma unroll
+      for (int k = 0; k < CG; ++k) s[k] += av * (double)bc[(size_t)j * nc + min(k, last)];
+    }
   }
-  part[w][lane] = s;
+#pragma unroll
+  for (int k = 0; k < CG; ++k) part[w][k][lane] = s[k];
   __syncthreads();
-  if (w == 0 && i < n) x[(size_t)i * nc + c] = (T)(((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+  if (w == 0 && i < n) {
+#pragma unroll
+    for (int k = 0; k < CG; ++k)
+      if (c0 + k < nc)
+        x[(size_t)i * nc + c0 + k] =
+            (T)(((part[0][k][lane] + part[1][k][lane]) + part[2][k][lane]) + part[3][k][lane]);
+  }
 }
 
 __global__ void k_fcg_beta(int nc, const double *zq, const double *pq, const uint8_t *active, double *beta) {
@@ -2427,18 +2444,30 @@ void diag_scale(hipStream_t s, int rows, int nc, const T *dinv, const T *b, T *x
   EMF_HIP(hipGetLastError());
 }
 template <typename T>
-void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const T *b, T *x) {
-  EMF_REQUIRE(n <= kDenseMax, kErrArg, "dense coarse solve larger than its LDS stage");
+void dense_solve(hipStream_t s, int n, int nc, const double *Ainv, const int *range, const T *b, T *x) {
+  EMF_REQUIRE(n <= kDenseMax, kErrArg, "dense coarse solve larger than kDenseMax");
   EMF_REQUIRE(nc <= 65535, kErrArg, "dense coarse solve: at most 65535 columns");
   if (n <= 0 || nc <= 0) return;
-  hipLaunchKernelGGL((k_dense_solve<T>), dim3((n + 63) / 64, nc), dim3(kBlock), 0, s, n, nc, Ainv, b, x);
+  const dim3 rows((n + 63) / 64);
+#define EMF_DS(CG)                                                                                              \
+  hipLaunchKernelGGL((k_dense_solve<T, CG>), dim3(rows.x, (nc + CG - 1) / CG), dim3(kBlock), 0, s, n, nc, Ainv, \
+                     range, b, x)
+  if (nc == 1)
+    EMF_DS(1);
+  else if (nc == 2)
+    EMF_DS(2);
+  else if (nc <= 4)
+    EMF_DS(4);
+  else
+    EMF_DS(8);
+#undef EMF_DS
   EMF_HIP(hipGetLastError());
 }
 
 #define EMF_INST(T)                                                                                        \
   template void cheb_first<T>(hipStream_t, int, int, const T *, const double *, T *, T *, double);         \
   template void diag_scale<T>(hipStream_t, int, int, const T *, const T *, T *);                           \
-  template void dense_solve<T>(hipStream_t, int, int, const double *, const T *, T *);
+  template void dense_solve<T>(hipStream_t, int, int, const double *, const int *, const T *, T *);
 EMF_INST(float)
 EMF_INST(double)
 #undef EMF_INST

@@ -510,6 +510,48 @@ def test_prepared_setup_matches_setup_in_turn(precision):
     b.close()
 
 
+@pytest.mark.parametrize("max_coarse,precision,pc", [(64, "fp32", "blockdiag"), (2048, "fp32", "blockdiag"),
+                                                     (2048, "fp32", "presb"), (2048, "fp64", "presb")])
+def test_coarse_size_reaches_the_exact_solution(max_coarse, precision, pc, oracle_mod):
+    """The auxiliary AMG down to 64 unknowns (a deep hierarchy) or stopped at 2048 (one
+    level, then the dense inverse of four components of a few hundred rows, read per
+    64-row block over its components' columns only, for 3 and 6 columns in groups):
+    each reaches the exact discrete solution to the bound its true residual sets.  At
+    the same 1e-10 true residual the PRESB iterates keep more error on this mesh at
+    every coarse size (2-7e-7 at 64, 256 and 2048, against 1-6e-8 block-diagonal),
+    hence its looser bound."""
+    m = mg.halfspace_mt(n=6, rho=100.0, freq=1.0, n_rx=2)
+    ctx = _ctx(m, "-inner_precision %s -ams_max_coarse %d -A_pc_type %s" % (precision, max_coarse, pc))
+    ctx.create_linear_system(3)
+    ctx.assemble_matrix(0)
+    ctx.assemble_rhs_mt(0, api.XY_POLAR, 0)
+    ctx.assemble_rhs_mt(0, api.YX_POLAR, 1)
+    ctx.assemble_rhs_dual(2)
+    ctx.create_pc()
+    rtol = 1e-10
+    its, rr = ctx.solve_linear_system(0, 3, 300, rtol)
+    print(max_coarse, precision, pc, "outer its", its, "true relres", rr, "inner its", ctx.stat("inner_its"))
+    assert np.all(rr <= rtol * 1.0001), rr
+    rp, ci, C, M = ctx.get_matrix()
+    for slot in range(3):
+        sr, si = ctx.get_rhs(slot)
+        e = oracle_mod.solve_direct(rp, ci, C, M, sr, si)
+        bound = (1e3 if pc == "blockdiag" else 1e4) * rtol
+        assert np.linalg.norm(ctx.get_solution(slot) - e) <= bound * np.linalg.norm(e)
+    ctx.close()
+
+
+def test_coarse_size_out_of_range_is_an_argument_error():
+    m = mg.halfspace_mt(n=3, n_rx=2)
+    for bad in ("0", "4097"):
+        ctx = _ctx(m, "-ams_max_coarse %s" % bad)
+        ctx.create_linear_system(1)
+        ctx.assemble_matrix(0)
+        with pytest.raises(api.EmfemError):
+            ctx.create_pc()
+        ctx.close()
+
+
 @pytest.mark.parametrize("cells,fits", [((64, 64, 8), True), ((72, 72, 8), False)])
 def test_16bit_columns_only_where_every_group_fits(cells, fits):
     """The 16-bit column form is taken when the columns of every aligned group of 64
