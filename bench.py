@@ -18,9 +18,11 @@ timed beside it for the speed-up.
 Beside the headline, in `extra`: the marine-CSEM workload at its full size (3.0M
 edges, 64 x-directed dipoles x 0.25 / 0.75 / 1.25 Hz = 192 right-hand sides, one
 64-column block per frequency) against the plain fp64 single-RHS path, with its own
-roofline; the goal-oriented adaptive loop's accuracy against wall time; the uneven
-multi-model batch; and the CPU stand-in.  A config-3 frequency takes about 40 s, so
-it is measured once per run rather than as the K timed steps.
+roofline; the goal-oriented adaptive loop's accuracy against wall time (plain fp64
+single-RHS, fp64 block, mixed and surrogate-guided loops, with the time to 1% median
+receiver error on a layered earth against its exact solution); the uneven
+multi-model batch; and the CPU stand-in.  A marine-CSEM frequency takes about 18 s,
+so it is measured once per run rather than as the K timed steps.
 """
 import argparse
 import json
@@ -326,14 +328,28 @@ def batch_leg(api, mg, n_models, n_single=8):
     return out
 
 
+# Modes of the adaptive legs.  plain_fp64 is the plain FP64 single-RHS path of the
+# comparisons (fp64 cycle, block-diagonal outer preconditioner, one solve per
+# right-hand side in the reference's order, src/em_fwd.cc:1345-1355); fp64_block
+# solves the same fp64 systems with the right-hand sides as one block (the mixed
+# modes' batching without their precision).
+def adaptive_modes():
+    return (("plain_fp64", "-inner_precision fp64 -forward_block_rhs false " + PLAIN_PC, False),
+            ("fp64_block", "-inner_precision fp64 " + PLAIN_PC, False),
+            ("mixed_fp32", "-inner_precision fp32 " + mixed_pc(), False),
+            ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true " + mixed_pc(), True))
+
+
 def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
     """Accuracy against wall time of the goal-oriented adaptive MT loop (em_forward):
     the adaptive-MT model (topography, anisotropic blocks) on a cells^3 grid at two
     frequencies with 36 receivers 1 m below the surface, Z_xy and Z_yx.  Target: the
     fine-mesh FP64 forward on the same model refined uniformly twice.  The loop runs
-    with k = 0..max_cycles refinement cycles (each run timed from scratch) in three
-    modes: plain (fp64 inner solve), mixed (fp32 inner solve), and mixed with the
-    learned indicator in the intermediate cycles (trained beforehand on plain cycles of
+    with k = 0..max_cycles refinement cycles (each run timed from scratch) in four
+    modes (adaptive_modes): plain (fp64 inner solve, one solve per right-hand side as
+    the reference runs them), fp64 with the right-hand sides as one block, mixed (fp32
+    inner solve, block), and mixed with the learned indicator in the intermediate
+    cycles (trained beforehand on plain cycles of
     16 uneven-batch models at 1e-3..10 Hz and 16 models of the same family with
     topography and anisotropic blocks; data generation and training are timed
     separately).
@@ -371,9 +387,7 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
            "surrogate_training": {"elements": int(F.shape[0]), "data_s": round(t1 - t0, 2),
                                   "train_s": round(t2 - t1, 2), "loss_first": round(float(loss[0]), 4),
                                   "loss_last": round(float(loss[-1]), 4)}}
-    modes = (("plain_fp64", "-inner_precision fp64 " + PLAIN_PC, False),
-             ("mixed_fp32", "-inner_precision fp32 " + mixed_pc(), False),
-             ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true " + mixed_pc(), True))
+    modes = adaptive_modes()
     for name, opts, use_sur in modes:
         if use_sur and params is None:
             continue
@@ -402,9 +416,14 @@ def adaptive_leg(api, mg, cells=10, max_cycles=7, fraction=0.2):
             hit = [t["s"] for t in trace if t[key] <= thr]
             res["time_to_%s_%g" % (key.split("_")[0], thr)] = hit[0] if hit else None
         out[name] = res
-    for name in ("mixed_fp32", "mixed_surrogate"):
+    for name in ("fp64_block", "mixed_fp32", "mixed_surrogate"):
+        if name not in out:
+            continue
         a, b = out[name]["time_to_median_0.05"], out["plain_fp64"]["time_to_median_0.05"]
         out[name + "_vs_plain_time_to_median_5pct"] = round(a / b, 3) if a and b else None
+        if name != "fp64_block":
+            c = out["fp64_block"]["time_to_median_0.05"]
+            out[name + "_vs_fp64_block_time_to_median_5pct"] = round(a / c, 3) if a and c else None
         # same number of cycles: wall time and median error relative to the plain loop
         pt, mt = out["plain_fp64"]["trace"], out[name]["trace"]
         out[name + "_vs_plain_same_cycles"] = {
@@ -422,15 +441,14 @@ def layered_leg(api, mg, params, max_cycles=6, fraction=0.2):
     solution is that field and the loop's FE solutions converge to it; the target is
     the closed-form impedance at each receiver (meshgen.layered_impedance).  Two frequencies
     (0.1 and 10 Hz), Z_xy and Z_yx at 36 receivers.  Each k = 0..max_cycles runs the
-    forward from scratch, plain (fp64 inner solve), mixed (fp32) and mixed with the
-    learned indicator in the intermediate cycles."""
+    forward from scratch in the modes of adaptive_modes (plain fp64 single-RHS, fp64
+    block, mixed fp32 block, mixed with the learned indicator in the intermediate
+    cycles)."""
     m, target = mg.layered_mt([0.1, 10.0])
     out = {"workload": "flat layered earth (100 / 10 / 300 ohm-m, interfaces 600 m and 1500 m) on a graded Kuhn "
                        "mesh of %d edges, 2 frequencies (0.1, 10 Hz), 36 receivers, Z_xy and Z_yx; target: the exact "
                        "1-D impedance" % (7 * m.n_vertices)}
-    modes = (("plain_fp64", "-inner_precision fp64 " + PLAIN_PC, False),
-             ("mixed_fp32", "-inner_precision fp32 " + mixed_pc(), False),
-             ("mixed_surrogate", "-inner_precision fp32 -surrogate_cycles true " + mixed_pc(), True))
+    modes = adaptive_modes()
     for name, opts, use_sur in modes:
         if use_sur and params is None:
             continue
@@ -462,11 +480,14 @@ def layered_leg(api, mg, params, max_cycles=6, fraction=0.2):
             hit = [t["s"] for t in trace if t[key] <= thr]
             res["time_to_%s_%g" % (key.split("_")[0], thr)] = hit[0] if hit else None
         out[name] = res
-    for name in ("mixed_fp32", "mixed_surrogate"):
+    for name in ("fp64_block", "mixed_fp32", "mixed_surrogate"):
         if name not in out:
             continue
         a, b = out[name]["time_to_median_0.01"], out["plain_fp64"]["time_to_median_0.01"]
         out[name + "_vs_plain_time_to_median_1pct"] = round(a / b, 3) if a and b else None
+        if name != "fp64_block":
+            c = out["fp64_block"]["time_to_median_0.01"]
+            out[name + "_vs_fp64_block_time_to_median_1pct"] = round(a / c, 3) if a and c else None
     return out
 
 

@@ -105,6 +105,10 @@ const std::map<std::string, std::string> &option_defaults() {
       {"ams_setup", "device"},
       {"surrogate_cycles", "false"},
       {"surrogate_min_corr", "0.9"},
+      // the forward driver's right-hand sides of one system (XY, YX and the dual; the
+      // transmitters of a CSEM group) as one block solve, or one solve each in the
+      // reference's order (src/em_fwd.cc:1345-1355)
+      {"forward_block_rhs", "true"},
   };
   return d;
 }
@@ -129,6 +133,7 @@ const std::set<std::string> &enum_values(const std::string &key) {
       {"ams_setup", {"device", "host"}},
       {"ams_cheb_kind", {"first", "fourth"}},
       {"surrogate_cycles", {"true", "false"}},
+      {"forward_block_rhs", {"true", "false"}},
       {"save_mesh", {"true", "false"}},
   };
   static const std::set<std::string> none;

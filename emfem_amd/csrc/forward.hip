@@ -250,6 +250,11 @@ void solve_primal_dual(emfem_context *c, int first, int n) {
 }
 
 void solve_slots(emfem_context *c, int first, int n, const std::vector<double> &rtols) {
+  if (n > 1 && c->opt("forward_block_rhs") == "false") {
+    // the reference's sequence: one solve per right-hand side (src/em_fwd.cc:1345-1355)
+    for (int k = 0; k < n; ++k) solve_slots(c, first + k, 1, {rtols[k]});
+    return;
+  }
   std::vector<int> its(n);
   std::vector<double> rr(n);
   const int rc = emfem_solve_linear_system(c, first, n, c->opti("K_max_it"), rtols[0], rtols.data(), its.data(), rr.data());

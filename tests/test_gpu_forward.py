@@ -18,9 +18,13 @@ def _oracle_fields(oracle_mod, m, pts, tets, attr, freq, rhs_list):
     return om, mm, sols
 
 
-def test_forward_mt_adaptive(oracle_mod, tmp_path):
+@pytest.mark.parametrize("block", ["true", "false"])
+def test_forward_mt_adaptive(block, oracle_mod, tmp_path):
+    """One adaptive cycle, the right-hand sides solved as one block or one at a time
+    (the reference's order): the final mesh's responses match the exact solution's."""
     m = mg.tiny_mt(n=3, aniso_form=1, seed=3, freq=0.5)
-    ctx = api.Context("-max_adaptive_refinements 1 -refine_fraction 0.3 -e_rtol 1e-10 -inner_precision fp32")
+    ctx = api.Context("-max_adaptive_refinements 1 -refine_fraction 0.3 -e_rtol 1e-10 -inner_precision fp32 "
+                      "-forward_block_rhs %s" % block)
     ctx.load_model(m)
     ctx.set_observations(m.obs)
     ctx.forward()
@@ -44,7 +48,8 @@ def test_forward_mt_adaptive(oracle_mod, tmp_path):
     ctx.save_rsp(tmp_path / "mt.rsp")
 
 
-def test_forward_csem_batched(oracle_mod):
+@pytest.mark.parametrize("block", ["true", "false"])
+def test_forward_csem_batched(block, oracle_mod):
     m = mg.tiny_mt(n=3, aniso_form=0, seed=1, freq=1.25)
     m.tx = np.array([[-800.0, 100.0, 400.0, 0.0, 0.0, 1.0, 0.0],
                      [0.0, -200.0, 600.0, 90.0, 0.0, 1.0, 0.0],
@@ -55,7 +60,7 @@ def test_forward_csem_batched(oracle_mod):
             for code in (mg.F_EX_RI, mg.F_EY_RI, mg.F_HZ_RI, mg.F_EX_AP):
                 obs.append((code, 0, t, r))
     m.obs = np.array(obs, dtype=np.int32)
-    ctx = api.Context("-e_rtol 1e-10 -n_tx_divisions 5")
+    ctx = api.Context("-e_rtol 1e-10 -n_tx_divisions 5 -forward_block_rhs %s" % block)
     ctx.load_model(m)
     ctx.set_observations(m.obs)
     ctx.forward()
