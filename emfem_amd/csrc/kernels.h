@@ -60,22 +60,24 @@ constexpr int kXnCols = 256;  // distinct columns per narrow x-block
 constexpr int kXnRows = 64;
 
 // Threads per workgroup of the row-block SpMM: one wavefront for fp32 blocks of 3 to
-// 6 columns (seven nonzeros per thread, all of whose loads are in flight together),
-// else 256.
+// 6 columns (seven or nine nonzeros per thread, all of whose loads are in flight
+// together), else 256.
 template <typename T, int NC>
 constexpr int stream_block() {
   return sizeof(T) == 4 && NC >= 3 && NC <= 6 ? 64 : 256;
 }
 
-// Nonzeros staged per workgroup by the row-block SpMM: 448 for the one-wavefront
-// blocks, 768 for fp64 blocks of 3 columns (measured: 80.6 us against 84.6 us for
-// 1024 at 1.55M edges), else 16 KB of products in fp32, 24 KB in fp64 (a multiple of
-// the 256-thread block, at most 2048).
+// Nonzeros staged per workgroup by the row-block SpMM: for the one-wavefront blocks
+// 576 at 3 columns (nine nonzeros per thread; at 1.54M edges 49.8 us per launch
+// against 51.6 for 448, 50.7 for 512 and 50.2 for 640) and 448 at 4 to 6 (seven),
+// 768 for fp64 blocks of 3 columns (80.6 us against 84.6 us for 1024), else 16 KB of
+// products in fp32, 24 KB in fp64 (a multiple of the 256-thread block, at most 2048).
 template <typename T, int NC>
 constexpr int stream_tile() {
   constexpr int bytes = sizeof(T) == 8 ? 24576 : 16384;
   constexpr int t = (bytes / (NC * (int)sizeof(T))) / 256 * 256;
-  return stream_block<T, NC>() == 64 ? 7 * 64 : (sizeof(T) == 8 && NC == 3 ? 768 : (t > 2048 ? 2048 : t));
+  return stream_block<T, NC>() == 64 ? (NC == 3 ? 9 * 64 : 7 * 64)
+                                     : (sizeof(T) == 8 && NC == 3 ? 768 : (t > 2048 ? 2048 : t));
 }
 int stream_tile_for(int nc, bool fp64);
 
