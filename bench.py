@@ -161,7 +161,7 @@ CSEM_SMOOTHER = "-ams_cheb_degree 8 -ams_cheb_ratio 120"
 
 
 def csem_leg(api, mg, cells, n_tx=64, n_plain=2, plain_fidx=1, n_freqs=None):
-    """Marine CSEM (the north star's third configuration): n_tx x-directed dipoles at
+    """Marine CSEM (3.0M edges at the default cells): n_tx x-directed dipoles at
     each of the model's frequencies (0.25 / 0.75 / 1.25 Hz), each frequency one block of
     n_tx right-hand sides (fp32 inner cycle in fp64 FGMRES, true residual 1e-8),
     against the plain fp64 single-RHS path on the first n_plain transmitters of one
@@ -271,7 +271,7 @@ def smoother_kernel_name(rows):
 
 
 def batch_leg(api, mg, n_models, n_single=8):
-    """Uneven multi-model batch (the north star's fourth configuration): n_models
+    """Uneven multi-model batch (MT models of 17k..178k edges): n_models
     independent MT models (edges log-uniform in 2e4..2e5, one random frequency each),
     XY and YX per model, solved as one block-diagonal batch with per-model Krylov
     processes, once in lock-step (the default) and once in stages that retire converged
@@ -627,7 +627,7 @@ def main():
         extra["speedup_vs_plain_solve"] = round(p_solve / mixed_solve, 3)
         ctxp.close()
     if not args.no_csem:
-        extra["config3_marine_csem"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")),
+        extra["marine_csem_64tx_3freq"] = csem_leg(api, mg, tuple(int(v) for v in args.csem_cells.split(",")),
                                                 n_plain=args.csem_plain, n_freqs=args.csem_freqs)
     if not args.no_adaptive:
         extra["adaptive"] = adaptive_leg(api, mg)

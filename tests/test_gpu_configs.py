@@ -1,4 +1,4 @@
-"""GPU parity at the north-star workloads' own settings, through the forward driver
+"""GPU parity at the plain-MT and marine-CSEM workloads' own settings, through the forward driver
 (em_forward, src/em_fwd.cc:1441-1471), against the committed golden responses of the
 oracle's exact discrete solutions (tests/golden/make_golden.py):
 
@@ -47,7 +47,7 @@ def _forward(m, opts):
 
 @pytest.mark.parametrize("pc", ["blockdiag", "presb"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_config1_plain_mt(precision, pc):
+def test_plain_mt_62k_edges(precision, pc):
     obs, ref, ne = _golden("plain_mt.npz")
     m = mg.plain_mt()
     assert np.array_equal(obs, m.obs)
@@ -57,7 +57,7 @@ def test_config1_plain_mt(precision, pc):
 
 
 @pytest.mark.parametrize("pc", ["blockdiag", "presb"])
-def test_config3_marine_csem_64tx_3freq(pc):
+def test_marine_csem_64tx_3freq(pc):
     obs, ref, ne = _golden("marine_csem_16x16x14.npz")
     m = G.marine_csem_model()
     assert np.array_equal(obs, m.obs) and len(m.tx) == 64 and len(m.freqs) == 3
@@ -73,4 +73,4 @@ def test_config3_marine_csem_64tx_3freq(pc):
                 err = np.max(np.abs(rsp[g] - ref[g])) / scale
                 worst = max(worst, err)
                 assert err <= 1e-6, (f, t, group, err)
-    print("config 3 (64 tx x 3 frequencies): worst receiver field error %.2e" % worst)
+    print("marine CSEM (64 tx x 3 frequencies): worst receiver field error %.2e" % worst)
