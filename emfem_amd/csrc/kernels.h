@@ -280,9 +280,17 @@ void interleaved_to_split(hipStream_t s, int rows, int n, const double *x, doubl
 // then the convergence test on the device: active[k] = 0 once rr[k] <= tol2 * bb[k].
 // rr and the updated flags are also written to h_rr and h_active, pinned host arrays
 // the caller reads after synchronising s.
+// With cf non-null the same pass also writes the fp32 auxiliary-space cycle's first
+// inputs for the new r: cf[0] = (float) r and cf[1] = (float) (c0 * (double) (dinv[row]
+// * (float) r)), the cycle's copy of its right-hand side and first Chebyshev iterate.
+struct CycleFill {
+  float *bt = nullptr, *x1 = nullptr;
+  const float *dinv = nullptr;
+  double c0 = 0.0;
+};
 void fcg_residual(hipStream_t s, int rows, int nc, const double *rz, const double *pq, uint8_t *active,
                   double *alpha, const double *q, double *r, double *rr, const double *bb, double tol2, double *h_rr,
-                  uint8_t *h_active, double *work, const Seg &sg = Seg());
+                  uint8_t *h_active, double *work, const Seg &sg = Seg(), const CycleFill &cf = CycleFill());
 // x += alpha[k] p ; p = z + beta p with beta = active[k] && pq[k] != 0 ? -zq[k] / pq[k]
 // : 0 formed per element (k = seg(row) * nc + column)
 template <typename TZ>

@@ -611,18 +611,20 @@ class DevAms {
   // out = M_AMS b: b fp64 compact [E][nc], out in T; out must not alias b.  With
   // use_graph the ~100 launches of one cycle are captured once per (nc, b, out)
   // and replayed as a single hipGraph launch (no memset nodes are captured).
-  void apply(hipStream_t s, int nc, const double *b, T *out, bool use_graph) {
+  // prefilled: the caller has written the cycle's copy of b and first Chebyshev
+  // iterate (fill), so the cycle starts at its first SpMM
+  void apply(hipStream_t s, int nc, const double *b, T *out, bool use_graph, bool prefilled = false) {
     tune(s, nc);
     if (!use_graph) {
-      launch_cycle(s, nc, b, out);
+      launch_cycle(s, nc, b, out, prefilled);
       return;
     }
-    const auto key = std::make_tuple(nc, (const void *)b, (const void *)out);
+    const auto key = std::make_tuple(nc, (const void *)b, (const void *)out, prefilled);
     auto it = graphs_.find(key);
     if (it == graphs_.end()) {
       hipGraph_t g;
       EMF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      launch_cycle(s, nc, b, out);
+      launch_cycle(s, nc, b, out, prefilled);
       EMF_HIP(hipStreamEndCapture(s, &g));
       hipGraphExec_t ex;
       EMF_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
@@ -630,6 +632,21 @@ class DevAms {
       it = graphs_.emplace(key, ex).first;
     }
     EMF_HIP(hipGraphLaunch(it->second, s));
+  }
+
+  // Where the fp32 cycle on nc columns reads its copy of b and its first Chebyshev
+  // iterate (x_1 = c0 Dinv b, in d), for a caller that forms them (apply with
+  // prefilled); valid until the cycle's buffers grow (a wider nc).
+  CycleFill fill(hipStream_t s, int nc) {
+    CycleFill f;
+    if constexpr (std::is_same<T, float>::value) {
+      tune(s, nc);
+      f.bt = bt_.get();
+      f.x1 = d_.get();
+      f.dinv = l1inv_.get();
+      f.c0 = c0_;
+    }
+    return f;
   }
 
   // ms per launch of the smoother SpMM in the cycle's form on nc columns (lanes per
@@ -660,7 +677,7 @@ class DevAms {
   // blocks, d and out: pre-smoothing starts in d, and the blocks are chosen so that
   // post-smoothing (deg writes) ends in out: the middle iterate lives in d for odd deg,
   // in out for even deg.
-  void launch_cycle(hipStream_t s, int nc, const double *b, T *out) {
+  void launch_cycle(hipStream_t s, int nc, const double *b, T *out, bool prefilled = false) {
     T *d = d_.get();
     const T *bt;
     T *btw = nullptr;  // the fp32 copy of b, when the cycle runs in fp32
@@ -671,7 +688,7 @@ class DevAms {
       bt = btw;
     }
     T *mid = (deg_ & 1) ? d : out, *other = (deg_ & 1) ? out : d;
-    cheb_first<T>(s, ne_, nc, l1inv_.get(), b, btw, d, c0_);
+    if (!prefilled) cheb_first<T>(s, ne_, nc, l1inv_.get(), b, btw, d, c0_);
     T *xm = cheb_x(s, nc, bt, out, d, true);
     EMF_REQUIRE(xm == mid, kErrState, "Chebyshev buffer plan");
     if (!additive_) aux_gradient(s, nc, b, mid);
@@ -837,7 +854,7 @@ class DevAms {
   DevBuf<int> vrp_, vcode_;
   DevBuf<T> th_;
   DevAmg<T> amgG_, amgPi_;
-  std::map<std::tuple<int, const void *, const void *>, hipGraphExec_t> graphs_;
+  std::map<std::tuple<int, const void *, const void *, bool>, hipGraphExec_t> graphs_;
   std::map<int, int> lpr_, lpr64_;
   std::map<int, DevBuf<int>> blk_, blk64_;
   bool xb_tried_ = false, xn_tried_ = false;
@@ -965,13 +982,14 @@ class InnerPcImpl : public InnerPc {
 
   // z = M r: the cycle in its precision, z in that precision (the FCG kernels widen
   // fp32 entries exactly as they read them)
-  void precondition(hipStream_t s, int nc, const double *r, double *z, SolveStats &st) {
+  void precondition(hipStream_t s, int nc, const double *r, double *z, SolveStats &st, bool prefilled = false) {
+    EMF_REQUIRE(!prefilled, kErrState, "the fp64 cycle forms its own first inputs");
     ++st.vcycles;
     a64_->apply(s, nc, r, z, o_.use_graph);
   }
-  void precondition(hipStream_t s, int nc, const double *r, float *z, SolveStats &st) {
+  void precondition(hipStream_t s, int nc, const double *r, float *z, SolveStats &st, bool prefilled = false) {
     ++st.vcycles;
-    a32_->apply(s, nc, r, z, o_.use_graph);
+    a32_->apply(s, nc, r, z, o_.use_graph, prefilled);
   }
   void solve(hipStream_t s, int nc, const double *b, double *x, SolveStats &st) {
     if (fp32_)
@@ -1024,10 +1042,14 @@ class InnerPcImpl : public InnerPc {
       bspmm(s, nc, p, q);
     coldot<double, double>(s, ne_, nc, p, nc, q, nc, pq_.get(), w, sg_);
     coldot<double, TZ>(s, ne_, nc, r, nc, z, nc, rz_.get(), w, sg_);
+    // the fp32 cycle's copy of r and first Chebyshev iterate are formed in the residual
+    // pass (the cycle's buffers are sized for nc by the first application above)
+    CycleFill cyc;
+    if constexpr (std::is_same<TZ, float>::value) cyc = a32_->fill(s, nc);
     // x += alpha p is delayed into the pass that forms the next p (or done on exit)
     for (int it = 1; it <= o_.inner_max_it; ++it) {
       fcg_residual(s, ne_, nc, rz_.get(), pq_.get(), active_.get(), alpha_.get(), q, r, rr_.get(), bb_.get(), tol2,
-                   h_rr_.get(), h_act_.get(), w, sg_);
+                   h_rr_.get(), h_act_.get(), w, sg_, cyc);
       ++st.inner_its;
       EMF_HIP(hipStreamSynchronize(s));
       any = false;
@@ -1040,7 +1062,7 @@ class InnerPcImpl : public InnerPc {
         axpby_cols<double, double>(s, ne_, nc, alpha_.get(), p, nc, ones_.get(), x, nc, sg_);
         break;
       }
-      precondition(s, nc, r, z, st);
+      precondition(s, nc, r, z, st, cyc.bt != nullptr);
       coldot2<TZ>(s, ne_, nc, z, nc, q, r, nc, zq_.get(), rz_.get(), w, sg_);
       fcg_px<TZ>(s, ne_, nc, alpha_.get(), zq_.get(), pq_.get(), active_.get(), z, p, x, sg_);
       if (from_z) {

@@ -2210,11 +2210,14 @@ __global__ void k_axpby(int rows, int nc, double alpha, const TX *__restrict__ x
 
 // The residual step of the flexible CG: alpha from rz, pq and the active flag (every
 // block of a segment forms the same value and stores it), r -= alpha q, and the block
-// partials of r . r in work[block][c].  A chunk of rows lies in one segment.
+// partials of r . r in work[block][c].  A chunk of rows lies in one segment.  With
+// cf.bt set, also the fp32 cycle's copy of r and its first Chebyshev iterate (the
+// values k_cheb_first forms from r).
 __global__ __launch_bounds__(kBlock) void k_fcg_res(int rows, int nc, const double *__restrict__ rz,
                                                     const double *__restrict__ pq,
                                                     const uint8_t *__restrict__ active, double *alpha,
-                                                    const double *__restrict__ q, double *r, double *work, Seg sg) {
+                                                    const double *__restrict__ q, double *r, double *work, Seg sg,
+                                                    CycleFill cf) {
   __shared__ double sh[kBlock];
   const int rpp = kBlock / nc;
   const int per_block = rpp * nc;
@@ -2243,6 +2246,11 @@ __global__ __launch_bounds__(kBlock) void k_fcg_res(int rows, int nc, const doub
           const double nr = rv[u] - a * qv[u];
           r[o] = nr;
           acc += nr * nr;
+          if (cf.bt) {
+            const float bv = (float)nr;
+            cf.bt[o] = bv;
+            cf.x1[o] = (float)(cf.c0 * (double)(cf.dinv[row + u * rpp] * bv));
+          }
         }
     }
   }
@@ -2408,10 +2416,12 @@ template void axpby<float, double>(hipStream_t, int, int, double, const float *,
 
 void fcg_residual(hipStream_t s, int rows, int nc, const double *rz, const double *pq, uint8_t *active,
                   double *alpha, const double *q, double *r, double *rr, const double *bb, double tol2, double *h_rr,
-                  uint8_t *h_active, double *work, const Seg &sg) {
+                  uint8_t *h_active, double *work, const Seg &sg, const CycleFill &cf) {
   EMF_REQUIRE(nc <= kBlock, kErrArg, "inner solve supports at most 256 columns");
+  EMF_REQUIRE((cf.bt == nullptr) == (cf.x1 == nullptr) && (cf.bt == nullptr || cf.dinv != nullptr), kErrArg,
+              "cycle fill needs its copy, first iterate and scaling");
   const int nb = reduce_blocks(rows, nc, sg);
-  hipLaunchKernelGGL(k_fcg_res, dim3(nb), dim3(kBlock), 0, s, rows, nc, rz, pq, active, alpha, q, r, work, sg);
+  hipLaunchKernelGGL(k_fcg_res, dim3(nb), dim3(kBlock), 0, s, rows, nc, rz, pq, active, alpha, q, r, work, sg, cf);
   hipLaunchKernelGGL(k_finish_conv, dim3(sg.nseg * nc), dim3(kBlock), 0, s, nb, nc, work, rr, sg, bb, tol2, active,
                      h_rr, h_active);
   EMF_HIP(hipGetLastError());
